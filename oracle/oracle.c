@@ -1,0 +1,786 @@
+/*
+ * oracle.c -- CPU restatement of wgpu-cpu's render-pass executor.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Built with -ffp-contract=off and
+ * SSE arithmetic so that every f32 operation rounds once, as the reference's
+ * Rust code and Cranelift JIT do on x86-64.
+ *
+ * The structure follows the reference, not the GPU design: one thread, the
+ * vertex shader once per index (state.rs:519-535), the clipper's plane-by-
+ * plane queue (clipper.rs:756-775), the scanline iterator row by row
+ * (scanline.rs:107-166) and the fragment shader once per emitted pixel.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct { float x, y, z, w; } v4;
+
+/* ------------------------------------------------------------------ */
+/* Scalar helpers with Rust semantics                                 */
+/* ------------------------------------------------------------------ */
+
+/* `f as u32` in Rust: NaN -> 0, saturating, truncating. */
+static uint32_t f32_as_u32(float f)
+{
+    if (!(f == f)) return 0;
+    if (f <= 0.0f) return 0;
+    if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)f;
+}
+
+/* texture.rs:376-379: (value * 255).clamp(0, 255) as u8 */
+uint8_t oracle_f32_to_unorm8(float v)
+{
+    float x = v * 255.0f;
+    if (x < 0.0f) x = 0.0f;     /* f32::clamp keeps NaN */
+    if (x > 255.0f) x = 255.0f;
+    if (!(x == x)) return 0;    /* `as u8` maps NaN to 0 */
+    return (uint8_t)x;
+}
+
+/* f32::rem_euclid (std): r = self % rhs; if r < 0 { r + |rhs| } */
+static float rem_euclid(float x, float m)
+{
+    float r = fmodf(x, m);
+    if (r < 0.0f) r = r + fabsf(m);
+    return r;
+}
+
+/* binding.rs:150-164 */
+uint32_t oracle_texel_coordinate(float x, uint32_t address_mode, uint32_t size)
+{
+    switch (address_mode) {
+    case OR_ADDR_CLAMP:
+        if (x < 0.0f) x = 0.0f;
+        if (x > 1.0f) x = 1.0f;
+        break;
+    case OR_ADDR_REPEAT:
+        x = rem_euclid(x, 1.0f);
+        break;
+    case OR_ADDR_MIRROR: {
+        float r = rem_euclid(x, 2.0f);
+        x = (r <= 1.0f) ? r : 2.0f - r;
+        break;
+    }
+    default:
+        break;
+    }
+    /* (x * (size - 1) as f32).round() as u32; round = half away from zero */
+    return f32_as_u32(roundf(x * (float)(size - 1u)));
+}
+
+static float srgb_table[256];
+static int srgb_ready = 0;
+
+const float *oracle_srgb_table(void)
+{
+    if (!srgb_ready) {
+        for (int i = 0; i < 256; ++i) {
+            double c = (double)i / 255.0;
+            double l = (c <= 0.04045) ? c / 12.92 : pow((c + 0.055) / 1.055, 2.4);
+            srgb_table[i] = (float)l;
+        }
+        srgb_ready = 1;
+    }
+    return srgb_table;
+}
+
+/* mod.rs:434-452 */
+static int compare(uint32_t fn, float value, float reference)
+{
+    switch (fn) {
+    case OR_CMP_NEVER: return 0;
+    case OR_CMP_LESS: return value < reference;
+    case OR_CMP_EQUAL: return value == reference;
+    case OR_CMP_LESS_EQUAL: return value <= reference;
+    case OR_CMP_GREATER: return value > reference;
+    case OR_CMP_NOT_EQUAL: return value != reference;
+    case OR_CMP_GREATER_EQUAL: return value >= reference;
+    case OR_CMP_ALWAYS: return 1;
+    default: return 0;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* util/sort.rs and util/scanline.rs                                  */
+/* ------------------------------------------------------------------ */
+
+typedef struct { int64_t x, y; } ip2;
+
+/* sort.rs:3-22: compare_and_swap(0,1), (0,2), (1,2); swap only on Greater */
+static void bubblesort3_by_y(ip2 v[3])
+{
+    static const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    for (int k = 0; k < 3; ++k) {
+        int l = pairs[k][0], r = pairs[k][1];
+        if (v[l].y > v[r].y) { ip2 t = v[l]; v[l] = v[r]; v[r] = t; }
+    }
+}
+
+static void bubblesort3_by_x(ip2 v[3])
+{
+    static const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    for (int k = 0; k < 3; ++k) {
+        int l = pairs[k][0], r = pairs[k][1];
+        if (v[l].x > v[r].x) { ip2 t = v[l]; v[l] = v[r]; v[r] = t; }
+    }
+}
+
+void oracle_bubblesort3_i64(int64_t v[3])
+{
+    static const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    for (int k = 0; k < 3; ++k) {
+        int l = pairs[k][0], r = pairs[k][1];
+        if (v[l] > v[r]) { int64_t t = v[l]; v[l] = v[r]; v[r] = t; }
+    }
+}
+
+/* One emitted scanline: row y, inclusive span [x1, x2]. */
+typedef struct { int64_t y, x1, x2; } scanline_t;
+
+/* Scanlines iterator state (scanline.rs:14-56, 77-171). */
+typedef struct {
+    int state; /* 0 degenerate line, 1 first half, 2 second half, 3 done */
+    int64_t ac_x, ac_y, ab_x, ab_y, bc_x, bc_y;
+    int64_t a_x, b_x, b_y, c_y, y, i, i2;
+    int64_t line_y, line_x1, line_x2;
+} scanlines_t;
+
+static void scanlines_init(scanlines_t *s, const uint32_t fb[3][2])
+{
+    ip2 tri[3];
+    for (int k = 0; k < 3; ++k) { tri[k].x = fb[k][0]; tri[k].y = fb[k][1]; }
+    bubblesort3_by_y(tri);
+    ip2 a = tri[0], b = tri[1], c = tri[2];
+    s->ac_x = c.x - a.x; s->ac_y = c.y - a.y;
+    s->ab_x = b.x - a.x; s->ab_y = b.y - a.y;
+    s->bc_x = c.x - b.x; s->bc_y = c.y - b.y;
+    if (s->ac_y == 0) {
+        bubblesort3_by_x(tri);
+        s->state = 0;
+        s->line_y = a.y;
+        s->line_x1 = tri[0].x;
+        s->line_x2 = tri[2].x;
+        return;
+    }
+    s->a_x = a.x; s->b_x = b.x; s->b_y = b.y; s->c_y = c.y;
+    s->y = a.y; s->i = 0; s->i2 = 0;
+    s->state = (s->ab_y == 0) ? 2 : 1;
+}
+
+/* Scanline::new_from_isize: order the span; a negative value panics. */
+static int make_line(scanline_t *out, int64_t y, int64_t x1, int64_t x2)
+{
+    if (x1 > x2) { int64_t t = x1; x1 = x2; x2 = t; }
+    if (y < 0 || x1 < 0 || x2 < 0 || y > 0xFFFFFFFFll || x2 > 0xFFFFFFFFll) return -1;
+    out->y = y; out->x1 = x1; out->x2 = x2;
+    return 0;
+}
+
+/* Returns 1 with a line, 0 when done, -1 where the reference panics.
+ * Rust's isize `/` truncates toward zero, as C's `/` does. */
+static int scanlines_next(scanlines_t *s, scanline_t *out)
+{
+    for (;;) {
+        switch (s->state) {
+        case 0:
+            s->state = 3;
+            return make_line(out, s->line_y, s->line_x1, s->line_x2) ? -1 : 1;
+        case 3:
+            return 0;
+        case 1:
+            if (s->y < s->b_y) {
+                int64_t x1 = s->a_x + (s->ac_x * s->i) / s->ac_y;
+                int64_t x2 = s->a_x + (s->ab_x * s->i) / s->ab_y;
+                int64_t y = s->y;
+                s->y += 1; s->i += 1;
+                return make_line(out, y, x1, x2) ? -1 : 1;
+            }
+            if (s->bc_y == 0) s->state = 3;
+            else { s->state = 2; s->i2 = 0; }
+            break;
+        case 2:
+            if (s->y <= s->c_y) {
+                int64_t x1 = s->a_x + (s->ac_x * s->i) / s->ac_y;
+                int64_t x2 = s->b_x + (s->bc_x * s->i2) / s->bc_y;
+                int64_t y = s->y;
+                s->i2 += 1;
+                s->y += 1; s->i += 1;
+                return make_line(out, y, x1, x2) ? -1 : 1;
+            }
+            s->state = 3;
+            break;
+        default:
+            return 0;
+        }
+    }
+}
+
+int64_t oracle_scanline_pixels(const uint32_t tri[6], uint32_t *out_xy, int64_t cap)
+{
+    uint32_t fb[3][2] = {{tri[0], tri[1]}, {tri[2], tri[3]}, {tri[4], tri[5]}};
+    scanlines_t s;
+    scanline_t line;
+    int64_t n = 0;
+    scanlines_init(&s, fb);
+    for (;;) {
+        int r = scanlines_next(&s, &line);
+        if (r < 0) return -1;
+        if (r == 0) break;
+        for (int64_t x = line.x1; x <= line.x2; ++x) {
+            if (n < cap && out_xy) {
+                out_xy[2 * n] = (uint32_t)x;
+                out_xy[2 * n + 1] = (uint32_t)line.y;
+            }
+            ++n;
+        }
+    }
+    return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* Clipper (clipper.rs:150-179, 625-802)                              */
+/* ------------------------------------------------------------------ */
+
+typedef struct { float p[4]; float b[3]; } cvert; /* position + barycentric */
+typedef struct { cvert v[3]; } ctri;
+
+/* ClipVolume::WEBGPU (clipper.rs:153-163): L, R, B, T, near (z >= 0), far (z <= w) */
+static const float planes[6][4] = {
+    { 1.0f, 0.0f, 0.0f, 1.0f},
+    {-1.0f, 0.0f, 0.0f, 1.0f},
+    { 0.0f, 1.0f, 0.0f, 1.0f},
+    { 0.0f, -1.0f, 0.0f, 1.0f},
+    { 0.0f, 0.0f, 1.0f, 0.0f},
+    { 0.0f, 0.0f, -1.0f, 1.0f},
+};
+
+/* nalgebra Vector4::dot: (p0*q0 + p2*q2) + (p1*q1 + p3*q3).  With the plane
+ * coefficients in {0, +-1} every grouping gives the same value up to the
+ * sign of an exact zero. */
+static float clip_distance(const float *pl, const float *p)
+{
+    float a = p[0] * pl[0];
+    float b = p[1] * pl[1];
+    float c = p[2] * pl[2];
+    float d = p[3] * pl[3];
+    a = a + c;
+    b = b + d;
+    return a + b;
+}
+
+/* interpolation.rs:84-89: lerp(x0, x1, t) = x0 * (1 - t) + x1 * t, on
+ * position and barycentric together (clipper.rs:602-622). */
+static cvert clip_lerp(const cvert *x0, const cvert *x1, float t)
+{
+    cvert r;
+    float s = 1.0f - t;
+    for (int k = 0; k < 4; ++k) r.p[k] = x0->p[k] * s + x1->p[k] * t;
+    for (int k = 0; k < 3; ++k) r.b[k] = x0->b[k] * s + x1->b[k] * t;
+    return r;
+}
+
+/* clipper.rs:625-728; returns the number of output triangles (0..2). */
+static int clip_against_plane(const ctri *in, const float *pl, ctri out[2])
+{
+    float d[3];
+    int o[3];
+    for (int k = 0; k < 3; ++k) {
+        d[k] = clip_distance(pl, in->v[k].p);
+        o[k] = d[k] < 0.0f;
+    }
+#define CLIP(i, j) clip_lerp(&in->v[i], &in->v[j], d[i] / (d[i] - d[j]))
+    const cvert *A = &in->v[0], *B = &in->v[1], *C = &in->v[2];
+    if (o[0] && o[1] && o[2]) return 0;
+    if (!o[0] && !o[1] && !o[2]) { out[0] = *in; return 1; }
+    if (o[0] && o[1] && !o[2]) {            /* clip(2,0), clip(1,2) */
+        cvert an = CLIP(2, 0), bn = CLIP(1, 2);
+        out[0].v[0] = an; out[0].v[1] = bn; out[0].v[2] = *C;
+        return 1;
+    }
+    if (!o[0] && o[1] && o[2]) {            /* clip(0,1), clip(2,0) */
+        cvert bn = CLIP(0, 1), cn = CLIP(2, 0);
+        out[0].v[0] = *A; out[0].v[1] = bn; out[0].v[2] = cn;
+        return 1;
+    }
+    if (o[0] && !o[1] && o[2]) {            /* clip(0,1), clip(1,2) */
+        cvert an = CLIP(0, 1), cn = CLIP(1, 2);
+        out[0].v[0] = an; out[0].v[1] = *B; out[0].v[2] = cn;
+        return 1;
+    }
+    if (o[0] && !o[1] && !o[2]) {           /* clip(0,1), clip(0,2) */
+        cvert bn = CLIP(0, 1), cn = CLIP(0, 2);
+        out[0].v[0] = bn; out[0].v[1] = *B; out[0].v[2] = cn;
+        out[1].v[0] = *B; out[1].v[1] = *C; out[1].v[2] = cn;
+        return 2;
+    }
+    if (!o[0] && o[1] && !o[2]) {           /* clip(0,1), clip(1,2) */
+        cvert an = CLIP(0, 1), cn = CLIP(1, 2);
+        out[0].v[0] = *A; out[0].v[1] = an; out[0].v[2] = cn;
+        out[1].v[0] = *A; out[1].v[1] = cn; out[1].v[2] = *C;
+        return 2;
+    }
+    /* !o[0] && !o[1] && o[2]: clip(1,2), clip(0,2) */
+    {
+        cvert bn = CLIP(1, 2), an = CLIP(0, 2);
+        out[0].v[0] = *A; out[0].v[1] = *B; out[0].v[2] = bn;
+        out[1].v[0] = *A; out[1].v[1] = bn; out[1].v[2] = an;
+        return 2;
+    }
+#undef CLIP
+}
+
+/* clipper.rs:756-775: a FIFO queue, processed plane by plane.  At most
+ * 2^6 triangles can come out. */
+#define CLIP_CAP 64
+static int clip_triangle(const float pos[3][4], ctri *out)
+{
+    ctri qa[CLIP_CAP], qb[CLIP_CAP];
+    ctri *cur = qa, *nxt = qb;
+    int n = 1;
+    for (int k = 0; k < 3; ++k) {
+        for (int c = 0; c < 4; ++c) cur[0].v[k].p[c] = pos[k][c];
+        for (int c = 0; c < 3; ++c) cur[0].v[k].b[c] = (c == k) ? 1.0f : 0.0f;
+    }
+    for (int pl = 0; pl < 6; ++pl) {
+        int m = 0;
+        for (int t = 0; t < n; ++t) {
+            ctri res[2];
+            int r = clip_against_plane(&cur[t], planes[pl], res);
+            for (int q = 0; q < r; ++q) nxt[m++] = res[q];
+        }
+        ctri *tmp = cur; cur = nxt; nxt = tmp;
+        n = m;
+    }
+    for (int t = 0; t < n; ++t) out[t] = cur[t];
+    return n;
+}
+
+int oracle_clip_triangle(const float in_xyzw[12], float *out, int cap)
+{
+    float pos[3][4];
+    ctri res[CLIP_CAP];
+    for (int k = 0; k < 3; ++k)
+        for (int c = 0; c < 4; ++c) pos[k][c] = in_xyzw[4 * k + c];
+    int n = clip_triangle(pos, res);
+    for (int t = 0; t < n && t < cap; ++t)
+        for (int k = 0; k < 3; ++k) {
+            float *o = out + (t * 3 + k) * 7;
+            for (int c = 0; c < 4; ++c) o[c] = res[t].v[k].p[c];
+            for (int c = 0; c < 3; ++c) o[4 + c] = res[t].v[k].b[c];
+        }
+    return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* Raster (raster.rs:129-160, 280-362)                                */
+/* ------------------------------------------------------------------ */
+
+/* ToRaster::new + to_raster */
+void oracle_to_raster(const float viewport[6], const float clip[4], uint32_t fb[2], float frag[4])
+{
+    float tx = viewport[0] + 0.5f * viewport[2];
+    float ty = viewport[1] + 0.5f * viewport[3];
+    float sx = 0.5f * viewport[2];
+    float sy = 0.5f * viewport[3];
+    sy = sy * -1.0f;
+    /* Point3::from_homogeneous: coords / w */
+    float nx = clip[0] / clip[3];
+    float ny = clip[1] / clip[3];
+    float nz = clip[2] / clip[3];
+    float pd = 1.0f / clip[3];
+    float vx = nx * sx + tx;
+    float vy = ny * sy + ty;
+    vx = fmaxf(vx, 0.0f);
+    vy = fmaxf(vy, 0.0f);
+    fb[0] = f32_as_u32(vx);
+    fb[1] = f32_as_u32(vy);
+    frag[0] = vx; frag[1] = vy; frag[2] = nz; frag[3] = pd;
+}
+
+/* shoelace(a, b, c) (raster.rs:294-298), evaluated left to right */
+static float shoelace(float ax, float ay, float bx, float by, float cx, float cy)
+{
+    float t1 = (by - ay) * (bx + ax);
+    float t2 = (cy - by) * (cx + bx);
+    float t3 = (ay - cy) * (ax + cx);
+    return (t1 + t2) + t3;
+}
+
+/* ------------------------------------------------------------------ */
+/* Vertex stage: attribute fetch + the built-in programs              */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    float clip[4];
+    float var[OR_MAX_VAR];
+    int nvar;
+} vout;
+
+/* vertex.rs:191-195 + :143-147: raw bytes of the attribute; the reference
+ * panics on an out-of-range slice. */
+static int fetch_attr(const OrDraw *d, uint32_t loc, uint32_t instance, uint32_t vertex,
+                      float *dst, int *ncomp)
+{
+    if (loc >= d->n_attr) return -1;
+    const OrAttribute *at = &d->attr[loc];
+    if (at->buffer >= d->n_vb) return -1;
+    const OrVertexBuffer *vb = &d->vb[at->buffer];
+    uint64_t start = vb->step_instance ? (uint64_t)instance * vb->stride
+                                       : (uint64_t)vertex * vb->stride;
+    uint64_t n = at->format;
+    if (start + at->offset + 4 * n > vb->size) return -1;
+    memcpy(dst, vb->data + start + at->offset, 4 * n);
+    *ncomp = (int)n;
+    return 0;
+}
+
+/* naga-cranelift binary.rs:299-321: sum_i column_i * v_i, separate fmul/fadd */
+static void mat4_mul_vec4(const float *m, const float *v, float *r)
+{
+    for (int k = 0; k < 4; ++k) {
+        float s = m[0 * 4 + k] * v[0];
+        s = s + m[1 * 4 + k] * v[1];
+        s = s + m[2 * 4 + k] * v[2];
+        s = s + m[3 * 4 + k] * v[3];
+        r[k] = s;
+    }
+}
+
+void oracle_mat4_mul_vec4(const float m[16], const float v[4], float r[4])
+{
+    mat4_mul_vec4(m, v, r);
+}
+
+static int run_vs(const OrDraw *d, uint32_t instance, uint32_t vertex, vout *o)
+{
+    int nc;
+    float pos[4];
+    o->nvar = 0;
+    switch (d->program) {
+    case OR_PROG_MVP_COLOR:
+    case OR_PROG_MVP_TEXTURE:
+        if (fetch_attr(d, 0, instance, vertex, pos, &nc) || nc != 4) return -1;
+        if (!d->uniform) return -1;
+        mat4_mul_vec4(d->uniform, pos, o->clip);
+        if (fetch_attr(d, 1, instance, vertex, o->var, &nc)) return -1;
+        o->nvar = nc;
+        return 0;
+    case OR_PROG_CLIP_COLOR:
+        if (fetch_attr(d, 0, instance, vertex, pos, &nc) || nc != 4) return -1;
+        memcpy(o->clip, pos, sizeof pos);
+        if (fetch_attr(d, 1, instance, vertex, o->var, &nc)) return -1;
+        o->nvar = nc;
+        return 0;
+    case OR_PROG_COLORED_TRIANGLE: {
+        /* colored_triangle.wgsl: vi = vertex_index % 3;
+         * x = f32(i32(vi) - 1); y = f32(i32(vi & 1u) * 2 - 1);
+         * colour = (f32(vi == 0), f32(vi == 1), f32(vi == 2), 1.0) */
+        uint32_t vi = vertex % 3u;
+        int32_t ix = (int32_t)vi - 1;
+        int32_t iy = (int32_t)(vi & 1u) * 2 - 1;
+        o->clip[0] = (float)ix;
+        o->clip[1] = (float)iy;
+        o->clip[2] = 0.0f;
+        o->clip[3] = 1.0f;
+        o->var[0] = (vi == 0) ? 1.0f : 0.0f;
+        o->var[1] = (vi == 1) ? 1.0f : 0.0f;
+        o->var[2] = (vi == 2) ? 1.0f : 0.0f;
+        o->var[3] = 1.0f;
+        o->nvar = 4;
+        return 0;
+    }
+    default:
+        return -1;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Fragment stage                                                     */
+/* ------------------------------------------------------------------ */
+
+static void fetch_texel(const OrDraw *d, uint32_t u, uint32_t v, float out[4])
+{
+    const uint8_t *t = d->texture + ((uint64_t)v * d->tex_w + u) * 4u;
+    if (d->srgb_decode && d->tex_srgb) {
+        const float *lut = oracle_srgb_table();
+        for (int k = 0; k < 3; ++k) out[k] = lut[t[k]];
+        out[3] = (float)t[3] / 255.0f;
+    } else {
+        for (int k = 0; k < 4; ++k) out[k] = (float)t[k] / 255.0f; /* texture.rs:173-175 */
+    }
+}
+
+/* Extension: bilinear filtering with the WebGPU texel-centre convention,
+ * u' = u*W - 0.5, on wrapped integer coordinates. */
+static int32_t wrap_index(int32_t i, uint32_t mode, uint32_t size)
+{
+    int32_t n = (int32_t)size;
+    switch (mode) {
+    case OR_ADDR_REPEAT: { int32_t r = i % n; return r < 0 ? r + n : r; }
+    case OR_ADDR_MIRROR: {
+        int32_t p = 2 * n;
+        int32_t r = i % p; if (r < 0) r += p;
+        return r < n ? r : p - 1 - r;
+    }
+    default: return i < 0 ? 0 : (i >= n ? n - 1 : i);
+    }
+}
+
+static void sample_bilinear(const OrDraw *d, float u, float v, float out[4])
+{
+    float fu = u * (float)d->tex_w - 0.5f;
+    float fv = v * (float)d->tex_h - 0.5f;
+    float u0f = floorf(fu), v0f = floorf(fv);
+    float au = fu - u0f, av = fv - v0f;
+    /* coordinates beyond +-2^30 are left to the clamp of the cast */
+    int32_t iu = (int32_t)fmaxf(fminf(u0f, 1073741824.0f), -1073741824.0f);
+    int32_t iv = (int32_t)fmaxf(fminf(v0f, 1073741824.0f), -1073741824.0f);
+    uint32_t x0 = (uint32_t)wrap_index(iu, d->addr_u, d->tex_w);
+    uint32_t x1 = (uint32_t)wrap_index(iu + 1, d->addr_u, d->tex_w);
+    uint32_t y0 = (uint32_t)wrap_index(iv, d->addr_v, d->tex_h);
+    uint32_t y1 = (uint32_t)wrap_index(iv + 1, d->addr_v, d->tex_h);
+    float t00[4], t10[4], t01[4], t11[4];
+    fetch_texel(d, x0, y0, t00);
+    fetch_texel(d, x1, y0, t10);
+    fetch_texel(d, x0, y1, t01);
+    fetch_texel(d, x1, y1, t11);
+    float su = 1.0f - au, sv = 1.0f - av;
+    for (int k = 0; k < 4; ++k) {
+        float top = t00[k] * su + t10[k] * au;
+        float bot = t01[k] * su + t11[k] * au;
+        out[k] = top * sv + bot * av;
+    }
+}
+
+/* binding.rs:93-141 (reference path) */
+static void texture_sample(const OrDraw *d, float u, float v, float out[4])
+{
+    if (d->bilinear) { sample_bilinear(d, u, v, out); return; }
+    uint32_t x = oracle_texel_coordinate(u, d->addr_u, d->tex_w);
+    uint32_t y = oracle_texel_coordinate(v, d->addr_v, d->tex_h);
+    fetch_texel(d, x, y, out);
+}
+
+static void run_fs(const OrDraw *d, const float *var, int nvar, float out[4])
+{
+    switch (d->program) {
+    case OR_PROG_MVP_TEXTURE:
+        texture_sample(d, var[0], var[1], out);
+        return;
+    default:
+        /* fs returns the colour varying; a vec3 becomes
+         * vec4f(c.x, c.y, c.z, 1.0) (scalar-composed vector) */
+        out[0] = var[0]; out[1] = var[1]; out[2] = var[2];
+        out[3] = (nvar >= 4) ? var[3] : 1.0f;
+        return;
+    }
+}
+
+/* texture.rs:213-218 + 373-411 */
+static void put_pixel(OrPass *p, uint32_t x, uint32_t y, const float c[4])
+{
+    uint8_t *t = p->color + ((uint64_t)y * p->width + x) * 4u;
+    uint8_t r = oracle_f32_to_unorm8(c[0]), g = oracle_f32_to_unorm8(c[1]);
+    uint8_t b = oracle_f32_to_unorm8(c[2]), a = oracle_f32_to_unorm8(c[3]);
+    if (p->color_format == OR_FMT_BGRA8UNORM || p->color_format == OR_FMT_BGRA8UNORM_SRGB) {
+        t[0] = b; t[1] = g; t[2] = r; t[3] = a;
+    } else {
+        t[0] = r; t[1] = g; t[2] = b; t[3] = a;
+    }
+}
+
+/* Extension: src-over blending, BlendState::ALPHA_BLENDING:
+ * rgb = src*srcA + dst*(1-srcA); a = srcA + dstA*(1-srcA), the destination
+ * read back as u8/255 and the result written with the truncating writer. */
+static void blend_pixel(OrPass *p, uint32_t x, uint32_t y, const float s[4])
+{
+    uint8_t *t = p->color + ((uint64_t)y * p->width + x) * 4u;
+    int bgra = (p->color_format == OR_FMT_BGRA8UNORM || p->color_format == OR_FMT_BGRA8UNORM_SRGB);
+    float dst[4];
+    dst[0] = (float)t[bgra ? 2 : 0] / 255.0f;
+    dst[1] = (float)t[1] / 255.0f;
+    dst[2] = (float)t[bgra ? 0 : 2] / 255.0f;
+    dst[3] = (float)t[3] / 255.0f;
+    float ia = 1.0f - s[3];
+    float o[4];
+    for (int k = 0; k < 3; ++k) o[k] = s[k] * s[3] + dst[k] * ia;
+    o[3] = s[3] + dst[3] * ia;
+    put_pixel(p, x, y, o);
+}
+
+/* ------------------------------------------------------------------ */
+/* Draw (state.rs:490-593)                                            */
+/* ------------------------------------------------------------------ */
+
+static int fail(OrPass *p, const char *msg)
+{
+    p->error = 1;
+    snprintf(p->message, sizeof p->message, "%s", msg);
+    return -1;
+}
+
+/* index.rs:45-70: u32::from(index).strict_add_signed(base_vertex) */
+static int resolve_index(const OrDraw *d, uint32_t i, uint32_t *vertex)
+{
+    if (!d->indexed) { *vertex = i; return 0; }
+    uint64_t off = (uint64_t)i * d->index_format;
+    if (off + d->index_format > d->index_size) return -1;
+    uint32_t idx;
+    if (d->index_format == 2) {
+        uint16_t s; memcpy(&s, d->index_data + off, 2); idx = s;
+    } else {
+        memcpy(&idx, d->index_data + off, 4);
+    }
+    int64_t v = (int64_t)idx + d->base_vertex;
+    if (v < 0 || v > 0xFFFFFFFFll) return -1;
+    *vertex = (uint32_t)v;
+    return 0;
+}
+
+static int draw(OrPass *p, const OrDraw *d)
+{
+    if (d->program == OR_PROG_MVP_TEXTURE && !d->texture) return fail(p, "no texture bound");
+    /* ToRaster scissor box (raster.rs:130-143) */
+    uint64_t sx0 = d->scissor[0], sy0 = d->scissor[1];
+    uint64_t sx1 = sx0 + d->scissor[2], sy1 = sy0 + d->scissor[3];
+    if (sx1 > p->width || sy1 > p->height) return fail(p, "scissor outside the framebuffer");
+
+    for (uint32_t inst = d->first_instance; inst < d->first_instance + d->instance_count; ++inst) {
+        uint32_t nprim = d->count / 3; /* array_chunks_::<3> drops a remainder */
+        for (uint32_t prim = 0; prim < nprim; ++prim) {
+            vout vo[3];
+            for (int k = 0; k < 3; ++k) {
+                uint32_t vertex;
+                if (resolve_index(d, d->first + prim * 3 + k, &vertex))
+                    return fail(p, "index out of range or vertex index overflow");
+                if (run_vs(d, inst, vertex, &vo[k])) return fail(p, "vertex fetch out of range");
+            }
+            /* TriFace::new (primitive.rs:169-184) */
+            float nd[3][3];
+            for (int k = 0; k < 3; ++k) {
+                if (vo[k].clip[3] == 0.0f) return fail(p, "clip position w = 0");
+                for (int c = 0; c < 3; ++c) nd[k][c] = vo[k].clip[c] / vo[k].clip[3];
+            }
+            float abx = nd[1][0] - nd[0][0], aby = nd[1][1] - nd[0][1];
+            float acx = nd[2][0] - nd[0][0], acy = nd[2][1] - nd[0][1];
+            float area_z = abx * acy - aby * acx;
+            int ccw = area_z > 0.0f;
+            int front_facing = ccw ? (d->front_face_cw == 0) : (d->front_face_cw == 1);
+            int cull = (d->cull_mode == OR_CULL_FRONT) ? front_facing
+                     : (d->cull_mode == OR_CULL_BACK) ? !front_facing : 0;
+            if (cull) continue;
+            (void)front_facing; /* the built-in programs do not read front_facing */
+
+            float pos[3][4];
+            for (int k = 0; k < 3; ++k) memcpy(pos[k], vo[k].clip, sizeof pos[k]);
+            ctri sub[CLIP_CAP];
+            int nsub = clip_triangle(pos, sub);
+            for (int t = 0; t < nsub; ++t) {
+                uint32_t fb[3][2];
+                float fr[3][4];
+                for (int k = 0; k < 3; ++k) oracle_to_raster(d->viewport, sub[t].v[k].p, fb[k], fr[k]);
+                float total = shoelace(fr[0][0], fr[0][1], fr[1][0], fr[1][1], fr[2][0], fr[2][1]);
+                if (total == 0.0f) continue;
+                scanlines_t sl;
+                scanline_t line;
+                scanlines_init(&sl, fb);
+                for (;;) {
+                    int r = scanlines_next(&sl, &line);
+                    if (r < 0) return fail(p, "scanline produced a negative coordinate");
+                    if (r == 0) break;
+                    for (int64_t xx = line.x1; xx <= line.x2; ++xx) {
+                        uint64_t x = (uint64_t)xx, y = (uint64_t)line.y;
+                        if (!(x >= sx0 && x < sx1 && y >= sy0 && y < sy1)) continue;
+                        p->fragments++;
+                        if (p->frag_count) p->frag_count[y * p->width + x]++;
+                        /* barycentric at the pixel corner (raster.rs:313-335) */
+                        float px = (float)x, py = (float)y;
+                        float l0 = shoelace(px, py, fr[1][0], fr[1][1], fr[2][0], fr[2][1]) / total;
+                        float l1 = shoelace(fr[0][0], fr[0][1], px, py, fr[2][0], fr[2][1]) / total;
+                        float l2 = shoelace(fr[0][0], fr[0][1], fr[1][0], fr[1][1], px, py) / total;
+                        /* compose with the clip barycentrics (raster.rs:338-343) */
+                        float lc[3];
+                        for (int c = 0; c < 3; ++c) {
+                            float acc = sub[t].v[0].b[c] * l0;
+                            acc = acc + sub[t].v[1].b[c] * l1;
+                            acc = acc + sub[t].v[2].b[c] * l2;
+                            lc[c] = acc;
+                        }
+                        /* fragment = lc . [a, b, c].fragment (raster.rs:345-346) */
+                        float frag_z = fr[0][2] * lc[0];
+                        frag_z = frag_z + fr[1][2] * lc[1];
+                        frag_z = frag_z + fr[2][2] * lc[2];
+                        /* varyings of the unclipped vertices (fragment.rs:150-152, 346-366) */
+                        float var[OR_MAX_VAR];
+                        int nvar = vo[0].nvar;
+                        if (d->interp == OR_INTERP_FLAT) {
+                            for (int c = 0; c < nvar; ++c) var[c] = vo[0].var[c];
+                        } else if (d->interp == OR_INTERP_PERSPECTIVE) {
+                            /* extension: perspective-correct weights q_i = lc_i * (1/w_i) */
+                            float q0 = lc[0] * (1.0f / vo[0].clip[3]);
+                            float q1 = lc[1] * (1.0f / vo[1].clip[3]);
+                            float q2 = lc[2] * (1.0f / vo[2].clip[3]);
+                            float qs = (q0 + q1) + q2;
+                            for (int c = 0; c < nvar; ++c) {
+                                float acc = vo[0].var[c] * q0;
+                                acc = acc + vo[1].var[c] * q1;
+                                acc = acc + vo[2].var[c] * q2;
+                                var[c] = acc / qs;
+                            }
+                        } else {
+                            for (int c = 0; c < nvar; ++c) {
+                                float acc = vo[0].var[c] * lc[0];
+                                acc = acc + vo[1].var[c] * lc[1];
+                                acc = acc + vo[2].var[c] * lc[2];
+                                var[c] = acc;
+                            }
+                        }
+                        float color[4];
+                        run_fs(d, var, nvar, color);
+                        /* late depth test on the @location write (fragment.rs:427-488) */
+                        int pass = 1;
+                        if (p->depth && d->depth_enabled) {
+                            float *dp = p->depth + y * p->width + x;
+                            if (compare(d->depth_compare, frag_z, *dp)) {
+                                if (d->depth_write) *dp = frag_z;
+                            } else {
+                                pass = 0;
+                            }
+                        }
+                        if (pass && p->color) {
+                            if (d->blend) blend_pixel(p, (uint32_t)x, (uint32_t)y, color);
+                            else put_pixel(p, (uint32_t)x, (uint32_t)y, color);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+/* RenderPassCommand::execute (render_pass/mod.rs:345-394) */
+int oracle_execute_pass(OrPass *p)
+{
+    p->error = 0;
+    p->message[0] = 0;
+    p->fragments = 0;
+    uint64_t npx = (uint64_t)p->width * p->height;
+    /* State::load (state.rs:135-145) */
+    if (p->color && p->color_clear) {
+        float c[4];
+        for (int k = 0; k < 4; ++k) c[k] = (float)p->clear_color[k]; /* f64 -> f32 cast */
+        uint32_t w = p->width;
+        for (uint64_t i = 0; i < npx; ++i) put_pixel(p, (uint32_t)(i % w), (uint32_t)(i / w), c);
+    }
+    if (p->depth && p->depth_clear)
+        for (uint64_t i = 0; i < npx; ++i) p->depth[i] = p->clear_depth;
+    for (uint32_t i = 0; i < p->n_draws; ++i)
+        if (draw(p, &p->draws[i])) return -1;
+    return 0;
+}

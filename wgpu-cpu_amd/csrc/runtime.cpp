@@ -1,0 +1,1269 @@
+// runtime.cpp -- libwgc C ABI: devices, resources, command recording and the
+// queue that turns recorded render passes into kernel launches.
+//
+// Mirrors the reference's backend objects (wgpu-cpu/src/{device,buffer,
+// texture,sampler,bind_group,shader,pipeline,command}.rs and
+// render_pass/mod.rs): handles are reference counted like its Arc clones,
+// and one HIP stream per device takes the place of its engine thread
+// (engine.rs:26-46).
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/wgc.h"
+#include "wgc_core.h"
+#include "wgc_launch.h"
+
+using namespace wgc;
+
+// ------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------
+
+static thread_local std::string g_last_error;
+
+static int32_t set_error(int32_t code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+static int32_t hip_fail(hipError_t e, const char *what)
+{
+    return set_error(WGC_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                              \
+    do {                                                 \
+        hipError_t _e = (expr);                          \
+        if (_e != hipSuccess) return hip_fail(_e, what); \
+    } while (0)
+
+extern "C" uint32_t wgc_abi_version(void) { return WGC_ABI_VERSION; }
+extern "C" const char *wgc_last_error(void) { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------------
+// objects
+// ------------------------------------------------------------------
+
+struct Object {
+    std::atomic<int> refs{1};
+    virtual ~Object() = default;
+};
+
+static void retain(Object *o)
+{
+    if (o) o->refs.fetch_add(1, std::memory_order_relaxed);
+}
+
+static void release(Object *o)
+{
+    if (o && o->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete o;
+}
+
+struct WgcDevice_T;
+
+struct WgcBuffer_T : Object {
+    WgcDevice_T *dev = nullptr;
+    uint8_t *ptr = nullptr;
+    uint64_t size = 0;
+    ~WgcBuffer_T() override;
+};
+
+struct WgcTexture_T : Object {
+    WgcDevice_T *dev = nullptr;
+    uint8_t *ptr = nullptr;
+    uint32_t width = 0, height = 0, format = 0;
+    uint64_t size = 0;
+    ~WgcTexture_T() override;
+};
+
+struct WgcSampler_T : Object {
+    WgcSamplerDescriptor desc{};
+};
+
+struct BindingSlot {
+    uint32_t binding = 0, kind = 0;
+    WgcBuffer_T *buffer = nullptr;
+    uint64_t offset = 0, size = 0;
+    WgcTexture_T *texture = nullptr;
+    WgcSampler_T *sampler = nullptr;
+};
+
+struct WgcBindGroup_T : Object {
+    std::vector<BindingSlot> entries;
+    ~WgcBindGroup_T() override
+    {
+        for (auto &e : entries) {
+            release(e.buffer);
+            release(e.texture);
+            release(e.sampler);
+        }
+    }
+    const BindingSlot *find(uint32_t binding) const
+    {
+        for (auto &e : entries)
+            if (e.binding == binding) return &e;
+        return nullptr;
+    }
+};
+
+struct WgcShaderModule_T : Object {
+    uint32_t program = 0, interpolation = 0;
+};
+
+struct WgcRenderPipeline_T : Object {
+    WgcRenderPipelineDescriptor desc{};
+    uint32_t program = 0, interp = 0, nvar = 0;
+    bool has_fragment = false;
+    std::vector<uint32_t> vb_stride, vb_step;
+    AttrDev attr[kMaxAttr]{};
+    uint32_t n_attr = 0;
+};
+
+struct BufferBinding {
+    WgcBuffer_T *buffer = nullptr;
+    uint64_t offset = 0, size = 0;
+    uint32_t format = 0;
+};
+
+enum class CmdKind { SetPipeline, SetBindGroup, SetVertexBuffer, SetIndexBuffer, SetViewport,
+                     SetScissor, SetBlendConstant, SetStencilReference, Draw, DrawIndexed };
+
+struct SubCommand {
+    CmdKind kind;
+    Object *obj = nullptr;     // pipeline / bind group / buffer (one reference held)
+    uint32_t slot = 0, format = 0;
+    uint64_t offset = 0, size = 0;
+    float viewport[6]{};
+    uint32_t rect[4]{};
+    uint32_t first = 0, count = 0, first_instance = 0, instance_count = 0;
+    int32_t base_vertex = 0;
+};
+
+struct PassCommand {
+    WgcTexture_T *color = nullptr;
+    uint32_t color_load = 0;
+    double clear_color[4]{};
+    WgcTexture_T *depth = nullptr;
+    uint32_t has_depth_ops = 0, depth_load = 0;
+    float clear_depth = 0.0f;
+    std::vector<SubCommand> cmds;
+
+    void drop()
+    {
+        release(color);
+        release(depth);
+        for (auto &c : cmds) release(c.obj);
+        cmds.clear();
+        color = depth = nullptr;
+    }
+};
+
+struct WgcCommandBuffer_T : Object {
+    std::vector<PassCommand> passes;
+    ~WgcCommandBuffer_T() override
+    {
+        for (auto &p : passes) p.drop();
+    }
+};
+
+struct WgcCommandEncoder_T : Object {
+    WgcDevice_T *dev = nullptr;
+    std::vector<PassCommand> passes;
+    bool pass_open = false;
+    ~WgcCommandEncoder_T() override
+    {
+        for (auto &p : passes) p.drop();
+    }
+};
+
+struct WgcRenderPass_T : Object {
+    WgcCommandEncoder_T *enc = nullptr;
+    PassCommand pass;
+    ~WgcRenderPass_T() override { pass.drop(); }
+};
+
+struct Inflight {
+    uint64_t index;
+    hipEvent_t done;
+    std::vector<Object *> keep;  // command buffers used by the submission
+};
+
+struct WgcDevice_T : Object {
+    int hip_device = 0;
+    uint32_t semantics = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    // per-draw scratch arena, reused by every draw (they run in stream order)
+    uint8_t *arena = nullptr;
+    size_t arena_size = 0;
+    // persistent device words
+    uint32_t *d_error = nullptr;
+    unsigned long long *d_frags = nullptr;
+    float *d_srgb = nullptr;
+    uint64_t next_index = 1;
+    uint64_t last_done = 0;
+    std::vector<Inflight> inflight;
+    std::vector<hipEvent_t> event_pool;
+    hipEvent_t ts[64]{};
+    bool ts_recorded[64]{};
+    bool timing = false;
+    KernelTimer timer;
+    std::map<std::string, std::pair<double, uint64_t>> ktimes;
+    uint64_t last_prims = 0, last_launches = 0;
+    uint32_t *frag_px = nullptr;   // optional per-pixel fragment counts (debug)
+    size_t frag_px_size = 0;
+
+    ~WgcDevice_T() override
+    {
+        (void)hipSetDevice(hip_device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &f : inflight) {
+            (void)hipEventDestroy(f.done);
+            for (auto *o : f.keep) release(o);
+        }
+        for (auto e : event_pool) (void)hipEventDestroy(e);
+        for (auto e : ts)
+            if (e) (void)hipEventDestroy(e);
+        for (auto &p : timer.pending) {
+            (void)hipEventDestroy(p.begin);
+            (void)hipEventDestroy(p.end);
+        }
+        for (auto e : timer.pool) (void)hipEventDestroy(e);
+        if (arena) (void)hipFree(arena);
+        if (d_error) (void)hipFree(d_error);
+        if (d_frags) (void)hipFree(d_frags);
+        if (d_srgb) (void)hipFree(d_srgb);
+        if (frag_px) (void)hipFree(frag_px);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+WgcBuffer_T::~WgcBuffer_T()
+{
+    if (ptr) {
+        (void)hipSetDevice(dev->hip_device);
+        (void)hipFree(ptr);
+    }
+    release(dev);
+}
+
+WgcTexture_T::~WgcTexture_T()
+{
+    if (ptr) {
+        (void)hipSetDevice(dev->hip_device);
+        (void)hipFree(ptr);
+    }
+    release(dev);
+}
+
+// ------------------------------------------------------------------
+// device
+// ------------------------------------------------------------------
+
+static float srgb_entry(int i)
+{
+    double c = (double)i / 255.0;
+    double l = (c <= 0.04045) ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4);
+    return (float)l;
+}
+
+extern "C" int32_t wgc_device_create(const WgcDeviceDescriptor *desc, WgcDevice *out)
+{
+    if (!desc || !out) return set_error(WGC_ERR_INVALID, "null argument");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (desc->hip_device < 0 || desc->hip_device >= n)
+        return set_error(WGC_ERR_INVALID, "no such HIP device");
+    if (desc->semantics > WGC_SEMANTICS_EXTENDED) return set_error(WGC_ERR_INVALID, "bad semantics");
+    auto *d = new WgcDevice_T();
+    d->hip_device = desc->hip_device;
+    d->semantics = desc->semantics;
+    hipError_t e = hipSetDevice(d->hip_device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&d->d_error, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&d->d_frags, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&d->d_srgb, 256 * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(d->d_error, 0, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(d->d_frags, 0, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) {
+        float lut[256];
+        for (int i = 0; i < 256; ++i) lut[i] = srgb_entry(i);
+        e = hipMemcpy(d->d_srgb, lut, sizeof lut, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        release(d);
+        return hip_fail(e, "device setup");
+    }
+    *out = d;
+    return WGC_OK;
+}
+
+extern "C" void wgc_device_release(WgcDevice device) { release(device); }
+
+// Drains finished submissions; with wait, blocks for `upto`.
+static int32_t retire(WgcDevice_T *d, uint64_t upto, bool wait, uint64_t timeout_ns)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    while (!d->inflight.empty() && d->inflight.front().index <= upto) {
+        Inflight &f = d->inflight.front();
+        hipError_t q = hipEventQuery(f.done);
+        if (q == hipErrorNotReady) {
+            if (!wait) return WGC_OK;
+            if (timeout_ns) {
+                auto el = std::chrono::duration_cast<std::chrono::nanoseconds>(
+                              std::chrono::steady_clock::now() - t0).count();
+                if ((uint64_t)el >= timeout_ns) return set_error(WGC_ERR_TIMEOUT, "poll timed out");
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+                continue;
+            }
+            q = hipEventSynchronize(f.done);
+        }
+        if (q != hipSuccess) return hip_fail(q, "submission");
+        d->last_done = f.index;
+        d->event_pool.push_back(f.done);
+        for (auto *o : f.keep) release(o);
+        d->inflight.erase(d->inflight.begin());
+    }
+    return WGC_OK;
+}
+
+static int32_t resolve_timers(WgcDevice_T *d)
+{
+    for (auto &p : d->timer.pending) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventSynchronize(p.end), "timer");
+        HIP_TRY(hipEventElapsedTime(&ms, p.begin, p.end), "timer");
+        auto &slot = d->ktimes[p.name];
+        slot.first += ms;
+        slot.second += 1;
+        d->timer.pool.push_back(p.begin);
+        d->timer.pool.push_back(p.end);
+    }
+    d->timer.pending.clear();
+    return WGC_OK;
+}
+
+static int32_t check_device_error(WgcDevice_T *d)
+{
+    uint32_t err = 0;
+    HIP_TRY(hipMemcpy(&err, d->d_error, sizeof err, hipMemcpyDeviceToHost), "error readback");
+    if (!err) return WGC_OK;
+    HIP_TRY(hipMemset(d->d_error, 0, sizeof err), "error reset");
+    std::string msg = "render pass failed on the device:";
+    if (err & ERR_INDEX) msg += " index out of the index buffer (reference: panic);";
+    if (err & ERR_VERTEX) msg += " vertex fetch out of the vertex buffer (reference: panic);";
+    if (err & ERR_W_ZERO) msg += " clip position w == 0 (reference: expect(\"w=0\"));";
+    if (err & ERR_CLIP_OVERFLOW) msg += " clipped sub-triangle buffer full;";
+    if (err & ERR_RECORD_OVERFLOW) msg += " triangle record buffer full;";
+    if (err & ERR_ORDERED_OVERFLOW) msg += " more than 4096 triangles on one tile in ordered (blend) mode;";
+    if (err & ERR_COORD_RANGE) msg += " framebuffer coordinate beyond 2^30;";
+    if (err & ERR_PRIM_RANGE) msg += " more than 2^25 primitives in one draw;";
+    return set_error(WGC_ERR_EXECUTION, msg);
+}
+
+extern "C" int32_t wgc_device_poll(WgcDevice device, uint64_t index, uint64_t timeout_ns)
+{
+    if (!device) return set_error(WGC_ERR_INVALID, "null device");
+    std::lock_guard<std::mutex> lk(device->mu);
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    int32_t r = retire(device, index, true, timeout_ns);
+    if (r != WGC_OK) return r;
+    if (device->timing) {
+        r = resolve_timers(device);
+        if (r != WGC_OK) return r;
+    }
+    return check_device_error(device);
+}
+
+static int32_t drain(WgcDevice_T *d)
+{
+    int32_t r = retire(d, UINT64_MAX, true, 0);
+    if (r != WGC_OK) return r;
+    HIP_TRY(hipStreamSynchronize(d->stream), "hipStreamSynchronize");
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_device_last_pass_stats(WgcDevice device, WgcPassStats *out)
+{
+    if (!device || !out) return set_error(WGC_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(device->mu);
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    int32_t r = drain(device);
+    if (r != WGC_OK) return r;
+    unsigned long long f[2];
+    HIP_TRY(hipMemcpy(f, device->d_frags, sizeof f, hipMemcpyDeviceToHost), "stats readback");
+    out->fragments = f[0];
+    out->primitives = device->last_prims;
+    out->kernel_launches = device->last_launches;
+    return WGC_OK;
+}
+
+// Optional per-pixel fragment counts: allocates a width*height u32 buffer
+// that every later pass adds its emitted fragments to (debug aid).
+extern "C" int32_t wgc_device_debug_fragment_counts(WgcDevice device, uint32_t width,
+                                                    uint32_t height, uint32_t *host_out)
+{
+    if (!device) return set_error(WGC_ERR_INVALID, "null device");
+    std::lock_guard<std::mutex> lk(device->mu);
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    int32_t r = drain(device);
+    if (r != WGC_OK) return r;
+    size_t need = (size_t)width * height * 4u;
+    if (host_out) {
+        if (!device->frag_px || need != device->frag_px_size)
+            return set_error(WGC_ERR_INVALID, "fragment counts not enabled for this size");
+        HIP_TRY(hipMemcpy(host_out, device->frag_px, need, hipMemcpyDeviceToHost), "readback");
+        return WGC_OK;
+    }
+    if (device->frag_px) {
+        HIP_TRY(hipFree(device->frag_px), "hipFree");
+        device->frag_px = nullptr;
+        device->frag_px_size = 0;
+    }
+    if (need) {
+        HIP_TRY(hipMalloc(&device->frag_px, need), "hipMalloc");
+        HIP_TRY(hipMemset(device->frag_px, 0, need), "hipMemset");
+        device->frag_px_size = need;
+    }
+    return WGC_OK;
+}
+
+// ------------------------------------------------------------------
+// buffers and textures
+// ------------------------------------------------------------------
+
+extern "C" int32_t wgc_buffer_create(WgcDevice device, uint64_t size, WgcBuffer *out)
+{
+    if (!device || !out) return set_error(WGC_ERR_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    auto *b = new WgcBuffer_T();
+    b->dev = device;
+    retain(device);
+    b->size = size;
+    if (size) {
+        hipError_t e = hipMalloc(&b->ptr, size);
+        if (e == hipSuccess) e = hipMemset(b->ptr, 0, size);
+        if (e != hipSuccess) {
+            release(b);
+            return set_error(WGC_ERR_OUT_OF_MEMORY, std::string("buffer: ") + hipGetErrorString(e));
+        }
+    }
+    *out = b;
+    return WGC_OK;
+}
+
+extern "C" void wgc_buffer_release(WgcBuffer buffer) { release(buffer); }
+
+extern "C" int32_t wgc_queue_write_buffer(WgcDevice device, WgcBuffer buffer, uint64_t offset,
+                                          const void *data, uint64_t size)
+{
+    if (!device || !buffer || (!data && size)) return set_error(WGC_ERR_INVALID, "null argument");
+    if (offset > buffer->size || size > buffer->size - offset)
+        return set_error(WGC_ERR_INVALID, "write_buffer out of range (reference: slice panic)");
+    if (!size) return WGC_OK;
+    std::lock_guard<std::mutex> lk(device->mu);
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    // Ordered after every earlier submission on the device stream; the
+    // caller's memory is free to reuse on return, like the reference's
+    // synchronous write (device.rs:332-344).
+    HIP_TRY(hipMemcpyAsync(buffer->ptr + offset, data, size, hipMemcpyHostToDevice, device->stream),
+            "write_buffer");
+    HIP_TRY(hipStreamSynchronize(device->stream), "write_buffer");
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_buffer_read(WgcBuffer buffer, uint64_t offset, void *dst, uint64_t size)
+{
+    if (!buffer || (!dst && size)) return set_error(WGC_ERR_INVALID, "null argument");
+    if (offset > buffer->size || size > buffer->size - offset)
+        return set_error(WGC_ERR_INVALID, "read out of range");
+    WgcDevice_T *d = buffer->dev;
+    std::lock_guard<std::mutex> lk(d->mu);
+    HIP_TRY(hipSetDevice(d->hip_device), "hipSetDevice");
+    int32_t r = drain(d);
+    if (r != WGC_OK) return r;
+    if (size) HIP_TRY(hipMemcpy(dst, buffer->ptr + offset, size, hipMemcpyDeviceToHost), "buffer read");
+    return WGC_OK;
+}
+
+static uint32_t texel_bytes(uint32_t format)
+{
+    switch (format) {
+    case WGC_TEXTURE_FORMAT_RGBA8_UNORM:
+    case WGC_TEXTURE_FORMAT_RGBA8_UNORM_SRGB:
+    case WGC_TEXTURE_FORMAT_BGRA8_UNORM:
+    case WGC_TEXTURE_FORMAT_BGRA8_UNORM_SRGB:
+    case WGC_TEXTURE_FORMAT_DEPTH32_FLOAT:
+        return 4;
+    default:
+        return 0;
+    }
+}
+
+extern "C" int32_t wgc_texture_create(WgcDevice device, uint32_t width, uint32_t height,
+                                      uint32_t format, WgcTexture *out)
+{
+    if (!device || !out) return set_error(WGC_ERR_INVALID, "null argument");
+    uint32_t tb = texel_bytes(format);
+    if (!tb) return set_error(WGC_ERR_UNSUPPORTED, "unsupported texture format");
+    if (width == 0 || height == 0 || width > 65535u * kTileW || height > 65535u * kTileH)
+        return set_error(WGC_ERR_INVALID, "bad texture size");
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    auto *t = new WgcTexture_T();
+    t->dev = device;
+    retain(device);
+    t->width = width;
+    t->height = height;
+    t->format = format;
+    t->size = (uint64_t)width * height * tb;
+    hipError_t e = hipMalloc(&t->ptr, t->size);
+    if (e == hipSuccess) e = hipMemset(t->ptr, 0, t->size);
+    if (e != hipSuccess) {
+        release(t);
+        return set_error(WGC_ERR_OUT_OF_MEMORY, std::string("texture: ") + hipGetErrorString(e));
+    }
+    *out = t;
+    return WGC_OK;
+}
+
+extern "C" void wgc_texture_release(WgcTexture texture) { release(texture); }
+
+extern "C" int32_t wgc_queue_write_texture(WgcDevice device, WgcTexture texture, const void *data,
+                                           uint64_t data_size, uint64_t offset, uint32_t bytes_per_row)
+{
+    if (!device || !texture || !data) return set_error(WGC_ERR_INVALID, "null argument");
+    // device.rs:406-430: rows of bytes_per_row bytes, copied back to back
+    uint64_t bpr = bytes_per_row ? bytes_per_row : data_size;
+    uint64_t rows = bytes_per_row ? texture->height : 1u;
+    uint64_t n = bpr * rows;
+    if (offset > data_size || n > data_size - offset)
+        return set_error(WGC_ERR_INVALID, "write_texture source too small (reference: slice panic)");
+    if (n > texture->size)
+        return set_error(WGC_ERR_INVALID, "write_texture past the texture (reference: slice panic)");
+    std::lock_guard<std::mutex> lk(device->mu);
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    HIP_TRY(hipMemcpyAsync(texture->ptr, (const uint8_t *)data + offset, n, hipMemcpyHostToDevice,
+                           device->stream), "write_texture");
+    HIP_TRY(hipStreamSynchronize(device->stream), "write_texture");
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_texture_read(WgcTexture texture, void *dst, uint64_t size)
+{
+    if (!texture || !dst) return set_error(WGC_ERR_INVALID, "null argument");
+    if (size != texture->size) return set_error(WGC_ERR_INVALID, "size must equal the texture size");
+    WgcDevice_T *d = texture->dev;
+    std::lock_guard<std::mutex> lk(d->mu);
+    HIP_TRY(hipSetDevice(d->hip_device), "hipSetDevice");
+    int32_t r = drain(d);
+    if (r != WGC_OK) return r;
+    HIP_TRY(hipMemcpy(dst, texture->ptr, size, hipMemcpyDeviceToHost), "texture read");
+    return WGC_OK;
+}
+
+// ------------------------------------------------------------------
+// samplers, bind groups, shaders, pipelines
+// ------------------------------------------------------------------
+
+extern "C" int32_t wgc_sampler_create(WgcDevice device, const WgcSamplerDescriptor *desc, WgcSampler *out)
+{
+    if (!device || !desc || !out) return set_error(WGC_ERR_INVALID, "null argument");
+    if (desc->address_mode_u > 2 || desc->address_mode_v > 2)
+        return set_error(WGC_ERR_UNSUPPORTED, "address mode (ClampToBorder is todo!() in the reference)");
+    auto *s = new WgcSampler_T();
+    s->desc = *desc;
+    *out = s;
+    return WGC_OK;
+}
+
+extern "C" void wgc_sampler_release(WgcSampler sampler) { release(sampler); }
+
+extern "C" int32_t wgc_bind_group_create(WgcDevice device, uint32_t n, const WgcBindGroupEntry *entries,
+                                         WgcBindGroup *out)
+{
+    if (!device || !out || (n && !entries)) return set_error(WGC_ERR_INVALID, "null argument");
+    auto *g = new WgcBindGroup_T();
+    for (uint32_t i = 0; i < n; ++i) {
+        const WgcBindGroupEntry &e = entries[i];
+        BindingSlot s;
+        s.binding = e.binding;
+        s.kind = e.kind;
+        if (e.kind == WGC_BINDING_BUFFER) {
+            if (!e.buffer || e.offset > e.buffer->size) {
+                release(g);
+                return set_error(WGC_ERR_INVALID, "bad buffer binding");
+            }
+            s.buffer = e.buffer;
+            retain(e.buffer);
+            s.offset = e.offset;
+            s.size = e.size ? e.size : e.buffer->size - e.offset;
+            if (s.size > e.buffer->size - e.offset) {
+                release(g);
+                return set_error(WGC_ERR_INVALID, "buffer binding past the end");
+            }
+        } else if (e.kind == WGC_BINDING_TEXTURE) {
+            if (!e.texture) { release(g); return set_error(WGC_ERR_INVALID, "null texture"); }
+            s.texture = e.texture;
+            retain(e.texture);
+        } else if (e.kind == WGC_BINDING_SAMPLER) {
+            if (!e.sampler) { release(g); return set_error(WGC_ERR_INVALID, "null sampler"); }
+            s.sampler = e.sampler;
+            retain(e.sampler);
+        } else {
+            release(g);
+            return set_error(WGC_ERR_INVALID, "bad binding kind");
+        }
+        g->entries.push_back(s);
+    }
+    *out = g;
+    return WGC_OK;
+}
+
+extern "C" void wgc_bind_group_release(WgcBindGroup group) { release(group); }
+
+extern "C" int32_t wgc_shader_module_create_builtin(WgcDevice device, uint32_t program,
+                                                    uint32_t interpolation, WgcShaderModule *out)
+{
+    if (!device || !out) return set_error(WGC_ERR_INVALID, "null argument");
+    if (program < PROG_MVP_COLOR || program > PROG_CLIP_COLOR)
+        return set_error(WGC_ERR_INVALID, "unknown built-in program");
+    if (interpolation > WGC_INTERPOLATE_FLAT) return set_error(WGC_ERR_INVALID, "bad interpolation");
+    auto *m = new WgcShaderModule_T();
+    m->program = program;
+    m->interpolation = interpolation;
+    *out = m;
+    return WGC_OK;
+}
+
+extern "C" void wgc_shader_module_release(WgcShaderModule module) { release(module); }
+
+extern "C" int32_t wgc_render_pipeline_create(WgcDevice device, const WgcRenderPipelineDescriptor *desc,
+                                              WgcRenderPipeline *out)
+{
+    if (!device || !desc || !out || !desc->vertex_module) return set_error(WGC_ERR_INVALID, "null argument");
+    if (desc->topology != WGC_TOPOLOGY_TRIANGLE_LIST)
+        return set_error(WGC_ERR_UNSUPPORTED, "only TriangleList is implemented");
+    if (desc->vertex_buffer_count > (uint32_t)kMaxVb) return set_error(WGC_ERR_UNSUPPORTED, "too many vertex buffers");
+    if (desc->fragment_module && desc->fragment_module->program != desc->vertex_module->program)
+        return set_error(WGC_ERR_INVALID, "vertex and fragment entry points come from different programs");
+    auto *p = new WgcRenderPipeline_T();
+    p->desc = *desc;
+    p->desc.vertex_buffers = nullptr;
+    p->program = desc->vertex_module->program;
+    p->interp = desc->vertex_module->interpolation;
+    p->has_fragment = desc->fragment_module != nullptr;
+    // vertex.rs:60-85: shader locations -> (buffer, offset, size)
+    bool seen[kMaxAttr] = {};
+    for (uint32_t b = 0; b < desc->vertex_buffer_count; ++b) {
+        const WgcVertexBufferLayout &l = desc->vertex_buffers[b];
+        p->vb_stride.push_back(l.array_stride);
+        p->vb_step.push_back(l.step_mode);
+        for (uint32_t a = 0; a < l.attribute_count; ++a) {
+            const WgcVertexAttribute &at = l.attributes[a];
+            if (at.shader_location >= (uint32_t)kMaxAttr || at.format < 1 || at.format > 4 || (at.offset & 3u)) {
+                release(p);
+                return set_error(WGC_ERR_UNSUPPORTED, "vertex attribute");
+            }
+            if (seen[at.shader_location]) {
+                release(p);
+                return set_error(WGC_ERR_INVALID, "duplicate vertex attribute location (reference: panic)");
+            }
+            seen[at.shader_location] = true;
+            p->attr[at.shader_location] = AttrDev{b, at.offset, at.format, 0u};
+            p->n_attr = std::max(p->n_attr, at.shader_location + 1u);
+        }
+    }
+    // the built-in programs' vertex inputs and varyings
+    auto need = [&](uint32_t loc, uint32_t lo, uint32_t hi) {
+        return loc < p->n_attr && seen[loc] && p->attr[loc].ncomp >= lo && p->attr[loc].ncomp <= hi;
+    };
+    switch (p->program) {
+    case PROG_MVP_COLOR:
+    case PROG_CLIP_COLOR:
+        if (!need(0, 4, 4) || !need(1, 3, 4)) {
+            release(p);
+            return set_error(WGC_ERR_INVALID, "program needs @location(0) vec4f position, @location(1) vec3f/vec4f colour");
+        }
+        p->nvar = p->attr[1].ncomp;
+        break;
+    case PROG_MVP_TEXTURE:
+        if (!need(0, 4, 4) || !need(1, 2, 2)) {
+            release(p);
+            return set_error(WGC_ERR_INVALID, "program needs @location(0) vec4f position, @location(1) vec2f uv");
+        }
+        p->nvar = 2;
+        break;
+    default:
+        p->nvar = 4;
+        break;
+    }
+    if (p->interp == WGC_INTERPOLATE_PERSPECTIVE && device->semantics == WGC_SEMANTICS_REFERENCE) {
+        release(p);
+        return set_error(WGC_ERR_UNSUPPORTED,
+                         "perspective interpolation is todo!() in the reference (fragment.rs:343-345); "
+                         "use WGC_SEMANTICS_EXTENDED");
+    }
+    if (desc->has_depth_stencil && desc->depth_format != WGC_TEXTURE_FORMAT_DEPTH32_FLOAT) {
+        release(p);
+        return set_error(WGC_ERR_UNSUPPORTED, "only Depth32Float depth is implemented");
+    }
+    if (desc->has_depth_stencil && (desc->depth_compare < 1 || desc->depth_compare > 8)) {
+        release(p);
+        return set_error(WGC_ERR_INVALID, "bad depth compare");
+    }
+    *out = p;
+    return WGC_OK;
+}
+
+extern "C" void wgc_render_pipeline_release(WgcRenderPipeline pipeline) { release(pipeline); }
+
+// ------------------------------------------------------------------
+// recording
+// ------------------------------------------------------------------
+
+extern "C" int32_t wgc_command_encoder_create(WgcDevice device, WgcCommandEncoder *out)
+{
+    if (!device || !out) return set_error(WGC_ERR_INVALID, "null argument");
+    auto *e = new WgcCommandEncoder_T();
+    e->dev = device;
+    *out = e;
+    return WGC_OK;
+}
+
+extern "C" void wgc_command_encoder_release(WgcCommandEncoder encoder) { release(encoder); }
+
+extern "C" int32_t wgc_command_encoder_begin_render_pass(WgcCommandEncoder enc, const WgcRenderPassDescriptor *desc,
+                                                         WgcRenderPass *out)
+{
+    if (!enc || !desc || !out) return set_error(WGC_ERR_INVALID, "null argument");
+    if (enc->pass_open) return set_error(WGC_ERR_INVALID, "a render pass is already open");
+    if (desc->color_attachment_count > 1)
+        return set_error(WGC_ERR_UNSUPPORTED, "more than one colour attachment");
+    auto *p = new WgcRenderPass_T();
+    p->enc = enc;
+    retain(enc);
+    uint32_t w = 0, h = 0;
+    bool sized = false;
+    // state.rs:84-96: all attachments must be the same size
+    auto size_of = [&](WgcTexture_T *t) -> bool {
+        if (!sized) { w = t->width; h = t->height; sized = true; return true; }
+        return t->width == w && t->height == h;
+    };
+    if (desc->color_attachment_count == 1) {
+        const WgcColorAttachment &c = desc->color_attachments[0];
+        if (!c.view || c.view->format == WGC_TEXTURE_FORMAT_DEPTH32_FLOAT) {
+            release(p);
+            return set_error(WGC_ERR_INVALID, "bad colour attachment");
+        }
+        size_of(c.view);
+        p->pass.color = c.view;
+        retain(c.view);
+        p->pass.color_load = c.load_op;
+        for (int k = 0; k < 4; ++k) p->pass.clear_color[k] = c.clear_value[k];
+    }
+    if (desc->depth_stencil_attachment) {
+        const WgcDepthStencilAttachment &dd = *desc->depth_stencil_attachment;
+        if (!dd.view || dd.view->format != WGC_TEXTURE_FORMAT_DEPTH32_FLOAT) {
+            release(p);
+            return set_error(WGC_ERR_INVALID, "bad depth attachment");
+        }
+        if (!size_of(dd.view)) {
+            release(p);
+            return set_error(WGC_ERR_INVALID, "All render attachments must be the same size (state.rs:88-91)");
+        }
+        p->pass.depth = dd.view;
+        retain(dd.view);
+        p->pass.has_depth_ops = dd.has_depth_ops;
+        p->pass.depth_load = dd.depth_load_op;
+        p->pass.clear_depth = dd.depth_clear_value;
+    }
+    enc->pass_open = true;
+    *out = p;
+    return WGC_OK;
+}
+
+static int32_t push(WgcRenderPass pass, SubCommand c)
+{
+    if (!pass) return set_error(WGC_ERR_INVALID, "null pass");
+    pass->pass.cmds.push_back(c);
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_render_pass_set_pipeline(WgcRenderPass pass, WgcRenderPipeline pipeline)
+{
+    if (!pipeline) return set_error(WGC_ERR_INVALID, "null pipeline");
+    SubCommand c{CmdKind::SetPipeline};
+    c.obj = pipeline;
+    retain(pipeline);
+    return push(pass, c);
+}
+
+extern "C" int32_t wgc_render_pass_set_bind_group(WgcRenderPass pass, uint32_t index, WgcBindGroup group)
+{
+    SubCommand c{CmdKind::SetBindGroup};
+    c.slot = index;
+    c.obj = group;
+    retain(group);
+    return push(pass, c);
+}
+
+extern "C" int32_t wgc_render_pass_set_vertex_buffer(WgcRenderPass pass, uint32_t slot, WgcBuffer buffer,
+                                                     uint64_t offset, uint64_t size)
+{
+    if (!buffer || slot >= (uint32_t)kMaxVb) return set_error(WGC_ERR_INVALID, "bad vertex buffer");
+    if (offset > buffer->size) return set_error(WGC_ERR_INVALID, "vertex buffer offset past the end");
+    SubCommand c{CmdKind::SetVertexBuffer};
+    c.obj = buffer;
+    retain(buffer);
+    c.slot = slot;
+    c.offset = offset;
+    c.size = size ? size : buffer->size - offset;
+    if (c.size > buffer->size - offset) { release(buffer); return set_error(WGC_ERR_INVALID, "vertex buffer range"); }
+    return push(pass, c);
+}
+
+extern "C" int32_t wgc_render_pass_set_index_buffer(WgcRenderPass pass, WgcBuffer buffer, uint32_t format,
+                                                    uint64_t offset, uint64_t size)
+{
+    if (!buffer || (format != WGC_INDEX_UINT16 && format != WGC_INDEX_UINT32))
+        return set_error(WGC_ERR_INVALID, "bad index buffer");
+    if (offset > buffer->size) return set_error(WGC_ERR_INVALID, "index buffer offset past the end");
+    SubCommand c{CmdKind::SetIndexBuffer};
+    c.obj = buffer;
+    retain(buffer);
+    c.format = format;
+    c.offset = offset;
+    c.size = size ? size : buffer->size - offset;
+    if (c.size > buffer->size - offset) { release(buffer); return set_error(WGC_ERR_INVALID, "index buffer range"); }
+    return push(pass, c);
+}
+
+extern "C" int32_t wgc_render_pass_set_viewport(WgcRenderPass pass, float x, float y, float w, float h,
+                                                float min_depth, float max_depth)
+{
+    SubCommand c{CmdKind::SetViewport};
+    c.viewport[0] = x; c.viewport[1] = y; c.viewport[2] = w; c.viewport[3] = h;
+    c.viewport[4] = min_depth; c.viewport[5] = max_depth;
+    return push(pass, c);
+}
+
+extern "C" int32_t wgc_render_pass_set_scissor_rect(WgcRenderPass pass, uint32_t x, uint32_t y, uint32_t w,
+                                                    uint32_t h)
+{
+    SubCommand c{CmdKind::SetScissor};
+    c.rect[0] = x; c.rect[1] = y; c.rect[2] = w; c.rect[3] = h;
+    return push(pass, c);
+}
+
+extern "C" int32_t wgc_render_pass_set_blend_constant(WgcRenderPass pass, const double *)
+{
+    return push(pass, SubCommand{CmdKind::SetBlendConstant});
+}
+
+extern "C" int32_t wgc_render_pass_set_stencil_reference(WgcRenderPass pass, uint32_t)
+{
+    return push(pass, SubCommand{CmdKind::SetStencilReference});
+}
+
+extern "C" int32_t wgc_render_pass_draw(WgcRenderPass pass, uint32_t first_vertex, uint32_t vertex_count,
+                                        uint32_t first_instance, uint32_t instance_count)
+{
+    SubCommand c{CmdKind::Draw};
+    c.first = first_vertex;
+    c.count = vertex_count;
+    c.first_instance = first_instance;
+    c.instance_count = instance_count;
+    return push(pass, c);
+}
+
+extern "C" int32_t wgc_render_pass_draw_indexed(WgcRenderPass pass, uint32_t first_index, uint32_t index_count,
+                                                int32_t base_vertex, uint32_t first_instance,
+                                                uint32_t instance_count)
+{
+    SubCommand c{CmdKind::DrawIndexed};
+    c.first = first_index;
+    c.count = index_count;
+    c.base_vertex = base_vertex;
+    c.first_instance = first_instance;
+    c.instance_count = instance_count;
+    return push(pass, c);
+}
+
+extern "C" int32_t wgc_render_pass_end(WgcRenderPass pass)
+{
+    if (!pass) return set_error(WGC_ERR_INVALID, "null pass");
+    WgcCommandEncoder_T *enc = pass->enc;
+    enc->passes.push_back(std::move(pass->pass));
+    pass->pass = PassCommand();
+    enc->pass_open = false;
+    release(enc);
+    pass->enc = nullptr;
+    release(pass);
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_command_encoder_finish(WgcCommandEncoder enc, WgcCommandBuffer *out)
+{
+    if (!enc || !out) return set_error(WGC_ERR_INVALID, "null argument");
+    if (enc->pass_open) return set_error(WGC_ERR_INVALID, "a render pass is still open");
+    auto *cb = new WgcCommandBuffer_T();
+    cb->passes = std::move(enc->passes);
+    enc->passes.clear();
+    *out = cb;
+    return WGC_OK;
+}
+
+extern "C" void wgc_command_buffer_release(WgcCommandBuffer buffer) { release(buffer); }
+
+// ------------------------------------------------------------------
+// execution
+// ------------------------------------------------------------------
+
+static uint8_t unorm8(float v)
+{
+    float x = v * 255.0f;
+    if (x < 0.0f) x = 0.0f;
+    if (x > 255.0f) x = 255.0f;
+    if (!(x == x)) return 0;
+    return (uint8_t)x;
+}
+
+static uint32_t pack_clear(const double c[4], uint32_t format)
+{
+    uint32_t r = unorm8((float)c[0]), g = unorm8((float)c[1]), b = unorm8((float)c[2]), a = unorm8((float)c[3]);
+    bool bgra = format == WGC_TEXTURE_FORMAT_BGRA8_UNORM || format == WGC_TEXTURE_FORMAT_BGRA8_UNORM_SRGB;
+    return bgra ? (b | (g << 8) | (r << 16) | (a << 24)) : (r | (g << 8) | (b << 16) | (a << 24));
+}
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Carves the per-draw scratch out of the device arena, growing it when needed.
+static int32_t plan_scratch(WgcDevice_T *dev, DrawParams &d)
+{
+    uint32_t n_tiles = d.tiles_x * d.tiles_y;
+    uint64_t prims = d.total_prims;
+    uint64_t clip_tri_cap = 2ull * prims + 4096ull;
+    uint64_t rec_cap = prims + clip_tri_cap;
+    uint64_t bins_cap = rec_cap * (uint64_t)kMaxBinTiles;
+    if (bins_cap > 0xFFFFFFFFull) return set_error(WGC_ERR_UNSUPPORTED, "draw too large for one launch");
+    struct Part { size_t bytes; size_t off; };
+    Part parts[] = {
+        {4ull * n_tiles, 0}, {4ull * (n_tiles + 1), 0}, {4ull * n_tiles, 0},
+        {4ull * bins_cap, 0}, {16ull * rec_cap, 0}, {16ull * rec_cap, 0},
+        {4ull * (prims ? prims : 1), 0}, {4ull * kClipFloats * clip_tri_cap, 0},
+        {4ull * (prims ? prims : 1), 0}, {4ull * C_NUM, 0},
+    };
+    size_t total = 0;
+    for (auto &p : parts) {
+        p.off = total;
+        total = align_up(total + p.bytes, 256);
+    }
+    if (total > dev->arena_size) {
+        HIP_TRY(hipStreamSynchronize(dev->stream), "arena");
+        if (dev->arena) HIP_TRY(hipFree(dev->arena), "arena");
+        dev->arena = nullptr;
+        size_t grow = std::max(total, dev->arena_size * 2);
+        hipError_t e = hipMalloc(&dev->arena, grow);
+        if (e != hipSuccess) {
+            dev->arena_size = 0;
+            return set_error(WGC_ERR_OUT_OF_MEMORY, "scratch arena");
+        }
+        dev->arena_size = grow;
+    }
+    uint8_t *a = dev->arena;
+    Scratch &s = d.s;
+    s.tile_count = (uint32_t *)(a + parts[0].off);
+    s.tile_offset = (uint32_t *)(a + parts[1].off);
+    s.tile_fill = (uint32_t *)(a + parts[2].off);
+    s.bins = (uint32_t *)(a + parts[3].off);
+    s.recs = (uint4 *)(a + parts[4].off);
+    s.big = (uint4 *)(a + parts[5].off);
+    s.clip_list = (uint32_t *)(a + parts[6].off);
+    s.clip_tris = (float *)(a + parts[7].off);
+    s.clip_base = (uint32_t *)(a + parts[8].off);
+    s.counters = (uint32_t *)(a + parts[9].off);
+    s.frags = dev->d_frags;
+    s.error = dev->d_error;
+    // the debug counts are only written when they were sized for this pass
+    s.frag_px = (dev->frag_px && dev->frag_px_size == (size_t)d.width * d.height * 4u) ? dev->frag_px : nullptr;
+    s.bins_cap = (uint32_t)bins_cap;
+    s.rec_cap = (uint32_t)rec_cap;
+    s.big_cap = (uint32_t)rec_cap;
+    s.clip_list_cap = (uint32_t)prims;
+    s.clip_tri_cap = (uint32_t)clip_tri_cap;
+    return WGC_OK;
+}
+
+struct PassState {
+    WgcRenderPipeline_T *pipeline = nullptr;
+    WgcBindGroup_T *groups[4] = {};
+    BufferBinding vbs[kMaxVb];
+    BufferBinding ib;
+    float viewport[6];
+    uint32_t scissor[4];
+};
+
+static uint32_t resolve_mode(const WgcRenderPipelineDescriptor &p, bool depth_test, bool blend)
+{
+    if (blend) return RES_ORDERED;
+    if (!depth_test) return RES_LAST;
+    bool w = p.depth_write_enabled != 0;
+    switch (p.depth_compare) {
+    case WGC_COMPARE_LESS: return w ? RES_MIN_FIRST : RES_PASS_LAST;
+    case WGC_COMPARE_LESS_EQUAL: return w ? RES_MIN_LAST : RES_PASS_LAST;
+    case WGC_COMPARE_GREATER: return w ? RES_MAX_FIRST : RES_PASS_LAST;
+    case WGC_COMPARE_GREATER_EQUAL: return w ? RES_MAX_LAST : RES_PASS_LAST;
+    case WGC_COMPARE_NOT_EQUAL: return w ? RES_ORDERED : RES_PASS_LAST;
+    default: return RES_PASS_LAST;
+    }
+}
+
+// Builds the kernel parameters of one draw from the pass state
+// (DrawCall::execute, state.rs:239-480).
+static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassState &st, const SubCommand &c,
+                          uint32_t w, uint32_t h, DrawParams &d)
+{
+    memset(&d, 0, sizeof d);
+    const WgcRenderPipeline_T *pl = st.pipeline;
+    if (!pl) return set_error(WGC_ERR_INVALID, "No pipeline bound (state.rs:243)");
+    const WgcRenderPipelineDescriptor &pd = pl->desc;
+    bool extended = dev->semantics == WGC_SEMANTICS_EXTENDED;
+    d.program = pl->program;
+    d.nvar = pl->nvar;
+    d.interp = pl->interp;
+    d.front_face_cw = pd.front_face;
+    d.cull_mode = pd.cull_mode;
+    d.depth_test = (pc.depth && pd.has_depth_stencil) ? 1u : 0u;
+    d.depth_compare = pd.depth_compare;
+    d.depth_write = pd.depth_write_enabled;
+    d.blend = (extended && pd.blend == WGC_BLEND_ALPHA && pc.color) ? 1u : 0u;
+    d.resolve = resolve_mode(pd, d.depth_test != 0, d.blend != 0);
+    // vertex buffers (vertex.rs:261-272)
+    uint32_t n_vb = (uint32_t)pl->vb_stride.size();
+    for (uint32_t b = 0; b < n_vb; ++b)
+        if (!st.vbs[b].buffer)
+            return set_error(WGC_ERR_INVALID, "Buffer " + std::to_string(b) + " not bound (vertex.rs:268)");
+    d.n_attr = pl->n_attr;
+    for (uint32_t a = 0; a < pl->n_attr; ++a) {
+        const AttrDev &at = pl->attr[a];
+        if (!at.ncomp) continue;  // location not provided by any buffer
+        const BufferBinding &vb = st.vbs[at.buffer];
+        d.attr[a] = AttrBind{vb.buffer->ptr + vb.offset, vb.size, pl->vb_stride[at.buffer],
+                             pl->vb_step[at.buffer], at.offset, at.ncomp};
+    }
+    d.indexed = c.kind == CmdKind::DrawIndexed;
+    if (d.indexed) {
+        if (!st.ib.buffer) return set_error(WGC_ERR_INVALID, "No index buffer bound (state.rs:396)");
+        d.index_data = st.ib.buffer->ptr + st.ib.offset;
+        d.index_size = st.ib.size;
+        d.index_format = st.ib.format;
+        d.base_vertex = c.base_vertex;
+    }
+    d.first = c.first;
+    d.count = c.count;
+    d.first_instance = c.first_instance;
+    d.instance_count = c.instance_count;
+    d.nprim = c.count / 3u;
+    uint64_t total = (uint64_t)d.nprim * c.instance_count;
+    if (total >= (1ull << 25)) return set_error(WGC_ERR_UNSUPPORTED, "more than 2^25 primitives in one draw");
+    d.total_prims = (uint32_t)total;
+    // bind groups of the built-in programs
+    if (d.program == PROG_MVP_COLOR || d.program == PROG_MVP_TEXTURE) {
+        const BindingSlot *u = st.groups[0] ? st.groups[0]->find(0) : nullptr;
+        if (!u || u->kind != WGC_BINDING_BUFFER || u->size < 64)
+            return set_error(WGC_ERR_INVALID, "@group(0) @binding(0) uniform mat4x4f not bound");
+        d.uniform = (const float *)(u->buffer->ptr + u->offset);
+    }
+    if (d.program == PROG_MVP_TEXTURE) {
+        const BindingSlot *t = st.groups[1] ? st.groups[1]->find(0) : nullptr;
+        const BindingSlot *s = st.groups[1] ? st.groups[1]->find(1) : nullptr;
+        if (!t || t->kind != WGC_BINDING_TEXTURE || !s || s->kind != WGC_BINDING_SAMPLER)
+            return set_error(WGC_ERR_INVALID, "@group(1) texture_2d / sampler not bound");
+        const WgcTexture_T *tex = t->texture;
+        if (tex->format == WGC_TEXTURE_FORMAT_DEPTH32_FLOAT || tex->format == WGC_TEXTURE_FORMAT_BGRA8_UNORM ||
+            tex->format == WGC_TEXTURE_FORMAT_BGRA8_UNORM_SRGB)
+            return set_error(WGC_ERR_UNSUPPORTED, "texture format (texture.rs:186 todo!())");
+        d.tex = tex->ptr;
+        d.tex_w = tex->width;
+        d.tex_h = tex->height;
+        d.tex_srgb = tex->format == WGC_TEXTURE_FORMAT_RGBA8_UNORM_SRGB;
+        d.addr_u = s->sampler->desc.address_mode_u;
+        d.addr_v = s->sampler->desc.address_mode_v;
+        d.bilinear = (extended && (s->sampler->desc.mag_filter == WGC_FILTER_LINEAR ||
+                                   s->sampler->desc.min_filter == WGC_FILTER_LINEAR)) ? 1u : 0u;
+        d.srgb = extended ? 1u : 0u;
+        d.srgb_lut = dev->d_srgb;
+    }
+    memcpy(d.viewport, st.viewport, sizeof d.viewport);
+    memcpy(d.scissor, st.scissor, sizeof d.scissor);
+    if ((uint64_t)d.scissor[0] + d.scissor[2] > w || (uint64_t)d.scissor[1] + d.scissor[3] > h)
+        return set_error(WGC_ERR_INVALID, "scissor rect outside the framebuffer");
+    d.width = w;
+    d.height = h;
+    d.tiles_x = (w + kTileW - 1) / kTileW;
+    d.tiles_y = (h + kTileH - 1) / kTileH;
+    d.color = pc.color ? pc.color->ptr : nullptr;
+    d.color_bgra = pc.color && (pc.color->format == WGC_TEXTURE_FORMAT_BGRA8_UNORM ||
+                                pc.color->format == WGC_TEXTURE_FORMAT_BGRA8_UNORM_SRGB);
+    d.depth = pc.depth ? (float *)pc.depth->ptr : nullptr;
+    return plan_scratch(dev, d);
+}
+
+static int32_t execute_pass(WgcDevice_T *dev, const PassCommand &pc)
+{
+    uint32_t w = 0, h = 0;
+    if (pc.color) { w = pc.color->width; h = pc.color->height; }
+    else if (pc.depth) { w = pc.depth->width; h = pc.depth->height; }
+    KernelTimer *kt = dev->timing ? &dev->timer : nullptr;
+    HIP_TRY(launch_reset(dev->d_frags, dev->stream), "pass");
+    dev->last_prims = 0;
+    dev->last_launches = 0;
+    PassState st;
+    st.viewport[0] = 0.0f; st.viewport[1] = 0.0f;
+    st.viewport[2] = (float)w; st.viewport[3] = (float)h;
+    st.viewport[4] = 0.0f; st.viewport[5] = 1.0f;
+    st.scissor[0] = 0; st.scissor[1] = 0; st.scissor[2] = w; st.scissor[3] = h;
+    bool clear_color = pc.color && pc.color_load == WGC_LOAD_OP_CLEAR;
+    bool clear_depth = pc.depth && pc.has_depth_ops && pc.depth_load == WGC_LOAD_OP_CLEAR;
+    bool clear_pending = clear_color || clear_depth;
+    uint32_t clear_rgba = pc.color ? pack_clear(pc.clear_color, pc.color->format) : 0u;
+    for (const SubCommand &c : pc.cmds) {
+        switch (c.kind) {
+        case CmdKind::SetPipeline: st.pipeline = (WgcRenderPipeline_T *)c.obj; break;
+        case CmdKind::SetBindGroup:
+            if (c.slot >= 4) return set_error(WGC_ERR_UNSUPPORTED, "bind group index >= 4");
+            st.groups[c.slot] = (WgcBindGroup_T *)c.obj;
+            break;
+        case CmdKind::SetVertexBuffer:
+            st.vbs[c.slot] = BufferBinding{(WgcBuffer_T *)c.obj, c.offset, c.size, 0};
+            break;
+        case CmdKind::SetIndexBuffer:
+            st.ib = BufferBinding{(WgcBuffer_T *)c.obj, c.offset, c.size, c.format};
+            break;
+        case CmdKind::SetViewport: memcpy(st.viewport, c.viewport, sizeof st.viewport); break;
+        case CmdKind::SetScissor: memcpy(st.scissor, c.rect, sizeof st.scissor); break;
+        case CmdKind::SetBlendConstant:
+        case CmdKind::SetStencilReference:
+            break;
+        case CmdKind::Draw:
+        case CmdKind::DrawIndexed: {
+            if (!st.pipeline) return set_error(WGC_ERR_INVALID, "No pipeline bound (state.rs:243)");
+            if (!st.pipeline->has_fragment || w == 0) break;  // only vertex shading, no visible effect
+            DrawParams d;
+            int32_t r = build_draw(dev, pc, st, c, w, h, d);
+            if (r != WGC_OK) return r;
+            dev->last_prims += d.total_prims;
+            d.init_clear = clear_pending ? 1u : 0u;
+            d.clear_color_on = clear_color;
+            d.clear_depth_on = clear_depth;
+            d.clear_rgba = clear_rgba;
+            d.clear_depth = pc.clear_depth;
+            HIP_TRY(launch_draw(d, dev->stream, kt), "draw launch");
+            dev->last_launches += 6;
+            clear_pending = false;
+            break;
+        }
+        }
+    }
+    if (clear_pending) {
+        HIP_TRY(launch_clear(clear_color ? (uint32_t *)pc.color->ptr : nullptr, clear_rgba,
+                             clear_depth ? (float *)pc.depth->ptr : nullptr, pc.clear_depth,
+                             (uint64_t)w * h, dev->stream, kt), "clear launch");
+        dev->last_launches += 1;
+    }
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcCommandBuffer *buffers,
+                                    uint64_t *submission_index)
+{
+    if (!device || (count && !buffers)) return set_error(WGC_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(device->mu);
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    Inflight f;
+    f.index = device->next_index++;
+    for (uint32_t i = 0; i < count; ++i) {
+        WgcCommandBuffer_T *cb = buffers[i];
+        if (!cb) return set_error(WGC_ERR_INVALID, "null command buffer");
+        for (const PassCommand &pc : cb->passes) {
+            int32_t r = execute_pass(device, pc);
+            if (r != WGC_OK) return r;
+        }
+        retain(cb);
+        f.keep.push_back(cb);
+    }
+    if (!device->event_pool.empty()) {
+        f.done = device->event_pool.back();
+        device->event_pool.pop_back();
+    } else {
+        HIP_TRY(hipEventCreateWithFlags(&f.done, hipEventDisableTiming), "hipEventCreate");
+    }
+    HIP_TRY(hipEventRecord(f.done, device->stream), "hipEventRecord");
+    if (submission_index) *submission_index = f.index;
+    device->inflight.push_back(std::move(f));
+    // keep the queue short: retire what is already done
+    return retire(device, UINT64_MAX, false, 0);
+}
+
+// ------------------------------------------------------------------
+// measurement
+// ------------------------------------------------------------------
+
+extern "C" int32_t wgc_device_timestamp(WgcDevice device, uint32_t slot)
+{
+    if (!device || slot >= 64) return set_error(WGC_ERR_INVALID, "bad slot");
+    std::lock_guard<std::mutex> lk(device->mu);
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    if (!device->ts[slot]) HIP_TRY(hipEventCreate(&device->ts[slot]), "hipEventCreate");
+    HIP_TRY(hipEventRecord(device->ts[slot], device->stream), "hipEventRecord");
+    device->ts_recorded[slot] = true;
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_device_elapsed_ms(WgcDevice device, uint32_t a, uint32_t b, float *ms)
+{
+    if (!device || a >= 64 || b >= 64 || !ms) return set_error(WGC_ERR_INVALID, "bad slot");
+    if (!device->ts_recorded[a] || !device->ts_recorded[b]) return set_error(WGC_ERR_INVALID, "slot not recorded");
+    HIP_TRY(hipEventSynchronize(device->ts[b]), "hipEventSynchronize");
+    HIP_TRY(hipEventElapsedTime(ms, device->ts[a], device->ts[b]), "hipEventElapsedTime");
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_device_set_kernel_timing(WgcDevice device, uint32_t enabled)
+{
+    if (!device) return set_error(WGC_ERR_INVALID, "null device");
+    std::lock_guard<std::mutex> lk(device->mu);
+    device->timing = enabled != 0;
+    device->ktimes.clear();
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_device_kernel_time(WgcDevice device, const char *name, double *total_ms, uint64_t *launches)
+{
+    if (!device || !name || !total_ms || !launches) return set_error(WGC_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(device->mu);
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    int32_t r = drain(device);
+    if (r != WGC_OK) return r;
+    r = resolve_timers(device);
+    if (r != WGC_OK) return r;
+    *total_ms = 0.0;
+    *launches = 0;
+    for (auto &kv : device->ktimes)
+        if (kv.first.find(name) != std::string::npos) {
+            *total_ms += kv.second.first;
+            *launches += kv.second.second;
+        }
+    return WGC_OK;
+}

@@ -1,0 +1,728 @@
+// wgc_core.h -- device-side core of the MI355X render-pass executor.
+//
+// Everything here performs the same IEEE-754 binary32 operations, in the
+// same order, as the reference's CPU executor, so that coverage, depth and
+// colour come out bit-identical.  The translation unit must be compiled
+// with -ffp-contract=off (no fused multiply-add: the reference never fuses,
+// naga-cranelift/src/expression/binary.rs:299-321) and correctly rounded
+// f32 division.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wgc {
+
+constexpr int kMaxVb = 4;
+constexpr int kMaxAttr = 8;
+constexpr int kMaxVar = 4;        // user varying floats of the built-in programs
+constexpr int kTileW = 16;
+constexpr int kTileH = 16;
+constexpr int kRasterThreads = kTileW * kTileH;
+constexpr int kBatch = 64;        // triangles staged in LDS per raster step
+constexpr int kMaxBinTiles = 16;  // larger footprints go to the big-triangle list
+constexpr int kOrderedCap = 4096; // LDS list capacity of the ordered (blend) path
+constexpr int kClipFloats = 24;   // one clipped sub-triangle: 3 x (xyzw + b0 b1 b2), padded
+
+enum Program : uint32_t {
+    PROG_MVP_COLOR = 1,
+    PROG_MVP_TEXTURE = 2,
+    PROG_COLORED_TRIANGLE = 3,
+    PROG_CLIP_COLOR = 4,
+};
+
+enum Interp : uint32_t { INTERP_LINEAR = 0, INTERP_PERSPECTIVE = 1, INTERP_FLAT = 2 };
+enum Compare : uint32_t {
+    CMP_NEVER = 1, CMP_LESS = 2, CMP_EQUAL = 3, CMP_LESS_EQUAL = 4,
+    CMP_GREATER = 5, CMP_NOT_EQUAL = 6, CMP_GREATER_EQUAL = 7, CMP_ALWAYS = 8,
+};
+enum Addr : uint32_t { ADDR_CLAMP = 0, ADDR_REPEAT = 1, ADDR_MIRROR = 2 };
+
+// How a pixel's sequence of fragments is resolved.  All but RES_ORDERED are
+// order-independent restatements of the sequential depth test + overwrite:
+// the winner is picked by (depth, primitive order) so that fragments of one
+// tile may be processed in any order and still give the sequential result.
+enum Resolve : uint32_t {
+    RES_LAST = 0,        // no depth test: the last fragment in API order wins
+    RES_MIN_FIRST = 1,   // Less + write: smallest depth, first in API order
+    RES_MIN_LAST = 2,    // LessEqual + write: smallest depth, last in API order
+    RES_MAX_FIRST = 3,   // Greater + write
+    RES_MAX_LAST = 4,    // GreaterEqual + write
+    RES_PASS_LAST = 5,   // compare against the depth before the draw; last passing wins
+    RES_ORDERED = 6,     // blending / NotEqual + write: sorted, sequential per pixel
+};
+
+enum ErrorBits : uint32_t {
+    ERR_INDEX = 1u << 0,          // index out of the bound index buffer
+    ERR_VERTEX = 1u << 1,         // vertex fetch out of the bound vertex buffer
+    ERR_W_ZERO = 1u << 2,         // clip position w == 0 (reference: expect("w=0"))
+    ERR_CLIP_OVERFLOW = 1u << 3,  // clipped sub-triangle buffer full
+    ERR_RECORD_OVERFLOW = 1u << 4,
+    ERR_ORDERED_OVERFLOW = 1u << 5,
+    ERR_COORD_RANGE = 1u << 6,    // framebuffer coordinate >= 2^30
+    ERR_PRIM_RANGE = 1u << 7,     // more than 2^25 primitives in a draw
+};
+
+// Pipeline-level attribute description (shader location -> buffer slot).
+struct AttrDev {
+    uint32_t buffer, offset, ncomp, pad;
+};
+
+// Draw-level attribute: the bound buffer slice resolved on the host, so the
+// kernels index nothing dynamically.
+struct AttrBind {
+    const uint8_t *data;  // buffer slice start
+    uint64_t size;        // slice size
+    uint32_t stride, step_instance, offset, ncomp;
+};
+
+struct Scratch {
+    uint32_t *tile_count;    // n_tiles
+    uint32_t *tile_offset;   // n_tiles + 1
+    uint32_t *tile_fill;     // n_tiles
+    uint32_t *bins;          // bins_cap keys
+    uint4 *recs;             // (key, tx0|ty0<<16, tx1|ty1<<16, 0)
+    uint4 *big;
+    uint32_t *clip_list;     // primitives that need the clipper
+    float *clip_tris;        // clip_tri_cap x kClipFloats
+    uint32_t *clip_base;     // per primitive: first sub-triangle slot
+    uint32_t *counters;      // see Counter
+    unsigned long long *frags;  // fragment counter of the pass
+    uint32_t *error;
+    uint32_t *frag_px;       // optional per-pixel fragment count
+    uint32_t bins_cap, rec_cap, big_cap, clip_list_cap, clip_tri_cap;
+};
+
+enum Counter { C_REC = 0, C_BIG = 1, C_CLIP_LIST = 2, C_CLIP_TRI = 3, C_NUM = 4 };
+
+struct DrawParams {
+    // pipeline
+    uint32_t program, nvar, interp, blend, bilinear, srgb;
+    uint32_t front_face_cw, cull_mode;
+    uint32_t depth_test, depth_compare, depth_write, resolve;
+    // vertex input
+    uint32_t n_attr;
+    AttrBind attr[kMaxAttr];
+    const uint8_t *index_data;
+    uint64_t index_size;
+    uint32_t index_format;
+    int32_t base_vertex;
+    uint32_t indexed, first, count, first_instance, instance_count;
+    uint32_t nprim, total_prims;
+    // bindings
+    const float *uniform;
+    const uint8_t *tex;
+    uint32_t tex_w, tex_h, tex_srgb, addr_u, addr_v;
+    const float *srgb_lut;
+    // dynamic state
+    float viewport[6];
+    uint32_t scissor[4];
+    // framebuffer
+    uint32_t width, height, tiles_x, tiles_y;
+    uint8_t *color;
+    uint32_t color_bgra;
+    float *depth;
+    uint32_t init_clear;     // this raster launch applies the LoadOp::Clear values
+    uint32_t clear_color_on, clear_depth_on;
+    uint32_t clear_rgba;     // packed in memory byte order
+    float clear_depth;
+    Scratch s;
+};
+
+// ------------------------------------------------------------------
+// Scalar helpers with Rust semantics
+// ------------------------------------------------------------------
+
+// `f as u32`: NaN -> 0, saturating, truncating.
+__device__ __forceinline__ uint32_t f32_as_u32(float f)
+{
+    if (!(f == f)) return 0u;
+    if (f <= 0.0f) return 0u;
+    if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)f;
+}
+
+// texture.rs:376-379: (v * 255).clamp(0, 255) as u8
+__device__ __forceinline__ uint32_t f32_to_unorm8(float v)
+{
+    float x = v * 255.0f;
+    if (x < 0.0f) x = 0.0f;
+    if (x > 255.0f) x = 255.0f;
+    if (!(x == x)) return 0u;
+    return (uint32_t)x;
+}
+
+// x % m for m in {1, 2}: x/m and m*trunc(x/m) are exact, and so is the
+// difference (Sterbenz), i.e. the same value as the IEEE fmod the reference's
+// `%` computes (up to the sign of an exact zero, which no later step sees).
+__device__ __forceinline__ float rem_pow2(float x, float m)
+{
+    float q = truncf(x / m);
+    return x - m * q;
+}
+
+// f32::rem_euclid for m in {1, 2}
+__device__ __forceinline__ float rem_euclid_pow2(float x, float m)
+{
+    float r = rem_pow2(x, m);
+    if (r < 0.0f) r = r + m;
+    return r;
+}
+
+// binding.rs:150-164
+__device__ __forceinline__ uint32_t texel_coordinate(float x, uint32_t mode, uint32_t size)
+{
+    if (mode == ADDR_CLAMP) {
+        if (x < 0.0f) x = 0.0f;
+        if (x > 1.0f) x = 1.0f;
+    } else if (mode == ADDR_REPEAT) {
+        x = rem_euclid_pow2(x, 1.0f);
+    } else {
+        float r = rem_euclid_pow2(x, 2.0f);
+        x = (r <= 1.0f) ? r : 2.0f - r;
+    }
+    return f32_as_u32(roundf(x * (float)(size - 1u)));
+}
+
+// render_pass/mod.rs:434-452
+__device__ __forceinline__ bool compare(uint32_t fn, float v, float r)
+{
+    switch (fn) {
+    case CMP_NEVER: return false;
+    case CMP_LESS: return v < r;
+    case CMP_EQUAL: return v == r;
+    case CMP_LESS_EQUAL: return v <= r;
+    case CMP_GREATER: return v > r;
+    case CMP_NOT_EQUAL: return v != r;
+    case CMP_GREATER_EQUAL: return v >= r;
+    case CMP_ALWAYS: return true;
+    default: return false;
+    }
+}
+
+// ------------------------------------------------------------------
+// Vertex stage (vertex.rs:128-196 fetch; the built-in programs)
+// ------------------------------------------------------------------
+
+struct VsOut {
+    float clip[4];
+    float var[kMaxVar];
+};
+
+// vertex.rs:191-195: buffer[instance * stride_i + vertex * stride_v + offset..]
+template <uint32_t LOC>
+__device__ __forceinline__ bool fetch_attr(const DrawParams &d, uint32_t instance, uint32_t vertex,
+                                           float *dst, uint32_t want)
+{
+    if (LOC >= d.n_attr) return false;
+    const AttrBind at = d.attr[LOC];
+    if (!at.data || at.ncomp < want) return false;
+    uint64_t start = at.step_instance ? (uint64_t)instance * at.stride : (uint64_t)vertex * at.stride;
+    uint64_t off = start + at.offset;
+    if (off + 4ull * at.ncomp > at.size) return false;
+    const float *p = reinterpret_cast<const float *>(at.data + off);
+#pragma unroll
+    for (uint32_t k = 0; k < (uint32_t)kMaxVar; ++k)
+        if (k < want) dst[k] = p[k];
+    return true;
+}
+
+// naga-cranelift binary.rs:299-321: column sum with separate fmul / fadd
+__device__ __forceinline__ void mat4_mul_vec4(const float *m, const float *v, float *r)
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float s = m[0 * 4 + k] * v[0];
+        s = s + m[1 * 4 + k] * v[1];
+        s = s + m[2 * 4 + k] * v[2];
+        s = s + m[3 * 4 + k] * v[3];
+        r[k] = s;
+    }
+}
+
+template <int PROG>
+__device__ __forceinline__ bool run_vs(const DrawParams &d, uint32_t instance, uint32_t vertex,
+                                       VsOut &o)
+{
+    if constexpr (PROG == PROG_MVP_COLOR || PROG == PROG_MVP_TEXTURE) {
+        float pos[4];
+        if (!fetch_attr<0>(d, instance, vertex, pos, 4)) return false;
+        float m[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) m[k] = d.uniform[k];
+        mat4_mul_vec4(m, pos, o.clip);
+        o.var[0] = o.var[1] = o.var[2] = o.var[3] = 0.0f;
+        if (!fetch_attr<1>(d, instance, vertex, o.var, d.nvar)) return false;
+        return true;
+    } else if constexpr (PROG == PROG_CLIP_COLOR) {
+        if (!fetch_attr<0>(d, instance, vertex, o.clip, 4)) return false;
+        o.var[0] = o.var[1] = o.var[2] = o.var[3] = 0.0f;
+        if (!fetch_attr<1>(d, instance, vertex, o.var, d.nvar)) return false;
+        return true;
+    } else {
+        // colored_triangle.wgsl
+        uint32_t vi = vertex % 3u;
+        int32_t ix = (int32_t)vi - 1;
+        int32_t iy = (int32_t)(vi & 1u) * 2 - 1;
+        o.clip[0] = (float)ix;
+        o.clip[1] = (float)iy;
+        o.clip[2] = 0.0f;
+        o.clip[3] = 1.0f;
+        o.var[0] = (vi == 0u) ? 1.0f : 0.0f;
+        o.var[1] = (vi == 1u) ? 1.0f : 0.0f;
+        o.var[2] = (vi == 2u) ? 1.0f : 0.0f;
+        o.var[3] = 1.0f;
+        return true;
+    }
+}
+
+// index.rs:45-70
+__device__ __forceinline__ bool resolve_index(const DrawParams &d, uint32_t i, uint32_t &vertex)
+{
+    if (!d.indexed) { vertex = i; return true; }
+    uint64_t off = (uint64_t)i * d.index_format;
+    if (off + d.index_format > d.index_size) return false;
+    uint32_t idx;
+    if (d.index_format == 2u) idx = *reinterpret_cast<const uint16_t *>(d.index_data + off);
+    else idx = *reinterpret_cast<const uint32_t *>(d.index_data + off);
+    int64_t v = (int64_t)idx + d.base_vertex;
+    if (v < 0 || v > 0xFFFFFFFFll) return false;
+    vertex = (uint32_t)v;
+    return true;
+}
+
+// Runs the vertex shader for the three vertices of primitive `p` (global
+// across instances), returns false and raises an error bit on failure.
+template <int PROG>
+__device__ __forceinline__ bool shade_primitive(const DrawParams &d, uint32_t p, VsOut v[3])
+{
+    uint32_t inst = d.first_instance + p / d.nprim;
+    uint32_t prim = p % d.nprim;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        uint32_t vertex;
+        if (!resolve_index(d, d.first + prim * 3u + (uint32_t)k, vertex)) {
+            atomicOr(d.s.error, (uint32_t)ERR_INDEX);
+            return false;
+        }
+        if (!run_vs<PROG>(d, inst, vertex, v[k])) {
+            atomicOr(d.s.error, (uint32_t)ERR_VERTEX);
+            return false;
+        }
+    }
+    return true;
+}
+
+// primitive.rs:169-197 + state.rs:542-561.  Returns false when w == 0.
+__device__ __forceinline__ bool face_and_cull(const DrawParams &d, const VsOut v[3],
+                                              bool &front_facing, bool &culled)
+{
+    float nd[3][2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (v[k].clip[3] == 0.0f) return false;
+        nd[k][0] = v[k].clip[0] / v[k].clip[3];
+        nd[k][1] = v[k].clip[1] / v[k].clip[3];
+    }
+    float abx = nd[1][0] - nd[0][0], aby = nd[1][1] - nd[0][1];
+    float acx = nd[2][0] - nd[0][0], acy = nd[2][1] - nd[0][1];
+    float z = abx * acy - aby * acx;
+    bool ccw = z > 0.0f;
+    front_facing = ccw ? (d.front_face_cw == 0u) : (d.front_face_cw == 1u);
+    culled = (d.cull_mode == 1u) ? front_facing : (d.cull_mode == 2u) ? !front_facing : false;
+    return true;
+}
+
+// ------------------------------------------------------------------
+// Clipper (clipper.rs:150-179, 625-802)
+// ------------------------------------------------------------------
+
+// nalgebra Vector4::dot with a WebGPU plane: (p0*q0 + p2*q2) + (p1*q1 + p3*q3)
+__device__ __forceinline__ float clip_distance(int plane, const float *p)
+{
+    float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f, q3 = 1.0f;
+    switch (plane) {
+    case 0: q0 = 1.0f; break;
+    case 1: q0 = -1.0f; break;
+    case 2: q1 = 1.0f; break;
+    case 3: q1 = -1.0f; break;
+    case 4: q2 = 1.0f; q3 = 0.0f; break;
+    default: q2 = -1.0f; break;
+    }
+    float a = p[0] * q0;
+    float b = p[1] * q1;
+    float c = p[2] * q2;
+    float e = p[3] * q3;
+    a = a + c;
+    b = b + e;
+    return a + b;
+}
+
+// True when no vertex is outside any plane: the clipper then returns the
+// triangle unchanged with identity barycentrics.
+__device__ __forceinline__ bool fully_inside(const VsOut v[3])
+{
+    bool inside = true;
+#pragma unroll
+    for (int pl = 0; pl < 6; ++pl)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) inside = inside && !(clip_distance(pl, v[k].clip) < 0.0f);
+    return inside;
+}
+
+struct CTri {
+    float p[3][4];
+    float b[3][3];
+};
+
+__device__ __forceinline__ void clip_lerp(const CTri &t, int i, int j, float tt, float *p, float *b)
+{
+    float s = 1.0f - tt;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) p[k] = t.p[i][k] * s + t.p[j][k] * tt;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) b[k] = t.b[i][k] * s + t.b[j][k] * tt;
+}
+
+__device__ __forceinline__ void set_vert(CTri &o, int slot, const float *p, const float *b)
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o.p[slot][k] = p[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.b[slot][k] = b[k];
+}
+
+// clipper.rs:625-728
+__device__ inline int clip_against_plane(const CTri &t, int plane, CTri out[2])
+{
+    float dd[3];
+    bool o[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        dd[k] = clip_distance(plane, t.p[k]);
+        o[k] = dd[k] < 0.0f;
+    }
+    float p1[4], b1[3], p2[4], b2[3];
+#define WGC_CLIP(i, j, P, B) clip_lerp(t, i, j, dd[i] / (dd[i] - dd[j]), P, B)
+    if (o[0] && o[1] && o[2]) return 0;
+    if (!o[0] && !o[1] && !o[2]) { out[0] = t; return 1; }
+    if (o[0] && o[1] && !o[2]) {  // clip(2,0), clip(1,2) -> [A', B', C]
+        WGC_CLIP(2, 0, p1, b1); WGC_CLIP(1, 2, p2, b2);
+        set_vert(out[0], 0, p1, b1); set_vert(out[0], 1, p2, b2); set_vert(out[0], 2, t.p[2], t.b[2]);
+        return 1;
+    }
+    if (!o[0] && o[1] && o[2]) {  // clip(0,1), clip(2,0) -> [A, B', C']
+        WGC_CLIP(0, 1, p1, b1); WGC_CLIP(2, 0, p2, b2);
+        set_vert(out[0], 0, t.p[0], t.b[0]); set_vert(out[0], 1, p1, b1); set_vert(out[0], 2, p2, b2);
+        return 1;
+    }
+    if (o[0] && !o[1] && o[2]) {  // clip(0,1), clip(1,2) -> [A', B, C']
+        WGC_CLIP(0, 1, p1, b1); WGC_CLIP(1, 2, p2, b2);
+        set_vert(out[0], 0, p1, b1); set_vert(out[0], 1, t.p[1], t.b[1]); set_vert(out[0], 2, p2, b2);
+        return 1;
+    }
+    if (o[0] && !o[1] && !o[2]) {  // clip(0,1), clip(0,2) -> [B', B, C'], [B, C, C']
+        WGC_CLIP(0, 1, p1, b1); WGC_CLIP(0, 2, p2, b2);
+        set_vert(out[0], 0, p1, b1); set_vert(out[0], 1, t.p[1], t.b[1]); set_vert(out[0], 2, p2, b2);
+        set_vert(out[1], 0, t.p[1], t.b[1]); set_vert(out[1], 1, t.p[2], t.b[2]); set_vert(out[1], 2, p2, b2);
+        return 2;
+    }
+    if (!o[0] && o[1] && !o[2]) {  // clip(0,1), clip(1,2) -> [A, A', C'], [A, C', C]
+        WGC_CLIP(0, 1, p1, b1); WGC_CLIP(1, 2, p2, b2);
+        set_vert(out[0], 0, t.p[0], t.b[0]); set_vert(out[0], 1, p1, b1); set_vert(out[0], 2, p2, b2);
+        set_vert(out[1], 0, t.p[0], t.b[0]); set_vert(out[1], 1, p2, b2); set_vert(out[1], 2, t.p[2], t.b[2]);
+        return 2;
+    }
+    // !o[0] && !o[1] && o[2]: clip(1,2), clip(0,2) -> [A, B, B'], [A, B', A']
+    WGC_CLIP(1, 2, p1, b1); WGC_CLIP(0, 2, p2, b2);
+    set_vert(out[0], 0, t.p[0], t.b[0]); set_vert(out[0], 1, t.p[1], t.b[1]); set_vert(out[0], 2, p1, b1);
+    set_vert(out[1], 0, t.p[0], t.b[0]); set_vert(out[1], 1, p1, b1); set_vert(out[1], 2, p2, b2);
+    return 2;
+#undef WGC_CLIP
+}
+
+// ------------------------------------------------------------------
+// Raster setup (raster.rs:129-160, 280-321; scanline.rs:14-56)
+// ------------------------------------------------------------------
+
+struct RasterTri {
+    float vpx[3], vpy[3], z[3];
+    uint32_t fbx[3], fby[3];
+    float total;
+};
+
+__device__ __forceinline__ void to_raster(const float *vp, const float *clip, uint32_t &fbx,
+                                          uint32_t &fby, float &vx, float &vy, float &nz)
+{
+    float tx = vp[0] + 0.5f * vp[2];
+    float ty = vp[1] + 0.5f * vp[3];
+    float sx = 0.5f * vp[2];
+    float sy = 0.5f * vp[3];
+    sy = sy * -1.0f;
+    float nx = clip[0] / clip[3];
+    float ny = clip[1] / clip[3];
+    nz = clip[2] / clip[3];
+    vx = nx * sx + tx;
+    vy = ny * sy + ty;
+    vx = fmaxf(vx, 0.0f);
+    vy = fmaxf(vy, 0.0f);
+    fbx = f32_as_u32(vx);
+    fby = f32_as_u32(vy);
+}
+
+__device__ __forceinline__ float shoelace(float ax, float ay, float bx, float by, float cx, float cy)
+{
+    float t1 = (by - ay) * (bx + ax);
+    float t2 = (cy - by) * (cx + bx);
+    float t3 = (ay - cy) * (ax + cx);
+    return (t1 + t2) + t3;
+}
+
+__device__ __forceinline__ void raster_setup(const DrawParams &d, const float pos[3][4], RasterTri &r)
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) to_raster(d.viewport, pos[k], r.fbx[k], r.fby[k], r.vpx[k], r.vpy[k], r.z[k]);
+    r.total = shoelace(r.vpx[0], r.vpy[0], r.vpx[1], r.vpy[1], r.vpx[2], r.vpy[2]);
+}
+
+// Vertices of the scanline rule after bubblesort3 by y, with the case the
+// iterator starts in (scanline.rs:24-53).
+struct ScanTri {
+    int32_t ax, ay, bx, by, cx, cy;
+    int32_t mode;   // 0: single row (ac.y == 0), 1: start in the second half (ab.y == 0), 2: general
+    int32_t pad;
+};
+
+__device__ __forceinline__ ScanTri scan_setup(const RasterTri &r)
+{
+    int32_t x[3], y[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { x[k] = (int32_t)r.fbx[k]; y[k] = (int32_t)r.fby[k]; }
+    // sort.rs:3-7: compare_and_swap (0,1), (0,2), (1,2), swap only on Greater
+#define WGC_CAS(l, rr) if (y[l] > y[rr]) { int32_t t = y[l]; y[l] = y[rr]; y[rr] = t; t = x[l]; x[l] = x[rr]; x[rr] = t; }
+    WGC_CAS(0, 1) WGC_CAS(0, 2) WGC_CAS(1, 2)
+#undef WGC_CAS
+    ScanTri s;
+    s.ax = x[0]; s.ay = y[0]; s.bx = x[1]; s.by = y[1]; s.cx = x[2]; s.cy = y[2];
+    s.pad = 0;
+    if (s.cy == s.ay) {
+        s.mode = 0;
+        // bubblesort3 by x; only the extremes are used
+        s.ax = min(min(x[0], x[1]), x[2]);
+        s.cx = max(max(x[0], x[1]), x[2]);
+    } else if (s.by == s.ay) {
+        s.mode = 1;
+    } else {
+        s.mode = 2;
+    }
+    return s;
+}
+
+// Rows the scanline iterator emits: [row0, row1] (row1 < row0 when none).
+__device__ __forceinline__ void scan_rows(const ScanTri &s, int32_t &row0, int32_t &row1)
+{
+    row0 = s.ay;
+    if (s.mode == 0) row1 = s.ay;
+    else if (s.mode == 1) row1 = s.cy;
+    else row1 = (s.cy == s.by) ? s.by - 1 : s.cy;   // flat bottom: row c.y is not emitted
+}
+
+__device__ __forceinline__ void scan_cols(const ScanTri &s, int32_t &col0, int32_t &col1)
+{
+    if (s.mode == 0) { col0 = s.ax; col1 = s.cx; }
+    else { col0 = min(min(s.ax, s.bx), s.cx); col1 = max(max(s.ax, s.bx), s.cx); }
+}
+
+// Inclusive span [x1, x2] of row y (scanline.rs:107-166).  Returns false
+// when the iterator emits no row y.  Rust's isize `/` truncates toward zero,
+// as does C++'s; products fit in 32 bits for coordinates below 2^15, and use
+// 64 bits above.
+__device__ __forceinline__ bool scan_span(const ScanTri &s, int32_t y, int32_t &x1, int32_t &x2, bool wide)
+{
+    int32_t a, b;
+    if (s.mode == 0) {
+        if (y != s.ay) return false;
+        x1 = s.ax; x2 = s.cx;
+        return true;
+    }
+    int32_t acx = s.cx - s.ax, acy = s.cy - s.ay;
+    int32_t i = y - s.ay;
+    if (s.mode == 1) {
+        if (y < s.ay || y > s.cy) return false;
+        int32_t bcx = s.cx - s.bx, bcy = s.cy - s.by;
+        if (wide) {
+            a = s.ax + (int32_t)(((int64_t)acx * i) / acy);
+            b = s.bx + (int32_t)(((int64_t)bcx * i) / bcy);
+        } else {
+            a = s.ax + (acx * i) / acy;
+            b = s.bx + (bcx * i) / bcy;
+        }
+    } else {
+        if (y < s.ay) return false;
+        if (y < s.by) {
+            int32_t abx = s.bx - s.ax, aby = s.by - s.ay;
+            if (wide) {
+                a = s.ax + (int32_t)(((int64_t)acx * i) / acy);
+                b = s.ax + (int32_t)(((int64_t)abx * i) / aby);
+            } else {
+                a = s.ax + (acx * i) / acy;
+                b = s.ax + (abx * i) / aby;
+            }
+        } else {
+            int32_t bcy = s.cy - s.by;
+            if (bcy == 0 || y > s.cy) return false;
+            int32_t bcx = s.cx - s.bx;
+            int32_t i2 = y - s.by;
+            if (wide) {
+                a = s.ax + (int32_t)(((int64_t)acx * i) / acy);
+                b = s.bx + (int32_t)(((int64_t)bcx * i2) / bcy);
+            } else {
+                a = s.ax + (acx * i) / acy;
+                b = s.bx + (bcx * i2) / bcy;
+            }
+        }
+    }
+    x1 = min(a, b);
+    x2 = max(a, b);
+    return true;
+}
+
+__device__ __forceinline__ bool scan_wide(const RasterTri &r)
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) m |= r.fbx[k] | r.fby[k];
+    return m >= 32768u;
+}
+
+__device__ __forceinline__ bool coord_ok(const RasterTri &r)
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) m |= r.fbx[k] | r.fby[k];
+    return m < (1u << 30);
+}
+
+// Tile footprint of the pixels the scanline rule emits inside the scissor.
+// Returns false when it is empty.
+__device__ __forceinline__ bool tile_bbox(const DrawParams &d, const ScanTri &s, uint32_t &tx0,
+                                          uint32_t &ty0, uint32_t &tx1, uint32_t &ty1)
+{
+    int32_t r0, r1, c0, c1;
+    scan_rows(s, r0, r1);
+    scan_cols(s, c0, c1);
+    int64_t sx0 = d.scissor[0], sy0 = d.scissor[1];
+    int64_t sx1 = sx0 + d.scissor[2] - 1, sy1 = sy0 + d.scissor[3] - 1;
+    int64_t x0 = max((int64_t)c0, sx0), x1 = min((int64_t)c1, sx1);
+    int64_t y0 = max((int64_t)r0, sy0), y1 = min((int64_t)r1, sy1);
+    if (x0 > x1 || y0 > y1) return false;
+    tx0 = (uint32_t)(x0 / kTileW); tx1 = (uint32_t)(x1 / kTileW);
+    ty0 = (uint32_t)(y0 / kTileH); ty1 = (uint32_t)(y1 / kTileH);
+    return true;
+}
+
+// ------------------------------------------------------------------
+// Fragment stage
+// ------------------------------------------------------------------
+
+__device__ __forceinline__ void fetch_texel(const DrawParams &d, uint32_t u, uint32_t v, float out[4])
+{
+    uint32_t t = *reinterpret_cast<const uint32_t *>(d.tex + ((uint64_t)v * d.tex_w + u) * 4u);
+    if (d.srgb && d.tex_srgb) {
+        out[0] = d.srgb_lut[t & 0xFFu];
+        out[1] = d.srgb_lut[(t >> 8) & 0xFFu];
+        out[2] = d.srgb_lut[(t >> 16) & 0xFFu];
+        out[3] = (float)(t >> 24) / 255.0f;
+    } else {
+        out[0] = (float)(t & 0xFFu) / 255.0f;   // texture.rs:173-175
+        out[1] = (float)((t >> 8) & 0xFFu) / 255.0f;
+        out[2] = (float)((t >> 16) & 0xFFu) / 255.0f;
+        out[3] = (float)(t >> 24) / 255.0f;
+    }
+}
+
+__device__ __forceinline__ int32_t wrap_index(int32_t i, uint32_t mode, uint32_t size)
+{
+    int32_t n = (int32_t)size;
+    if (mode == ADDR_REPEAT) { int32_t r = i % n; return r < 0 ? r + n : r; }
+    if (mode == ADDR_MIRROR) {
+        int32_t p = 2 * n;
+        int32_t r = i % p;
+        if (r < 0) r += p;
+        return r < n ? r : p - 1 - r;
+    }
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+// Extension: bilinear, WebGPU texel-centre convention.
+__device__ __forceinline__ void sample_bilinear(const DrawParams &d, float u, float v, float out[4])
+{
+    float fu = u * (float)d.tex_w - 0.5f;
+    float fv = v * (float)d.tex_h - 0.5f;
+    float u0f = floorf(fu), v0f = floorf(fv);
+    float au = fu - u0f, av = fv - v0f;
+    int32_t iu = (int32_t)fmaxf(fminf(u0f, 1073741824.0f), -1073741824.0f);
+    int32_t iv = (int32_t)fmaxf(fminf(v0f, 1073741824.0f), -1073741824.0f);
+    uint32_t x0 = (uint32_t)wrap_index(iu, d.addr_u, d.tex_w);
+    uint32_t x1 = (uint32_t)wrap_index(iu + 1, d.addr_u, d.tex_w);
+    uint32_t y0 = (uint32_t)wrap_index(iv, d.addr_v, d.tex_h);
+    uint32_t y1 = (uint32_t)wrap_index(iv + 1, d.addr_v, d.tex_h);
+    float t00[4], t10[4], t01[4], t11[4];
+    fetch_texel(d, x0, y0, t00);
+    fetch_texel(d, x1, y0, t10);
+    fetch_texel(d, x0, y1, t01);
+    fetch_texel(d, x1, y1, t11);
+    float su = 1.0f - au, sv = 1.0f - av;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float top = t00[k] * su + t10[k] * au;
+        float bot = t01[k] * su + t11[k] * au;
+        out[k] = top * sv + bot * av;
+    }
+}
+
+template <int PROG>
+__device__ __forceinline__ void run_fs(const DrawParams &d, const float *var, float out[4])
+{
+    if constexpr (PROG == PROG_MVP_TEXTURE) {
+        if (d.bilinear) {
+            sample_bilinear(d, var[0], var[1], out);
+        } else {
+            uint32_t x = texel_coordinate(var[0], d.addr_u, d.tex_w);
+            uint32_t y = texel_coordinate(var[1], d.addr_v, d.tex_h);
+            fetch_texel(d, x, y, out);
+        }
+    } else {
+        out[0] = var[0];
+        out[1] = var[1];
+        out[2] = var[2];
+        out[3] = (d.nvar >= 4u) ? var[3] : 1.0f;
+    }
+}
+
+// TexelWriter::from_color packed in memory byte order.
+__device__ __forceinline__ uint32_t pack_color(const float c[4], uint32_t bgra)
+{
+    uint32_t r = f32_to_unorm8(c[0]), g = f32_to_unorm8(c[1]);
+    uint32_t b = f32_to_unorm8(c[2]), a = f32_to_unorm8(c[3]);
+    return bgra ? (b | (g << 8) | (r << 16) | (a << 24)) : (r | (g << 8) | (b << 16) | (a << 24));
+}
+
+// Extension: src-over (BlendState::ALPHA_BLENDING) against the stored unorm8.
+__device__ __forceinline__ uint32_t blend_over(const float s[4], uint32_t dst, uint32_t bgra)
+{
+    float dc[4];
+    uint32_t c0 = dst & 0xFFu, c1 = (dst >> 8) & 0xFFu, c2 = (dst >> 16) & 0xFFu, c3 = dst >> 24;
+    dc[0] = (float)(bgra ? c2 : c0) / 255.0f;
+    dc[1] = (float)c1 / 255.0f;
+    dc[2] = (float)(bgra ? c0 : c2) / 255.0f;
+    dc[3] = (float)c3 / 255.0f;
+    float ia = 1.0f - s[3];
+    float o[4];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o[k] = s[k] * s[3] + dc[k] * ia;
+    o[3] = s[3] + dc[3] * ia;
+    return pack_color(o, bgra);
+}
+
+}  // namespace wgc
