@@ -125,6 +125,8 @@ def render(spec, frag_counts: bool = False) -> OracleResult:
     fc = np.zeros(w * h, np.uint32) if frag_counts else None
     ext = spec.semantics == SEMANTICS_EXTENDED
     draws = (OrDraw * max(1, len(spec.draws)))()
+    cur_vp = (0.0, 0.0, float(w), float(h), 0.0, 1.0)
+    cur_sc = (0, 0, w, h)
     for i, d in enumerate(spec.draws):
         o = draws[i]
         o.program = d.program
@@ -166,12 +168,16 @@ def render(spec, frag_counts: bool = False) -> OracleResult:
             o.texture = C.addressof(tbuf)
             o.tex_w, o.tex_h, o.tex_srgb = tw, th, 1 if srgb else 0
             o.addr_u, o.addr_v = d.address_u, d.address_v
-        vp = d.viewport if d.viewport is not None else (0.0, 0.0, float(w), float(h), 0.0, 1.0)
+        # viewport and scissor are sticky pass state (state.rs:211-217): a
+        # draw without its own keeps what an earlier draw of the pass set
+        if d.viewport is not None:
+            cur_vp = d.viewport
+        if d.scissor is not None:
+            cur_sc = d.scissor
         for k in range(6):
-            o.viewport[k] = vp[k]
-        sc = d.scissor if d.scissor is not None else (0, 0, w, h)
+            o.viewport[k] = cur_vp[k]
         for k in range(4):
-            o.scissor[k] = sc[k]
+            o.scissor[k] = cur_sc[k]
         o.first, o.count = d.first, d.count
         o.base_vertex = d.base_vertex
         o.first_instance, o.instance_count = d.first_instance, d.instance_count
