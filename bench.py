@@ -112,6 +112,7 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c2", "c2ref", "c1", "c3", "c3blend"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     args = ap.parse_args()
 
@@ -163,6 +164,10 @@ def main():
             extra[name] = {"workload": d2, "Mfragments_per_s": round(f2 * n2 / (e2 * 1e-3) / 1e6, 2),
                            "frame_ms": round(e2 / n2, 4), "fragments": f2}
 
+    batched = None
+    if not args.no_batch:
+        batched = run_batched(devs, args)
+
     line = {
         "metric": "Mfragments/s (render pass throughput; 1080p frame ms beside it)",
         "value": round(value, 2),
@@ -185,8 +190,56 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "extra": extra,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
+
+
+def run_batched(devs, args):
+    """256 independent passes (W, H log-uniform in [64, 1024], 100-50k
+    triangles, half textured) in one submission: shared launches replayed as
+    a hipGraph, raster tile shape autotuned; against the same passes launched
+    one launch set at a time without a graph."""
+    import wgc
+    from wgc import autotune, runner, scenes
+    seed = scenes.SEED_BASE + 4
+    specs = [scenes.batched_pass(seed, i, *a) for i, a in enumerate(scenes.batched_sizes(seed, 256))]
+    if wgc.SEMANTICS_REFERENCE not in devs:
+        devs[wgc.SEMANTICS_REFERENCE] = wgc.Device(0, wgc.SEMANTICS_REFERENCE)
+    dev = devs[wgc.SEMANTICS_REFERENCE]
+    pp = runner.PreparedPasses(dev, specs)
+    pp.submit()
+    dev.poll()
+    frags = pp.total_fragments()
+    tune = autotune.tune(pp, reps=3)
+    best = dev.get_option(wgc.OPTION_RASTER_TILE)
+    reps = max(3, args.steps // 20)
+    ms_batched = autotune.time_submissions(pp, reps)
+    prof = kernel_profile(dev, pp, reps=2)
+    dev.set_option(wgc.OPTION_BATCHING, 0)
+    dev.set_option(wgc.OPTION_GRAPHS, 0)
+    ms_plain = autotune.time_submissions(pp, max(2, reps // 2), warmup=1)
+    dev.set_option(wgc.OPTION_BATCHING, 1)
+    dev.set_option(wgc.OPTION_GRAPHS, 1)
+    alg = sum(scenes.algorithmic_bytes(s, count_texture=False) for s in specs) + 4 * 1024 * 1024
+    raster = prof.get("k_raster")
+    return {
+        "workload": "256 passes, W,H log-uniform [64,1024], spheres of 100-50k triangles, half textured "
+                    "(one shared 1024^2 texture), Rgba8Unorm + Depth32Float Less",
+        "fragments_per_submission": frags,
+        "ms_per_submission": round(ms_batched, 4),
+        "Mfragments_per_s": round(frags / (ms_batched * 1e-3) / 1e6, 2),
+        "plain_ms_per_submission": round(ms_plain, 4),
+        "plain_Mfragments_per_s": round(frags / (ms_plain * 1e-3) / 1e6, 2),
+        "speedup_vs_plain": round(ms_plain / ms_batched, 2),
+        "algorithmic_GBps": round(alg / (ms_batched * 1e-3) / 1e9, 1),
+        "hbm_frac_of_submission": round(alg / (ms_batched * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "raster_kernel_hbm_frac": round(alg / (raster["avg_us"] * 1e-6 * raster["launches"] / 2) / 1e9 / HBM_PEAK_GBS, 4)
+        if raster else None,
+        "autotune_ms": {wgc.RASTER_TILES[k].__repr__(): round(v, 4) for k, v in tune.items()},
+        "raster_tile": wgc.RASTER_TILES[best],
+        "kernels_us": {k: round(v["avg_us"], 2) for k, v in prof.items()},
+    }
 
 
 def cpu_model() -> str:

@@ -275,8 +275,25 @@ int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcCommandBuffe
 /* Waits until `submission_index` has executed (UINT64_MAX: everything).
  * timeout_ns 0 = no timeout. Returns WGC_ERR_EXECUTION if a pass failed. */
 int32_t wgc_device_poll(WgcDevice device, uint64_t submission_index, uint64_t timeout_ns);
-/* Counters of the last executed render pass (waits for the queue). */
+/* Counters of the last submission (waits for the queue): fragments of all its
+ * passes, primitives assembled, kernel launches. */
 int32_t wgc_device_last_pass_stats(WgcDevice device, WgcPassStats *out);
+
+/* ---- execution options (no reference counterpart: the reference runs one
+ *      pass at a time on one thread, engine.rs:26-46) ---- */
+typedef enum WgcOption {
+    /* raster tile shape: 0 = 16x16 px / 256 lanes, 1 = 8x8 / 64, 2 = 16x16 / 64,
+     * 3 = 32x32 / 256 (the autotuner's search space) */
+    WGC_OPTION_RASTER_TILE = 1,
+    /* 1 (default): consecutive independent passes with one draw each share
+     * their launches (the batched form); 0: one launch set per draw */
+    WGC_OPTION_BATCHING = 2,
+    /* 1 (default): a command buffer's launches are captured once into a
+     * hipGraph and replayed on every later submission */
+    WGC_OPTION_GRAPHS = 3,
+} WgcOption;
+int32_t wgc_device_set_option(WgcDevice device, uint32_t option, uint32_t value);
+int32_t wgc_device_get_option(WgcDevice device, uint32_t option, uint32_t *value);
 
 /* Debug aid with no reference counterpart: with host_out == NULL, (re)allocates
  * a zeroed width*height u32 buffer to which every later pass adds, per pixel,

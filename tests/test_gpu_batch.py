@@ -1,0 +1,122 @@
+"""The batched form: many independent passes in one submission share their
+launches and are replayed as a hipGraph.  Every pass must equal the
+oracle's render of it alone, for every raster tile shape."""
+import sys
+
+import numpy as np
+import pytest
+
+import wgc
+from oracle import bridge
+from wgc import runner, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(name):
+    print(f"\n[gpu case] {name}", flush=True)
+    sys.stdout.flush()
+
+
+def small_batch(n=20, seed=0x5EED_0004):
+    specs = []
+    for i, (w, h, tris, tex) in enumerate(scenes.batched_sizes(seed, n)):
+        # keep the oracle quick: targets up to 256 px, up to 5k triangles
+        w, h, tris = max(16, w // 4), max(16, h // 4), min(tris, 5000.0)
+        specs.append(scenes.batched_pass(seed, i, w, h, tris, tex))
+    return specs
+
+
+def check_against_oracle(specs, results, total):
+    want_total = 0
+    for s, r in zip(specs, results):
+        ref = bridge.render(s)
+        want_total += ref.fragments
+        assert np.array_equal(r.color, ref.color), s.name
+        assert np.array_equal(r.depth.view(np.uint32), ref.depth.view(np.uint32)), s.name
+    assert total == want_total
+
+
+@pytest.mark.parametrize("cfg", sorted(wgc.RASTER_TILES))
+def test_batch_matches_oracle_per_pass(gpu_device, cfg):
+    announce(f"batch raster cfg {cfg}")
+    gpu_device.set_option(wgc.OPTION_RASTER_TILE, cfg)
+    try:
+        specs = small_batch()
+        res, total = runner.render_batch(specs, gpu_device)
+        check_against_oracle(specs, res, total)
+    finally:
+        gpu_device.set_option(wgc.OPTION_RASTER_TILE, 0)
+
+
+def test_graph_replay_and_batching_switch(gpu_device):
+    announce("graph replay")
+    specs = small_batch(12, seed=99)
+    pp = runner.PreparedPasses(gpu_device, specs)
+    pp.submit()
+    first = pp.results()
+    for _ in range(3):
+        pp.submit()
+    again = pp.results()
+    for a, b in zip(first, again):
+        assert np.array_equal(a.color, b.color) and np.array_equal(a.depth, b.depth)
+    # the same passes one launch set at a time, no graph
+    gpu_device.set_option(wgc.OPTION_BATCHING, 0)
+    gpu_device.set_option(wgc.OPTION_GRAPHS, 0)
+    try:
+        pp.submit()
+        plain = pp.results()
+    finally:
+        gpu_device.set_option(wgc.OPTION_BATCHING, 1)
+        gpu_device.set_option(wgc.OPTION_GRAPHS, 1)
+    for a, b in zip(first, plain):
+        assert np.array_equal(a.color, b.color) and np.array_equal(a.depth, b.depth)
+
+
+def test_dependent_passes_stay_ordered(gpu_device):
+    """Two passes on the same target: the second loads what the first drew,
+    so they must not share a launch group."""
+    announce("dependent passes")
+    a = scenes.config1_plain_small()
+    b = scenes.config1_plain_small(seed=scenes.SEED_BASE + 11)
+    b.clear_color = None
+    b.clear_depth = None
+    b.draws[0].depth = (wgc.LESS_EQUAL, True)
+    pp = runner.PreparedPasses(gpu_device, [a])
+    # record both passes against one pair of attachments
+    enc = gpu_device.create_command_encoder()
+    cache = {}
+    keep = []
+    runner.record_pass(gpu_device, enc, a, pp.res[0], keep, cache)
+    runner.record_pass(gpu_device, enc, b, pp.res[0], keep, cache)
+    cb = enc.finish()
+    gpu_device.submit([cb])
+    gpu_device.poll()
+    color, depth = pp.res[0].read()
+    ref_a = bridge.render(a)
+    b2 = scenes.config1_plain_small(seed=scenes.SEED_BASE + 11)
+    b2.clear_color = None
+    b2.clear_depth = None
+    b2.draws[0].depth = (wgc.LESS_EQUAL, True)
+    b2.color_init = ref_a.color.tobytes()
+    b2.depth_init = ref_a.depth.tobytes()
+    ref = bridge.render(b2)
+    assert np.array_equal(color, ref.color)
+    assert np.array_equal(depth.view(np.uint32), ref.depth.view(np.uint32))
+
+
+def test_autotuner_picks_a_shape(gpu_device):
+    announce("autotune")
+    from wgc import autotune
+    specs = small_batch(8, seed=7)
+    pp = runner.PreparedPasses(gpu_device, specs)
+    timings = autotune.tune(pp, reps=3)
+    try:
+        assert set(timings) == set(wgc.RASTER_TILES)
+        best = gpu_device.get_option(wgc.OPTION_RASTER_TILE)
+        assert timings[best] == min(timings.values())
+        pp.submit()
+        res = pp.results()
+        check_against_oracle(specs, res, pp.total_fragments())
+    finally:
+        gpu_device.set_option(wgc.OPTION_RASTER_TILE, 0)

@@ -1,6 +1,9 @@
 // kernels.hip -- HIP kernels of the render-pass executor for gfx950.
 //
-// Per draw: k_zero -> k_setup -> k_clip -> k_tile_scan -> k_bin_fill -> k_raster.
+// Per launch group: k_zero -> k_setup -> k_clip -> k_tile_scan -> k_bin_fill
+// -> k_raster.  A group is either one draw (parameters passed by value) or
+// a table of independent draws -- the batched form, where hundreds of render
+// passes share every launch (MULTI).
 //
 //  k_setup     one thread per primitive: vertex shader for its three indices
 //              (state.rs:519-535), winding and culling (state.rs:540-561),
@@ -10,9 +13,9 @@
 //  k_clip      primitives that cross a clip plane: the 6-plane clipper as an
 //              in-order depth-first walk (same output order as the
 //              reference's plane-by-plane queue, clipper.rs:756-775).
-//  k_tile_scan exclusive scan of per-tile counts.
+//  k_tile_scan exclusive scan of per-tile counts (all draws of the group).
 //  k_bin_fill  scatters record keys into per-tile lists.
-//  k_raster    one workgroup per 16x16 tile, one pixel per lane; colour and
+//  k_raster    one workgroup per TSxTS tile, PPT pixels per lane; colour and
 //              depth live in registers from load (or clear) to one coalesced
 //              store.  Triangles are staged 64 at a time in LDS, with per-row
 //              spans of the reference's integer scanline rule.
@@ -29,7 +32,42 @@ namespace wgc {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Wave-aggregated append: one atomic per wave, slots in lane order.
+// Compile-time loop: f(IC<I>{}) for I in [B, E), so per-pixel arrays are only
+// ever indexed by constants and live in registers.
+template <int I>
+struct IC {
+    static constexpr int value = I;
+};
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(IC<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+template <bool MULTI>
+__device__ __forceinline__ const DrawParams &draw_at(const DrawParams &d0, const BatchArgs &b, uint32_t i)
+{
+    if constexpr (MULTI) return b.draws[i];
+    else return d0;
+}
+
+// Largest i < n with base[i] <= g (empty ranges resolve to the last of them).
+__device__ __forceinline__ uint32_t find_slot(const uint32_t *base, uint32_t n, uint32_t g)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Wave-aggregated append: one atomic per wave, slots in lane order.  Every
+// lane of the wave must pass the same counter.
 __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
 {
     uint64_t mask = __ballot(pred);
@@ -60,24 +98,34 @@ __device__ __forceinline__ void add_tile_counts(const DrawParams &d, uint32_t tx
 }
 
 // ------------------------------------------------------------------
-// k_zero
+// k_zero: tile counts of the group, per-draw counters, record counter
 // ------------------------------------------------------------------
 
-__global__ void k_zero(uint32_t *tile_count, uint32_t n_tiles, uint32_t *counters)
+__global__ void k_zero(uint32_t *tile_count, uint32_t n_tiles, uint32_t *counters, uint32_t n_counters,
+                       uint32_t *rec_count)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_tiles) tile_count[i] = 0u;
-    if (i < C_NUM) counters[i] = 0u;
+    if (i < n_counters) counters[i] = 0u;
+    if (i == 0) *rec_count = 0u;
 }
 
 // ------------------------------------------------------------------
 // k_setup
 // ------------------------------------------------------------------
 
-template <int PROG>
-__global__ __launch_bounds__(256) void k_setup(DrawParams d)
+template <int PROG, bool MULTI>
+__global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t di = 0, p = g;
+    if constexpr (MULTI) {
+        // draw ranges are padded to whole waves, so a wave never spans draws
+        if (g >= b.total_prims) return;
+        di = find_slot(b.prim_base, b.n, g);
+        p = g - b.prim_base[di];
+    }
+    const DrawParams &d = draw_at<MULTI>(d0, b, di);
     bool active = p < d.total_prims;
     bool has_rec = false, needs_clip = false;
     uint4 rec = make_uint4(0, 0, 0, 0);
@@ -104,7 +152,7 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d)
                         uint32_t tx0, ty0, tx1, ty1;
                         if (tile_bbox(d, s, tx0, ty0, tx1, ty1)) {
                             has_rec = true;
-                            rec = make_uint4(p << 7, tx0 | (ty0 << 16), tx1 | (ty1 << 16), 0u);
+                            rec = make_uint4(p << 7, tx0 | (ty0 << 16), tx1 | (ty1 << 16), di);
                             ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
                         }
                     }
@@ -121,7 +169,7 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d)
         if (slot < d.s.clip_list_cap) d.s.clip_list[slot] = p;
         else atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
     }
-    slot = wave_append(&d.s.counters[C_REC], small);
+    slot = wave_append(d.s.rec_count, small);
     if (small) {
         if (slot < d.s.rec_cap) {
             d.s.recs[slot] = rec;
@@ -142,7 +190,8 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d)
 // ------------------------------------------------------------------
 
 // Depth-first walk over the plane-by-plane splits; leaves come out in the
-// order of the reference's queue (lexicographic in the split choices).
+// order of the reference's queue (lexicographic in the split choices).  At
+// most one pending sibling per level: 7 stack entries.
 template <class F>
 __device__ int clip_walk(const CTri &root, F &&emit)
 {
@@ -174,59 +223,71 @@ __device__ int clip_walk(const CTri &root, F &&emit)
 }
 
 template <int PROG>
-__global__ __launch_bounds__(256) void k_clip(DrawParams d)
+__device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p)
 {
-    uint32_t n = min(d.s.counters[C_CLIP_LIST], d.s.clip_list_cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t p = d.s.clip_list[i];
-        VsOut v[3];
-        if (!shade_primitive<PROG>(d, p, v)) continue;
-        CTri root;
+    VsOut v[3];
+    if (!shade_primitive<PROG>(d, p, v)) return;
+    CTri root;
 #pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) root.p[k][c] = v[k].clip[c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) root.b[k][c] = (c == k) ? 1.0f : 0.0f;
+    }
+    int nleaf = clip_walk(root, [](const CTri &, int) {});
+    if (nleaf == 0) return;
+    uint32_t base = atomicAdd(&d.s.counters[C_CLIP_TRI], (uint32_t)nleaf);
+    if (base + (uint32_t)nleaf > d.s.clip_tri_cap) {
+        atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
+        return;
+    }
+    d.s.clip_base[p] = base;
+    clip_walk(root, [&](const CTri &t, int leaf) {
+        float *o = d.s.clip_tris + (size_t)(base + (uint32_t)leaf) * kClipFloats;
         for (int k = 0; k < 3; ++k) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) root.p[k][c] = v[k].clip[c];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) root.b[k][c] = (c == k) ? 1.0f : 0.0f;
+            for (int c = 0; c < 4; ++c) o[k * 7 + c] = t.p[k][c];
+            for (int c = 0; c < 3; ++c) o[k * 7 + 4 + c] = t.b[k][c];
         }
-        int nleaf = clip_walk(root, [](const CTri &, int) {});
-        if (nleaf == 0) continue;
-        uint32_t base = atomicAdd(&d.s.counters[C_CLIP_TRI], (uint32_t)nleaf);
-        if (base + (uint32_t)nleaf > d.s.clip_tri_cap) {
-            atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
-            continue;
-        }
-        d.s.clip_base[p] = base;
-        clip_walk(root, [&](const CTri &t, int leaf) {
-            float *o = d.s.clip_tris + (size_t)(base + (uint32_t)leaf) * kClipFloats;
-            for (int k = 0; k < 3; ++k) {
-                for (int c = 0; c < 4; ++c) o[k * 7 + c] = t.p[k][c];
-                for (int c = 0; c < 3; ++c) o[k * 7 + 4 + c] = t.b[k][c];
-            }
-            RasterTri r;
-            raster_setup(d, t.p, r);
-            if (!coord_ok(r)) { atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE); return; }
-            if (r.total == 0.0f) return;
-            ScanTri s = scan_setup(r);
-            uint32_t tx0, ty0, tx1, ty1;
-            if (!tile_bbox(d, s, tx0, ty0, tx1, ty1)) return;
-            uint4 rec = make_uint4((p << 7) | ((uint32_t)leaf << 1) | 1u, tx0 | (ty0 << 16),
-                                   tx1 | (ty1 << 16), 0u);
-            uint32_t nt = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-            if (nt <= (uint32_t)kMaxBinTiles) {
-                uint32_t slot = atomicAdd(&d.s.counters[C_REC], 1u);
-                if (slot < d.s.rec_cap) {
-                    d.s.recs[slot] = rec;
-                    add_tile_counts(d, tx0, ty0, tx1, ty1);
-                } else {
-                    atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
-                }
+        RasterTri r;
+        raster_setup(d, t.p, r);
+        if (!coord_ok(r)) { atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE); return; }
+        if (r.total == 0.0f) return;
+        ScanTri s = scan_setup(r);
+        uint32_t tx0, ty0, tx1, ty1;
+        if (!tile_bbox(d, s, tx0, ty0, tx1, ty1)) return;
+        uint4 rec = make_uint4((p << 7) | ((uint32_t)leaf << 1) | 1u, tx0 | (ty0 << 16),
+                               tx1 | (ty1 << 16), di);
+        uint32_t nt = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+        if (nt <= (uint32_t)kMaxBinTiles) {
+            uint32_t slot = atomicAdd(d.s.rec_count, 1u);
+            if (slot < d.s.rec_cap) {
+                d.s.recs[slot] = rec;
+                add_tile_counts(d, tx0, ty0, tx1, ty1);
             } else {
-                uint32_t slot = atomicAdd(&d.s.counters[C_BIG], 1u);
-                if (slot < d.s.big_cap) d.s.big[slot] = rec;
-                else atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
+                atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
             }
-        });
+        } else {
+            uint32_t slot = atomicAdd(&d.s.counters[C_BIG], 1u);
+            if (slot < d.s.big_cap) d.s.big[slot] = rec;
+            else atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
+        }
+    });
+}
+
+template <int PROG, bool MULTI>
+__global__ __launch_bounds__(256) void k_clip(DrawParams d0, BatchArgs b)
+{
+    uint32_t total = MULTI ? b.total_prims : d0.total_prims;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+        uint32_t di = 0, i = g;
+        if constexpr (MULTI) {
+            di = find_slot(b.prim_base, b.n, g);
+            i = g - b.prim_base[di];
+        }
+        const DrawParams &d = draw_at<MULTI>(d0, b, di);
+        uint32_t n = min(d.s.counters[C_CLIP_LIST], d.s.clip_list_cap);
+        if (i < n) clip_primitive<PROG>(d, di, d.s.clip_list[i]);
     }
 }
 
@@ -264,17 +325,20 @@ __global__ __launch_bounds__(1024) void k_tile_scan(uint32_t *tile_count, uint32
 // k_bin_fill
 // ------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_bin_fill(DrawParams d)
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_bin_fill(DrawParams d0, BatchArgs b, const uint4 *recs,
+                                                  const uint32_t *rec_count, uint32_t rec_cap, uint32_t *bins)
 {
-    uint32_t n = min(d.s.counters[C_REC], d.s.rec_cap);
+    uint32_t n = min(*rec_count, rec_cap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint4 r = d.s.recs[i];
+        uint4 r = recs[i];
+        const DrawParams &d = draw_at<MULTI>(d0, b, r.w);
         uint32_t tx0 = r.y & 0xFFFFu, ty0 = r.y >> 16, tx1 = r.z & 0xFFFFu, ty1 = r.z >> 16;
         for (uint32_t ty = ty0; ty <= ty1; ++ty)
             for (uint32_t tx = tx0; tx <= tx1; ++tx) {
                 uint32_t t = ty * d.tiles_x + tx;
                 uint32_t slot = atomicAdd(&d.s.tile_fill[t], 1u);
-                d.s.bins[d.s.tile_offset[t] + slot] = r.x;
+                bins[d.s.tile_offset[t] + slot] = r.x;
             }
     }
 }
@@ -294,11 +358,12 @@ struct TriLds {
     uint32_t valid;
 };
 
+template <int TS>
 struct RasterShared {
     TriLds tri[kBatch];
     ScanTri scan[kBatch];
     uint32_t wide[kBatch];
-    uint32_t span[kBatch][kTileH];
+    uint32_t span[kBatch][TS];
     uint32_t keys[kBatch];
     uint32_t nkeys;
 };
@@ -350,6 +415,8 @@ __device__ void stage_entry(const DrawParams &d, uint32_t key, TriLds &T, ScanTr
         for (int c = 0; c < 3; ++c) T.cb[k * 3 + c] = cb[k][c];
     }
     T.total = r.total;
+    // constant terms of the three shoelace sums (the same products the
+    // per-pixel evaluation would form)
     T.k0 = (r.vpy[2] - r.vpy[1]) * (r.vpx[2] + r.vpx[1]);
     T.k1 = (r.vpy[0] - r.vpy[2]) * (r.vpx[0] + r.vpx[2]);
     T.k2 = (r.vpy[1] - r.vpy[0]) * (r.vpx[1] + r.vpx[0]);
@@ -364,202 +431,206 @@ struct PixelState {
     float init_dep;
     uint32_t key;
     bool has;
-    uint32_t nfrag;
 };
 
-// Stages sh.keys[0..n) into LDS and applies them to this lane's pixel.
+// One fragment of triangle T at pixel (pxf, pyf): barycentric at the pixel
+// corner (raster.rs:313-346), the depth test or its order-independent
+// restatement, varyings (fragment.rs:319-377), the fragment shader and the
+// colour write (texture.rs:213-218).
 template <int PROG, bool ORDERED>
-__device__ void raster_batch(const DrawParams &d, RasterShared &sh, uint32_t n, uint32_t x0,
-                             uint32_t y0, uint32_t lx, uint32_t ly, PixelState &ps)
+__device__ __forceinline__ void shade_fragment(const DrawParams &d, const TriLds &T, float pxf, float pyf,
+                                               PixelState &ps)
 {
+    float l0 = (((T.vpy[1] - pyf) * (T.vpx[1] + pxf) + T.k0) + (pyf - T.vpy[2]) * (pxf + T.vpx[2])) / T.total;
+    float l1 = (((pyf - T.vpy[0]) * (pxf + T.vpx[0]) + (T.vpy[2] - pyf) * (T.vpx[2] + pxf)) + T.k1) / T.total;
+    float l2 = ((T.k2 + (pyf - T.vpy[1]) * (pxf + T.vpx[1])) + (T.vpy[0] - pyf) * (T.vpx[0] + pxf)) / T.total;
+    float lc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float acc = T.cb[0 * 3 + c] * l0;
+        acc = acc + T.cb[1 * 3 + c] * l1;
+        acc = acc + T.cb[2 * 3 + c] * l2;
+        lc[c] = acc;
+    }
+    float z = T.z[0] * lc[0];
+    z = z + T.z[1] * lc[1];
+    z = z + T.z[2] * lc[2];
+
+    bool win;
+    if (ORDERED) {
+        win = d.depth_test ? compare(d.depth_compare, z, ps.dep) : true;
+    } else {
+        switch (d.resolve) {
+        case RES_LAST: win = !ps.has || T.key > ps.key; break;
+        case RES_MIN_FIRST:
+            win = (z < ps.init_dep) && (!ps.has || z < ps.dep || (z == ps.dep && T.key < ps.key));
+            break;
+        case RES_MIN_LAST:
+            win = (z <= ps.init_dep) && (!ps.has || z < ps.dep || (z == ps.dep && T.key > ps.key));
+            break;
+        case RES_MAX_FIRST:
+            win = (z > ps.init_dep) && (!ps.has || z > ps.dep || (z == ps.dep && T.key < ps.key));
+            break;
+        case RES_MAX_LAST:
+            win = (z >= ps.init_dep) && (!ps.has || z > ps.dep || (z == ps.dep && T.key > ps.key));
+            break;
+        default:  // RES_PASS_LAST
+            win = compare(d.depth_compare, z, ps.init_dep) && (!ps.has || T.key > ps.key);
+            break;
+        }
+    }
+    if (!win) return;
+    float var[kMaxVar];
+    if (d.interp == INTERP_FLAT) {
+#pragma unroll
+        for (int c = 0; c < kMaxVar; ++c) var[c] = T.var[0][c];
+    } else if (d.interp == INTERP_PERSPECTIVE) {
+        float q0 = lc[0] * T.iw[0];
+        float q1 = lc[1] * T.iw[1];
+        float q2 = lc[2] * T.iw[2];
+        float qs = (q0 + q1) + q2;
+#pragma unroll
+        for (int c = 0; c < kMaxVar; ++c) {
+            float acc = T.var[0][c] * q0;
+            acc = acc + T.var[1][c] * q1;
+            acc = acc + T.var[2][c] * q2;
+            var[c] = acc / qs;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < kMaxVar; ++c) {
+            float acc = T.var[0][c] * lc[0];
+            acc = acc + T.var[1][c] * lc[1];
+            acc = acc + T.var[2][c] * lc[2];
+            var[c] = acc;
+        }
+    }
+    float color[4];
+    run_fs<PROG>(d, var, color);
+    if (ORDERED) {
+        if (d.depth_test && d.depth_write) ps.dep = z;
+        ps.col = d.blend ? blend_over(color, ps.col, d.color_bgra) : pack_color(color, d.color_bgra);
+        ps.has = true;
+    } else {
+        ps.col = pack_color(color, d.color_bgra);
+        ps.dep = z;
+        ps.key = T.key;
+        ps.has = true;
+    }
+}
+
+// Stages sh.keys[0..n) into LDS and applies them to this lane's pixels.
+template <int PROG, bool ORDERED, int TS, int NT>
+__device__ __forceinline__ void raster_batch(const DrawParams &d, RasterShared<TS> &sh, uint32_t n, uint32_t x0, uint32_t y0,
+                             PixelState (&ps)[TS * TS / NT], uint32_t (&nfrag)[TS * TS / NT])
+{
+    constexpr int PPT = TS * TS / NT;
     uint32_t tid = threadIdx.x;
     if (tid < n) stage_entry<PROG>(d, sh.keys[tid], sh.tri[tid], sh.scan[tid], sh.wide[tid]);
     __syncthreads();
     // per-row spans inside the scissor and the tile, tile-relative
     int64_t sx0 = d.scissor[0], sy0 = d.scissor[1];
     int64_t sx1 = sx0 + d.scissor[2], sy1 = sy0 + d.scissor[3];
-    for (uint32_t idx = tid; idx < n * (uint32_t)kTileH; idx += blockDim.x) {
-        uint32_t e = idx / kTileH, row = idx % kTileH;
+    for (uint32_t idx = tid; idx < n * (uint32_t)TS; idx += NT) {
+        uint32_t e = idx / TS, row = idx % TS;
         int32_t y = (int32_t)(y0 + row);
         uint32_t span = 1u;  // empty: lo = 1, hi = 0
         int32_t x1, x2;
         if (sh.tri[e].valid && (int64_t)y >= sy0 && (int64_t)y < sy1 &&
             scan_span(sh.scan[e], y, x1, x2, sh.wide[e] != 0u)) {
             int64_t lo = max((int64_t)x1, max(sx0, (int64_t)x0));
-            int64_t hi = min((int64_t)x2, min(sx1 - 1, (int64_t)x0 + kTileW - 1));
+            int64_t hi = min((int64_t)x2, min(sx1 - 1, (int64_t)x0 + TS - 1));
             if (lo <= hi) span = (uint32_t)(lo - x0) | ((uint32_t)(hi - x0) << 16);
         }
         sh.span[e][row] = span;
     }
     __syncthreads();
-    float pxf = (float)(x0 + lx), pyf = (float)(y0 + ly);
-    for (uint32_t e = 0; e < n; ++e) {
-        uint32_t sp = sh.span[e][ly];
-        if (lx < (sp & 0xFFFFu) || lx > (sp >> 16)) continue;
-        const TriLds &T = sh.tri[e];
-        ps.nfrag++;
-        // barycentric at the pixel corner (raster.rs:313-335)
-        float l0 = (((T.vpy[1] - pyf) * (T.vpx[1] + pxf) + T.k0) + (pyf - T.vpy[2]) * (pxf + T.vpx[2])) / T.total;
-        float l1 = (((pyf - T.vpy[0]) * (pxf + T.vpx[0]) + (T.vpy[2] - pyf) * (T.vpx[2] + pxf)) + T.k1) / T.total;
-        float l2 = ((T.k2 + (pyf - T.vpy[1]) * (pxf + T.vpx[1])) + (T.vpy[0] - pyf) * (T.vpx[0] + pxf)) / T.total;
-        float lc[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            float acc = T.cb[0 * 3 + c] * l0;
-            acc = acc + T.cb[1 * 3 + c] * l1;
-            acc = acc + T.cb[2 * 3 + c] * l2;
-            lc[c] = acc;
+    // pixels outer (unrolled, so the per-pixel state stays in registers),
+    // triangles inner in list order
+    static_for<0, PPT>([&](auto kc) __attribute__((always_inline)) {
+        constexpr int k = decltype(kc)::value;
+        uint32_t q = tid + (uint32_t)k * NT;
+        uint32_t lx = q % TS, ly = q / TS;
+        float pxf = (float)(x0 + lx), pyf = (float)(y0 + ly);
+        for (uint32_t e = 0; e < n; ++e) {
+            uint32_t sp = sh.span[e][ly];
+            if (lx < (sp & 0xFFFFu) || lx > (sp >> 16)) continue;
+            nfrag[k]++;
+            shade_fragment<PROG, ORDERED>(d, sh.tri[e], pxf, pyf, ps[k]);
         }
-        float z = T.z[0] * lc[0];
-        z = z + T.z[1] * lc[1];
-        z = z + T.z[2] * lc[2];
-
-        bool win;
-        if (ORDERED) {
-            win = d.depth_test ? compare(d.depth_compare, z, ps.dep) : true;
-        } else {
-            switch (d.resolve) {
-            case RES_LAST: win = !ps.has || T.key > ps.key; break;
-            case RES_MIN_FIRST:
-                win = (z < ps.init_dep) && (!ps.has || z < ps.dep || (z == ps.dep && T.key < ps.key));
-                break;
-            case RES_MIN_LAST:
-                win = (z <= ps.init_dep) && (!ps.has || z < ps.dep || (z == ps.dep && T.key > ps.key));
-                break;
-            case RES_MAX_FIRST:
-                win = (z > ps.init_dep) && (!ps.has || z > ps.dep || (z == ps.dep && T.key < ps.key));
-                break;
-            case RES_MAX_LAST:
-                win = (z >= ps.init_dep) && (!ps.has || z > ps.dep || (z == ps.dep && T.key > ps.key));
-                break;
-            default:  // RES_PASS_LAST
-                win = compare(d.depth_compare, z, ps.init_dep) && (!ps.has || T.key > ps.key);
-                break;
-            }
-        }
-        if (!win) continue;
-        // varyings of the unclipped vertices (fragment.rs:319-377)
-        float var[kMaxVar];
-        if (d.interp == INTERP_FLAT) {
-#pragma unroll
-            for (int c = 0; c < kMaxVar; ++c) var[c] = T.var[0][c];
-        } else if (d.interp == INTERP_PERSPECTIVE) {
-            float q0 = lc[0] * T.iw[0];
-            float q1 = lc[1] * T.iw[1];
-            float q2 = lc[2] * T.iw[2];
-            float qs = (q0 + q1) + q2;
-#pragma unroll
-            for (int c = 0; c < kMaxVar; ++c) {
-                float acc = T.var[0][c] * q0;
-                acc = acc + T.var[1][c] * q1;
-                acc = acc + T.var[2][c] * q2;
-                var[c] = acc / qs;
-            }
-        } else {
-#pragma unroll
-            for (int c = 0; c < kMaxVar; ++c) {
-                float acc = T.var[0][c] * lc[0];
-                acc = acc + T.var[1][c] * lc[1];
-                acc = acc + T.var[2][c] * lc[2];
-                var[c] = acc;
-            }
-        }
-        float color[4];
-        run_fs<PROG>(d, var, color);
-        if (ORDERED) {
-            if (d.depth_test && d.depth_write) ps.dep = z;
-            ps.col = d.blend ? blend_over(color, ps.col, d.color_bgra) : pack_color(color, d.color_bgra);
-            ps.has = true;
-        } else {
-            ps.col = pack_color(color, d.color_bgra);
-            ps.dep = z;
-            ps.key = T.key;
-            ps.has = true;
-        }
-    }
+    });
     __syncthreads();
 }
 
-template <int PROG, bool ORDERED>
-__global__ __launch_bounds__(kRasterThreads) void k_raster(DrawParams d)
+template <int PROG, bool ORDERED, bool MULTI, int TS, int NT>
+__global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
 {
-    __shared__ RasterShared sh;
+    constexpr int PPT = TS * TS / NT;
+    static_assert(NT >= kBatch && PPT >= 1 && TS * TS == PPT * NT, "tile shape");
+    __shared__ RasterShared<TS> sh;
     __shared__ uint32_t olist[ORDERED ? kOrderedCap : 1];
     __shared__ uint32_t olist_n;
 
-    uint32_t n_tiles = d.tiles_x * d.tiles_y;
-    uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
+    uint32_t n_tiles = MULTI ? b.total_tiles : d0.tiles_x * d0.tiles_y;
+    uint32_t gt = xcd_tile(blockIdx.x, n_tiles);
+    uint32_t di = 0, tile = gt;
+    if constexpr (MULTI) {
+        di = find_slot(b.tile_base, b.n, gt);
+        tile = gt - b.tile_base[di];
+    }
+    const DrawParams &d = draw_at<MULTI>(d0, b, di);
     uint32_t tx = tile % d.tiles_x, ty = tile / d.tiles_x;
-    uint32_t x0 = tx * kTileW, y0 = ty * kTileH;
+    uint32_t x0 = tx * TS, y0 = ty * TS;
     uint32_t tid = threadIdx.x;
-    uint32_t lx = tid % kTileW, ly = tid / kTileW;
-    uint32_t px = x0 + lx, py = y0 + ly;
-    bool inside = px < d.width && py < d.height;
-    size_t pix = (size_t)py * d.width + px;
 
     uint32_t seg0 = d.s.tile_offset[tile];
     uint32_t seg_n = d.s.tile_offset[tile + 1] - seg0;
     uint32_t nbig = min(d.s.counters[C_BIG], d.s.big_cap);
     if (!d.init_clear && seg_n == 0 && nbig == 0) return;
 
-    PixelState ps;
-    ps.has = false;
-    ps.key = 0u;
-    ps.nfrag = 0u;
-    ps.col = 0u;
-    ps.dep = 0.0f;
-    if (inside) {
-        if (d.color) ps.col = (d.init_clear && d.clear_color_on) ? d.clear_rgba
-                                                                  : reinterpret_cast<const uint32_t *>(d.color)[pix];
-        if (d.depth) ps.dep = (d.init_clear && d.clear_depth_on) ? d.clear_depth : d.depth[pix];
-    }
-    ps.init_dep = ps.dep;
-    uint32_t init_col = ps.col;
+    PixelState ps[PPT];
+    uint32_t init_col[PPT];
+    uint32_t nfrag[PPT];
+    static_for<0, PPT>([&](auto kc) __attribute__((always_inline)) {
+        constexpr int k = decltype(kc)::value;
+        nfrag[k] = 0u;
+        uint32_t q = tid + (uint32_t)k * NT;
+        uint32_t px = x0 + q % TS, py = y0 + q / TS;
+        bool inside = px < d.width && py < d.height;
+        size_t pix = (size_t)py * d.width + px;
+        ps[k].has = false;
+        ps[k].key = 0u;
+        ps[k].col = 0u;
+        ps[k].dep = 0.0f;
+        if (inside) {
+            if (d.color)
+                ps[k].col = (d.init_clear && d.clear_color_on) ? d.clear_rgba
+                                                               : reinterpret_cast<const uint32_t *>(d.color)[pix];
+            if (d.depth) ps[k].dep = (d.init_clear && d.clear_depth_on) ? d.clear_depth : d.depth[pix];
+        }
+        ps[k].init_dep = ps[k].dep;
+        init_col[k] = ps[k].col;
+    });
 
-    if (!ORDERED) {
-        for (uint32_t b = 0; b < seg_n; b += kBatch) {
-            uint32_t n = min((uint32_t)kBatch, seg_n - b);
-            if (tid < n) sh.keys[tid] = d.s.bins[seg0 + b + tid];
-            raster_batch<PROG, false>(d, sh, n, x0, y0, lx, ly, ps);
-        }
-        for (uint32_t b = 0; b < nbig; b += kBatch) {
-            // filter the big-triangle list by tile overlap (wave 0 compacts)
-            if (tid < kBatch) {
-                bool hit = false;
-                uint32_t key = 0u;
-                if (b + tid < nbig) {
-                    uint4 r = d.s.big[b + tid];
-                    hit = tx >= (r.y & 0xFFFFu) && tx <= (r.z & 0xFFFFu) && ty >= (r.y >> 16) && ty <= (r.z >> 16);
-                    key = r.x;
-                }
-                uint64_t mask = __ballot(hit);
-                uint64_t below = (tid == 0) ? 0ull : (mask & ((~0ull) >> (64 - tid)));
-                if (hit) sh.keys[__popcll(below)] = key;
-                if (tid == 0) sh.nkeys = (uint32_t)__popcll(mask);
-            }
-            __syncthreads();
-            uint32_t n = sh.nkeys;
-            if (n) raster_batch<PROG, false>(d, sh, n, x0, y0, lx, ly, ps);
-            else __syncthreads();
-        }
-    } else {
-        // gather the tile's entries, sort them into API order, apply in order
+    uint32_t total = 0;  // ordered: length of the sorted list
+    if (ORDERED) {
+        // gather the tile's entries, sort them into API order
         if (tid == 0) olist_n = 0u;
         __syncthreads();
         bool overflow = seg_n > (uint32_t)kOrderedCap;
         if (!overflow) {
-            for (uint32_t i = tid; i < seg_n; i += blockDim.x) olist[i] = d.s.bins[seg0 + i];
+            for (uint32_t i = tid; i < seg_n; i += NT) olist[i] = d.s.bins[seg0 + i];
             if (tid == 0) olist_n = seg_n;
         }
         __syncthreads();
-        for (uint32_t b = 0; b < nbig && !overflow; b += kBatch) {
+        for (uint32_t bb = 0; bb < nbig && !overflow; bb += kBatch) {
             uint32_t base = olist_n;
             __syncthreads();  // every thread has read olist_n before thread 0 moves it
             if (tid < kBatch) {
                 bool hit = false;
                 uint32_t key = 0u;
-                if (b + tid < nbig) {
-                    uint4 r = d.s.big[b + tid];
+                if (bb + tid < nbig) {
+                    uint4 r = d.s.big[bb + tid];
                     hit = tx >= (r.y & 0xFFFFu) && tx <= (r.z & 0xFFFFu) && ty >= (r.y >> 16) && ty <= (r.z >> 16);
                     key = r.x;
                 }
@@ -572,7 +643,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(DrawParams d)
             __syncthreads();
             overflow = olist_n > (uint32_t)kOrderedCap;
         }
-        uint32_t total = olist_n;
+        total = olist_n;
         if (overflow || total > (uint32_t)kOrderedCap) {
             if (tid == 0) atomicOr(d.s.error, (uint32_t)ERR_ORDERED_OVERFLOW);
             total = 0u;
@@ -580,11 +651,11 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(DrawParams d)
         // bitonic sort of olist[0..P), padded with UINT32_MAX
         uint32_t P = 1u;
         while (P < total) P <<= 1;
-        for (uint32_t i = total + tid; i < P; i += blockDim.x) olist[i] = 0xFFFFFFFFu;
+        for (uint32_t i = total + tid; i < P; i += NT) olist[i] = 0xFFFFFFFFu;
         __syncthreads();
         for (uint32_t k = 2; k <= P; k <<= 1) {
             for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = tid; i < P; i += blockDim.x) {
+                for (uint32_t i = tid; i < P; i += NT) {
                     uint32_t ixj = i ^ j;
                     if (ixj > i) {
                         uint32_t a = olist[i], c = olist[ixj];
@@ -595,35 +666,78 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(DrawParams d)
                 __syncthreads();
             }
         }
-        for (uint32_t b = 0; b < total; b += kBatch) {
-            uint32_t n = min((uint32_t)kBatch, total - b);
-            if (tid < n) sh.keys[tid] = olist[b + tid];
-            raster_batch<PROG, true>(d, sh, n, x0, y0, lx, ly, ps);
-        }
     }
 
-    if (inside) {
+    // One loop, one call site (so the per-pixel state stays in registers):
+    // ordered -> the sorted list; otherwise the tile's bin segment, then the
+    // big-triangle list filtered by tile overlap.
+    uint32_t fpos = 0, bpos = 0;
+    for (;;) {
+        uint32_t n;
+        if (ORDERED) {
+            if (fpos >= total) break;
+            n = min((uint32_t)kBatch, total - fpos);
+            if (tid < n) sh.keys[tid] = olist[fpos + tid];
+            fpos += n;
+        } else if (fpos < seg_n) {
+            n = min((uint32_t)kBatch, seg_n - fpos);
+            if (tid < n) sh.keys[tid] = d.s.bins[seg0 + fpos + tid];
+            fpos += n;
+        } else if (bpos < nbig) {
+            if (tid < kBatch) {  // wave 0 compacts the hits
+                bool hit = false;
+                uint32_t key = 0u;
+                if (bpos + tid < nbig) {
+                    uint4 r = d.s.big[bpos + tid];
+                    hit = tx >= (r.y & 0xFFFFu) && tx <= (r.z & 0xFFFFu) && ty >= (r.y >> 16) && ty <= (r.z >> 16);
+                    key = r.x;
+                }
+                uint64_t mask = __ballot(hit);
+                uint64_t below = (tid == 0) ? 0ull : (mask & ((~0ull) >> (64 - tid)));
+                if (hit) sh.keys[__popcll(below)] = key;
+                if (tid == 0) sh.nkeys = (uint32_t)__popcll(mask);
+            }
+            __syncthreads();
+            n = sh.nkeys;
+            bpos += kBatch;
+            if (n == 0) {
+                __syncthreads();  // sh.nkeys is rewritten next round
+                continue;
+            }
+        } else {
+            break;
+        }
+        raster_batch<PROG, ORDERED, TS, NT>(d, sh, n, x0, y0, ps, nfrag);
+    }
+
+    uint32_t f = 0;
+    static_for<0, PPT>([&](auto kc) __attribute__((always_inline)) {
+        constexpr int k = decltype(kc)::value;
+        f += nfrag[k];
+        uint32_t q = tid + (uint32_t)k * NT;
+        uint32_t px = x0 + q % TS, py = y0 + q / TS;
+        if (px >= d.width || py >= d.height) return;
+        size_t pix = (size_t)py * d.width + px;
         bool write_depth;
         if (ORDERED) {
             write_depth = d.depth_test && d.depth_write;
         } else {
-            write_depth = ps.has && d.depth_test && d.depth_write &&
+            write_depth = ps[k].has && d.depth_test && d.depth_write &&
                           (d.resolve != RES_PASS_LAST || d.depth_compare == CMP_EQUAL ||
                            d.depth_compare == CMP_ALWAYS);
-            if (!ps.has) ps.col = init_col;
+            if (!ps[k].has) ps[k].col = init_col[k];
         }
-        if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = ps.col;
-        if (d.depth) d.depth[pix] = write_depth ? ps.dep : ps.init_dep;
-        if (d.s.frag_px && ps.nfrag) d.s.frag_px[pix] += ps.nfrag;
-    }
+        if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = ps[k].col;
+        if (d.depth) d.depth[pix] = write_depth ? ps[k].dep : ps[k].init_dep;
+        if (d.s.frag_px && nfrag[k]) d.s.frag_px[pix] += nfrag[k];
+    });
     // fragment count: wave reduction, one atomic per wave
-    uint32_t f = ps.nfrag;
     for (int off = 32; off > 0; off >>= 1) f += __shfl_xor(f, off);
     if (lane_id() == 0 && f) atomicAdd(d.s.frags, (unsigned long long)f);
 }
 
 // ------------------------------------------------------------------
-// k_clear: LoadOp::Clear for passes without a rasterising draw
+// k_clear / k_reset
 // ------------------------------------------------------------------
 
 __global__ void k_reset(unsigned long long *frags)
@@ -643,51 +757,76 @@ __global__ void k_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, 
 // launchers
 // ------------------------------------------------------------------
 
-template <int PROG>
-static hipError_t launch_draw_prog(const DrawParams &d, hipStream_t st, KernelTimer *kt)
-{
-    uint32_t n_tiles = d.tiles_x * d.tiles_y;
-    uint32_t zero_n = max(n_tiles, (uint32_t)C_NUM);
-    uint32_t g;
-    hipError_t e;
 #define WGC_LAUNCH(name, kern, grid, block, ...)                                        \
     do {                                                                                \
         if (kt) kt->begin(name, st);                                                    \
         hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, __VA_ARGS__);          \
-        e = hipGetLastError();                                                          \
+        hipError_t e_ = hipGetLastError();                                              \
         if (kt) kt->end(st);                                                            \
-        if (e != hipSuccess) return e;                                                  \
+        if (e_ != hipSuccess) return e_;                                                \
     } while (0)
 
-    WGC_LAUNCH("k_zero", k_zero, (zero_n + 255) / 256, 256, d.s.tile_count, n_tiles, d.s.counters);
-    if (d.total_prims > 0) {
-        g = (d.total_prims + 255u) / 256u;
-        WGC_LAUNCH("k_setup", (k_setup<PROG>), g, 256, d);
-        WGC_LAUNCH("k_clip", (k_clip<PROG>), min(g, 512u), 256, d);
+template <int PROG, bool ORDERED, bool MULTI>
+static hipError_t launch_raster(const DrawParams &d, const BatchArgs &b, uint32_t tiles, uint32_t cfg,
+                                hipStream_t st, KernelTimer *kt)
+{
+    if (ORDERED) {
+        WGC_LAUNCH("k_raster", (k_raster<PROG, true, MULTI, 16, 256>), tiles, 256, d, b);
+        return hipSuccess;
     }
-    WGC_LAUNCH("k_tile_scan", k_tile_scan, 1, 1024, d.s.tile_count, d.s.tile_offset, d.s.tile_fill, n_tiles);
-    if (d.total_prims > 0) {
-        g = min((d.s.rec_cap + 255u) / 256u, 4096u);
-        WGC_LAUNCH("k_bin_fill", k_bin_fill, g, 256, d);
+    switch (cfg) {
+    case RASTER_8x8_64: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 8, 64>), tiles, 64, d, b); break;
+    case RASTER_16x16_64: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 16, 64>), tiles, 64, d, b); break;
+    case RASTER_32x32_256: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 32, 256>), tiles, 256, d, b); break;
+    default: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 16, 256>), tiles, 256, d, b); break;
     }
-    if (d.resolve == RES_ORDERED) {
-        WGC_LAUNCH("k_raster", (k_raster<PROG, true>), n_tiles, kRasterThreads, d);
-    } else {
-        WGC_LAUNCH("k_raster", (k_raster<PROG, false>), n_tiles, kRasterThreads, d);
-    }
-#undef WGC_LAUNCH
     return hipSuccess;
 }
 
-hipError_t launch_draw(const DrawParams &d, hipStream_t st, KernelTimer *kt)
+template <int PROG, bool MULTI>
+static hipError_t launch_group(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
+                               KernelTimer *kt)
 {
-    switch (d.program) {
-    case PROG_MVP_COLOR: return launch_draw_prog<PROG_MVP_COLOR>(d, st, kt);
-    case PROG_MVP_TEXTURE: return launch_draw_prog<PROG_MVP_TEXTURE>(d, st, kt);
-    case PROG_COLORED_TRIANGLE: return launch_draw_prog<PROG_COLORED_TRIANGLE>(d, st, kt);
-    case PROG_CLIP_COLOR: return launch_draw_prog<PROG_CLIP_COLOR>(d, st, kt);
+    uint32_t zero_n = max(max(gl.total_tiles, gl.n_counters), 1u);
+    WGC_LAUNCH("k_zero", k_zero, (zero_n + 255) / 256, 256, gl.tile_count, gl.total_tiles, gl.counters,
+               gl.n_counters, gl.rec_count);
+    if (gl.total_prims > 0) {
+        uint32_t g = (gl.total_prims + 255u) / 256u;
+        WGC_LAUNCH("k_setup", (k_setup<PROG, MULTI>), g, 256, d, b);
+        WGC_LAUNCH("k_clip", (k_clip<PROG, MULTI>), min(g, 1024u), 256, d, b);
+    }
+    WGC_LAUNCH("k_tile_scan", k_tile_scan, 1, 1024, gl.tile_count, gl.tile_offset, gl.tile_fill, gl.total_tiles);
+    if (gl.total_prims > 0) {
+        uint32_t g = min((gl.rec_cap + 255u) / 256u, 4096u);
+        WGC_LAUNCH("k_bin_fill", k_bin_fill<MULTI>, g, 256, d, b, gl.recs, gl.rec_count, gl.rec_cap, gl.bins);
+    }
+    if (gl.ordered) return launch_raster<PROG, true, MULTI>(d, b, gl.total_tiles, gl.raster_cfg, st, kt);
+    return launch_raster<PROG, false, MULTI>(d, b, gl.total_tiles, gl.raster_cfg, st, kt);
+}
+
+template <bool MULTI>
+static hipError_t launch_group_prog(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl,
+                                    hipStream_t st, KernelTimer *kt)
+{
+    switch (gl.program) {
+    case PROG_MVP_COLOR: return launch_group<PROG_MVP_COLOR, MULTI>(d, b, gl, st, kt);
+    case PROG_MVP_TEXTURE: return launch_group<PROG_MVP_TEXTURE, MULTI>(d, b, gl, st, kt);
+    case PROG_COLORED_TRIANGLE: return launch_group<PROG_COLORED_TRIANGLE, MULTI>(d, b, gl, st, kt);
+    case PROG_CLIP_COLOR: return launch_group<PROG_CLIP_COLOR, MULTI>(d, b, gl, st, kt);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_draw(const DrawParams &d, const GroupLaunch &gl, hipStream_t st, KernelTimer *kt)
+{
+    BatchArgs b{};
+    return launch_group_prog<false>(d, b, gl, st, kt);
+}
+
+hipError_t launch_batch(const BatchArgs &b, const GroupLaunch &gl, hipStream_t st, KernelTimer *kt)
+{
+    DrawParams d{};
+    return launch_group_prog<true>(d, b, gl, st, kt);
 }
 
 hipError_t launch_reset(unsigned long long *frags, hipStream_t st)
@@ -700,11 +839,8 @@ hipError_t launch_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, 
                         hipStream_t st, KernelTimer *kt)
 {
     if (n == 0) return hipSuccess;
-    if (kt) kt->begin("k_clear", st);
-    hipLaunchKernelGGL(k_clear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, color, rgba, depth, dv, n);
-    hipError_t e = hipGetLastError();
-    if (kt) kt->end(st);
-    return e;
+    WGC_LAUNCH("k_clear", k_clear, (unsigned)((n + 255) / 256), 256, color, rgba, depth, dv, n);
+    return hipSuccess;
 }
 
 }  // namespace wgc

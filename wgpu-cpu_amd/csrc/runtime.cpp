@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -169,10 +170,60 @@ struct PassCommand {
     }
 };
 
+// One rasterising draw with the pass it belongs to.
+struct DrawWork {
+    DrawParams d;
+    const PassCommand *pass;
+};
+
+enum class StepKind { Group, Clear };
+
+struct Step {
+    StepKind kind = StepKind::Group;
+    bool multi = false;
+    std::vector<DrawParams> draws;   // the group's draws (scratch filled when laid out)
+    GroupLaunch gl{};
+    BatchArgs batch{};
+    size_t table_off = 0;            // offset of this group's tables in the plan's device block
+    // clear
+    uint32_t *color = nullptr;
+    uint32_t rgba = 0;
+    float *depth = nullptr;
+    float dv = 0.0f;
+    uint64_t n = 0;
+};
+
+struct Plan {
+    uint64_t arena_gen = UINT64_MAX;
+    uint32_t raster_cfg = UINT32_MAX;
+    bool batching = false;
+    std::vector<Step> steps;
+    uint8_t *d_tables = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t prims = 0, launches = 0;
+    int hip_device = 0;
+
+    void reset()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (d_tables) (void)hipFree(d_tables);
+        exec = nullptr;
+        graph = nullptr;
+        d_tables = nullptr;
+        steps.clear();
+        arena_gen = UINT64_MAX;
+    }
+    ~Plan() { reset(); }
+};
+
 struct WgcCommandBuffer_T : Object {
     std::vector<PassCommand> passes;
+    std::unique_ptr<Plan> plan{new Plan()};  // launch plan cached for resubmission
     ~WgcCommandBuffer_T() override
     {
+        plan.reset();
         for (auto &p : passes) p.drop();
     }
 };
@@ -221,6 +272,10 @@ struct WgcDevice_T : Object {
     KernelTimer timer;
     std::map<std::string, std::pair<double, uint64_t>> ktimes;
     uint64_t last_prims = 0, last_launches = 0;
+    uint64_t arena_gen = 0;        // bumped whenever memory a cached plan points at moves
+    uint32_t raster_cfg = RASTER_16x16_256;
+    bool batching = true;          // merge independent single-draw passes into launch groups
+    bool use_graphs = true;        // replay cached plans as hipGraphs
     uint32_t *frag_px = nullptr;   // optional per-pixel fragment counts (debug)
     size_t frag_px_size = 0;
 
@@ -426,6 +481,7 @@ extern "C" int32_t wgc_device_debug_fragment_counts(WgcDevice device, uint32_t w
         HIP_TRY(hipMemcpy(host_out, device->frag_px, need, hipMemcpyDeviceToHost), "readback");
         return WGC_OK;
     }
+    device->arena_gen++;  // cached plans point at the counts buffer
     if (device->frag_px) {
         HIP_TRY(hipFree(device->frag_px), "hipFree");
         device->frag_px = nullptr;
@@ -517,7 +573,8 @@ extern "C" int32_t wgc_texture_create(WgcDevice device, uint32_t width, uint32_t
     if (!device || !out) return set_error(WGC_ERR_INVALID, "null argument");
     uint32_t tb = texel_bytes(format);
     if (!tb) return set_error(WGC_ERR_UNSUPPORTED, "unsupported texture format");
-    if (width == 0 || height == 0 || width > 65535u * kTileW || height > 65535u * kTileH)
+    // tile coordinates are 16-bit with 8-pixel tiles
+    if (width == 0 || height == 0 || width > 65535u * 8u || height > 65535u * 8u)
         return set_error(WGC_ERR_INVALID, "bad texture size");
     HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
     auto *t = new WgcTexture_T();
@@ -927,8 +984,15 @@ extern "C" int32_t wgc_command_encoder_finish(WgcCommandEncoder enc, WgcCommandB
 extern "C" void wgc_command_buffer_release(WgcCommandBuffer buffer) { release(buffer); }
 
 // ------------------------------------------------------------------
-// execution
+// execution: planning (host) and launching (device stream)
 // ------------------------------------------------------------------
+//
+// A command buffer is planned once per device state (arena generation,
+// raster tile shape, batching switch): every draw becomes a DrawParams, and
+// consecutive independent passes with one draw each are merged into launch
+// groups that share all kernels (the batched form).  The plan is cached in
+// the command buffer; a resubmission only launches it, as a hipGraph when
+// graphs are enabled.
 
 static uint8_t unorm8(float v)
 {
@@ -947,63 +1011,6 @@ static uint32_t pack_clear(const double c[4], uint32_t format)
 }
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-// Carves the per-draw scratch out of the device arena, growing it when needed.
-static int32_t plan_scratch(WgcDevice_T *dev, DrawParams &d)
-{
-    uint32_t n_tiles = d.tiles_x * d.tiles_y;
-    uint64_t prims = d.total_prims;
-    uint64_t clip_tri_cap = 2ull * prims + 4096ull;
-    uint64_t rec_cap = prims + clip_tri_cap;
-    uint64_t bins_cap = rec_cap * (uint64_t)kMaxBinTiles;
-    if (bins_cap > 0xFFFFFFFFull) return set_error(WGC_ERR_UNSUPPORTED, "draw too large for one launch");
-    struct Part { size_t bytes; size_t off; };
-    Part parts[] = {
-        {4ull * n_tiles, 0}, {4ull * (n_tiles + 1), 0}, {4ull * n_tiles, 0},
-        {4ull * bins_cap, 0}, {16ull * rec_cap, 0}, {16ull * rec_cap, 0},
-        {4ull * (prims ? prims : 1), 0}, {4ull * kClipFloats * clip_tri_cap, 0},
-        {4ull * (prims ? prims : 1), 0}, {4ull * C_NUM, 0},
-    };
-    size_t total = 0;
-    for (auto &p : parts) {
-        p.off = total;
-        total = align_up(total + p.bytes, 256);
-    }
-    if (total > dev->arena_size) {
-        HIP_TRY(hipStreamSynchronize(dev->stream), "arena");
-        if (dev->arena) HIP_TRY(hipFree(dev->arena), "arena");
-        dev->arena = nullptr;
-        size_t grow = std::max(total, dev->arena_size * 2);
-        hipError_t e = hipMalloc(&dev->arena, grow);
-        if (e != hipSuccess) {
-            dev->arena_size = 0;
-            return set_error(WGC_ERR_OUT_OF_MEMORY, "scratch arena");
-        }
-        dev->arena_size = grow;
-    }
-    uint8_t *a = dev->arena;
-    Scratch &s = d.s;
-    s.tile_count = (uint32_t *)(a + parts[0].off);
-    s.tile_offset = (uint32_t *)(a + parts[1].off);
-    s.tile_fill = (uint32_t *)(a + parts[2].off);
-    s.bins = (uint32_t *)(a + parts[3].off);
-    s.recs = (uint4 *)(a + parts[4].off);
-    s.big = (uint4 *)(a + parts[5].off);
-    s.clip_list = (uint32_t *)(a + parts[6].off);
-    s.clip_tris = (float *)(a + parts[7].off);
-    s.clip_base = (uint32_t *)(a + parts[8].off);
-    s.counters = (uint32_t *)(a + parts[9].off);
-    s.frags = dev->d_frags;
-    s.error = dev->d_error;
-    // the debug counts are only written when they were sized for this pass
-    s.frag_px = (dev->frag_px && dev->frag_px_size == (size_t)d.width * d.height * 4u) ? dev->frag_px : nullptr;
-    s.bins_cap = (uint32_t)bins_cap;
-    s.rec_cap = (uint32_t)rec_cap;
-    s.big_cap = (uint32_t)rec_cap;
-    s.clip_list_cap = (uint32_t)prims;
-    s.clip_tri_cap = (uint32_t)clip_tri_cap;
-    return WGC_OK;
-}
 
 struct PassState {
     WgcRenderPipeline_T *pipeline = nullptr;
@@ -1030,9 +1037,9 @@ static uint32_t resolve_mode(const WgcRenderPipelineDescriptor &p, bool depth_te
 }
 
 // Builds the kernel parameters of one draw from the pass state
-// (DrawCall::execute, state.rs:239-480).
+// (DrawCall::execute, state.rs:239-480); scratch pointers come later.
 static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassState &st, const SubCommand &c,
-                          uint32_t w, uint32_t h, DrawParams &d)
+                          uint32_t w, uint32_t h, uint32_t tile_shift, DrawParams &d)
 {
     memset(&d, 0, sizeof d);
     const WgcRenderPipeline_T *pl = st.pipeline;
@@ -1111,24 +1118,110 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
         return set_error(WGC_ERR_INVALID, "scissor rect outside the framebuffer");
     d.width = w;
     d.height = h;
-    d.tiles_x = (w + kTileW - 1) / kTileW;
-    d.tiles_y = (h + kTileH - 1) / kTileH;
+    // the ordered (sorted, sequential) raster exists for 16x16 tiles only
+    d.tile_shift = d.resolve == RES_ORDERED ? 4u : tile_shift;
+    tile_shift = d.tile_shift;
+    d.tiles_x = (w + (1u << tile_shift) - 1) >> tile_shift;
+    d.tiles_y = (h + (1u << tile_shift) - 1) >> tile_shift;
     d.color = pc.color ? pc.color->ptr : nullptr;
     d.color_bgra = pc.color && (pc.color->format == WGC_TEXTURE_FORMAT_BGRA8_UNORM ||
                                 pc.color->format == WGC_TEXTURE_FORMAT_BGRA8_UNORM_SRGB);
     d.depth = pc.depth ? (float *)pc.depth->ptr : nullptr;
-    return plan_scratch(dev, d);
+    return WGC_OK;
 }
 
-static int32_t execute_pass(WgcDevice_T *dev, const PassCommand &pc)
+// Lays a group's scratch out at the start of the arena (groups run one after
+// another on the stream and reuse it).  Returns the bytes it needs; with a
+// base pointer, fills the pointers.
+static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t frag_px_size, WgcDevice_T *dev)
+{
+    uint32_t n = (uint32_t)st.draws.size();
+    uint64_t total_tiles = 0, rec_cap = 0;
+    std::vector<uint64_t> clip_cap(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const DrawParams &d = st.draws[i];
+        total_tiles += (uint64_t)d.tiles_x * d.tiles_y;
+        clip_cap[i] = 2ull * d.total_prims + 4096ull;
+        rec_cap += d.total_prims + clip_cap[i];
+    }
+    uint64_t bins_cap = rec_cap * (uint64_t)kMaxBinTiles;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off = align_up(off + std::max<size_t>(bytes, 4), 256);
+        return o;
+    };
+    size_t o_tc = take(4 * total_tiles), o_to = take(4 * (total_tiles + 1)), o_tf = take(4 * total_tiles);
+    size_t o_bins = take(4 * bins_cap), o_recs = take(16 * rec_cap);
+    size_t o_ctr = take(4ull * C_NUM * n), o_rc = take(4);
+    std::vector<size_t> o_big(n), o_cl(n), o_ct(n), o_cb(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const DrawParams &d = st.draws[i];
+        uint64_t rc = d.total_prims + clip_cap[i];
+        o_big[i] = take(16 * rc);
+        o_cl[i] = take(4ull * d.total_prims);
+        o_ct[i] = take(4ull * kClipFloats * clip_cap[i]);
+        o_cb[i] = take(4ull * d.total_prims);
+    }
+    if (!base) return off;
+    GroupLaunch &gl = st.gl;
+    gl.total_tiles = (uint32_t)total_tiles;
+    gl.n_counters = C_NUM * n;
+    gl.tile_count = (uint32_t *)(base + o_tc);
+    gl.tile_offset = (uint32_t *)(base + o_to);
+    gl.tile_fill = (uint32_t *)(base + o_tf);
+    gl.bins = (uint32_t *)(base + o_bins);
+    gl.recs = (uint4 *)(base + o_recs);
+    gl.counters = (uint32_t *)(base + o_ctr);
+    gl.rec_count = (uint32_t *)(base + o_rc);
+    gl.rec_cap = (uint32_t)rec_cap;
+    uint64_t tile_base = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        DrawParams &d = st.draws[i];
+        Scratch &s = d.s;
+        s.tile_count = gl.tile_count + tile_base;
+        s.tile_offset = gl.tile_offset + tile_base;
+        s.tile_fill = gl.tile_fill + tile_base;
+        s.bins = gl.bins;
+        s.recs = gl.recs;
+        s.rec_count = gl.rec_count;
+        s.big = (uint4 *)(base + o_big[i]);
+        s.clip_list = (uint32_t *)(base + o_cl[i]);
+        s.clip_tris = (float *)(base + o_ct[i]);
+        s.clip_base = (uint32_t *)(base + o_cb[i]);
+        s.counters = gl.counters + C_NUM * i;
+        s.frags = dev->d_frags;
+        s.error = dev->d_error;
+        // debug counts are only written when they were sized for this pass
+        s.frag_px = (frag_px && frag_px_size == (size_t)d.width * d.height * 4u) ? frag_px : nullptr;
+        s.bins_cap = (uint32_t)bins_cap;
+        s.rec_cap = (uint32_t)rec_cap;
+        s.big_cap = (uint32_t)(d.total_prims + clip_cap[i]);
+        s.clip_list_cap = d.total_prims;
+        s.clip_tri_cap = (uint32_t)clip_cap[i];
+        tile_base += (uint64_t)d.tiles_x * d.tiles_y;
+    }
+    return off;
+}
+
+// Passes whose attachments and texture bindings do not meet may share a
+// launch group.
+static void pass_textures(const PassCommand &pc, std::vector<const void *> &writes, std::vector<const void *> &reads)
+{
+    if (pc.color) writes.push_back(pc.color);
+    if (pc.depth) writes.push_back(pc.depth);
+    for (const SubCommand &c : pc.cmds)
+        if (c.kind == CmdKind::SetBindGroup && c.obj)
+            for (const BindingSlot &e : ((WgcBindGroup_T *)c.obj)->entries)
+                if (e.texture) reads.push_back(e.texture);
+}
+
+static int32_t collect_pass(WgcDevice_T *dev, const PassCommand &pc, uint32_t tile_shift, std::vector<DrawWork> &out,
+                            bool &clear_only, Step &clear)
 {
     uint32_t w = 0, h = 0;
     if (pc.color) { w = pc.color->width; h = pc.color->height; }
     else if (pc.depth) { w = pc.depth->width; h = pc.depth->height; }
-    KernelTimer *kt = dev->timing ? &dev->timer : nullptr;
-    HIP_TRY(launch_reset(dev->d_frags, dev->stream), "pass");
-    dev->last_prims = 0;
-    dev->last_launches = 0;
     PassState st;
     st.viewport[0] = 0.0f; st.viewport[1] = 0.0f;
     st.viewport[2] = (float)w; st.viewport[3] = (float)h;
@@ -1145,43 +1238,214 @@ static int32_t execute_pass(WgcDevice_T *dev, const PassCommand &pc)
             if (c.slot >= 4) return set_error(WGC_ERR_UNSUPPORTED, "bind group index >= 4");
             st.groups[c.slot] = (WgcBindGroup_T *)c.obj;
             break;
-        case CmdKind::SetVertexBuffer:
-            st.vbs[c.slot] = BufferBinding{(WgcBuffer_T *)c.obj, c.offset, c.size, 0};
-            break;
-        case CmdKind::SetIndexBuffer:
-            st.ib = BufferBinding{(WgcBuffer_T *)c.obj, c.offset, c.size, c.format};
-            break;
+        case CmdKind::SetVertexBuffer: st.vbs[c.slot] = BufferBinding{(WgcBuffer_T *)c.obj, c.offset, c.size, 0}; break;
+        case CmdKind::SetIndexBuffer: st.ib = BufferBinding{(WgcBuffer_T *)c.obj, c.offset, c.size, c.format}; break;
         case CmdKind::SetViewport: memcpy(st.viewport, c.viewport, sizeof st.viewport); break;
         case CmdKind::SetScissor: memcpy(st.scissor, c.rect, sizeof st.scissor); break;
         case CmdKind::SetBlendConstant:
-        case CmdKind::SetStencilReference:
-            break;
+        case CmdKind::SetStencilReference: break;
         case CmdKind::Draw:
         case CmdKind::DrawIndexed: {
             if (!st.pipeline) return set_error(WGC_ERR_INVALID, "No pipeline bound (state.rs:243)");
-            if (!st.pipeline->has_fragment || w == 0) break;  // only vertex shading, no visible effect
-            DrawParams d;
-            int32_t r = build_draw(dev, pc, st, c, w, h, d);
+            if (!st.pipeline->has_fragment || w == 0) break;  // vertex shading only: no visible effect
+            DrawWork dw;
+            int32_t r = build_draw(dev, pc, st, c, w, h, tile_shift, dw.d);
             if (r != WGC_OK) return r;
-            dev->last_prims += d.total_prims;
-            d.init_clear = clear_pending ? 1u : 0u;
-            d.clear_color_on = clear_color;
-            d.clear_depth_on = clear_depth;
-            d.clear_rgba = clear_rgba;
-            d.clear_depth = pc.clear_depth;
-            HIP_TRY(launch_draw(d, dev->stream, kt), "draw launch");
-            dev->last_launches += 6;
+            dw.pass = &pc;
+            dw.d.init_clear = clear_pending ? 1u : 0u;
+            dw.d.clear_color_on = clear_color;
+            dw.d.clear_depth_on = clear_depth;
+            dw.d.clear_rgba = clear_rgba;
+            dw.d.clear_depth = pc.clear_depth;
             clear_pending = false;
+            out.push_back(dw);
             break;
         }
         }
     }
+    clear_only = clear_pending;
     if (clear_pending) {
-        HIP_TRY(launch_clear(clear_color ? (uint32_t *)pc.color->ptr : nullptr, clear_rgba,
-                             clear_depth ? (float *)pc.depth->ptr : nullptr, pc.clear_depth,
-                             (uint64_t)w * h, dev->stream, kt), "clear launch");
-        dev->last_launches += 1;
+        clear.kind = StepKind::Clear;
+        clear.color = clear_color ? (uint32_t *)pc.color->ptr : nullptr;
+        clear.rgba = clear_rgba;
+        clear.depth = clear_depth ? (float *)pc.depth->ptr : nullptr;
+        clear.dv = pc.clear_depth;
+        clear.n = (uint64_t)w * h;
     }
+    return WGC_OK;
+}
+
+static void add_group(std::vector<Step> &steps, std::vector<DrawParams> draws, uint32_t cfg)
+{
+    Step s;
+    s.kind = StepKind::Group;
+    s.multi = draws.size() > 1;
+    s.gl.program = draws[0].program;
+    s.gl.ordered = draws[0].resolve == RES_ORDERED;
+    s.gl.raster_cfg = cfg;
+    uint64_t prims = 0;
+    for (auto &d : draws) prims += s.multi ? align_up(d.total_prims, 64) : d.total_prims;
+    s.gl.total_prims = (uint32_t)prims;
+    s.draws = std::move(draws);
+    steps.push_back(std::move(s));
+}
+
+static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
+{
+    plan.reset();
+    plan.hip_device = dev->hip_device;
+    uint32_t cfg = dev->raster_cfg;
+    uint32_t shift = raster_tile_shift(cfg);
+    // 1. draws per pass, in order
+    struct PassWork {
+        std::vector<DrawWork> draws;
+        bool clear_only = false;
+        Step clear;
+        std::vector<const void *> writes, reads;
+    };
+    std::vector<PassWork> passes(cb->passes.size());
+    for (size_t i = 0; i < cb->passes.size(); ++i) {
+        int32_t r = collect_pass(dev, cb->passes[i], shift, passes[i].draws, passes[i].clear_only, passes[i].clear);
+        if (r != WGC_OK) return r;
+        pass_textures(cb->passes[i], passes[i].writes, passes[i].reads);
+    }
+    // 2. steps: runs of independent single-draw passes become launch groups
+    //    (one per program and resolve kind); everything else runs in order
+    size_t i = 0;
+    while (i < passes.size()) {
+        PassWork &pw = passes[i];
+        bool batchable = dev->batching && pw.draws.size() == 1;
+        if (!batchable) {
+            for (auto &dw : pw.draws) add_group(plan.steps, {dw.d}, cfg);
+            if (pw.clear_only) plan.steps.push_back(pw.clear);
+            ++i;
+            continue;
+        }
+        std::vector<const void *> written, read;
+        size_t j = i;
+        while (j < passes.size() && passes[j].draws.size() == 1) {
+            PassWork &q = passes[j];
+            bool clash = false;
+            for (auto *t : q.writes) {
+                for (auto *u : written) clash = clash || t == u;
+                for (auto *u : read) clash = clash || t == u;
+            }
+            for (auto *t : q.reads)
+                for (auto *u : written) clash = clash || t == u;
+            if (clash) break;
+            written.insert(written.end(), q.writes.begin(), q.writes.end());
+            read.insert(read.end(), q.reads.begin(), q.reads.end());
+            ++j;
+        }
+        if (j - i == 1) {
+            add_group(plan.steps, {pw.draws[0].d}, cfg);
+        } else {
+            // group by (program, ordered); ordered groups use the 16x16 raster
+            std::map<std::pair<uint32_t, bool>, std::vector<DrawParams>> groups;
+            for (size_t k = i; k < j; ++k) {
+                const DrawParams &d = passes[k].draws[0].d;
+                groups[{d.program, d.resolve == RES_ORDERED}].push_back(d);
+            }
+            for (auto &kv : groups) add_group(plan.steps, std::move(kv.second), cfg);
+        }
+        i = j;
+    }
+    // 3. scratch: the arena must hold the largest group
+    size_t need = 0;
+    size_t tables = 0;
+    for (Step &s : plan.steps) {
+        if (s.kind != StepKind::Group) continue;
+        need = std::max(need, layout_group(s, nullptr, nullptr, 0, dev));
+        if (s.multi) {
+            s.table_off = tables;
+            size_t n = s.draws.size();
+            tables = align_up(tables + n * sizeof(DrawParams) + 8 * (n + 1), 256);
+        }
+    }
+    if (need > dev->arena_size) {
+        HIP_TRY(hipStreamSynchronize(dev->stream), "arena");
+        if (dev->arena) HIP_TRY(hipFree(dev->arena), "arena");
+        dev->arena = nullptr;
+        dev->arena_size = 0;
+        size_t grow = std::max(need, (size_t)64 << 20);
+        if (hipMalloc(&dev->arena, grow) != hipSuccess) return set_error(WGC_ERR_OUT_OF_MEMORY, "scratch arena");
+        dev->arena_size = grow;
+        dev->arena_gen++;
+    }
+    std::vector<uint8_t> host_tables(tables);
+    if (tables) HIP_TRY(hipMalloc(&plan.d_tables, tables), "draw tables");
+    for (Step &s : plan.steps) {
+        if (s.kind != StepKind::Group) {
+            plan.launches += 1;
+            continue;
+        }
+        layout_group(s, dev->arena, dev->frag_px, dev->frag_px_size, dev);
+        plan.launches += 6;
+        for (auto &d : s.draws) plan.prims += d.total_prims;
+        if (!s.multi) continue;
+        uint32_t n = (uint32_t)s.draws.size();
+        uint8_t *h = host_tables.data() + s.table_off;
+        uint8_t *dptr = plan.d_tables + s.table_off;
+        memcpy(h, s.draws.data(), n * sizeof(DrawParams));
+        uint32_t *pb = (uint32_t *)(h + n * sizeof(DrawParams));
+        uint32_t *tb = pb + (n + 1);
+        uint32_t acc_p = 0, acc_t = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            pb[k] = acc_p;
+            tb[k] = acc_t;
+            acc_p += (uint32_t)align_up(s.draws[k].total_prims, 64);
+            acc_t += s.draws[k].tiles_x * s.draws[k].tiles_y;
+        }
+        pb[n] = acc_p;
+        tb[n] = acc_t;
+        s.batch.draws = (const DrawParams *)dptr;
+        s.batch.prim_base = (const uint32_t *)(dptr + n * sizeof(DrawParams));
+        s.batch.tile_base = s.batch.prim_base + (n + 1);
+        s.batch.n = n;
+        s.batch.total_prims = acc_p;
+        s.batch.total_tiles = acc_t;
+    }
+    if (tables) HIP_TRY(hipMemcpy(plan.d_tables, host_tables.data(), tables, hipMemcpyHostToDevice), "draw tables");
+    plan.arena_gen = dev->arena_gen;
+    plan.raster_cfg = cfg;
+    plan.batching = dev->batching;
+    return WGC_OK;
+}
+
+static hipError_t launch_plan(WgcDevice_T *dev, const Plan &plan, KernelTimer *kt)
+{
+    hipError_t e = launch_reset(dev->d_frags, dev->stream);
+    if (e != hipSuccess) return e;
+    for (const Step &s : plan.steps) {
+        if (s.kind == StepKind::Clear) e = launch_clear(s.color, s.rgba, s.depth, s.dv, s.n, dev->stream, kt);
+        else if (s.multi) e = launch_batch(s.batch, s.gl, dev->stream, kt);
+        else e = launch_draw(s.draws[0], s.gl, dev->stream, kt);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+static int32_t submit_plan(WgcDevice_T *dev, Plan &plan)
+{
+    KernelTimer *kt = dev->timing ? &dev->timer : nullptr;
+    if (!dev->use_graphs || kt) {
+        HIP_TRY(launch_plan(dev, plan, kt), "launch");
+        return WGC_OK;
+    }
+    if (!plan.exec) {
+        HIP_TRY(hipStreamBeginCapture(dev->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+        hipError_t e = launch_plan(dev, plan, nullptr);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(dev->stream, &g);
+        if (e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return hip_fail(e, "graph capture launch");
+        }
+        if (e2 != hipSuccess) return hip_fail(e2, "graph capture");
+        plan.graph = g;
+        HIP_TRY(hipGraphInstantiate(&plan.exec, g, nullptr, nullptr, 0), "graph instantiate");
+    }
+    HIP_TRY(hipGraphLaunch(plan.exec, dev->stream), "graph launch");
     return WGC_OK;
 }
 
@@ -1193,13 +1457,29 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
     HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
     Inflight f;
     f.index = device->next_index++;
+    device->last_prims = 0;
+    device->last_launches = 0;
     for (uint32_t i = 0; i < count; ++i) {
         WgcCommandBuffer_T *cb = buffers[i];
         if (!cb) return set_error(WGC_ERR_INVALID, "null command buffer");
-        for (const PassCommand &pc : cb->passes) {
-            int32_t r = execute_pass(device, pc);
-            if (r != WGC_OK) return r;
+        Plan &plan = *cb->plan;
+        if (plan.arena_gen != device->arena_gen || plan.raster_cfg != device->raster_cfg ||
+            plan.batching != device->batching || plan.hip_device != device->hip_device) {
+            int32_t r = build_plan(device, cb, plan);
+            if (r != WGC_OK) {
+                plan.reset();
+                return r;
+            }
+            // growing the arena may have invalidated plans built before
+            if (plan.arena_gen != device->arena_gen) {
+                r = build_plan(device, cb, plan);
+                if (r != WGC_OK) return r;
+            }
         }
+        int32_t r = submit_plan(device, plan);
+        if (r != WGC_OK) return r;
+        device->last_prims += plan.prims;
+        device->last_launches += plan.launches;
         retain(cb);
         f.keep.push_back(cb);
     }
@@ -1214,6 +1494,32 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
     device->inflight.push_back(std::move(f));
     // keep the queue short: retire what is already done
     return retire(device, UINT64_MAX, false, 0);
+}
+
+extern "C" int32_t wgc_device_set_option(WgcDevice device, uint32_t option, uint32_t value)
+{
+    if (!device) return set_error(WGC_ERR_INVALID, "null device");
+    std::lock_guard<std::mutex> lk(device->mu);
+    switch (option) {
+    case WGC_OPTION_RASTER_TILE:
+        if (value >= RASTER_NUM_CFG) return set_error(WGC_ERR_INVALID, "bad raster tile shape");
+        device->raster_cfg = value;
+        return WGC_OK;
+    case WGC_OPTION_BATCHING: device->batching = value != 0; return WGC_OK;
+    case WGC_OPTION_GRAPHS: device->use_graphs = value != 0; return WGC_OK;
+    default: return set_error(WGC_ERR_INVALID, "unknown option");
+    }
+}
+
+extern "C" int32_t wgc_device_get_option(WgcDevice device, uint32_t option, uint32_t *value)
+{
+    if (!device || !value) return set_error(WGC_ERR_INVALID, "null argument");
+    switch (option) {
+    case WGC_OPTION_RASTER_TILE: *value = device->raster_cfg; return WGC_OK;
+    case WGC_OPTION_BATCHING: *value = device->batching ? 1u : 0u; return WGC_OK;
+    case WGC_OPTION_GRAPHS: *value = device->use_graphs ? 1u : 0u; return WGC_OK;
+    default: return set_error(WGC_ERR_INVALID, "unknown option");
+    }
 }
 
 // ------------------------------------------------------------------

@@ -16,9 +16,7 @@ namespace wgc {
 constexpr int kMaxVb = 4;
 constexpr int kMaxAttr = 8;
 constexpr int kMaxVar = 4;        // user varying floats of the built-in programs
-constexpr int kTileW = 16;
-constexpr int kTileH = 16;
-constexpr int kRasterThreads = kTileW * kTileH;
+constexpr int kMaxTileDim = 32;   // tiles are 8x8, 16x16 or 32x32 pixels
 constexpr int kBatch = 64;        // triangles staged in LDS per raster step
 constexpr int kMaxBinTiles = 16;  // larger footprints go to the big-triangle list
 constexpr int kOrderedCap = 4096; // LDS list capacity of the ordered (blend) path
@@ -76,12 +74,16 @@ struct AttrBind {
     uint32_t stride, step_instance, offset, ncomp;
 };
 
+// Scratch of one draw.  The tile arrays, the record list and the bins are
+// slices of arrays shared by every draw of a launch (tile offsets and bin
+// indices are global); the rest is per draw.
 struct Scratch {
-    uint32_t *tile_count;    // n_tiles
-    uint32_t *tile_offset;   // n_tiles + 1
+    uint32_t *tile_count;    // n_tiles (this draw's slice)
+    uint32_t *tile_offset;   // n_tiles + 1 (global bin offsets)
     uint32_t *tile_fill;     // n_tiles
-    uint32_t *bins;          // bins_cap keys
-    uint4 *recs;             // (key, tx0|ty0<<16, tx1|ty1<<16, 0)
+    uint32_t *bins;          // shared, bins_cap keys
+    uint4 *recs;             // shared: (key, tx0|ty0<<16, tx1|ty1<<16, draw)
+    uint32_t *rec_count;     // shared record counter
     uint4 *big;
     uint32_t *clip_list;     // primitives that need the clipper
     float *clip_tris;        // clip_tri_cap x kClipFloats
@@ -93,7 +95,7 @@ struct Scratch {
     uint32_t bins_cap, rec_cap, big_cap, clip_list_cap, clip_tri_cap;
 };
 
-enum Counter { C_REC = 0, C_BIG = 1, C_CLIP_LIST = 2, C_CLIP_TRI = 3, C_NUM = 4 };
+enum Counter { C_BIG = 0, C_CLIP_LIST = 1, C_CLIP_TRI = 2, C_NUM = 4 };
 
 struct DrawParams {
     // pipeline
@@ -117,8 +119,8 @@ struct DrawParams {
     // dynamic state
     float viewport[6];
     uint32_t scissor[4];
-    // framebuffer
-    uint32_t width, height, tiles_x, tiles_y;
+    // framebuffer and its tiling (tile = 1 << tile_shift pixels square)
+    uint32_t width, height, tiles_x, tiles_y, tile_shift;
     uint8_t *color;
     uint32_t color_bgra;
     float *depth;
@@ -616,8 +618,8 @@ __device__ __forceinline__ bool tile_bbox(const DrawParams &d, const ScanTri &s,
     int64_t x0 = max((int64_t)c0, sx0), x1 = min((int64_t)c1, sx1);
     int64_t y0 = max((int64_t)r0, sy0), y1 = min((int64_t)r1, sy1);
     if (x0 > x1 || y0 > y1) return false;
-    tx0 = (uint32_t)(x0 / kTileW); tx1 = (uint32_t)(x1 / kTileW);
-    ty0 = (uint32_t)(y0 / kTileH); ty1 = (uint32_t)(y1 / kTileH);
+    tx0 = (uint32_t)(x0 >> d.tile_shift); tx1 = (uint32_t)(x1 >> d.tile_shift);
+    ty0 = (uint32_t)(y0 >> d.tile_shift); ty1 = (uint32_t)(y1 >> d.tile_shift);
     return true;
 }
 

@@ -66,6 +66,12 @@ PROGRAM_MVP_TEXTURE = 2
 PROGRAM_COLORED_TRIANGLE = 3
 PROGRAM_CLIP_COLOR = 4
 
+OPTION_RASTER_TILE = 1
+OPTION_BATCHING = 2
+OPTION_GRAPHS = 3
+# raster tile shapes (tile side, lanes per workgroup)
+RASTER_TILES = {0: (16, 256), 1: (8, 64), 2: (16, 64), 3: (32, 256)}
+
 BINDING_BUFFER = 1
 BINDING_TEXTURE = 2
 BINDING_SAMPLER = 3
@@ -174,6 +180,8 @@ SIGNATURES = {
     "wgc_queue_submit": (C.c_int32, [_P, C.c_uint32, _PP, C.POINTER(C.c_uint64)]),
     "wgc_device_poll": (C.c_int32, [_P, C.c_uint64, C.c_uint64]),
     "wgc_device_last_pass_stats": (C.c_int32, [_P, C.POINTER(PassStats)]),
+    "wgc_device_set_option": (C.c_int32, [_P, C.c_uint32, C.c_uint32]),
+    "wgc_device_get_option": (C.c_int32, [_P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "wgc_device_debug_fragment_counts": (C.c_int32, [_P, C.c_uint32, C.c_uint32, C.c_void_p]),
     "wgc_device_timestamp": (C.c_int32, [_P, C.c_uint32]),
     "wgc_device_elapsed_ms": (C.c_int32, [_P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
@@ -504,6 +512,14 @@ class Device:
     def poll(self, submission_index: Optional[int] = None, timeout_ns: int = 0):
         idx = (1 << 64) - 1 if submission_index is None else submission_index
         _check(self._lib.wgc_device_poll(self._ptr, idx, timeout_ns))
+
+    def set_option(self, option: int, value: int):
+        _check(self._lib.wgc_device_set_option(self._ptr, option, value))
+
+    def get_option(self, option: int) -> int:
+        v = C.c_uint32()
+        _check(self._lib.wgc_device_get_option(self._ptr, option, C.byref(v)))
+        return v.value
 
     def last_pass_stats(self) -> PassStats:
         s = PassStats()

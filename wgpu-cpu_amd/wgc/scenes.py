@@ -132,19 +132,22 @@ def uv_sphere(segments: int, rings: int):
     pos = np.stack([st * np.cos(phi)[None, :], np.broadcast_to(ct, (rings + 1, segments + 1)),
                     -st * np.sin(phi)[None, :]], axis=-1).reshape(-1, 3)
     uv = np.stack(np.meshgrid(s / segments, r / rings, indexing="xy"), axis=-1).reshape(-1, 2)
-    idx = []
     W = segments + 1
-    for ri in range(rings):
-        for si in range(segments):
-            a = ri * W + si
-            b = a + 1
-            c = (ri + 1) * W + si
-            dd = c + 1
-            if ri != 0:
-                idx.append((a, c, b))
-            if ri != rings - 1:
-                idx.append((b, c, dd))
-    return pos, uv, np.array(idx, dtype=np.uint32).reshape(-1)
+    ri, si = np.meshgrid(np.arange(rings), np.arange(segments), indexing="ij")
+    a = ri * W + si
+    b = a + 1
+    c = (ri + 1) * W + si
+    dd = c + 1
+    # per quad, in order: the upper triangle (absent on the top band), then
+    # the lower one (absent on the bottom band)
+    up = np.stack([a, c, b], axis=-1)
+    lo = np.stack([b, c, dd], axis=-1)
+    has_up = (ri != 0)[..., None]
+    has_lo = (ri != rings - 1)[..., None]
+    tri = np.concatenate([up[..., None, :], lo[..., None, :]], axis=-2)   # (R, S, 2, 3)
+    keep = np.concatenate([has_up, has_lo], axis=-1)                      # (R, S, 2)
+    idx = tri[np.broadcast_to(keep, tri.shape[:-1])]
+    return pos, uv, idx.astype(np.uint32).reshape(-1)
 
 
 def _value_noise(p: np.ndarray, seed: int, lattice: float = 8.0) -> np.ndarray:
@@ -216,7 +219,8 @@ def displaced_sphere(segments: int, rings: int, seed: int, amplitude: float = 0.
 
 
 def config2_textured(seed: int = SEED_BASE + 2, extended: bool = False, width: int = 1920,
-                     height: int = 1080, segments: int = 256, rings: int = 128) -> PassSpec:
+                     height: int = 1080, segments: int = 256, rings: int = 128,
+                     texture: Optional[bytes] = None) -> PassSpec:
     """Sphere 256x128 (65,024 triangles) radially displaced by seeded value
     noise (amplitude 0.05), f32x2 UVs; 1024x1024 Rgba8UnormSrgb checker (32 px
     cells) plus +-16 noise, Repeat addressing; 1920x1080 Rgba8Unorm target,
@@ -228,7 +232,7 @@ def config2_textured(seed: int = SEED_BASE + 2, extended: bool = False, width: i
     vb[:, 0:3] = pos
     vb[:, 3] = 1.0
     vb[:, 4:6] = uv
-    tex = checker_texture(1024, 32, 16, sub_seed(seed, 2))
+    tex = texture if texture is not None else checker_texture(1024, 32, 16, sub_seed(seed, 2))
     draw = DrawSpec(program=PROGRAM_MVP_TEXTURE,
                     interp=INTERP_PERSPECTIVE if extended else INTERP_LINEAR,
                     vertex_data=vb.tobytes(), vertex_stride=24,
@@ -310,11 +314,23 @@ def sphere_for_triangles(tris: float) -> Tuple[int, int]:
     return 2 * rings, rings
 
 
+_SHARED_TEXTURE = {}
+
+
+def shared_texture() -> bytes:
+    """The textured workload's 1024x1024 checker, generated once and shared
+    by every textured pass of the batched and serving workloads."""
+    if "tex" not in _SHARED_TEXTURE:
+        _SHARED_TEXTURE["tex"] = checker_texture(1024, 32, 16, sub_seed(SEED_BASE + 2, 2))
+    return _SHARED_TEXTURE["tex"]
+
+
 def batched_pass(seed: int, index: int, width: int, height: int, tris: float, textured: bool) -> PassSpec:
     segs, rings = sphere_for_triangles(tris)
     s = sub_seed(seed, index)
     if textured:
-        spec = config2_textured(seed=s, width=width, height=height, segments=segs, rings=rings)
+        spec = config2_textured(seed=s, width=width, height=height, segments=segs, rings=rings,
+                                texture=shared_texture())
     else:
         pos, _, idx = displaced_sphere(segs, rings, sub_seed(s, 1))
         col = splitmix_f32(sub_seed(s, 3), len(pos) * 3).reshape(-1, 3)
