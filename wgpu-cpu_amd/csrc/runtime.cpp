@@ -49,6 +49,25 @@ static int32_t hip_fail(hipError_t e, const char *what)
         if (_e != hipSuccess) return hip_fail(_e, what); \
     } while (0)
 
+// Every transfer and fill goes through the device's own (non-blocking)
+// stream and is waited for: null-stream memsets and pageable copies are not
+// ordered with that stream and may complete after later work on it.
+static hipError_t stream_copy(hipStream_t st, void *dst, const void *src, size_t n, hipMemcpyKind kind)
+{
+    if (!n) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, n, kind, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
+}
+
+static hipError_t stream_zero(hipStream_t st, void *dst, size_t n)
+{
+    if (!n) return hipSuccess;
+    hipError_t e = hipMemsetAsync(dst, 0, n, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
+}
+
 extern "C" uint32_t wgc_abi_version(void) { return WGC_ABI_VERSION; }
 extern "C" const char *wgc_last_error(void) { return g_last_error.c_str(); }
 
@@ -349,12 +368,12 @@ extern "C" int32_t wgc_device_create(const WgcDeviceDescriptor *desc, WgcDevice 
     if (e == hipSuccess) e = hipMalloc(&d->d_error, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&d->d_frags, 2 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&d->d_srgb, 256 * sizeof(float));
-    if (e == hipSuccess) e = hipMemset(d->d_error, 0, sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(d->d_frags, 0, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = stream_zero(d->stream, d->d_error, sizeof(uint32_t));
+    if (e == hipSuccess) e = stream_zero(d->stream, d->d_frags, 2 * sizeof(unsigned long long));
     if (e == hipSuccess) {
         float lut[256];
         for (int i = 0; i < 256; ++i) lut[i] = srgb_entry(i);
-        e = hipMemcpy(d->d_srgb, lut, sizeof lut, hipMemcpyHostToDevice);
+        e = stream_copy(d->stream, d->d_srgb, lut, sizeof lut, hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
         release(d);
@@ -412,9 +431,9 @@ static int32_t resolve_timers(WgcDevice_T *d)
 static int32_t check_device_error(WgcDevice_T *d)
 {
     uint32_t err = 0;
-    HIP_TRY(hipMemcpy(&err, d->d_error, sizeof err, hipMemcpyDeviceToHost), "error readback");
+    HIP_TRY(stream_copy(d->stream, &err, d->d_error, sizeof err, hipMemcpyDeviceToHost), "error readback");
     if (!err) return WGC_OK;
-    HIP_TRY(hipMemset(d->d_error, 0, sizeof err), "error reset");
+    HIP_TRY(stream_zero(d->stream, d->d_error, sizeof err), "error reset");
     std::string msg = "render pass failed on the device:";
     if (err & ERR_INDEX) msg += " index out of the index buffer (reference: panic);";
     if (err & ERR_VERTEX) msg += " vertex fetch out of the vertex buffer (reference: panic);";
@@ -457,7 +476,7 @@ extern "C" int32_t wgc_device_last_pass_stats(WgcDevice device, WgcPassStats *ou
     int32_t r = drain(device);
     if (r != WGC_OK) return r;
     unsigned long long f[2];
-    HIP_TRY(hipMemcpy(f, device->d_frags, sizeof f, hipMemcpyDeviceToHost), "stats readback");
+    HIP_TRY(stream_copy(device->stream, f, device->d_frags, sizeof f, hipMemcpyDeviceToHost), "stats readback");
     out->fragments = f[0];
     out->primitives = device->last_prims;
     out->kernel_launches = device->last_launches;
@@ -478,7 +497,7 @@ extern "C" int32_t wgc_device_debug_fragment_counts(WgcDevice device, uint32_t w
     if (host_out) {
         if (!device->frag_px || need != device->frag_px_size)
             return set_error(WGC_ERR_INVALID, "fragment counts not enabled for this size");
-        HIP_TRY(hipMemcpy(host_out, device->frag_px, need, hipMemcpyDeviceToHost), "readback");
+        HIP_TRY(stream_copy(device->stream, host_out, device->frag_px, need, hipMemcpyDeviceToHost), "readback");
         return WGC_OK;
     }
     device->arena_gen++;  // cached plans point at the counts buffer
@@ -489,7 +508,7 @@ extern "C" int32_t wgc_device_debug_fragment_counts(WgcDevice device, uint32_t w
     }
     if (need) {
         HIP_TRY(hipMalloc(&device->frag_px, need), "hipMalloc");
-        HIP_TRY(hipMemset(device->frag_px, 0, need), "hipMemset");
+        HIP_TRY(stream_zero(device->stream, device->frag_px, need), "hipMemset");
         device->frag_px_size = need;
     }
     return WGC_OK;
@@ -509,7 +528,10 @@ extern "C" int32_t wgc_buffer_create(WgcDevice device, uint64_t size, WgcBuffer 
     b->size = size;
     if (size) {
         hipError_t e = hipMalloc(&b->ptr, size);
-        if (e == hipSuccess) e = hipMemset(b->ptr, 0, size);
+        if (e == hipSuccess) {
+            std::lock_guard<std::mutex> lk(device->mu);
+            e = stream_zero(device->stream, b->ptr, size);
+        }
         if (e != hipSuccess) {
             release(b);
             return set_error(WGC_ERR_OUT_OF_MEMORY, std::string("buffer: ") + hipGetErrorString(e));
@@ -549,7 +571,7 @@ extern "C" int32_t wgc_buffer_read(WgcBuffer buffer, uint64_t offset, void *dst,
     HIP_TRY(hipSetDevice(d->hip_device), "hipSetDevice");
     int32_t r = drain(d);
     if (r != WGC_OK) return r;
-    if (size) HIP_TRY(hipMemcpy(dst, buffer->ptr + offset, size, hipMemcpyDeviceToHost), "buffer read");
+    HIP_TRY(stream_copy(d->stream, dst, buffer->ptr + offset, size, hipMemcpyDeviceToHost), "buffer read");
     return WGC_OK;
 }
 
@@ -585,7 +607,10 @@ extern "C" int32_t wgc_texture_create(WgcDevice device, uint32_t width, uint32_t
     t->format = format;
     t->size = (uint64_t)width * height * tb;
     hipError_t e = hipMalloc(&t->ptr, t->size);
-    if (e == hipSuccess) e = hipMemset(t->ptr, 0, t->size);
+    if (e == hipSuccess) {
+        std::lock_guard<std::mutex> lk(device->mu);
+        e = stream_zero(device->stream, t->ptr, t->size);
+    }
     if (e != hipSuccess) {
         release(t);
         return set_error(WGC_ERR_OUT_OF_MEMORY, std::string("texture: ") + hipGetErrorString(e));
@@ -625,7 +650,7 @@ extern "C" int32_t wgc_texture_read(WgcTexture texture, void *dst, uint64_t size
     HIP_TRY(hipSetDevice(d->hip_device), "hipSetDevice");
     int32_t r = drain(d);
     if (r != WGC_OK) return r;
-    HIP_TRY(hipMemcpy(dst, texture->ptr, size, hipMemcpyDeviceToHost), "texture read");
+    HIP_TRY(stream_copy(d->stream, dst, texture->ptr, size, hipMemcpyDeviceToHost), "texture read");
     return WGC_OK;
 }
 
@@ -1405,7 +1430,9 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         s.batch.total_prims = acc_p;
         s.batch.total_tiles = acc_t;
     }
-    if (tables) HIP_TRY(hipMemcpy(plan.d_tables, host_tables.data(), tables, hipMemcpyHostToDevice), "draw tables");
+    if (tables)
+        HIP_TRY(stream_copy(dev->stream, plan.d_tables, host_tables.data(), tables, hipMemcpyHostToDevice),
+                "draw tables");
     plan.arena_gen = dev->arena_gen;
     plan.raster_cfg = cfg;
     plan.batching = dev->batching;
