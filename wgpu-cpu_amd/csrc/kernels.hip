@@ -1,24 +1,29 @@
 // kernels.hip -- HIP kernels of the render-pass executor for gfx950.
 //
-// Per launch group: k_zero -> k_setup -> k_clip -> k_tile_scan -> k_bin_fill
-// -> k_raster.  A group is either one draw (parameters passed by value) or
-// a table of independent draws -- the batched form, where hundreds of render
-// passes share every launch (MULTI).
+// Per launch group: k_zero -> k_setup -> k_clip -> k_scan_blocks ->
+// k_scan_totals -> k_scan_tiles -> k_bin_fill -> k_raster.  A group is one
+// draw (parameters passed by value) or a table of independent draws -- the
+// batched form, where hundreds of render passes share every launch (MULTI).
 //
 //  k_setup     one thread per primitive: vertex shader for its three indices
 //              (state.rs:519-535), winding and culling (state.rs:540-561),
-//              the clipper's pass-through test, raster setup and the tile
-//              footprint; wave-aggregated appends (ballot + prefix count)
-//              to the record list, the big-triangle list and the clip list.
+//              the clipper's pass-through test and raster setup; writes the
+//              triangle record the raster reads and the primitive's bin
+//              record, counts it into its tiles (keeping its bin positions),
+//              and appends (ballot + prefix count, one atomic per wave) to
+//              the big-triangle list and the clip list.
 //  k_clip      primitives that cross a clip plane: the 6-plane clipper as an
 //              in-order depth-first walk (same output order as the
 //              reference's plane-by-plane queue, clipper.rs:756-775).
-//  k_tile_scan exclusive scan of per-tile counts (all draws of the group).
-//  k_bin_fill  scatters record keys into per-tile lists.
-//  k_raster    one workgroup per TSxTS tile, PPT pixels per lane; colour and
-//              depth live in registers from load (or clear) to one coalesced
-//              store.  Triangles are staged 64 at a time in LDS, with per-row
-//              spans of the reference's integer scanline rule.
+//  k_scan_*    per-tile bin offsets (exclusive scan over all tiles of the
+//              group) and the compacted list of tiles the raster must visit.
+//  k_bin_fill  scatters record keys into per-tile lists (no atomics).
+//  k_raster    persistent workgroups, statically scheduled: first
+//              the active tiles (TSxTS pixels, PPT pixels per lane; colour and
+//              depth in registers from load or clear to one coalesced store;
+//              triangles staged 64 at a time in LDS with per-row spans of the
+//              reference's integer scanline rule), then chunks of untouched
+//              tiles that only need the LoadOp::Clear values stored.
 #pragma clang fp contract(off)
 
 #include "wgc_core.h"
@@ -31,21 +36,6 @@ namespace wgc {
 // ------------------------------------------------------------------
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-// Compile-time loop: f(IC<I>{}) for I in [B, E), so per-pixel arrays are only
-// ever indexed by constants and live in registers.
-template <int I>
-struct IC {
-    static constexpr int value = I;
-};
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F &&f)
-{
-    if constexpr (B < E) {
-        f(IC<B>{});
-        static_for<B + 1, E>(f);
-    }
-}
 
 template <bool MULTI>
 __device__ __forceinline__ const DrawParams &draw_at(const DrawParams &d0, const BatchArgs &b, uint32_t i)
@@ -81,38 +71,103 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
     return pred ? base + (uint32_t)__popcll(below) : 0xFFFFFFFFu;
 }
 
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t i, uint32_t n)
-{
-    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-    // contiguous band of tiles so neighbouring tiles share its L2.
-    uint32_t xcd = i & 7u, local = i >> 3;
-    uint32_t per = n >> 3, rem = n & 7u;
-    return xcd * per + min(xcd, rem) + local;
-}
-
-__device__ __forceinline__ void add_tile_counts(const DrawParams &d, uint32_t tx0, uint32_t ty0,
+// Counts a record into the bins of its footprint (row-major) and keeps its
+// position in each, so that the bins can be filled without atomics.
+__device__ __forceinline__ void add_tile_counts(const DrawParams &d, uint32_t rec, uint32_t tx0, uint32_t ty0,
                                                 uint32_t tx1, uint32_t ty1)
 {
+    uint32_t *slots = d.s.rec_slots + (size_t)rec * kMaxBinTiles;
+    uint32_t k = 0;
     for (uint32_t ty = ty0; ty <= ty1; ++ty)
-        for (uint32_t tx = tx0; tx <= tx1; ++tx) atomicAdd(&d.s.tile_count[ty * d.tiles_x + tx], 1u);
+        for (uint32_t tx = tx0; tx <= tx1; ++tx) slots[k++] = atomicAdd(&d.s.tile_count[ty * d.tiles_x + tx], 1u);
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+}
+
+// Compile-time loop: f(IC<I>{}) for I in [B, E).
+template <int I>
+struct IC {
+    static constexpr int value = I;
+};
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(IC<B>{});
+        static_for<B + 1, E>(f);
+    }
 }
 
 // ------------------------------------------------------------------
-// k_zero: tile counts of the group, per-draw counters, record counter
+// triangle records
+// ------------------------------------------------------------------
+
+__device__ __forceinline__ void write_record(uint32_t *rec, const RasterTri &r, const VsOut v[3], uint32_t key)
+{
+    ScanTri s = scan_setup(r);
+    uint32_t w[kRecWords];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        w[R_VP + 2 * k] = __float_as_uint(r.vpx[k]);
+        w[R_VP + 2 * k + 1] = __float_as_uint(r.vpy[k]);
+        w[R_Z + k] = __float_as_uint(r.z[k]);
+        w[R_IW + k] = __float_as_uint(1.0f / v[k].clip[3]);
+#pragma unroll
+        for (int c = 0; c < kMaxVar; ++c) w[R_VAR + k * kMaxVar + c] = __float_as_uint(v[k].var[c]);
+    }
+    // constant terms of the three shoelace sums (the same products the
+    // per-pixel evaluation would form)
+    w[R_K + 0] = __float_as_uint((r.vpy[2] - r.vpy[1]) * (r.vpx[2] + r.vpx[1]));
+    w[R_K + 1] = __float_as_uint((r.vpy[0] - r.vpy[2]) * (r.vpx[0] + r.vpx[2]));
+    w[R_K + 2] = __float_as_uint((r.vpy[1] - r.vpy[0]) * (r.vpx[1] + r.vpx[0]));
+    w[R_K + 3] = __float_as_uint(r.total);
+    w[R_SCAN + 0] = (uint32_t)s.ax; w[R_SCAN + 1] = (uint32_t)s.ay;
+    w[R_SCAN + 2] = (uint32_t)s.bx; w[R_SCAN + 3] = (uint32_t)s.by;
+    w[R_SCAN + 4] = (uint32_t)s.cx; w[R_SCAN + 5] = (uint32_t)s.cy;
+    w[R_MODE] = (uint32_t)s.mode | ((scan_wide(r) ? 1u : 0u) << 8);
+    w[R_KEY] = key;
+    uint4 *o = reinterpret_cast<uint4 *>(rec);
+#pragma unroll
+    for (int q = 0; q < kRecWords / 4; ++q) o[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+}
+
+// ------------------------------------------------------------------
+// k_zero: tile counts of the group, per-draw counters, group header
 // ------------------------------------------------------------------
 
 __global__ void k_zero(uint32_t *tile_count, uint32_t n_tiles, uint32_t *counters, uint32_t n_counters,
-                       uint32_t *rec_count)
+                       uint32_t *header)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_tiles) tile_count[i] = 0u;
     if (i < n_counters) counters[i] = 0u;
-    if (i == 0) *rec_count = 0u;
+    if (i < H_NUM) header[i] = 0u;
 }
 
 // ------------------------------------------------------------------
 // k_setup
 // ------------------------------------------------------------------
+
+// Grows the draw's big-triangle footprint union; kept as maxima of
+// (65535 - min) and max.  Every lane of the wave calls it (same draw).
+__device__ __forceinline__ void note_big_wave(const DrawParams &d, bool big, const uint4 &rec)
+{
+    if (!__ballot(big)) return;
+    uint32_t nx0 = wave_max(big ? 65535u - (rec.y & 0xFFFFu) : 0u);
+    uint32_t ny0 = wave_max(big ? 65535u - (rec.y >> 16) : 0u);
+    uint32_t x1 = wave_max(big ? (rec.z & 0xFFFFu) : 0u);
+    uint32_t y1 = wave_max(big ? (rec.z >> 16) : 0u);
+    if (lane_id() == 0) {
+        atomicMax(&d.s.counters[C_BIG_NX0], nx0);
+        atomicMax(&d.s.counters[C_BIG_NY0], ny0);
+        atomicMax(&d.s.counters[C_BIG_X1], x1);
+        atomicMax(&d.s.counters[C_BIG_Y1], y1);
+    }
+}
 
 template <int PROG, bool MULTI>
 __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
@@ -126,6 +181,9 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
         p = g - b.prim_base[di];
     }
     const DrawParams &d = draw_at<MULTI>(d0, b, di);
+    if constexpr (!MULTI) {
+        if (g >= d.total_prims) return;  // whole waves past the end only matter in the batched form
+    }
     bool active = p < d.total_prims;
     bool has_rec = false, needs_clip = false;
     uint4 rec = make_uint4(0, 0, 0, 0);
@@ -154,6 +212,7 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
                             has_rec = true;
                             rec = make_uint4(p << 7, tx0 | (ty0 << 16), tx1 | (ty1 << 16), di);
                             ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+                            write_record(d.s.tri + (size_t)p * kRecWords, r, v, p << 7);
                         }
                     }
                 } else {
@@ -164,25 +223,20 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
     }
     bool small = has_rec && ntiles <= (uint32_t)kMaxBinTiles;
     bool big = has_rec && !small;
-    uint32_t slot = wave_append(&d.s.counters[C_CLIP_LIST], needs_clip);
+    uint32_t slot = wave_append(&d.s.header[H_CLIP], needs_clip);
     if (needs_clip) {
-        if (slot < d.s.clip_list_cap) d.s.clip_list[slot] = p;
+        if (slot < d.s.clip_list_cap) d.s.clip_list[slot] = make_uint2(di, p);
         else atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
     }
-    slot = wave_append(d.s.rec_count, small);
-    if (small) {
-        if (slot < d.s.rec_cap) {
-            d.s.recs[slot] = rec;
-            add_tile_counts(d, rec.y & 0xFFFFu, rec.y >> 16, rec.z & 0xFFFFu, rec.z >> 16);
-        } else {
-            atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
-        }
-    }
+    // the per-primitive record slot: an empty footprint unless binned here
+    d.s.recs[g] = small ? rec : make_uint4(0u, 0xFFFFFFFFu, 0u, di);
+    if (small) add_tile_counts(d, g, rec.y & 0xFFFFu, rec.y >> 16, rec.z & 0xFFFFu, rec.z >> 16);
     slot = wave_append(&d.s.counters[C_BIG], big);
     if (big) {
         if (slot < d.s.big_cap) d.s.big[slot] = rec;
         else atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
     }
+    note_big_wave(d, big && slot < d.s.big_cap, rec);
 }
 
 // ------------------------------------------------------------------
@@ -237,6 +291,10 @@ __device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p)
     }
     int nleaf = clip_walk(root, [](const CTri &, int) {});
     if (nleaf == 0) return;
+    if (nleaf > 63) {  // the record key holds 6 bits of sub-triangle index
+        atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
+        return;
+    }
     uint32_t base = atomicAdd(&d.s.counters[C_CLIP_TRI], (uint32_t)nleaf);
     if (base + (uint32_t)nleaf > d.s.clip_tri_cap) {
         atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
@@ -244,7 +302,8 @@ __device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p)
     }
     d.s.clip_base[p] = base;
     clip_walk(root, [&](const CTri &t, int leaf) {
-        float *o = d.s.clip_tris + (size_t)(base + (uint32_t)leaf) * kClipFloats;
+        uint32_t slot = base + (uint32_t)leaf;
+        float *o = d.s.clip_tris + (size_t)slot * kClipFloats;
         for (int k = 0; k < 3; ++k) {
             for (int c = 0; c < 4; ++c) o[k * 7 + c] = t.p[k][c];
             for (int c = 0; c < 3; ++c) o[k * 7 + 4 + c] = t.b[k][c];
@@ -256,89 +315,193 @@ __device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p)
         ScanTri s = scan_setup(r);
         uint32_t tx0, ty0, tx1, ty1;
         if (!tile_bbox(d, s, tx0, ty0, tx1, ty1)) return;
-        uint4 rec = make_uint4((p << 7) | ((uint32_t)leaf << 1) | 1u, tx0 | (ty0 << 16),
-                               tx1 | (ty1 << 16), di);
+        uint32_t key = (p << 7) | ((uint32_t)leaf << 1) | 1u;
+        write_record(d.s.clip_tri + (size_t)slot * kRecWords, r, v, key);
+        uint4 rec = make_uint4(key, tx0 | (ty0 << 16), tx1 | (ty1 << 16), di);
         uint32_t nt = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
         if (nt <= (uint32_t)kMaxBinTiles) {
-            uint32_t slot = atomicAdd(d.s.rec_count, 1u);
-            if (slot < d.s.rec_cap) {
-                d.s.recs[slot] = rec;
-                add_tile_counts(d, tx0, ty0, tx1, ty1);
+            uint32_t rs = atomicAdd(&d.s.header[H_REC], 1u);
+            if (rs < d.s.rec_clip_cap) {
+                uint32_t ri = d.s.rec_prims + rs;
+                d.s.recs[ri] = rec;
+                add_tile_counts(d, ri, tx0, ty0, tx1, ty1);
             } else {
                 atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
             }
         } else {
-            uint32_t slot = atomicAdd(&d.s.counters[C_BIG], 1u);
-            if (slot < d.s.big_cap) d.s.big[slot] = rec;
-            else atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
+            uint32_t bs = atomicAdd(&d.s.counters[C_BIG], 1u);
+            if (bs < d.s.big_cap) {
+                d.s.big[bs] = rec;
+                atomicMax(&d.s.counters[C_BIG_NX0], 65535u - tx0);
+                atomicMax(&d.s.counters[C_BIG_NY0], 65535u - ty0);
+                atomicMax(&d.s.counters[C_BIG_X1], tx1);
+                atomicMax(&d.s.counters[C_BIG_Y1], ty1);
+            } else {
+                atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
+            }
         }
     });
 }
 
 template <int PROG, bool MULTI>
-__global__ __launch_bounds__(256) void k_clip(DrawParams d0, BatchArgs b)
+__global__ __launch_bounds__(256) void k_clip(DrawParams d0, BatchArgs b, const uint2 *clip_list, uint32_t cap)
 {
-    uint32_t total = MULTI ? b.total_prims : d0.total_prims;
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
-        uint32_t di = 0, i = g;
-        if constexpr (MULTI) {
-            di = find_slot(b.prim_base, b.n, g);
-            i = g - b.prim_base[di];
-        }
-        const DrawParams &d = draw_at<MULTI>(d0, b, di);
-        uint32_t n = min(d.s.counters[C_CLIP_LIST], d.s.clip_list_cap);
-        if (i < n) clip_primitive<PROG>(d, di, d.s.clip_list[i]);
+    uint32_t n = min(b.header[H_CLIP], cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint2 e = clip_list[i];
+        clip_primitive<PROG>(draw_at<MULTI>(d0, b, e.x), e.x, e.y);
     }
 }
 
 // ------------------------------------------------------------------
-// k_tile_scan: one workgroup of 1024 threads
+// k_scan_*: bin offsets and the active-tile list
 // ------------------------------------------------------------------
 
-__global__ __launch_bounds__(1024) void k_tile_scan(uint32_t *tile_count, uint32_t *tile_offset,
-                                                    uint32_t *tile_fill, uint32_t n)
+// Whether the raster must visit a tile: binned triangles, or inside the
+// footprint union of the draw's big triangles.
+template <bool MULTI>
+__device__ __forceinline__ bool tile_is_active(const DrawParams &d0, const BatchArgs &b, uint32_t gt, uint32_t count)
 {
-    __shared__ uint32_t sums[1024];
+    if (count) return true;
+    uint32_t di = 0, t = gt;
+    if constexpr (MULTI) {
+        di = find_slot(b.tile_base, b.n, gt);
+        t = gt - b.tile_base[di];
+    }
+    const DrawParams &d = draw_at<MULTI>(d0, b, di);
+    if (d.s.counters[C_BIG] == 0) return false;
+    uint32_t tx = t % d.tiles_x, ty = t / d.tiles_x;
+    return tx >= 65535u - d.s.counters[C_BIG_NX0] && tx <= d.s.counters[C_BIG_X1] &&
+           ty >= 65535u - d.s.counters[C_BIG_NY0] && ty <= d.s.counters[C_BIG_Y1];
+}
+
+// Block-wide exclusive scan of two values per thread (256 threads).
+__device__ __forceinline__ void block_scan2(uint32_t &a, uint32_t &c, uint32_t &tot_a, uint32_t &tot_c)
+{
+    __shared__ uint32_t sa[256], sc[256];
     uint32_t t = threadIdx.x;
-    uint32_t chunk = (n + 1023u) / 1024u;
-    uint32_t b = t * chunk, e = min(b + chunk, n);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += tile_count[i];
-    sums[t] = s;
+    sa[t] = a;
+    sc[t] = c;
     __syncthreads();
-    for (uint32_t off = 1; off < 1024u; off <<= 1) {
-        uint32_t v = (t >= off) ? sums[t - off] : 0u;
+    for (uint32_t off = 1; off < 256u; off <<= 1) {
+        uint32_t va = (t >= off) ? sa[t - off] : 0u;
+        uint32_t vc = (t >= off) ? sc[t - off] : 0u;
         __syncthreads();
-        sums[t] += v;
+        sa[t] += va;
+        sc[t] += vc;
         __syncthreads();
     }
-    uint32_t run = sums[t] - s;
-    for (uint32_t i = b; i < e; ++i) {
-        tile_offset[i] = run;
-        run += tile_count[i];
-        tile_fill[i] = 0u;
+    tot_a = sa[255];
+    tot_c = sc[255];
+    a = sa[t] - a;
+    c = sc[t] - c;
+    __syncthreads();
+}
+
+// Phase 1: per block of kScanBlock tiles, the bin count and active count.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_scan_blocks(DrawParams d0, BatchArgs b)
+{
+    uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * (kScanBlock / 256);
+    uint32_t cnt = 0, act = 0;
+    for (uint32_t k = 0; k < kScanBlock / 256; ++k) {
+        uint32_t gt = base + k;
+        if (gt >= b.total_tiles) break;
+        uint32_t c = b.tile_count[gt];
+        cnt += c;
+        act += tile_is_active<MULTI>(d0, b, gt, c) ? 1u : 0u;
     }
-    if (t == 1023u) tile_offset[n] = sums[1023];
+    uint32_t ta, tc;
+    block_scan2(cnt, act, ta, tc);
+    if (threadIdx.x == 0) {
+        b.block_sums[2 * blockIdx.x] = ta;
+        b.block_sums[2 * blockIdx.x + 1] = tc;
+    }
+}
+
+// Phase 2: one workgroup scans the block sums.
+__global__ __launch_bounds__(256) void k_scan_totals(BatchArgs b)
+{
+    uint32_t run_c = 0, run_a = 0;
+    for (uint32_t base = 0; base < b.n_blocks; base += 256) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t c = i < b.n_blocks ? b.block_sums[2 * i] : 0u;
+        uint32_t a = i < b.n_blocks ? b.block_sums[2 * i + 1] : 0u;
+        uint32_t tc, ta;
+        block_scan2(c, a, tc, ta);
+        if (i < b.n_blocks) {
+            b.block_sums[2 * i] = run_c + c;
+            b.block_sums[2 * i + 1] = run_a + a;
+        }
+        run_c += tc;
+        run_a += ta;
+    }
+    if (threadIdx.x == 0) {
+        b.tile_offset[b.total_tiles] = run_c;
+        b.header[H_ACTIVE] = run_a;
+    }
+}
+
+// Phase 3: per tile, its bin offset, the active flag and list slot.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_scan_tiles(DrawParams d0, BatchArgs b)
+{
+    constexpr uint32_t PER = kScanBlock / 256;
+    uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * PER;
+    uint32_t c[PER];
+    bool a[PER];
+    uint32_t cnt = 0, act = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        uint32_t gt = base + k;
+        c[k] = 0u;
+        a[k] = false;
+        if (gt < b.total_tiles) {
+            c[k] = b.tile_count[gt];
+            a[k] = tile_is_active<MULTI>(d0, b, gt, c[k]);
+        }
+        cnt += c[k];
+        act += a[k] ? 1u : 0u;
+    }
+    uint32_t ta, tc;
+    block_scan2(cnt, act, ta, tc);
+    uint32_t off = b.block_sums[2 * blockIdx.x] + cnt;
+    uint32_t aoff = b.block_sums[2 * blockIdx.x + 1] + act;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        uint32_t gt = base + k;
+        if (gt < b.total_tiles) {
+            b.tile_offset[gt] = off;
+            b.tile_active[gt] = a[k] ? 1u : 0u;
+            off += c[k];
+            if (a[k]) b.active_list[aoff++] = gt;
+        }
+    }
 }
 
 // ------------------------------------------------------------------
 // k_bin_fill
 // ------------------------------------------------------------------
 
+// Scatters every record's key into the bins of its footprint, at the
+// positions taken when the tiles were counted.
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_bin_fill(DrawParams d0, BatchArgs b, const uint4 *recs,
-                                                  const uint32_t *rec_count, uint32_t rec_cap, uint32_t *bins)
+                                                  const uint32_t *rec_slots, uint32_t rec_prims,
+                                                  uint32_t rec_clip_cap, uint32_t *bins)
 {
-    uint32_t n = min(*rec_count, rec_cap);
+    uint32_t n = rec_prims + min(b.header[H_REC], rec_clip_cap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint4 r = recs[i];
-        const DrawParams &d = draw_at<MULTI>(d0, b, r.w);
         uint32_t tx0 = r.y & 0xFFFFu, ty0 = r.y >> 16, tx1 = r.z & 0xFFFFu, ty1 = r.z >> 16;
+        if (tx0 > tx1) continue;
+        const DrawParams &d = draw_at<MULTI>(d0, b, r.w);
+        const uint32_t *slots = rec_slots + (size_t)i * kMaxBinTiles;
+        uint32_t k = 0;
         for (uint32_t ty = ty0; ty <= ty1; ++ty)
             for (uint32_t tx = tx0; tx <= tx1; ++tx) {
                 uint32_t t = ty * d.tiles_x + tx;
-                uint32_t slot = atomicAdd(&d.s.tile_fill[t], 1u);
-                bins[d.s.tile_offset[t] + slot] = r.x;
+                bins[d.s.tile_offset[t] + slots[k++]] = r.x;
             }
     }
 }
@@ -355,7 +518,7 @@ struct TriLds {
     float var[3][kMaxVar];
     float iw[3];
     uint32_t key;
-    uint32_t valid;
+    uint32_t pad;
 };
 
 template <int TS>
@@ -368,61 +531,52 @@ struct RasterShared {
     uint32_t nkeys;
 };
 
-// Stage one list entry (thread-local): the vertex shader for the unclipped
-// vertices (varyings, 1/w), the clipped sub-triangle when the key says so,
-// raster setup and the scanline case.
-template <int PROG>
-__device__ void stage_entry(const DrawParams &d, uint32_t key, TriLds &T, ScanTri &S, uint32_t &wide)
+// Loads one list entry's triangle record into LDS.
+__device__ __forceinline__ void stage_record(const DrawParams &d, uint32_t key, TriLds &T, ScanTri &S, uint32_t &wide)
 {
     uint32_t p = key >> 7, sub = (key >> 1) & 63u;
     bool clipped = key & 1u;
-    VsOut v[3];
-    T.key = key;
-    T.valid = 0u;
-    wide = 0u;
-    if (!shade_primitive<PROG>(d, p, v)) return;
-    float pos[3][4];
-    float cb[3][3];
-    if (clipped) {
-        const float *ct = d.s.clip_tris + (size_t)(d.s.clip_base[p] + sub) * kClipFloats;
+    uint32_t slot = clipped ? d.s.clip_base[p] + sub : 0u;
+    const uint4 *r = reinterpret_cast<const uint4 *>(clipped ? d.s.clip_tri + (size_t)slot * kRecWords
+                                                             : d.s.tri + (size_t)p * kRecWords);
+    uint32_t w[kRecWords];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) pos[k][c] = ct[k * 7 + c];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) cb[k][c] = ct[k * 7 + 4 + c];
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) pos[k][c] = v[k].clip[c];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) cb[k][c] = (c == k) ? 1.0f : 0.0f;
-        }
+    for (int q = 0; q < kRecWords / 4; ++q) {
+        uint4 v = r[q];
+        w[4 * q] = v.x; w[4 * q + 1] = v.y; w[4 * q + 2] = v.z; w[4 * q + 3] = v.w;
     }
-    RasterTri r;
-    raster_setup(d, pos, r);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        T.vpx[k] = r.vpx[k];
-        T.vpy[k] = r.vpy[k];
-        T.z[k] = r.z[k];
-        T.iw[k] = 1.0f / v[k].clip[3];
+        T.vpx[k] = __uint_as_float(w[R_VP + 2 * k]);
+        T.vpy[k] = __uint_as_float(w[R_VP + 2 * k + 1]);
+        T.z[k] = __uint_as_float(w[R_Z + k]);
+        T.iw[k] = __uint_as_float(w[R_IW + k]);
 #pragma unroll
-        for (int c = 0; c < kMaxVar; ++c) T.var[k][c] = v[k].var[c];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) T.cb[k * 3 + c] = cb[k][c];
+        for (int c = 0; c < kMaxVar; ++c) T.var[k][c] = __uint_as_float(w[R_VAR + k * kMaxVar + c]);
     }
-    T.total = r.total;
-    // constant terms of the three shoelace sums (the same products the
-    // per-pixel evaluation would form)
-    T.k0 = (r.vpy[2] - r.vpy[1]) * (r.vpx[2] + r.vpx[1]);
-    T.k1 = (r.vpy[0] - r.vpy[2]) * (r.vpx[0] + r.vpx[2]);
-    T.k2 = (r.vpy[1] - r.vpy[0]) * (r.vpx[1] + r.vpx[0]);
-    S = scan_setup(r);
-    wide = scan_wide(r) ? 1u : 0u;
-    T.valid = (r.total != 0.0f && coord_ok(r)) ? 1u : 0u;
+    T.k0 = __uint_as_float(w[R_K + 0]);
+    T.k1 = __uint_as_float(w[R_K + 1]);
+    T.k2 = __uint_as_float(w[R_K + 2]);
+    T.total = __uint_as_float(w[R_K + 3]);
+    T.key = key;
+    if (clipped) {
+        const float *ct = d.s.clip_tris + (size_t)slot * kClipFloats;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) T.cb[k * 3 + c] = ct[k * 7 + 4 + c];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) T.cb[k * 3 + c] = (c == k) ? 1.0f : 0.0f;
+    }
+    S.ax = (int32_t)w[R_SCAN + 0]; S.ay = (int32_t)w[R_SCAN + 1];
+    S.bx = (int32_t)w[R_SCAN + 2]; S.by = (int32_t)w[R_SCAN + 3];
+    S.cx = (int32_t)w[R_SCAN + 4]; S.cy = (int32_t)w[R_SCAN + 5];
+    S.mode = (int32_t)(w[R_MODE] & 0xFFu);
+    S.pad = 0;
+    wide = (w[R_MODE] >> 8) & 1u;
 }
 
 struct PixelState {
@@ -521,12 +675,13 @@ __device__ __forceinline__ void shade_fragment(const DrawParams &d, const TriLds
 
 // Stages sh.keys[0..n) into LDS and applies them to this lane's pixels.
 template <int PROG, bool ORDERED, int TS, int NT>
-__device__ __forceinline__ void raster_batch(const DrawParams &d, RasterShared<TS> &sh, uint32_t n, uint32_t x0, uint32_t y0,
-                             PixelState (&ps)[TS * TS / NT], uint32_t (&nfrag)[TS * TS / NT])
+__device__ __forceinline__ void raster_batch(const DrawParams &d, RasterShared<TS> &sh, uint32_t n, uint32_t x0,
+                                             uint32_t y0, PixelState (&ps)[TS * TS / NT],
+                                             uint32_t (&nfrag)[TS * TS / NT])
 {
     constexpr int PPT = TS * TS / NT;
     uint32_t tid = threadIdx.x;
-    if (tid < n) stage_entry<PROG>(d, sh.keys[tid], sh.tri[tid], sh.scan[tid], sh.wide[tid]);
+    if (tid < n) stage_record(d, sh.keys[tid], sh.tri[tid], sh.scan[tid], sh.wide[tid]);
     __syncthreads();
     // per-row spans inside the scissor and the tile, tile-relative
     int64_t sx0 = d.scissor[0], sy0 = d.scissor[1];
@@ -536,8 +691,7 @@ __device__ __forceinline__ void raster_batch(const DrawParams &d, RasterShared<T
         int32_t y = (int32_t)(y0 + row);
         uint32_t span = 1u;  // empty: lo = 1, hi = 0
         int32_t x1, x2;
-        if (sh.tri[e].valid && (int64_t)y >= sy0 && (int64_t)y < sy1 &&
-            scan_span(sh.scan[e], y, x1, x2, sh.wide[e] != 0u)) {
+        if ((int64_t)y >= sy0 && (int64_t)y < sy1 && scan_span(sh.scan[e], y, x1, x2, sh.wide[e] != 0u)) {
             int64_t lo = max((int64_t)x1, max(sx0, (int64_t)x0));
             int64_t hi = min((int64_t)x2, min(sx1 - 1, (int64_t)x0 + TS - 1));
             if (lo <= hi) span = (uint32_t)(lo - x0) | ((uint32_t)(hi - x0) << 16);
@@ -545,7 +699,7 @@ __device__ __forceinline__ void raster_batch(const DrawParams &d, RasterShared<T
         sh.span[e][row] = span;
     }
     __syncthreads();
-    // pixels outer (unrolled, so the per-pixel state stays in registers),
+    // pixels outer (compile-time, so the per-pixel state stays in registers),
     // triangles inner in list order
     static_for<0, PPT>([&](auto kc) __attribute__((always_inline)) {
         constexpr int k = decltype(kc)::value;
@@ -562,31 +716,18 @@ __device__ __forceinline__ void raster_batch(const DrawParams &d, RasterShared<T
     __syncthreads();
 }
 
-template <int PROG, bool ORDERED, bool MULTI, int TS, int NT>
-__global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
+// One active tile: load or clear, apply its triangles, store.
+template <int PROG, bool ORDERED, int TS, int NT>
+__device__ __forceinline__ void raster_tile(const DrawParams &d, uint32_t tile, RasterShared<TS> &sh,
+                                            uint32_t *olist, uint32_t &olist_n, uint32_t &frag_acc)
 {
     constexpr int PPT = TS * TS / NT;
-    static_assert(NT >= kBatch && PPT >= 1 && TS * TS == PPT * NT, "tile shape");
-    __shared__ RasterShared<TS> sh;
-    __shared__ uint32_t olist[ORDERED ? kOrderedCap : 1];
-    __shared__ uint32_t olist_n;
-
-    uint32_t n_tiles = MULTI ? b.total_tiles : d0.tiles_x * d0.tiles_y;
-    uint32_t gt = xcd_tile(blockIdx.x, n_tiles);
-    uint32_t di = 0, tile = gt;
-    if constexpr (MULTI) {
-        di = find_slot(b.tile_base, b.n, gt);
-        tile = gt - b.tile_base[di];
-    }
-    const DrawParams &d = draw_at<MULTI>(d0, b, di);
     uint32_t tx = tile % d.tiles_x, ty = tile / d.tiles_x;
     uint32_t x0 = tx * TS, y0 = ty * TS;
     uint32_t tid = threadIdx.x;
-
     uint32_t seg0 = d.s.tile_offset[tile];
     uint32_t seg_n = d.s.tile_offset[tile + 1] - seg0;
     uint32_t nbig = min(d.s.counters[C_BIG], d.s.big_cap);
-    if (!d.init_clear && seg_n == 0 && nbig == 0) return;
 
     PixelState ps[PPT];
     uint32_t init_col[PPT];
@@ -668,9 +809,8 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
         }
     }
 
-    // One loop, one call site (so the per-pixel state stays in registers):
-    // ordered -> the sorted list; otherwise the tile's bin segment, then the
-    // big-triangle list filtered by tile overlap.
+    // One loop, one call site: ordered -> the sorted list; otherwise the
+    // tile's bin segment, then the big-triangle list filtered by overlap.
     uint32_t fpos = 0, bpos = 0;
     for (;;) {
         uint32_t n;
@@ -731,9 +871,77 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
         if (d.depth) d.depth[pix] = write_depth ? ps[k].dep : ps[k].init_dep;
         if (d.s.frag_px && nfrag[k]) d.s.frag_px[pix] += nfrag[k];
     });
-    // fragment count: wave reduction, one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) f += __shfl_xor(f, off);
-    if (lane_id() == 0 && f) atomicAdd(d.s.frags, (unsigned long long)f);
+    frag_acc += f;
+}
+
+// Stores the clear values into the untouched tiles of a chunk.
+template <bool MULTI, int TS, int NT>
+__device__ __forceinline__ void clear_chunk(const DrawParams &d0, const BatchArgs &b, uint32_t chunk)
+{
+    constexpr int PPT = TS * TS / NT;
+    uint32_t g0 = chunk * kClearChunk, g1 = min(g0 + kClearChunk, b.total_tiles);
+    for (uint32_t gt = g0; gt < g1; ++gt) {
+        if (b.tile_active[gt]) continue;
+        uint32_t di = 0, tile = gt;
+        if constexpr (MULTI) {
+            di = find_slot(b.tile_base, b.n, gt);
+            tile = gt - b.tile_base[di];
+        }
+        const DrawParams &d = draw_at<MULTI>(d0, b, di);
+        if (!d.init_clear) continue;
+        uint32_t x0 = (tile % d.tiles_x) * TS, y0 = (tile / d.tiles_x) * TS;
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            uint32_t q = threadIdx.x + (uint32_t)k * NT;
+            uint32_t px = x0 + q % TS, py = y0 + q / TS;
+            if (px >= d.width || py >= d.height) continue;
+            size_t pix = (size_t)py * d.width + px;
+            if (d.color && d.clear_color_on) reinterpret_cast<uint32_t *>(d.color)[pix] = d.clear_rgba;
+            if (d.depth && d.clear_depth_on) d.depth[pix] = d.clear_depth;
+        }
+    }
+}
+
+// Persistent, statically scheduled (no shared work counter): the active
+// tiles are cut into 8 contiguous ranges, one per XCD (workgroup i runs on
+// XCD i mod 8 and the grid is a multiple of 8), so that neighbouring tiles
+// share an L2; the clear chunks are strided over the whole grid.  The
+// fragment count is reduced per workgroup into one of kFragSlots counters.
+template <int PROG, bool ORDERED, bool MULTI, int TS, int NT>
+__global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
+{
+    constexpr int PPT = TS * TS / NT;
+    static_assert(NT >= kBatch && PPT >= 1 && TS * TS == PPT * NT, "tile shape");
+    __shared__ RasterShared<TS> sh;
+    __shared__ uint32_t olist[ORDERED ? kOrderedCap : 1];
+    __shared__ uint32_t olist_n;
+    __shared__ uint32_t wave_frags[NT / 64];
+
+    uint32_t n_active = b.header[H_ACTIVE];
+    uint32_t per_xcd = (n_active + 7u) / 8u;
+    uint32_t xcd = blockIdx.x & 7u, lw = blockIdx.x >> 3, nw = gridDim.x >> 3;
+    uint32_t frag_acc = 0;
+    for (uint32_t i = lw; i < per_xcd; i += nw) {
+        uint32_t item = xcd * per_xcd + i;
+        if (item >= n_active) break;
+        uint32_t gt = b.active_list[item];
+        uint32_t di = 0, tile = gt;
+        if constexpr (MULTI) {
+            di = find_slot(b.tile_base, b.n, gt);
+            tile = gt - b.tile_base[di];
+        }
+        raster_tile<PROG, ORDERED, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, olist, olist_n, frag_acc);
+    }
+    for (uint32_t c = blockIdx.x; c < b.n_chunks; c += gridDim.x) clear_chunk<MULTI, TS, NT>(d0, b, c);
+
+    for (int off = 32; off > 0; off >>= 1) frag_acc += __shfl_xor(frag_acc, off);
+    if (lane_id() == 0) wave_frags[threadIdx.x >> 6] = frag_acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t f = 0;
+        for (int w = 0; w < NT / 64; ++w) f += wave_frags[w];
+        if (f) atomicAdd(draw_at<MULTI>(d0, b, 0).s.frags + (blockIdx.x % kFragSlots), (unsigned long long)f);
+    }
 }
 
 // ------------------------------------------------------------------
@@ -742,7 +950,7 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
 
 __global__ void k_reset(unsigned long long *frags)
 {
-    if (threadIdx.x == 0) frags[0] = 0ull;
+    if (threadIdx.x < kFragSlots) frags[threadIdx.x] = 0ull;
 }
 
 __global__ void k_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, uint64_t n)
@@ -767,18 +975,18 @@ __global__ void k_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, 
     } while (0)
 
 template <int PROG, bool ORDERED, bool MULTI>
-static hipError_t launch_raster(const DrawParams &d, const BatchArgs &b, uint32_t tiles, uint32_t cfg,
+static hipError_t launch_raster(const DrawParams &d, const BatchArgs &b, uint32_t grid, uint32_t cfg,
                                 hipStream_t st, KernelTimer *kt)
 {
     if (ORDERED) {
-        WGC_LAUNCH("k_raster", (k_raster<PROG, true, MULTI, 16, 256>), tiles, 256, d, b);
+        WGC_LAUNCH("k_raster", (k_raster<PROG, true, MULTI, 16, 256>), grid, 256, d, b);
         return hipSuccess;
     }
     switch (cfg) {
-    case RASTER_8x8_64: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 8, 64>), tiles, 64, d, b); break;
-    case RASTER_16x16_64: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 16, 64>), tiles, 64, d, b); break;
-    case RASTER_32x32_256: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 32, 256>), tiles, 256, d, b); break;
-    default: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 16, 256>), tiles, 256, d, b); break;
+    case RASTER_8x8_64: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 8, 64>), grid, 64, d, b); break;
+    case RASTER_16x16_64: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 16, 64>), grid, 64, d, b); break;
+    case RASTER_32x32_256: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 32, 256>), grid, 256, d, b); break;
+    default: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 16, 256>), grid, 256, d, b); break;
     }
     return hipSuccess;
 }
@@ -787,21 +995,24 @@ template <int PROG, bool MULTI>
 static hipError_t launch_group(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
                                KernelTimer *kt)
 {
-    uint32_t zero_n = max(max(gl.total_tiles, gl.n_counters), 1u);
+    uint32_t zero_n = max(max(gl.total_tiles, gl.n_counters), (uint32_t)H_NUM);
     WGC_LAUNCH("k_zero", k_zero, (zero_n + 255) / 256, 256, gl.tile_count, gl.total_tiles, gl.counters,
-               gl.n_counters, gl.rec_count);
+               gl.n_counters, gl.header);
     if (gl.total_prims > 0) {
         uint32_t g = (gl.total_prims + 255u) / 256u;
         WGC_LAUNCH("k_setup", (k_setup<PROG, MULTI>), g, 256, d, b);
-        WGC_LAUNCH("k_clip", (k_clip<PROG, MULTI>), min(g, 1024u), 256, d, b);
+        WGC_LAUNCH("k_clip", (k_clip<PROG, MULTI>), 64, 256, d, b, gl.clip_list, gl.clip_cap);
     }
-    WGC_LAUNCH("k_tile_scan", k_tile_scan, 1, 1024, gl.tile_count, gl.tile_offset, gl.tile_fill, gl.total_tiles);
+    WGC_LAUNCH("k_scan_blocks", k_scan_blocks<MULTI>, b.n_blocks, 256, d, b);
+    WGC_LAUNCH("k_scan_totals", k_scan_totals, 1, 256, b);
+    WGC_LAUNCH("k_scan_tiles", k_scan_tiles<MULTI>, b.n_blocks, 256, d, b);
     if (gl.total_prims > 0) {
-        uint32_t g = min((gl.rec_cap + 255u) / 256u, 4096u);
-        WGC_LAUNCH("k_bin_fill", k_bin_fill<MULTI>, g, 256, d, b, gl.recs, gl.rec_count, gl.rec_cap, gl.bins);
+        uint32_t g = min((gl.rec_prims + gl.rec_clip_cap + 255u) / 256u, 4096u);
+        WGC_LAUNCH("k_bin_fill", k_bin_fill<MULTI>, g, 256, d, b, gl.recs, gl.rec_slots, gl.rec_prims,
+                   gl.rec_clip_cap, gl.bins);
     }
-    if (gl.ordered) return launch_raster<PROG, true, MULTI>(d, b, gl.total_tiles, gl.raster_cfg, st, kt);
-    return launch_raster<PROG, false, MULTI>(d, b, gl.total_tiles, gl.raster_cfg, st, kt);
+    if (gl.ordered) return launch_raster<PROG, true, MULTI>(d, b, gl.raster_grid, gl.raster_cfg, st, kt);
+    return launch_raster<PROG, false, MULTI>(d, b, gl.raster_grid, gl.raster_cfg, st, kt);
 }
 
 template <bool MULTI>
@@ -817,16 +1028,16 @@ static hipError_t launch_group_prog(const DrawParams &d, const BatchArgs &b, con
     }
 }
 
-hipError_t launch_draw(const DrawParams &d, const GroupLaunch &gl, hipStream_t st, KernelTimer *kt)
+hipError_t launch_draw(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
+                       KernelTimer *kt)
 {
-    BatchArgs b{};
     return launch_group_prog<false>(d, b, gl, st, kt);
 }
 
-hipError_t launch_batch(const BatchArgs &b, const GroupLaunch &gl, hipStream_t st, KernelTimer *kt)
+hipError_t launch_batch(const DrawParams &d0, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
+                        KernelTimer *kt)
 {
-    DrawParams d{};
-    return launch_group_prog<true>(d, b, gl, st, kt);
+    return launch_group_prog<true>(d0, b, gl, st, kt);
 }
 
 hipError_t launch_reset(unsigned long long *frags, hipStream_t st)

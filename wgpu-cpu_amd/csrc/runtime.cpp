@@ -366,10 +366,10 @@ extern "C" int32_t wgc_device_create(const WgcDeviceDescriptor *desc, WgcDevice 
     hipError_t e = hipSetDevice(d->hip_device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&d->d_error, sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&d->d_frags, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&d->d_frags, kFragSlots * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&d->d_srgb, 256 * sizeof(float));
     if (e == hipSuccess) e = stream_zero(d->stream, d->d_error, sizeof(uint32_t));
-    if (e == hipSuccess) e = stream_zero(d->stream, d->d_frags, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = stream_zero(d->stream, d->d_frags, kFragSlots * sizeof(unsigned long long));
     if (e == hipSuccess) {
         float lut[256];
         for (int i = 0; i < 256; ++i) lut[i] = srgb_entry(i);
@@ -475,9 +475,10 @@ extern "C" int32_t wgc_device_last_pass_stats(WgcDevice device, WgcPassStats *ou
     HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
     int32_t r = drain(device);
     if (r != WGC_OK) return r;
-    unsigned long long f[2];
+    unsigned long long f[kFragSlots];
     HIP_TRY(stream_copy(device->stream, f, device->d_frags, sizeof f, hipMemcpyDeviceToHost), "stats readback");
-    out->fragments = f[0];
+    out->fragments = 0;
+    for (int i = 0; i < kFragSlots; ++i) out->fragments += f[i];
     out->primitives = device->last_prims;
     out->kernel_launches = device->last_launches;
     return WGC_OK;
@@ -1155,63 +1156,108 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
     return WGC_OK;
 }
 
+// Persistent raster grid: enough workgroups to fill every CU a few times
+// over, never more than there are work items.
+static uint32_t raster_grid(uint32_t cfg, bool ordered, uint64_t items)
+{
+    uint64_t cap = (!ordered && (cfg == RASTER_8x8_64 || cfg == RASTER_16x16_64)) ? 4096u : 2048u;
+    uint64_t g = std::max<uint64_t>(1u, std::min<uint64_t>(items, cap));
+    return (uint32_t)align_up(g, 8);  // the raster splits its work over the 8 XCDs
+}
+
 // Lays a group's scratch out at the start of the arena (groups run one after
 // another on the stream and reuse it).  Returns the bytes it needs; with a
 // base pointer, fills the pointers.
+//
+// Group-wide: tile counts / offsets / fill cursors / active flags / active
+// list (one word per tile of every draw), scan block sums, bin records and
+// bins, the clip list, the header and per-draw counters.  Per draw: triangle
+// records by primitive, clipped sub-triangle records and their clip data,
+// the big-triangle list and the first clip slot per primitive.
 static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t frag_px_size, WgcDevice_T *dev)
 {
     uint32_t n = (uint32_t)st.draws.size();
-    uint64_t total_tiles = 0, rec_cap = 0;
+    uint64_t total_tiles = 0, clip_rec_cap = 0, prims = 0, prims_padded = 0;
+    bool any_clear = false;
     std::vector<uint64_t> clip_cap(n);
     for (uint32_t i = 0; i < n; ++i) {
         const DrawParams &d = st.draws[i];
         total_tiles += (uint64_t)d.tiles_x * d.tiles_y;
-        clip_cap[i] = 2ull * d.total_prims + 4096ull;
-        rec_cap += d.total_prims + clip_cap[i];
+        clip_cap[i] = 2ull * d.total_prims + 1024ull;
+        clip_rec_cap += clip_cap[i];
+        prims += d.total_prims;
+        prims_padded += st.multi ? align_up(d.total_prims, 64) : d.total_prims;
+        any_clear = any_clear || d.init_clear;
     }
+    uint64_t rec_cap = prims_padded + clip_rec_cap;
     uint64_t bins_cap = rec_cap * (uint64_t)kMaxBinTiles;
+    uint64_t n_blocks = (total_tiles + kScanBlock - 1) / kScanBlock;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         size_t o = off;
         off = align_up(off + std::max<size_t>(bytes, 4), 256);
         return o;
     };
-    size_t o_tc = take(4 * total_tiles), o_to = take(4 * (total_tiles + 1)), o_tf = take(4 * total_tiles);
-    size_t o_bins = take(4 * bins_cap), o_recs = take(16 * rec_cap);
-    size_t o_ctr = take(4ull * C_NUM * n), o_rc = take(4);
-    std::vector<size_t> o_big(n), o_cl(n), o_ct(n), o_cb(n);
+    size_t o_tc = take(4 * total_tiles), o_to = take(4 * (total_tiles + 1));
+    size_t o_ta = take(4 * total_tiles), o_al = take(4 * total_tiles), o_bs = take(8 * n_blocks);
+    size_t o_bins = take(4 * bins_cap), o_recs = take(16 * rec_cap), o_rs = take(4ull * kMaxBinTiles * rec_cap);
+    size_t o_ctr = take(4ull * C_NUM * n), o_hdr = take(4ull * H_NUM), o_cl = take(8 * prims);
+    std::vector<size_t> o_big(n), o_tri(n), o_ctri(n), o_ct(n), o_cb(n);
     for (uint32_t i = 0; i < n; ++i) {
         const DrawParams &d = st.draws[i];
         uint64_t rc = d.total_prims + clip_cap[i];
         o_big[i] = take(16 * rc);
-        o_cl[i] = take(4ull * d.total_prims);
+        o_tri[i] = take(4ull * kRecWords * d.total_prims);
+        o_ctri[i] = take(4ull * kRecWords * clip_cap[i]);
         o_ct[i] = take(4ull * kClipFloats * clip_cap[i]);
         o_cb[i] = take(4ull * d.total_prims);
     }
     if (!base) return off;
     GroupLaunch &gl = st.gl;
+    BatchArgs &b = st.batch;
     gl.total_tiles = (uint32_t)total_tiles;
     gl.n_counters = C_NUM * n;
     gl.tile_count = (uint32_t *)(base + o_tc);
-    gl.tile_offset = (uint32_t *)(base + o_to);
-    gl.tile_fill = (uint32_t *)(base + o_tf);
     gl.bins = (uint32_t *)(base + o_bins);
     gl.recs = (uint4 *)(base + o_recs);
     gl.counters = (uint32_t *)(base + o_ctr);
-    gl.rec_count = (uint32_t *)(base + o_rc);
-    gl.rec_cap = (uint32_t)rec_cap;
+    gl.header = (uint32_t *)(base + o_hdr);
+    gl.rec_slots = (uint32_t *)(base + o_rs);
+    gl.rec_prims = (uint32_t)prims_padded;
+    gl.rec_clip_cap = (uint32_t)clip_rec_cap;
+    gl.clip_list = (uint2 *)(base + o_cl);
+    gl.clip_cap = (uint32_t)prims;
+    b.n = n;
+    b.total_tiles = (uint32_t)total_tiles;
+    b.n_chunks = any_clear ? (uint32_t)((total_tiles + kClearChunk - 1) / kClearChunk) : 0u;
+    b.header = gl.header;
+    b.tile_count = gl.tile_count;
+    b.tile_offset = (uint32_t *)(base + o_to);
+    b.tile_active = (uint32_t *)(base + o_ta);
+    b.active_list = (uint32_t *)(base + o_al);
+    b.block_sums = (uint32_t *)(base + o_bs);
+    b.n_blocks = (uint32_t)n_blocks;
+    if (!st.multi) {
+        b.draws = nullptr;
+        b.prim_base = nullptr;
+        b.tile_base = nullptr;
+        b.total_prims = st.draws[0].total_prims;
+    }
+    gl.raster_grid = raster_grid(gl.raster_cfg, gl.ordered != 0, total_tiles + b.n_chunks);
     uint64_t tile_base = 0;
     for (uint32_t i = 0; i < n; ++i) {
         DrawParams &d = st.draws[i];
         Scratch &s = d.s;
         s.tile_count = gl.tile_count + tile_base;
-        s.tile_offset = gl.tile_offset + tile_base;
-        s.tile_fill = gl.tile_fill + tile_base;
+        s.tile_offset = b.tile_offset + tile_base;
         s.bins = gl.bins;
         s.recs = gl.recs;
-        s.rec_count = gl.rec_count;
+        s.rec_slots = gl.rec_slots;
+        s.header = gl.header;
+        s.clip_list = gl.clip_list;
         s.big = (uint4 *)(base + o_big[i]);
-        s.clip_list = (uint32_t *)(base + o_cl[i]);
+        s.tri = (uint32_t *)(base + o_tri[i]);
+        s.clip_tri = (uint32_t *)(base + o_ctri[i]);
         s.clip_tris = (float *)(base + o_ct[i]);
         s.clip_base = (uint32_t *)(base + o_cb[i]);
         s.counters = gl.counters + C_NUM * i;
@@ -1220,9 +1266,10 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
         // debug counts are only written when they were sized for this pass
         s.frag_px = (frag_px && frag_px_size == (size_t)d.width * d.height * 4u) ? frag_px : nullptr;
         s.bins_cap = (uint32_t)bins_cap;
-        s.rec_cap = (uint32_t)rec_cap;
+        s.rec_prims = gl.rec_prims;
+        s.rec_clip_cap = gl.rec_clip_cap;
         s.big_cap = (uint32_t)(d.total_prims + clip_cap[i]);
-        s.clip_list_cap = d.total_prims;
+        s.clip_list_cap = (uint32_t)prims;
         s.clip_tri_cap = (uint32_t)clip_cap[i];
         tile_base += (uint64_t)d.tiles_x * d.tiles_y;
     }
@@ -1405,7 +1452,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
             continue;
         }
         layout_group(s, dev->arena, dev->frag_px, dev->frag_px_size, dev);
-        plan.launches += 6;
+        plan.launches += 8;
         for (auto &d : s.draws) plan.prims += d.total_prims;
         if (!s.multi) continue;
         uint32_t n = (uint32_t)s.draws.size();
@@ -1445,8 +1492,8 @@ static hipError_t launch_plan(WgcDevice_T *dev, const Plan &plan, KernelTimer *k
     if (e != hipSuccess) return e;
     for (const Step &s : plan.steps) {
         if (s.kind == StepKind::Clear) e = launch_clear(s.color, s.rgba, s.depth, s.dv, s.n, dev->stream, kt);
-        else if (s.multi) e = launch_batch(s.batch, s.gl, dev->stream, kt);
-        else e = launch_draw(s.draws[0], s.gl, dev->stream, kt);
+        else if (s.multi) e = launch_batch(s.draws[0], s.batch, s.gl, dev->stream, kt);
+        else e = launch_draw(s.draws[0], s.batch, s.gl, dev->stream, kt);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -74,28 +74,62 @@ struct AttrBind {
     uint32_t stride, step_instance, offset, ncomp;
 };
 
-// Scratch of one draw.  The tile arrays, the record list and the bins are
-// slices of arrays shared by every draw of a launch (tile offsets and bin
-// indices are global); the rest is per draw.
+// Scratch of one draw.  The tile arrays, the bin records, the bins, the clip
+// list and the group header are slices of arrays shared by every draw of a
+// launch group (tile offsets and bin indices are global); the rest is per
+// draw.
+//
+// Bin records: one slot per primitive of the group (slot = group primitive
+// index; an empty footprint marks "no record"), followed by a region that
+// clipped sub-triangles append to.  For every tile of a record's footprint
+// (row-major, at most kMaxBinTiles) rec_slots holds its position in that
+// tile's bin, taken from the tile count when it was incremented.
 struct Scratch {
     uint32_t *tile_count;    // n_tiles (this draw's slice)
     uint32_t *tile_offset;   // n_tiles + 1 (global bin offsets)
-    uint32_t *tile_fill;     // n_tiles
     uint32_t *bins;          // shared, bins_cap keys
-    uint4 *recs;             // shared: (key, tx0|ty0<<16, tx1|ty1<<16, draw)
-    uint32_t *rec_count;     // shared record counter
-    uint4 *big;
-    uint32_t *clip_list;     // primitives that need the clipper
-    float *clip_tris;        // clip_tri_cap x kClipFloats
+    uint4 *recs;             // shared bin records: (key, tx0|ty0<<16, tx1|ty1<<16, draw)
+    uint32_t *rec_slots;     // shared, kMaxBinTiles per bin record
+    uint32_t *header;        // shared group header, see Header
+    uint2 *clip_list;        // shared: (draw, primitive) that need the clipper
+    uint4 *big;              // this draw's big-triangle bin records
+    uint32_t *tri;           // triangle records of unclipped primitives (kRecWords each, by primitive)
+    uint32_t *clip_tri;      // triangle records of clipped sub-triangles (by clip slot)
+    float *clip_tris;        // clip_tri_cap x kClipFloats: positions and clip barycentrics
     uint32_t *clip_base;     // per primitive: first sub-triangle slot
     uint32_t *counters;      // see Counter
-    unsigned long long *frags;  // fragment counter of the pass
+    unsigned long long *frags;  // kFragSlots fragment counters of the submission (summed on read)
     uint32_t *error;
     uint32_t *frag_px;       // optional per-pixel fragment count
-    uint32_t bins_cap, rec_cap, big_cap, clip_list_cap, clip_tri_cap;
+    uint32_t bins_cap;
+    uint32_t rec_prims;      // size of the per-primitive record region (group-wide)
+    uint32_t rec_clip_cap;   // capacity of the appended region
+    uint32_t big_cap, clip_list_cap, clip_tri_cap;
 };
 
-enum Counter { C_BIG = 0, C_CLIP_LIST = 1, C_CLIP_TRI = 2, C_NUM = 4 };
+constexpr int kFragSlots = 64;  // fragment counters, spread so that atomics rarely meet
+
+// Per-draw counters.  The big-triangle footprint union is kept as maxima of
+// (65535 - min) and max so that zero-initialised counters work.
+enum Counter { C_BIG = 0, C_CLIP_TRI = 1, C_BIG_NX0 = 2, C_BIG_NY0 = 3, C_BIG_X1 = 4, C_BIG_Y1 = 5, C_NUM = 8 };
+
+// Group header words.
+// H_REC counts records appended after the per-primitive region.
+enum Header { H_REC = 0, H_ACTIVE = 1, H_CLIP = 2, H_NUM = 4 };
+
+// Triangle record: everything the raster needs about one (sub-)triangle,
+// computed once in setup (the same values the raster would compute).
+constexpr int kRecWords = 36;
+enum RecWord {
+    R_VP = 0,      // vpx0 vpy0 vpx1 vpy1 vpx2 vpy2
+    R_K = 6,       // k0 k1 k2 total
+    R_Z = 10,      // z0 z1 z2
+    R_IW = 13,     // 1/w of the unclipped vertices
+    R_SCAN = 16,   // ax ay bx by cx cy (after bubblesort3 by y)
+    R_MODE = 22,   // scan mode | wide << 8
+    R_KEY = 23,
+    R_VAR = 24,    // 3 x kMaxVar varyings of the unclipped vertices
+};
 
 struct DrawParams {
     // pipeline
@@ -627,6 +661,21 @@ __device__ __forceinline__ bool tile_bbox(const DrawParams &d, const ScanTri &s,
 // Fragment stage
 // ------------------------------------------------------------------
 
+// u8 / 255.0 (texture.rs:173-175) for every byte, folded by the compiler with
+// IEEE round-to-nearest, i.e. the same value as the division.
+struct U8Table {
+    float v[256];
+};
+constexpr U8Table make_u8_table()
+{
+    U8Table t{};
+    for (int i = 0; i < 256; ++i) t.v[i] = (float)i / 255.0f;
+    return t;
+}
+__constant__ constexpr U8Table kU8ToF = make_u8_table();
+
+__device__ __forceinline__ float u8f(uint32_t b) { return kU8ToF.v[b & 0xFFu]; }
+
 __device__ __forceinline__ void fetch_texel(const DrawParams &d, uint32_t u, uint32_t v, float out[4])
 {
     uint32_t t = *reinterpret_cast<const uint32_t *>(d.tex + ((uint64_t)v * d.tex_w + u) * 4u);
@@ -634,12 +683,12 @@ __device__ __forceinline__ void fetch_texel(const DrawParams &d, uint32_t u, uin
         out[0] = d.srgb_lut[t & 0xFFu];
         out[1] = d.srgb_lut[(t >> 8) & 0xFFu];
         out[2] = d.srgb_lut[(t >> 16) & 0xFFu];
-        out[3] = (float)(t >> 24) / 255.0f;
+        out[3] = u8f(t >> 24);
     } else {
-        out[0] = (float)(t & 0xFFu) / 255.0f;   // texture.rs:173-175
-        out[1] = (float)((t >> 8) & 0xFFu) / 255.0f;
-        out[2] = (float)((t >> 16) & 0xFFu) / 255.0f;
-        out[3] = (float)(t >> 24) / 255.0f;
+        out[0] = u8f(t);   // texture.rs:173-175
+        out[1] = u8f(t >> 8);
+        out[2] = u8f(t >> 16);
+        out[3] = u8f(t >> 24);
     }
 }
 
@@ -715,10 +764,10 @@ __device__ __forceinline__ uint32_t blend_over(const float s[4], uint32_t dst, u
 {
     float dc[4];
     uint32_t c0 = dst & 0xFFu, c1 = (dst >> 8) & 0xFFu, c2 = (dst >> 16) & 0xFFu, c3 = dst >> 24;
-    dc[0] = (float)(bgra ? c2 : c0) / 255.0f;
-    dc[1] = (float)c1 / 255.0f;
-    dc[2] = (float)(bgra ? c0 : c2) / 255.0f;
-    dc[3] = (float)c3 / 255.0f;
+    dc[0] = u8f(bgra ? c2 : c0);
+    dc[1] = u8f(c1);
+    dc[2] = u8f(bgra ? c0 : c2);
+    dc[3] = u8f(c3);
     float ia = 1.0f - s[3];
     float o[4];
 #pragma unroll

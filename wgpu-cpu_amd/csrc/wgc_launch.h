@@ -11,13 +11,24 @@ namespace wgc {
 
 struct DrawParams;
 
-// A table of independent draws that share every launch (batched form).
+// Group-level arguments of every kernel: the table of independent draws
+// that share the launches (batched form; empty for a single draw passed by
+// value) and the group's tile bookkeeping.
 struct BatchArgs {
     const DrawParams *draws;     // device array of n draws
     const uint32_t *prim_base;   // n + 1 prefix of primitives, padded to multiples of 64
     const uint32_t *tile_base;   // n + 1 prefix of tiles
-    uint32_t n, total_prims, total_tiles, pad;
+    uint32_t n, total_prims, total_tiles, n_chunks;
+    uint32_t *header;            // H_* words
+    uint32_t *tile_count, *tile_offset;
+    uint32_t *tile_active;       // 1 when the raster must visit the tile
+    uint32_t *active_list;       // compacted active tiles
+    uint32_t *block_sums;        // 2 words per scan block
+    uint32_t n_blocks, pad;
 };
+
+constexpr uint32_t kScanBlock = 1024;   // tiles per scan block
+constexpr uint32_t kClearChunk = 8;     // tiles per clear work item
 
 // Raster tile shapes: tile side x threads per workgroup.
 enum RasterCfg : uint32_t {
@@ -36,10 +47,13 @@ inline uint32_t raster_tile_shift(uint32_t cfg)
 // Group-wide scratch and launch shape.
 struct GroupLaunch {
     uint32_t program, ordered, raster_cfg;
-    uint32_t total_prims, total_tiles, n_counters;
-    uint32_t *tile_count, *tile_offset, *tile_fill, *bins, *counters, *rec_count;
+    uint32_t total_prims, total_tiles, n_counters, raster_grid;
+    uint32_t *tile_count, *bins, *counters, *header;
     uint4 *recs;
-    uint32_t rec_cap;
+    uint32_t *rec_slots;
+    uint32_t rec_prims, rec_clip_cap;   // record regions: per primitive, appended
+    uint2 *clip_list;            // (draw, primitive) pairs that cross a clip plane
+    uint32_t clip_cap;
 };
 
 // Brackets launches with HIP events on the launching stream; the runtime
@@ -76,8 +90,12 @@ struct KernelTimer {
     }
 };
 
-hipError_t launch_draw(const DrawParams &d, const GroupLaunch &gl, hipStream_t st, KernelTimer *kt);
-hipError_t launch_batch(const BatchArgs &b, const GroupLaunch &gl, hipStream_t st, KernelTimer *kt);
+// A single draw passes its parameters by value (b.draws unused, b.n == 1);
+// a batch passes the device table in b.draws (d0 unused).
+hipError_t launch_draw(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
+                       KernelTimer *kt);
+hipError_t launch_batch(const DrawParams &d0, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
+                        KernelTimer *kt);
 hipError_t launch_reset(unsigned long long *frags, hipStream_t st);
 hipError_t launch_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, uint64_t n,
                         hipStream_t st, KernelTimer *kt);
