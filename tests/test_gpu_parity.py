@@ -108,6 +108,11 @@ def test_c2_textured_extended(gpu_device_ext):
     assert_parity(spec, gpu_device_ext)
 
 
+@pytest.mark.parametrize("make", [scenes.perspective_quad, scenes.bilinear_strip])
+def test_extension_closed_form_scenes(gpu_device_ext, make):
+    assert_parity(make(), gpu_device_ext)
+
+
 def test_c3_overdraw_reference_small(gpu_device):
     spec = scenes.config3_overdraw(n=20_000, width=640, height=610)
     assert_parity(spec, gpu_device)

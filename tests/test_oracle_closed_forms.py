@@ -152,3 +152,45 @@ def test_src_over_closed_form():
     once = r.frag_count == 1
     assert once.any()
     assert (r.color[once] == np.array(exp, np.uint8)).all()
+
+
+
+def test_perspective_correct_closed_form():
+    """Extension: along an edge from clip (x=-1, w=1) to (x=4, w=4) the clip
+    parameter s at NDC n is (n+1)/(5-3n); a varying that is 0 on the left and
+    1 on the right must come out as s, not as the screen-linear (n+1)/2."""
+    spec = scenes.perspective_quad()
+    W = spec.width
+    r = bridge.render(spec, frag_counts=True)
+    x = np.arange(W, dtype=np.float64)
+    n = 2.0 * x / W - 1.0                      # pixel corner in NDC
+    s = (n + 1.0) / (5.0 - 3.0 * n)
+    exp = np.floor(255.0 * s)
+    got = r.color[:, :, 0].astype(np.int64)
+    assert (r.frag_count > 0).all()
+    assert np.all(np.abs(got - exp[None, :]) <= 1)
+    # and it is far from screen-linear interpolation in the middle
+    assert abs(int(got[0, W // 2]) - int(255 * 0.5)) > 40
+
+
+def test_bilinear_repeat_closed_form():
+    """Extension: bilinear filtering with the u*W - 0.5 texel-centre
+    convention and Repeat wrap on a 2x2 texture (texel values 200 and 40 in
+    red), sampled along u = x/W at the pixel corners of a screen-aligned quad
+    (w = 1, so linear = perspective)."""
+    spec = scenes.bilinear_strip()
+    W = spec.width
+    r = bridge.render(spec, frag_counts=True)
+    u = np.arange(W, dtype=np.float64) / W
+    fu = u * 2.0 - 0.5
+    i0 = np.floor(fu)
+    a = fu - i0
+    t = np.array([200.0, 40.0]) / 255.0
+    c0 = t[(i0.astype(np.int64)) % 2]
+    c1 = t[(i0.astype(np.int64) + 1) % 2]
+    exp = np.floor(255.0 * (c0 * (1 - a) + c1 * a))
+    got = r.color[:, :, 0].astype(np.int64)
+    assert (r.frag_count > 0).all()
+    assert np.all(np.abs(got - exp[None, :]) <= 1)
+    # u = 0 lies between the last and the first texel (wrapped): the mean
+    assert abs(int(got[0, 0]) - 120) <= 1

@@ -247,6 +247,40 @@ def config2_textured(seed: int = SEED_BASE + 2, extended: bool = False, width: i
                     name="c2_textured_" + ("ext" if extended else "ref"))
 
 
+def perspective_quad(width: int = 64, height: int = 8) -> PassSpec:
+    """Full-target quad in clip space, left edge w = 1, right edge w = 4
+    (x = +-w, y = +-w, z = w/2); red varies 0 -> 1 from left to right with
+    perspective-correct interpolation (extension)."""
+    L, R = 1.0, 4.0
+    verts = np.array([[-L, -L, 0.5 * L, L], [R, -R, 0.5 * R, R], [R, R, 0.5 * R, R],
+                      [-L, -L, 0.5 * L, L], [R, R, 0.5 * R, R], [-L, L, 0.5 * L, L]], np.float32)
+    cl, cr = [0, 0, 0, 1], [1, 0, 0, 1]
+    cols = np.array([cl, cr, cr, cl, cr, cl], np.float32)
+    vb = np.concatenate([verts, cols], axis=1)
+    d = DrawSpec(program=PROGRAM_CLIP_COLOR, interp=INTERP_PERSPECTIVE, vertex_data=vb.tobytes(),
+                 vertex_stride=32, attributes=[(FLOAT32X4, 0, 0), (FLOAT32X4, 16, 1)], count=6)
+    return PassSpec(width, height, RGBA8_UNORM, clear_color=(0, 0, 0, 0), draws=[d],
+                    semantics=SEMANTICS_EXTENDED, name="perspective_quad")
+
+
+def bilinear_strip(width: int = 32, height: int = 4) -> PassSpec:
+    """Screen-aligned quad with u = 0 -> 1 left to right over a 2x2 texture
+    whose columns hold red 200 and 40, bilinear + Repeat (extension)."""
+    texels = np.array([[[200, 0, 0, 255], [40, 0, 0, 255]],
+                       [[200, 0, 0, 255], [40, 0, 0, 255]]], np.uint8)
+    verts = np.array([[-1, -1, 0.5, 1], [1, -1, 0.5, 1], [1, 1, 0.5, 1],
+                      [-1, -1, 0.5, 1], [1, 1, 0.5, 1], [-1, 1, 0.5, 1]], np.float32)
+    uv = np.array([[0, 1], [1, 1], [1, 0], [0, 1], [1, 0], [0, 0]], np.float32)
+    vb = np.concatenate([verts, uv], axis=1)
+    ident = np.eye(4, dtype=np.float32).T.tobytes()
+    d = DrawSpec(program=PROGRAM_MVP_TEXTURE, interp=INTERP_PERSPECTIVE, vertex_data=vb.tobytes(),
+                 vertex_stride=24, attributes=[(FLOAT32X4, 0, 0), (FLOAT32X2, 16, 1)], count=6,
+                 uniform=ident, texture=(2, 2, texels.tobytes(), False), address_u=REPEAT, address_v=REPEAT,
+                 filter=FILTER_LINEAR)
+    return PassSpec(width, height, RGBA8_UNORM, clear_color=(0, 0, 0, 0), draws=[d],
+                    semantics=SEMANTICS_EXTENDED, name="bilinear_strip")
+
+
 def config3_overdraw(seed: int = SEED_BASE + 3, blend: bool = False, n: int = 200_000,
                      width: int = 2560, height: int = 2440) -> PassSpec:
     """n screen-space equilateral triangles, centres uniform over the target,
