@@ -114,6 +114,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of each serving run (the full workload is 60 s)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration for profiles)")
@@ -182,6 +186,10 @@ def main():
     if not args.no_batch:
         batched = run_batched(devs, args)
 
+    serve = None
+    if not args.no_serve:
+        serve = run_serving(devs, args)
+
     line = {
         "metric": "Mfragments/s (render pass throughput; 1080p frame ms beside it)",
         "value": round(value, 2),
@@ -205,6 +213,7 @@ def main():
         "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "extra": extra,
         "batched": batched,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
 
@@ -254,6 +263,40 @@ def run_batched(devs, args):
         "raster_tile": wgc.RASTER_TILES[best],
         "kernels_us": {k: round(v["avg_us"], 2) for k, v in prof.items()},
     }
+
+
+def run_serving(devs, args):
+    """Poisson arrivals (default 2000 req/s) of requests of the batched
+    workload's distribution against 32 resident meshes, targets and uniforms
+    sub-allocated from a 64 GB heap (the HBM budget): continuous batching
+    against plain per-request launches of the same trace; then the batched
+    form at a rate beyond what the plain form sustains."""
+    import wgc
+    from wgc import scenes, serving
+    if wgc.SEMANTICS_REFERENCE not in devs:
+        devs[wgc.SEMANTICS_REFERENCE] = wgc.Device(0, wgc.SEMANTICS_REFERENCE)
+    dev = devs[wgc.SEMANTICS_REFERENCE]
+    dev.set_option(wgc.OPTION_RASTER_TILE, 0)
+    pool = serving.MeshPool(dev, 32, seed=scenes.SEED_BASE + 5)
+    budget = 64 << 30
+    trace = serving.make_trace(scenes.SEED_BASE + 5, args.serve_rate, args.serve_seconds, len(pool.meshes))
+    warm = serving.make_trace(scenes.SEED_BASE + 6, args.serve_rate, 0.5, len(pool.meshes))
+    out = {"workload": f"Poisson {args.serve_rate:g} req/s for {args.serve_seconds:g} s, W,H log-uniform "
+                       f"[64,1024], 32 resident spheres of 100-50k triangles (half textured), "
+                       f"64 GB heap budget", "requests": len(trace)}
+    for mode, batched in (("batched", True), ("plain", False)):
+        srv = serving.Server(dev, pool, budget, max_batch=256, max_inflight=2)
+        srv.run(warm, batched=batched)
+        out[mode] = srv.run(trace, batched=batched).summary()
+        srv.heap.release()
+    # saturation: 10x the rate for a shorter run
+    hi = serving.make_trace(scenes.SEED_BASE + 7, args.serve_rate * 10, max(1.0, args.serve_seconds / 5),
+                            len(pool.meshes))
+    for mode, batched in (("batched_10x", True), ("plain_10x", False)):
+        srv = serving.Server(dev, pool, budget, max_batch=256, max_inflight=2)
+        out[mode] = srv.run(hi, batched=batched).summary()
+        srv.heap.release()
+    return out
 
 
 def cpu_model() -> str:

@@ -57,6 +57,7 @@ typedef struct WgcRenderPipeline_T *WgcRenderPipeline;
 typedef struct WgcCommandEncoder_T *WgcCommandEncoder;
 typedef struct WgcRenderPass_T *WgcRenderPass;
 typedef struct WgcCommandBuffer_T *WgcCommandBuffer;
+typedef struct WgcHeap_T *WgcHeap;
 
 /* ---- enums (numbering follows wgpu-types where it has one) ---- */
 
@@ -219,6 +220,22 @@ int32_t wgc_queue_write_texture(WgcDevice device, WgcTexture texture, const void
 int32_t wgc_texture_read(WgcTexture texture, void *dst, uint64_t size);
 void wgc_texture_release(WgcTexture texture);
 
+/* ---- heaps (no reference counterpart: the reference allocates every
+ *      resource separately on the host, buffer.rs:24-52, texture.rs:35-50).
+ *      For callers that create and drop resources at a high rate (a serving
+ *      loop's per-request targets): one device allocation up front, resources
+ *      sub-allocated from it (first fit, 256-byte granules), zero-filled on
+ *      the device stream without a host wait, and returned to the heap when
+ *      released (after every submission using them has executed). A full heap
+ *      returns WGC_ERR_OUT_OF_MEMORY, which a caller can use for admission
+ *      control. ---- */
+int32_t wgc_heap_create(WgcDevice device, uint64_t size, WgcHeap *out);
+void wgc_heap_release(WgcHeap heap);
+int32_t wgc_heap_stats(WgcHeap heap, uint64_t *used, uint64_t *capacity, uint64_t *largest_free);
+int32_t wgc_buffer_create_in_heap(WgcHeap heap, uint64_t size, WgcBuffer *out);
+int32_t wgc_texture_create_in_heap(WgcHeap heap, uint32_t width, uint32_t height, uint32_t format,
+                                   WgcTexture *out);
+
 /* ---- samplers, bind groups (device.rs:114-127, sampler.rs:3-29,
  *      bind_group.rs:16-117) ---- */
 int32_t wgc_sampler_create(WgcDevice device, const WgcSamplerDescriptor *desc, WgcSampler *out);
@@ -273,11 +290,17 @@ void wgc_command_buffer_release(WgcCommandBuffer buffer);
 int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcCommandBuffer *buffers,
                          uint64_t *submission_index);
 /* Waits until `submission_index` has executed (UINT64_MAX: everything).
- * timeout_ns 0 = no timeout. Returns WGC_ERR_EXECUTION if a pass failed. */
+ * timeout_ns 0 = no timeout; a timeout of 1 ns checks without waiting
+ * (WGC_ERR_TIMEOUT while it is still running). Returns WGC_ERR_EXECUTION if a
+ * pass of a submission retired by this call failed. Waiting for one
+ * submission does not wait for later ones. */
 int32_t wgc_device_poll(WgcDevice device, uint64_t submission_index, uint64_t timeout_ns);
 /* Counters of the last submission (waits for the queue): fragments of all its
- * passes, primitives assembled, kernel launches. */
+ * command buffers' passes, primitives assembled, kernel launches. */
 int32_t wgc_device_last_pass_stats(WgcDevice device, WgcPassStats *out);
+/* Fragments and submission count summed over every submission that has
+ * executed so far; does not wait (no reference counterpart). */
+int32_t wgc_device_totals(WgcDevice device, uint64_t *fragments, uint64_t *submissions);
 
 /* ---- execution options (no reference counterpart: the reference runs one
  *      pass at a time on one thread, engine.rs:26-46) ---- */

@@ -34,3 +34,16 @@ def test_checker_texture():
     t = np.frombuffer(scenes.checker_texture(64, 32, 16, 5), np.uint8).reshape(64, 64, 4)
     assert (t[..., 3] == 255).all()
     assert t[0:32, 0:32, 0].min() >= 239 and t[0:32, 32:64, 0].max() <= 16
+
+
+def test_serving_trace_is_poisson_and_sorted():
+    from wgc import serving
+    tr = serving.make_trace(0x5EED_0005, rate=2000.0, seconds=2.0, n_meshes=8)
+    t = np.array([r.arrival for r in tr])
+    assert np.all(np.diff(t) >= 0) and t[-1] <= 2.0
+    # 4000 expected arrivals; Poisson sd = 63
+    assert abs(len(tr) - 4000) < 5 * 63
+    gaps = np.diff(t)
+    assert abs(gaps.mean() - 1 / 2000.0) < 0.1 / 2000.0
+    assert all(64 <= r.width <= 1024 and 64 <= r.height <= 1024 for r in tr)
+    assert {r.mesh for r in tr} == set(range(8))

@@ -579,6 +579,15 @@ __device__ __forceinline__ void stage_record(const DrawParams &d, uint32_t key, 
     wide = (w[R_MODE] >> 8) & 1u;
 }
 
+// Pins a value to a register: keeps the compiler from turning a select of
+// two fields of the per-pixel state into a load through a selected address,
+// which would move that state to scratch memory.
+__device__ __forceinline__ float in_vgpr(float x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 struct PixelState {
     uint32_t col;
     float dep;
@@ -587,18 +596,14 @@ struct PixelState {
     bool has;
 };
 
-// One fragment of triangle T at pixel (pxf, pyf): barycentric at the pixel
-// corner (raster.rs:313-346), the depth test or its order-independent
-// restatement, varyings (fragment.rs:319-377), the fragment shader and the
-// colour write (texture.rs:213-218).
-template <int PROG, bool ORDERED>
-__device__ __forceinline__ void shade_fragment(const DrawParams &d, const TriLds &T, float pxf, float pyf,
-                                               PixelState &ps)
+// Barycentric at the pixel corner (raster.rs:313-336) composed with the
+// clip barycentrics (raster.rs:337-343); returns the interpolated depth
+// (raster.rs:345-346).
+__device__ __forceinline__ float bary_depth(const TriLds &T, float pxf, float pyf, float lc[3])
 {
     float l0 = (((T.vpy[1] - pyf) * (T.vpx[1] + pxf) + T.k0) + (pyf - T.vpy[2]) * (pxf + T.vpx[2])) / T.total;
     float l1 = (((pyf - T.vpy[0]) * (pxf + T.vpx[0]) + (T.vpy[2] - pyf) * (T.vpx[2] + pxf)) + T.k1) / T.total;
     float l2 = ((T.k2 + (pyf - T.vpy[1]) * (pxf + T.vpx[1])) + (T.vpy[0] - pyf) * (T.vpx[0] + pxf)) / T.total;
-    float lc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         float acc = T.cb[0 * 3 + c] * l0;
@@ -609,31 +614,14 @@ __device__ __forceinline__ void shade_fragment(const DrawParams &d, const TriLds
     float z = T.z[0] * lc[0];
     z = z + T.z[1] * lc[1];
     z = z + T.z[2] * lc[2];
+    return z;
+}
 
-    bool win;
-    if (ORDERED) {
-        win = d.depth_test ? compare(d.depth_compare, z, ps.dep) : true;
-    } else {
-        switch (d.resolve) {
-        case RES_LAST: win = !ps.has || T.key > ps.key; break;
-        case RES_MIN_FIRST:
-            win = (z < ps.init_dep) && (!ps.has || z < ps.dep || (z == ps.dep && T.key < ps.key));
-            break;
-        case RES_MIN_LAST:
-            win = (z <= ps.init_dep) && (!ps.has || z < ps.dep || (z == ps.dep && T.key > ps.key));
-            break;
-        case RES_MAX_FIRST:
-            win = (z > ps.init_dep) && (!ps.has || z > ps.dep || (z == ps.dep && T.key < ps.key));
-            break;
-        case RES_MAX_LAST:
-            win = (z >= ps.init_dep) && (!ps.has || z > ps.dep || (z == ps.dep && T.key > ps.key));
-            break;
-        default:  // RES_PASS_LAST
-            win = compare(d.depth_compare, z, ps.init_dep) && (!ps.has || T.key > ps.key);
-            break;
-        }
-    }
-    if (!win) return;
+// Varyings (fragment.rs:319-377) and the fragment shader.
+template <int PROG>
+__device__ __forceinline__ void fragment_color(const DrawParams &d, const TriLds &T, const float lc[3],
+                                               float color[4])
+{
     float var[kMaxVar];
     if (d.interp == INTERP_FLAT) {
 #pragma unroll
@@ -659,18 +647,72 @@ __device__ __forceinline__ void shade_fragment(const DrawParams &d, const TriLds
             var[c] = acc;
         }
     }
-    float color[4];
     run_fs<PROG>(d, var, color);
+}
+
+// One fragment of triangle T at pixel (pxf, pyf).  Ordered: the depth test
+// against the running depth, the fragment shader and the colour write or
+// blend (fragment.rs:427-488, texture.rs:213-218), in API order.  Otherwise
+// only visibility: the order-independent restatement of the depth test picks
+// the winning (depth, key); its colour is shaded once after the tile's last
+// triangle (the fragment shader has no side effects, so only the winner's
+// output can reach the target).
+template <int PROG, bool ORDERED>
+__device__ __forceinline__ void shade_fragment(const DrawParams &d, const TriLds &T, float pxf, float pyf,
+                                               PixelState &ps)
+{
+    float lc[3];
+    float z = bary_depth(T, pxf, pyf, lc);
     if (ORDERED) {
+        if (d.depth_test && !compare(d.depth_compare, z, ps.dep)) return;
+        float color[4];
+        fragment_color<PROG>(d, T, lc, color);
         if (d.depth_test && d.depth_write) ps.dep = z;
         ps.col = d.blend ? blend_over(color, ps.col, d.color_bgra) : pack_color(color, d.color_bgra);
         ps.has = true;
-    } else {
-        ps.col = pack_color(color, d.color_bgra);
+        return;
+    }
+    bool win;
+    switch (d.resolve) {
+    case RES_LAST: win = !ps.has || T.key > ps.key; break;
+    case RES_MIN_FIRST:
+        win = (z < ps.init_dep) && (!ps.has || z < ps.dep || (z == ps.dep && T.key < ps.key));
+        break;
+    case RES_MIN_LAST:
+        win = (z <= ps.init_dep) && (!ps.has || z < ps.dep || (z == ps.dep && T.key > ps.key));
+        break;
+    case RES_MAX_FIRST:
+        win = (z > ps.init_dep) && (!ps.has || z > ps.dep || (z == ps.dep && T.key < ps.key));
+        break;
+    case RES_MAX_LAST:
+        win = (z >= ps.init_dep) && (!ps.has || z > ps.dep || (z == ps.dep && T.key > ps.key));
+        break;
+    default:  // RES_PASS_LAST
+        win = compare(d.depth_compare, z, ps.init_dep) && (!ps.has || T.key > ps.key);
+        break;
+    }
+    if (win) {
         ps.dep = z;
         ps.key = T.key;
         ps.has = true;
     }
+}
+
+// Colour of a pixel's winning fragment: its triangle record is read back
+// (from L2, shared by the neighbouring pixels it won) and the same
+// barycentrics are evaluated again, bit for bit.
+template <int PROG>
+__device__ __forceinline__ uint32_t winner_color(const DrawParams &d, uint32_t key, float pxf, float pyf)
+{
+    TriLds T;
+    ScanTri S;
+    uint32_t wide;
+    stage_record(d, key, T, S, wide);
+    float lc[3];
+    (void)bary_depth(T, pxf, pyf, lc);
+    float color[4];
+    fragment_color<PROG>(d, T, lc, color);
+    return pack_color(color, d.color_bgra);
 }
 
 // Stages sh.keys[0..n) into LDS and applies them to this lane's pixels.
@@ -865,10 +907,10 @@ __device__ __forceinline__ void raster_tile(const DrawParams &d, uint32_t tile, 
             write_depth = ps[k].has && d.depth_test && d.depth_write &&
                           (d.resolve != RES_PASS_LAST || d.depth_compare == CMP_EQUAL ||
                            d.depth_compare == CMP_ALWAYS);
-            if (!ps[k].has) ps[k].col = init_col[k];
+            ps[k].col = ps[k].has ? winner_color<PROG>(d, ps[k].key, (float)px, (float)py) : init_col[k];
         }
         if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = ps[k].col;
-        if (d.depth) d.depth[pix] = write_depth ? ps[k].dep : ps[k].init_dep;
+        if (d.depth) d.depth[pix] = write_depth ? in_vgpr(ps[k].dep) : in_vgpr(ps[k].init_dep);
         if (d.s.frag_px && nfrag[k]) d.s.frag_px[pix] += nfrag[k];
     });
     frag_acc += f;
@@ -948,9 +990,10 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
 // k_clear / k_reset
 // ------------------------------------------------------------------
 
-__global__ void k_reset(unsigned long long *frags)
+__global__ void k_reset(unsigned long long *frags, uint32_t *error)
 {
     if (threadIdx.x < kFragSlots) frags[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) *error = 0u;
 }
 
 __global__ void k_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, uint64_t n)
@@ -1001,7 +1044,9 @@ static hipError_t launch_group(const DrawParams &d, const BatchArgs &b, const Gr
     if (gl.total_prims > 0) {
         uint32_t g = (gl.total_prims + 255u) / 256u;
         WGC_LAUNCH("k_setup", (k_setup<PROG, MULTI>), g, 256, d, b);
-        WGC_LAUNCH("k_clip", (k_clip<PROG, MULTI>), 64, 256, d, b, gl.clip_list, gl.clip_cap);
+        // one thread per clipped primitive at most; the count is only known on the device
+        uint32_t gc = min(max(g / 4u, 1u), 1024u);
+        WGC_LAUNCH("k_clip", (k_clip<PROG, MULTI>), gc, 256, d, b, gl.clip_list, gl.clip_cap);
     }
     WGC_LAUNCH("k_scan_blocks", k_scan_blocks<MULTI>, b.n_blocks, 256, d, b);
     WGC_LAUNCH("k_scan_totals", k_scan_totals, 1, 256, b);
@@ -1040,9 +1085,9 @@ hipError_t launch_batch(const DrawParams &d0, const BatchArgs &b, const GroupLau
     return launch_group_prog<true>(d0, b, gl, st, kt);
 }
 
-hipError_t launch_reset(unsigned long long *frags, hipStream_t st)
+hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, frags);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, frags, error);
     return hipGetLastError();
 }
 

@@ -11,6 +11,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -68,6 +69,124 @@ static hipError_t stream_zero(hipStream_t st, void *dst, size_t n)
     return e;
 }
 
+// Zero fill ordered on the device stream without a host wait: later work on
+// the stream (submissions, reads) sees it, and nothing else touches device
+// memory.  Never called while the stream is being captured.
+static hipError_t stream_zero_async(hipStream_t st, void *dst, size_t n)
+{
+    if (!n) return hipSuccess;
+    return hipMemsetAsync(dst, 0, n, st);
+}
+
+// First-fit allocator over [0, capacity) in 256-byte granules with
+// coalescing of neighbouring free ranges.
+class RangeAllocator {
+public:
+    static constexpr uint64_t kAlign = 256;
+    void init(uint64_t capacity)
+    {
+        free_.clear();
+        cap_ = capacity / kAlign * kAlign;
+        if (cap_) free_[0] = cap_;
+        used_ = 0;
+    }
+    bool alloc(uint64_t size, uint64_t &off)
+    {
+        size = std::max<uint64_t>(kAlign, (size + kAlign - 1) / kAlign * kAlign);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < size) continue;
+            off = it->first;
+            uint64_t rest = it->second - size;
+            free_.erase(it);
+            if (rest) free_[off + size] = rest;
+            used_ += size;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t size)
+    {
+        size = std::max<uint64_t>(kAlign, (size + kAlign - 1) / kAlign * kAlign);
+        used_ -= size;
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && off + size == next->first) {
+            size += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += size;
+                return;
+            }
+        }
+        free_[off] = size;
+    }
+    uint64_t used() const { return used_; }
+    uint64_t capacity() const { return cap_; }
+    uint64_t largest_free() const
+    {
+        uint64_t m = 0;
+        for (auto &kv : free_) m = std::max(m, kv.second);
+        return m;
+    }
+
+private:
+    std::map<uint64_t, uint64_t> free_;
+    uint64_t cap_ = 0, used_ = 0;
+};
+
+// Blocks of device or pinned host memory carved by RangeAllocator; grows by
+// whole blocks, never frees a block before the device goes away (freeing
+// would synchronise the device).
+struct BlockPool {
+    struct Block {
+        uint8_t *base;
+        RangeAllocator ra;
+    };
+    bool pinned = false;
+    uint64_t block_size = 0;
+    std::vector<Block> blocks;
+
+    hipError_t alloc(uint64_t size, uint8_t *&ptr)
+    {
+        for (auto &b : blocks) {
+            uint64_t off;
+            if (b.ra.alloc(size, off)) {
+                ptr = b.base + off;
+                return hipSuccess;
+            }
+        }
+        uint64_t bs = std::max<uint64_t>(block_size, (size + RangeAllocator::kAlign - 1) / RangeAllocator::kAlign *
+                                                         RangeAllocator::kAlign);
+        void *p = nullptr;
+        hipError_t e = pinned ? hipHostMalloc(&p, bs, hipHostMallocDefault) : hipMalloc(&p, bs);
+        if (e != hipSuccess) return e;
+        blocks.push_back(Block{(uint8_t *)p, RangeAllocator()});
+        blocks.back().ra.init(bs);
+        uint64_t off = 0;
+        blocks.back().ra.alloc(size, off);
+        ptr = (uint8_t *)p + off;
+        return hipSuccess;
+    }
+    void release(uint8_t *ptr, uint64_t size)
+    {
+        for (auto &b : blocks)
+            if (ptr >= b.base && ptr < b.base + b.ra.capacity()) {
+                b.ra.release((uint64_t)(ptr - b.base), size);
+                return;
+            }
+    }
+    void destroy()
+    {
+        for (auto &b : blocks) {
+            if (pinned) (void)hipHostFree(b.base);
+            else (void)hipFree(b.base);
+        }
+        blocks.clear();
+    }
+};
+
 extern "C" uint32_t wgc_abi_version(void) { return WGC_ABI_VERSION; }
 extern "C" const char *wgc_last_error(void) { return g_last_error.c_str(); }
 
@@ -92,8 +211,19 @@ static void release(Object *o)
 
 struct WgcDevice_T;
 
+// Device memory pool that resources are sub-allocated from (serving: targets
+// that come and go at request rate, without hipMalloc / hipFree per request).
+struct WgcHeap_T : Object {
+    WgcDevice_T *dev = nullptr;
+    uint8_t *base = nullptr;
+    std::mutex mu;
+    RangeAllocator ra;
+    ~WgcHeap_T() override;
+};
+
 struct WgcBuffer_T : Object {
     WgcDevice_T *dev = nullptr;
+    WgcHeap_T *heap = nullptr;   // set when sub-allocated from a heap
     uint8_t *ptr = nullptr;
     uint64_t size = 0;
     ~WgcBuffer_T() override;
@@ -101,6 +231,7 @@ struct WgcBuffer_T : Object {
 
 struct WgcTexture_T : Object {
     WgcDevice_T *dev = nullptr;
+    WgcHeap_T *heap = nullptr;
     uint8_t *ptr = nullptr;
     uint32_t width = 0, height = 0, format = 0;
     uint64_t size = 0;
@@ -213,27 +344,21 @@ struct Step {
 };
 
 struct Plan {
+    WgcDevice_T *dev = nullptr;      // set (and retained) while the plan holds pooled memory
     uint64_t arena_gen = UINT64_MAX;
     uint32_t raster_cfg = UINT32_MAX;
     bool batching = false;
     std::vector<Step> steps;
-    uint8_t *d_tables = nullptr;
+    uint8_t *d_tables = nullptr;     // from the device's table pool
+    uint8_t *h_tables = nullptr;     // pinned staging of the upload, from the staging pool
+    size_t tables_size = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t prims = 0, launches = 0;
+    uint64_t submits = 0;            // the graph is captured on the second submission
     int hip_device = 0;
 
-    void reset()
-    {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (d_tables) (void)hipFree(d_tables);
-        exec = nullptr;
-        graph = nullptr;
-        d_tables = nullptr;
-        steps.clear();
-        arena_gen = UINT64_MAX;
-    }
+    void reset();
     ~Plan() { reset(); }
 };
 
@@ -263,10 +388,30 @@ struct WgcRenderPass_T : Object {
     ~WgcRenderPass_T() override { pass.drop(); }
 };
 
+// Device-side counters of a submission; copied into a pinned host slot at its
+// end, so that finishing one submission can be observed without draining the
+// stream.
+struct DevCounters {
+    unsigned long long frags[kFragSlots];
+    uint32_t error;
+    uint32_t pad[63];
+};
+
 struct Inflight {
     uint64_t index;
     hipEvent_t done;
     std::vector<Object *> keep;  // command buffers used by the submission
+    DevCounters *slot;           // pinned host copy of the counters
+    uint64_t prims, launches;
+};
+
+// Pooled memory whose release waits until the submissions before it have
+// executed (they may still read it).
+struct Deferred {
+    uint64_t after;              // submission index that must have retired
+    BlockPool *pool;
+    uint8_t *ptr;
+    uint64_t size;
 };
 
 struct WgcDevice_T : Object {
@@ -278,13 +423,22 @@ struct WgcDevice_T : Object {
     uint8_t *arena = nullptr;
     size_t arena_size = 0;
     // persistent device words
-    uint32_t *d_error = nullptr;
-    unsigned long long *d_frags = nullptr;
+    DevCounters *d_counters = nullptr;
+    uint32_t *d_error = nullptr;            // &d_counters->error
+    unsigned long long *d_frags = nullptr;  // d_counters->frags
     float *d_srgb = nullptr;
     uint64_t next_index = 1;
     uint64_t last_done = 0;
     std::vector<Inflight> inflight;
     std::vector<hipEvent_t> event_pool;
+    std::vector<DevCounters *> slot_pool;   // free pinned counter slots
+    std::vector<DevCounters *> slot_blocks;
+    BlockPool table_pool;                   // device draw tables of plans
+    BlockPool staging_pool;                 // pinned staging of their uploads
+    std::vector<Deferred> deferred;
+    uint32_t pending_error = 0;             // error bits of retired submissions not yet reported
+    uint64_t last_frags = 0;                // fragments of the last retired submission
+    uint64_t total_frags = 0, total_submissions = 0;
     hipEvent_t ts[64]{};
     bool ts_recorded[64]{};
     bool timing = false;
@@ -306,6 +460,9 @@ struct WgcDevice_T : Object {
             (void)hipEventDestroy(f.done);
             for (auto *o : f.keep) release(o);
         }
+        table_pool.destroy();
+        staging_pool.destroy();
+        for (auto *b : slot_blocks) (void)hipHostFree(b);
         for (auto e : event_pool) (void)hipEventDestroy(e);
         for (auto e : ts)
             if (e) (void)hipEventDestroy(e);
@@ -315,17 +472,29 @@ struct WgcDevice_T : Object {
         }
         for (auto e : timer.pool) (void)hipEventDestroy(e);
         if (arena) (void)hipFree(arena);
-        if (d_error) (void)hipFree(d_error);
-        if (d_frags) (void)hipFree(d_frags);
+        if (d_counters) (void)hipFree(d_counters);
         if (d_srgb) (void)hipFree(d_srgb);
         if (frag_px) (void)hipFree(frag_px);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
+// Gives a sub-allocation back to its heap.  Every submission that used the
+// resource held a reference until it executed, so the bytes are free now.
+static void heap_give_back(WgcHeap_T *heap, uint8_t *ptr, uint64_t size)
+{
+    {
+        std::lock_guard<std::mutex> lk(heap->mu);
+        heap->ra.release((uint64_t)(ptr - heap->base), size);
+    }
+    release(heap);
+}
+
 WgcBuffer_T::~WgcBuffer_T()
 {
-    if (ptr) {
+    if (heap) {
+        heap_give_back(heap, ptr, size);
+    } else if (ptr) {
         (void)hipSetDevice(dev->hip_device);
         (void)hipFree(ptr);
     }
@@ -334,11 +503,45 @@ WgcBuffer_T::~WgcBuffer_T()
 
 WgcTexture_T::~WgcTexture_T()
 {
-    if (ptr) {
+    if (heap) {
+        heap_give_back(heap, ptr, size);
+    } else if (ptr) {
         (void)hipSetDevice(dev->hip_device);
         (void)hipFree(ptr);
     }
     release(dev);
+}
+
+WgcHeap_T::~WgcHeap_T()
+{
+    if (base) {
+        (void)hipSetDevice(dev->hip_device);
+        (void)hipFree(base);
+    }
+    release(dev);
+}
+
+// Releases a plan's pooled memory once the submissions enqueued so far have
+// executed, and the graph.
+void Plan::reset()
+{
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (dev) {
+        uint64_t after = dev->next_index - 1;
+        if (d_tables) dev->deferred.push_back(Deferred{after, &dev->table_pool, d_tables, tables_size});
+        if (h_tables) dev->deferred.push_back(Deferred{after, &dev->staging_pool, h_tables, tables_size});
+        release(dev);
+    }
+    dev = nullptr;
+    exec = nullptr;
+    graph = nullptr;
+    d_tables = nullptr;
+    h_tables = nullptr;
+    tables_size = 0;
+    submits = 0;
+    steps.clear();
+    arena_gen = UINT64_MAX;
 }
 
 // ------------------------------------------------------------------
@@ -365,11 +568,16 @@ extern "C" int32_t wgc_device_create(const WgcDeviceDescriptor *desc, WgcDevice 
     d->semantics = desc->semantics;
     hipError_t e = hipSetDevice(d->hip_device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&d->d_error, sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&d->d_frags, kFragSlots * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&d->d_counters, sizeof(DevCounters));
     if (e == hipSuccess) e = hipMalloc(&d->d_srgb, 256 * sizeof(float));
-    if (e == hipSuccess) e = stream_zero(d->stream, d->d_error, sizeof(uint32_t));
-    if (e == hipSuccess) e = stream_zero(d->stream, d->d_frags, kFragSlots * sizeof(unsigned long long));
+    if (e == hipSuccess) e = stream_zero(d->stream, d->d_counters, sizeof(DevCounters));
+    if (e == hipSuccess) {
+        d->d_error = &d->d_counters->error;
+        d->d_frags = d->d_counters->frags;
+        d->table_pool.block_size = 16ull << 20;
+        d->staging_pool.pinned = true;
+        d->staging_pool.block_size = 16ull << 20;
+    }
     if (e == hipSuccess) {
         float lut[256];
         for (int i = 0; i < 256; ++i) lut[i] = srgb_entry(i);
@@ -383,7 +591,34 @@ extern "C" int32_t wgc_device_create(const WgcDeviceDescriptor *desc, WgcDevice 
     return WGC_OK;
 }
 
-extern "C" void wgc_device_release(WgcDevice device) { release(device); }
+static int32_t retire(WgcDevice_T *d, uint64_t upto, bool wait, uint64_t timeout_ns);
+
+// Waits for the device's submissions first, so that they release the command
+// buffers (and plans) they hold.
+extern "C" void wgc_device_release(WgcDevice device)
+{
+    if (!device) return;
+    {
+        std::lock_guard<std::mutex> lk(device->mu);
+        (void)hipSetDevice(device->hip_device);
+        (void)retire(device, UINT64_MAX, true, 0);
+    }
+    release(device);
+}
+
+static hipError_t take_slot(WgcDevice_T *d, DevCounters *&slot)
+{
+    if (d->slot_pool.empty()) {
+        void *p = nullptr;
+        hipError_t e = hipHostMalloc(&p, 64 * sizeof(DevCounters), hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        d->slot_blocks.push_back((DevCounters *)p);
+        for (int i = 0; i < 64; ++i) d->slot_pool.push_back((DevCounters *)p + i);
+    }
+    slot = d->slot_pool.back();
+    d->slot_pool.pop_back();
+    return hipSuccess;
+}
 
 // Drains finished submissions; with wait, blocks for `upto`.
 static int32_t retire(WgcDevice_T *d, uint64_t upto, bool wait, uint64_t timeout_ns)
@@ -406,8 +641,29 @@ static int32_t retire(WgcDevice_T *d, uint64_t upto, bool wait, uint64_t timeout
         if (q != hipSuccess) return hip_fail(q, "submission");
         d->last_done = f.index;
         d->event_pool.push_back(f.done);
-        for (auto *o : f.keep) release(o);
+        d->pending_error |= f.slot->error;
+        uint64_t frags = 0;
+        for (int i = 0; i < kFragSlots; ++i) frags += f.slot->frags[i];
+        d->last_frags = frags;
+        d->total_frags += frags;
+        d->total_submissions += 1;
+        d->last_prims = f.prims;
+        d->last_launches = f.launches;
+        d->slot_pool.push_back(f.slot);
+        std::vector<Object *> keep = std::move(f.keep);
         d->inflight.erase(d->inflight.begin());
+        for (auto *o : keep) release(o);  // may reset plans, which defer their memory
+    }
+    // memory whose last reader has executed goes back to its pool
+    for (size_t i = 0; i < d->deferred.size();) {
+        Deferred &x = d->deferred[i];
+        if (x.after <= d->last_done) {
+            x.pool->release(x.ptr, x.size);
+            x = d->deferred.back();
+            d->deferred.pop_back();
+        } else {
+            ++i;
+        }
     }
     return WGC_OK;
 }
@@ -428,12 +684,12 @@ static int32_t resolve_timers(WgcDevice_T *d)
     return WGC_OK;
 }
 
+// Reports (once) the error bits of the submissions retired so far.
 static int32_t check_device_error(WgcDevice_T *d)
 {
-    uint32_t err = 0;
-    HIP_TRY(stream_copy(d->stream, &err, d->d_error, sizeof err, hipMemcpyDeviceToHost), "error readback");
+    uint32_t err = d->pending_error;
     if (!err) return WGC_OK;
-    HIP_TRY(stream_zero(d->stream, d->d_error, sizeof err), "error reset");
+    d->pending_error = 0;
     std::string msg = "render pass failed on the device:";
     if (err & ERR_INDEX) msg += " index out of the index buffer (reference: panic);";
     if (err & ERR_VERTEX) msg += " vertex fetch out of the vertex buffer (reference: panic);";
@@ -468,6 +724,19 @@ static int32_t drain(WgcDevice_T *d)
     return WGC_OK;
 }
 
+// Totals over every retired submission; does not wait.
+extern "C" int32_t wgc_device_totals(WgcDevice device, uint64_t *fragments, uint64_t *submissions)
+{
+    if (!device) return set_error(WGC_ERR_INVALID, "null device");
+    std::lock_guard<std::mutex> lk(device->mu);
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    int32_t r = retire(device, UINT64_MAX, false, 0);
+    if (r != WGC_OK) return r;
+    if (fragments) *fragments = device->total_frags;
+    if (submissions) *submissions = device->total_submissions;
+    return WGC_OK;
+}
+
 extern "C" int32_t wgc_device_last_pass_stats(WgcDevice device, WgcPassStats *out)
 {
     if (!device || !out) return set_error(WGC_ERR_INVALID, "null argument");
@@ -475,10 +744,7 @@ extern "C" int32_t wgc_device_last_pass_stats(WgcDevice device, WgcPassStats *ou
     HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
     int32_t r = drain(device);
     if (r != WGC_OK) return r;
-    unsigned long long f[kFragSlots];
-    HIP_TRY(stream_copy(device->stream, f, device->d_frags, sizeof f, hipMemcpyDeviceToHost), "stats readback");
-    out->fragments = 0;
-    for (int i = 0; i < kFragSlots; ++i) out->fragments += f[i];
+    out->fragments = device->last_frags;
     out->primitives = device->last_prims;
     out->kernel_launches = device->last_launches;
     return WGC_OK;
@@ -531,7 +797,7 @@ extern "C" int32_t wgc_buffer_create(WgcDevice device, uint64_t size, WgcBuffer 
         hipError_t e = hipMalloc(&b->ptr, size);
         if (e == hipSuccess) {
             std::lock_guard<std::mutex> lk(device->mu);
-            e = stream_zero(device->stream, b->ptr, size);
+            e = stream_zero_async(device->stream, b->ptr, size);
         }
         if (e != hipSuccess) {
             release(b);
@@ -544,6 +810,28 @@ extern "C" int32_t wgc_buffer_create(WgcDevice device, uint64_t size, WgcBuffer 
 
 extern "C" void wgc_buffer_release(WgcBuffer buffer) { release(buffer); }
 
+// Queue writes are ordered after every earlier submission on the device
+// stream and before every later one; the caller's memory is free to reuse on
+// return, like the reference's synchronous write (device.rs:332-344).  Up to
+// 64 MiB the bytes go through pooled pinned staging and the host does not
+// wait (the staging returns to its pool once the next submission has
+// executed); larger writes copy from the caller's memory and wait.
+static hipError_t staged_upload(WgcDevice_T *d, uint8_t *dst, const void *src, uint64_t n)
+{
+    if (n <= (64ull << 20)) {
+        uint8_t *stage = nullptr;
+        hipError_t e = d->staging_pool.alloc(n, stage);
+        if (e != hipSuccess) return e;
+        memcpy(stage, src, n);
+        e = hipMemcpyAsync(dst, stage, n, hipMemcpyHostToDevice, d->stream);
+        d->deferred.push_back(Deferred{d->next_index, &d->staging_pool, stage, n});
+        return e;
+    }
+    hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    return e;
+}
+
 extern "C" int32_t wgc_queue_write_buffer(WgcDevice device, WgcBuffer buffer, uint64_t offset,
                                           const void *data, uint64_t size)
 {
@@ -553,12 +841,7 @@ extern "C" int32_t wgc_queue_write_buffer(WgcDevice device, WgcBuffer buffer, ui
     if (!size) return WGC_OK;
     std::lock_guard<std::mutex> lk(device->mu);
     HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
-    // Ordered after every earlier submission on the device stream; the
-    // caller's memory is free to reuse on return, like the reference's
-    // synchronous write (device.rs:332-344).
-    HIP_TRY(hipMemcpyAsync(buffer->ptr + offset, data, size, hipMemcpyHostToDevice, device->stream),
-            "write_buffer");
-    HIP_TRY(hipStreamSynchronize(device->stream), "write_buffer");
+    HIP_TRY(staged_upload(device, buffer->ptr + offset, data, size), "write_buffer");
     return WGC_OK;
 }
 
@@ -610,7 +893,7 @@ extern "C" int32_t wgc_texture_create(WgcDevice device, uint32_t width, uint32_t
     hipError_t e = hipMalloc(&t->ptr, t->size);
     if (e == hipSuccess) {
         std::lock_guard<std::mutex> lk(device->mu);
-        e = stream_zero(device->stream, t->ptr, t->size);
+        e = stream_zero_async(device->stream, t->ptr, t->size);
     }
     if (e != hipSuccess) {
         release(t);
@@ -621,6 +904,110 @@ extern "C" int32_t wgc_texture_create(WgcDevice device, uint32_t width, uint32_t
 }
 
 extern "C" void wgc_texture_release(WgcTexture texture) { release(texture); }
+
+// ------------------------------------------------------------------
+// heaps
+// ------------------------------------------------------------------
+
+extern "C" int32_t wgc_heap_create(WgcDevice device, uint64_t size, WgcHeap *out)
+{
+    if (!device || !out || !size) return set_error(WGC_ERR_INVALID, "bad heap arguments");
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    auto *h = new WgcHeap_T();
+    h->dev = device;
+    retain(device);
+    size = (size + RangeAllocator::kAlign - 1) / RangeAllocator::kAlign * RangeAllocator::kAlign;
+    if (hipMalloc(&h->base, size) != hipSuccess) {
+        h->base = nullptr;
+        release(h);
+        return set_error(WGC_ERR_OUT_OF_MEMORY, "heap: cannot allocate " + std::to_string(size) + " bytes");
+    }
+    h->ra.init(size);
+    *out = h;
+    return WGC_OK;
+}
+
+extern "C" void wgc_heap_release(WgcHeap heap) { release(heap); }
+
+extern "C" int32_t wgc_heap_stats(WgcHeap heap, uint64_t *used, uint64_t *capacity, uint64_t *largest_free)
+{
+    if (!heap) return set_error(WGC_ERR_INVALID, "null heap");
+    std::lock_guard<std::mutex> lk(heap->mu);
+    if (used) *used = heap->ra.used();
+    if (capacity) *capacity = heap->ra.capacity();
+    if (largest_free) *largest_free = heap->ra.largest_free();
+    return WGC_OK;
+}
+
+static int32_t heap_take(WgcHeap_T *heap, uint64_t size, uint8_t *&ptr)
+{
+    std::lock_guard<std::mutex> lk(heap->mu);
+    uint64_t off = 0;
+    if (!heap->ra.alloc(size, off))
+        return set_error(WGC_ERR_OUT_OF_MEMORY, "heap full (" + std::to_string(heap->ra.used()) + " of " +
+                                                    std::to_string(heap->ra.capacity()) + " bytes in use)");
+    ptr = heap->base + off;
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_buffer_create_in_heap(WgcHeap heap, uint64_t size, WgcBuffer *out)
+{
+    if (!heap || !out || !size) return set_error(WGC_ERR_INVALID, "bad argument");
+    WgcDevice_T *device = heap->dev;
+    uint8_t *ptr = nullptr;
+    int32_t r = heap_take(heap, size, ptr);
+    if (r != WGC_OK) return r;
+    auto *b = new WgcBuffer_T();
+    b->dev = device;
+    retain(device);
+    b->heap = heap;
+    retain(heap);
+    b->ptr = ptr;
+    b->size = size;
+    std::lock_guard<std::mutex> lk(device->mu);
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    hipError_t e = stream_zero_async(device->stream, ptr, size);
+    if (e != hipSuccess) {
+        release(b);
+        return hip_fail(e, "buffer zero fill");
+    }
+    *out = b;
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_texture_create_in_heap(WgcHeap heap, uint32_t width, uint32_t height, uint32_t format,
+                                              WgcTexture *out)
+{
+    if (!heap || !out) return set_error(WGC_ERR_INVALID, "null argument");
+    uint32_t tb = texel_bytes(format);
+    if (!tb) return set_error(WGC_ERR_UNSUPPORTED, "unsupported texture format");
+    if (width == 0 || height == 0 || width > 65535u * 8u || height > 65535u * 8u)
+        return set_error(WGC_ERR_INVALID, "bad texture size");
+    WgcDevice_T *device = heap->dev;
+    uint64_t size = (uint64_t)width * height * tb;
+    uint8_t *ptr = nullptr;
+    int32_t r = heap_take(heap, size, ptr);
+    if (r != WGC_OK) return r;
+    auto *t = new WgcTexture_T();
+    t->dev = device;
+    retain(device);
+    t->heap = heap;
+    retain(heap);
+    t->ptr = ptr;
+    t->width = width;
+    t->height = height;
+    t->format = format;
+    t->size = size;
+    std::lock_guard<std::mutex> lk(device->mu);
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    hipError_t e = stream_zero_async(device->stream, ptr, size);
+    if (e != hipSuccess) {
+        release(t);
+        return hip_fail(e, "texture zero fill");
+    }
+    *out = t;
+    return WGC_OK;
+}
 
 extern "C" int32_t wgc_queue_write_texture(WgcDevice device, WgcTexture texture, const void *data,
                                            uint64_t data_size, uint64_t offset, uint32_t bytes_per_row)
@@ -636,9 +1023,7 @@ extern "C" int32_t wgc_queue_write_texture(WgcDevice device, WgcTexture texture,
         return set_error(WGC_ERR_INVALID, "write_texture past the texture (reference: slice panic)");
     std::lock_guard<std::mutex> lk(device->mu);
     HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
-    HIP_TRY(hipMemcpyAsync(texture->ptr, (const uint8_t *)data + offset, n, hipMemcpyHostToDevice,
-                           device->stream), "write_texture");
-    HIP_TRY(hipStreamSynchronize(device->stream), "write_texture");
+    HIP_TRY(staged_upload(device, texture->ptr, (const uint8_t *)data + offset, n), "write_texture");
     return WGC_OK;
 }
 
@@ -1444,8 +1829,20 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         dev->arena_size = grow;
         dev->arena_gen++;
     }
-    std::vector<uint8_t> host_tables(tables);
-    if (tables) HIP_TRY(hipMalloc(&plan.d_tables, tables), "draw tables");
+    // draw tables: pooled device memory, uploaded from pooled pinned staging
+    // without a host wait (the upload is ordered before the plan's launches)
+    if (tables) {
+        HIP_TRY(dev->table_pool.alloc(tables, plan.d_tables), "draw tables");
+        hipError_t e = dev->staging_pool.alloc(tables, plan.h_tables);
+        if (e != hipSuccess) {
+            dev->table_pool.release(plan.d_tables, tables);
+            plan.d_tables = nullptr;
+            return hip_fail(e, "draw table staging");
+        }
+        plan.tables_size = tables;
+        plan.dev = dev;
+        retain(dev);
+    }
     for (Step &s : plan.steps) {
         if (s.kind != StepKind::Group) {
             plan.launches += 1;
@@ -1456,7 +1853,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         for (auto &d : s.draws) plan.prims += d.total_prims;
         if (!s.multi) continue;
         uint32_t n = (uint32_t)s.draws.size();
-        uint8_t *h = host_tables.data() + s.table_off;
+        uint8_t *h = plan.h_tables + s.table_off;
         uint8_t *dptr = plan.d_tables + s.table_off;
         memcpy(h, s.draws.data(), n * sizeof(DrawParams));
         uint32_t *pb = (uint32_t *)(h + n * sizeof(DrawParams));
@@ -1478,7 +1875,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         s.batch.total_tiles = acc_t;
     }
     if (tables)
-        HIP_TRY(stream_copy(dev->stream, plan.d_tables, host_tables.data(), tables, hipMemcpyHostToDevice),
+        HIP_TRY(hipMemcpyAsync(plan.d_tables, plan.h_tables, tables, hipMemcpyHostToDevice, dev->stream),
                 "draw tables");
     plan.arena_gen = dev->arena_gen;
     plan.raster_cfg = cfg;
@@ -1488,8 +1885,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
 
 static hipError_t launch_plan(WgcDevice_T *dev, const Plan &plan, KernelTimer *kt)
 {
-    hipError_t e = launch_reset(dev->d_frags, dev->stream);
-    if (e != hipSuccess) return e;
+    hipError_t e = hipSuccess;
     for (const Step &s : plan.steps) {
         if (s.kind == StepKind::Clear) e = launch_clear(s.color, s.rgba, s.depth, s.dv, s.n, dev->stream, kt);
         else if (s.multi) e = launch_batch(s.draws[0], s.batch, s.gl, dev->stream, kt);
@@ -1499,10 +1895,14 @@ static hipError_t launch_plan(WgcDevice_T *dev, const Plan &plan, KernelTimer *k
     return hipSuccess;
 }
 
+// The first submission of a plan launches directly; a command buffer that
+// is submitted again is captured into a hipGraph then and replayed.
 static int32_t submit_plan(WgcDevice_T *dev, Plan &plan)
 {
     KernelTimer *kt = dev->timing ? &dev->timer : nullptr;
-    if (!dev->use_graphs || kt) {
+    bool graph = dev->use_graphs && !kt && (plan.exec || plan.submits > 0);
+    plan.submits++;
+    if (!graph) {
         HIP_TRY(launch_plan(dev, plan, kt), "launch");
         return WGC_OK;
     }
@@ -1530,9 +1930,14 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
     std::lock_guard<std::mutex> lk(device->mu);
     HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
     Inflight f;
+    f.prims = f.launches = 0;
+    HIP_TRY(take_slot(device, f.slot), "pinned counter slot");
+    hipError_t e0 = launch_reset(device->d_frags, device->d_error, device->stream);
+    if (e0 != hipSuccess) {
+        device->slot_pool.push_back(f.slot);
+        return hip_fail(e0, "counter reset");
+    }
     f.index = device->next_index++;
-    device->last_prims = 0;
-    device->last_launches = 0;
     for (uint32_t i = 0; i < count; ++i) {
         WgcCommandBuffer_T *cb = buffers[i];
         if (!cb) return set_error(WGC_ERR_INVALID, "null command buffer");
@@ -1552,11 +1957,14 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
         }
         int32_t r = submit_plan(device, plan);
         if (r != WGC_OK) return r;
-        device->last_prims += plan.prims;
-        device->last_launches += plan.launches;
+        f.prims += plan.prims;
+        f.launches += plan.launches;
         retain(cb);
         f.keep.push_back(cb);
     }
+    // the submission's counters, read by the host once its event has passed
+    HIP_TRY(hipMemcpyAsync(f.slot, device->d_counters, offsetof(DevCounters, pad), hipMemcpyDeviceToHost,
+                           device->stream), "counter readback");
     if (!device->event_pool.empty()) {
         f.done = device->event_pool.back();
         device->event_pool.pop_back();

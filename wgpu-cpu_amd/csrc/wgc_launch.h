@@ -96,7 +96,8 @@ hipError_t launch_draw(const DrawParams &d, const BatchArgs &b, const GroupLaunc
                        KernelTimer *kt);
 hipError_t launch_batch(const DrawParams &d0, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
                         KernelTimer *kt);
-hipError_t launch_reset(unsigned long long *frags, hipStream_t st);
+// Zeroes the submission counters (fragment slots and error bits).
+hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t st);
 hipError_t launch_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, uint64_t n,
                         hipStream_t st, KernelTimer *kt);
 

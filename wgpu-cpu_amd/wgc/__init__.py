@@ -24,6 +24,14 @@ _LIB_PATH = Path(__file__).resolve().parent / "libwgc.so"
 SEMANTICS_REFERENCE = 0
 SEMANTICS_EXTENDED = 1
 
+OK = 0
+ERR_INVALID = -1
+ERR_UNSUPPORTED = -2
+ERR_OUT_OF_MEMORY = -3
+ERR_DEVICE = -4
+ERR_TIMEOUT = -5
+ERR_EXECUTION = -6
+
 RGBA8_UNORM = 0
 RGBA8_UNORM_SRGB = 1
 BGRA8_UNORM = 2
@@ -153,6 +161,11 @@ SIGNATURES = {
     "wgc_queue_write_texture": (C.c_int32, [_P, _P, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32]),
     "wgc_texture_read": (C.c_int32, [_P, C.c_void_p, C.c_uint64]),
     "wgc_texture_release": (None, [_P]),
+    "wgc_heap_create": (C.c_int32, [_P, C.c_uint64, _PP]),
+    "wgc_heap_release": (None, [_P]),
+    "wgc_heap_stats": (C.c_int32, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "wgc_buffer_create_in_heap": (C.c_int32, [_P, C.c_uint64, _PP]),
+    "wgc_texture_create_in_heap": (C.c_int32, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _PP]),
     "wgc_sampler_create": (C.c_int32, [_P, C.POINTER(SamplerDescriptor), _PP]),
     "wgc_sampler_release": (None, [_P]),
     "wgc_bind_group_create": (C.c_int32, [_P, C.c_uint32, C.POINTER(BindGroupEntry), _PP]),
@@ -180,6 +193,7 @@ SIGNATURES = {
     "wgc_queue_submit": (C.c_int32, [_P, C.c_uint32, _PP, C.POINTER(C.c_uint64)]),
     "wgc_device_poll": (C.c_int32, [_P, C.c_uint64, C.c_uint64]),
     "wgc_device_last_pass_stats": (C.c_int32, [_P, C.POINTER(PassStats)]),
+    "wgc_device_totals": (C.c_int32, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "wgc_device_set_option": (C.c_int32, [_P, C.c_uint32, C.c_uint32]),
     "wgc_device_get_option": (C.c_int32, [_P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "wgc_device_debug_fragment_counts": (C.c_int32, [_P, C.c_uint32, C.c_uint32, C.c_void_p]),
@@ -277,6 +291,38 @@ class Texture(_Handle):
 
 class Sampler(_Handle):
     _release = "wgc_sampler_release"
+
+
+class Heap(_Handle):
+    """Device memory pool for resources created at a high rate (no reference
+    counterpart).  Creation returns None when the heap is full, so that a
+    caller can hold work back until memory frees up."""
+    _release = "wgc_heap_release"
+
+    def __init__(self, ptr, size):
+        super().__init__(ptr)
+        self.size = size
+
+    def stats(self):
+        used, cap, largest = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(load_library().wgc_heap_stats(self._ptr, C.byref(used), C.byref(cap), C.byref(largest)))
+        return used.value, cap.value, largest.value
+
+    def create_texture(self, width: int, height: int, format: int) -> Optional["Texture"]:
+        out = _out()
+        code = load_library().wgc_texture_create_in_heap(self._ptr, width, height, format, C.byref(out))
+        if code == ERR_OUT_OF_MEMORY:
+            return None
+        _check(code)
+        return Texture(out, width, height, format)
+
+    def create_buffer(self, size: int) -> Optional["Buffer"]:
+        out = _out()
+        code = load_library().wgc_buffer_create_in_heap(self._ptr, size, C.byref(out))
+        if code == ERR_OUT_OF_MEMORY:
+            return None
+        _check(code)
+        return Buffer(out, size)
 
 
 class BindGroup(_Handle):
@@ -410,6 +456,11 @@ class Device:
         _check(self._lib.wgc_buffer_create(self._ptr, size, C.byref(out)))
         return Buffer(out, size)
 
+    def create_heap(self, size: int) -> Heap:
+        out = _out()
+        _check(self._lib.wgc_heap_create(self._ptr, size, C.byref(out)))
+        return Heap(out, size)
+
     def create_buffer_init(self, data: bytes) -> Buffer:
         b = self.create_buffer(len(data))
         self.write_buffer(b, 0, data)
@@ -513,6 +564,15 @@ class Device:
         idx = (1 << 64) - 1 if submission_index is None else submission_index
         _check(self._lib.wgc_device_poll(self._ptr, idx, timeout_ns))
 
+    def is_done(self, submission_index: int) -> bool:
+        """Non-blocking: whether the submission has executed (raises if one of
+        the submissions it retires failed)."""
+        code = self._lib.wgc_device_poll(self._ptr, submission_index, 1)
+        if code == ERR_TIMEOUT:
+            return False
+        _check(code)
+        return True
+
     def set_option(self, option: int, value: int):
         _check(self._lib.wgc_device_set_option(self._ptr, option, value))
 
@@ -525,6 +585,12 @@ class Device:
         s = PassStats()
         _check(self._lib.wgc_device_last_pass_stats(self._ptr, C.byref(s)))
         return s
+
+    def totals(self):
+        """(fragments, submissions) over every executed submission; no wait."""
+        f, n = C.c_uint64(), C.c_uint64()
+        _check(self._lib.wgc_device_totals(self._ptr, C.byref(f), C.byref(n)))
+        return f.value, n.value
 
     def enable_fragment_counts(self, width: int, height: int):
         _check(self._lib.wgc_device_debug_fragment_counts(self._ptr, width, height, None))
