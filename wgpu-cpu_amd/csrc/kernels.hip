@@ -526,9 +526,13 @@ struct RasterShared {
     TriLds tri[kBatch];
     ScanTri scan[kBatch];
     uint32_t wide[kBatch];
-    uint32_t span[kBatch][TS];
+    uint32_t span[kBatch][TS];      // ordered path: per-row spans of the staged triangles
     uint32_t keys[kBatch];
     uint32_t nkeys;
+    // order-independent path: per pixel of the tile
+    unsigned long long best[TS * TS];  // encoded winner (see oi_value), ~0 = none
+    float idep[TS * TS];               // depth before the draw
+    uint32_t cnt[TS * TS];             // emitted fragments (debug counts only)
 };
 
 // Loads one list entry's triangle record into LDS.
@@ -698,21 +702,64 @@ __device__ __forceinline__ void shade_fragment(const DrawParams &d, const TriLds
     }
 }
 
-// Colour of a pixel's winning fragment: its triangle record is read back
-// (from L2, shared by the neighbouring pixels it won) and the same
+// Colour and depth of a pixel's winning fragment: its triangle record is
+// read back (from L2, shared by the neighbouring pixels it won) and the same
 // barycentrics are evaluated again, bit for bit.
 template <int PROG>
-__device__ __forceinline__ uint32_t winner_color(const DrawParams &d, uint32_t key, float pxf, float pyf)
+__device__ __forceinline__ uint32_t winner_color(const DrawParams &d, uint32_t key, float pxf, float pyf, float &z)
 {
     TriLds T;
     ScanTri S;
     uint32_t wide;
     stage_record(d, key, T, S, wide);
     float lc[3];
-    (void)bary_depth(T, pxf, pyf, lc);
+    z = bary_depth(T, pxf, pyf, lc);
     float color[4];
     fragment_color<PROG>(d, T, lc, color);
     return pack_color(color, d.color_bgra);
+}
+
+// Order-independent resolve as a minimum over 64-bit values: the depth in
+// an unsigned encoding that keeps IEEE order (with -0 equal to +0, as the
+// comparison treats them) in the high word, the primitive key (or its
+// complement, to prefer the later fragment) in the low word.  ~0 means no
+// fragment; no candidate encodes to it (NaN depths never pass the test).
+__device__ __forceinline__ uint32_t depth_order(float z)
+{
+    uint32_t u = __float_as_uint(z);
+    if ((u << 1) == 0u) u = 0u;
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ unsigned long long oi_value(uint32_t resolve, float z, uint32_t key)
+{
+    switch (resolve) {
+    case RES_MIN_FIRST: return ((unsigned long long)depth_order(z) << 32) | key;
+    case RES_MIN_LAST: return ((unsigned long long)depth_order(z) << 32) | (~key);
+    case RES_MAX_FIRST: return ((unsigned long long)(~depth_order(z)) << 32) | key;
+    case RES_MAX_LAST: return ((unsigned long long)(~depth_order(z)) << 32) | (~key);
+    default: return (unsigned long long)(~key);  // last in API order
+    }
+}
+
+__device__ __forceinline__ uint32_t oi_key(uint32_t resolve, unsigned long long v)
+{
+    uint32_t lo = (uint32_t)v;
+    return (resolve == RES_MIN_FIRST || resolve == RES_MAX_FIRST) ? lo : ~lo;
+}
+
+// The depth test of the order-independent modes, against the depth the
+// pixel held before the draw.
+__device__ __forceinline__ bool oi_pass(const DrawParams &d, float z, float init)
+{
+    switch (d.resolve) {
+    case RES_LAST: return true;
+    case RES_MIN_FIRST: return z < init;
+    case RES_MIN_LAST: return z <= init;
+    case RES_MAX_FIRST: return z > init;
+    case RES_MAX_LAST: return z >= init;
+    default: return compare(d.depth_compare, z, init);  // RES_PASS_LAST
+    }
 }
 
 // Stages sh.keys[0..n) into LDS and applies them to this lane's pixels.
@@ -907,12 +954,146 @@ __device__ __forceinline__ void raster_tile(const DrawParams &d, uint32_t tile, 
             write_depth = ps[k].has && d.depth_test && d.depth_write &&
                           (d.resolve != RES_PASS_LAST || d.depth_compare == CMP_EQUAL ||
                            d.depth_compare == CMP_ALWAYS);
-            ps[k].col = ps[k].has ? winner_color<PROG>(d, ps[k].key, (float)px, (float)py) : init_col[k];
+            float zw;
+            ps[k].col = ps[k].has ? winner_color<PROG>(d, ps[k].key, (float)px, (float)py, zw) : init_col[k];
         }
         if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = ps[k].col;
         if (d.depth) d.depth[pix] = write_depth ? in_vgpr(ps[k].dep) : in_vgpr(ps[k].init_dep);
         if (d.s.frag_px && nfrag[k]) d.s.frag_px[pix] += nfrag[k];
     });
+    frag_acc += f;
+}
+
+// Stages sh.keys[0..n) and merges their fragments into the tile's winners:
+// one lane per (triangle, row) computes the row's span (the reference's
+// integer scanline rule, clipped to the scissor and the tile) and walks its
+// pixels, so the work follows the fragments rather than triangles x pixels.
+template <int TS, int NT>
+__device__ __forceinline__ uint32_t oi_batch(const DrawParams &d, RasterShared<TS> &sh, uint32_t n, uint32_t x0,
+                                             uint32_t y0)
+{
+    uint32_t tid = threadIdx.x;
+    if (tid < n) stage_record(d, sh.keys[tid], sh.tri[tid], sh.scan[tid], sh.wide[tid]);
+    __syncthreads();
+    int64_t sx0 = d.scissor[0], sy0 = d.scissor[1];
+    int64_t sx1 = sx0 + d.scissor[2], sy1 = sy0 + d.scissor[3];
+    bool counts = d.s.frag_px != nullptr;
+    uint32_t f = 0;
+    for (uint32_t idx = tid; idx < n * (uint32_t)TS; idx += NT) {
+        uint32_t e = idx / TS, row = idx % TS;
+        int32_t y = (int32_t)(y0 + row);
+        int32_t x1, x2;
+        if ((int64_t)y < sy0 || (int64_t)y >= sy1 || !scan_span(sh.scan[e], y, x1, x2, sh.wide[e] != 0u)) continue;
+        int64_t lo = max((int64_t)x1, max(sx0, (int64_t)x0));
+        int64_t hi = min((int64_t)x2, min(sx1 - 1, (int64_t)x0 + TS - 1));
+        if (lo > hi) continue;
+        const TriLds &T = sh.tri[e];
+        float pyf = (float)y;
+        for (int64_t x = lo; x <= hi; ++x) {
+            uint32_t q = row * TS + (uint32_t)(x - x0);
+            float lc[3];
+            float z = bary_depth(T, (float)x, pyf, lc);
+            ++f;
+            if (counts) atomicAdd(&sh.cnt[q], 1u);
+            if (oi_pass(d, z, sh.idep[q])) atomicMin(&sh.best[q], oi_value(d.resolve, z, T.key));
+        }
+    }
+    __syncthreads();
+    return f;
+}
+
+// One active tile of an order-independent draw: load or clear, merge the
+// winners of its bin segment and of the overlapping big triangles, shade
+// each pixel's winner once, store colour and depth.
+template <int PROG, int TS, int NT>
+__device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t tile, RasterShared<TS> &sh,
+                                               uint32_t &frag_acc)
+{
+    constexpr int PPT = TS * TS / NT;
+    uint32_t tx = tile % d.tiles_x, ty = tile / d.tiles_x;
+    uint32_t x0 = tx * TS, y0 = ty * TS;
+    uint32_t tid = threadIdx.x;
+    uint32_t seg0 = d.s.tile_offset[tile];
+    uint32_t seg_n = d.s.tile_offset[tile + 1] - seg0;
+    uint32_t nbig = min(d.s.counters[C_BIG], d.s.big_cap);
+
+    uint32_t init_col[PPT];
+    float init_dep[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        uint32_t q = tid + (uint32_t)k * NT;
+        uint32_t px = x0 + q % TS, py = y0 + q / TS;
+        size_t pix = (size_t)py * d.width + px;
+        uint32_t c = 0u;
+        float z = 0.0f;
+        if (px < d.width && py < d.height) {
+            if (d.color)
+                c = (d.init_clear && d.clear_color_on) ? d.clear_rgba : reinterpret_cast<const uint32_t *>(d.color)[pix];
+            if (d.depth) z = (d.init_clear && d.clear_depth_on) ? d.clear_depth : d.depth[pix];
+        }
+        init_col[k] = c;
+        init_dep[k] = z;
+        sh.idep[q] = z;
+        sh.best[q] = ~0ull;
+        sh.cnt[q] = 0u;
+    }
+    __syncthreads();
+
+    uint32_t f = 0, fpos = 0, bpos = 0;
+    for (;;) {
+        uint32_t n;
+        if (fpos < seg_n) {
+            n = min((uint32_t)kBatch, seg_n - fpos);
+            if (tid < n) sh.keys[tid] = d.s.bins[seg0 + fpos + tid];
+            fpos += n;
+        } else if (bpos < nbig) {
+            if (tid < kBatch) {  // wave 0 compacts the hits
+                bool hit = false;
+                uint32_t key = 0u;
+                if (bpos + tid < nbig) {
+                    uint4 r = d.s.big[bpos + tid];
+                    hit = tx >= (r.y & 0xFFFFu) && tx <= (r.z & 0xFFFFu) && ty >= (r.y >> 16) && ty <= (r.z >> 16);
+                    key = r.x;
+                }
+                uint64_t mask = __ballot(hit);
+                uint64_t below = (tid == 0) ? 0ull : (mask & ((~0ull) >> (64 - tid)));
+                if (hit) sh.keys[__popcll(below)] = key;
+                if (tid == 0) sh.nkeys = (uint32_t)__popcll(mask);
+            }
+            __syncthreads();
+            n = sh.nkeys;
+            bpos += kBatch;
+            if (n == 0) {
+                __syncthreads();  // sh.nkeys is rewritten next round
+                continue;
+            }
+        } else {
+            break;
+        }
+        f += oi_batch<TS, NT>(d, sh, n, x0, y0);
+    }
+
+    bool write_depth_mode = d.depth_test && d.depth_write &&
+                            (d.resolve != RES_PASS_LAST || d.depth_compare == CMP_EQUAL ||
+                             d.depth_compare == CMP_ALWAYS);
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        uint32_t q = tid + (uint32_t)k * NT;
+        uint32_t px = x0 + q % TS, py = y0 + q / TS;
+        if (px >= d.width || py >= d.height) continue;
+        size_t pix = (size_t)py * d.width + px;
+        unsigned long long v = sh.best[q];
+        uint32_t col = init_col[k];
+        float dep = init_dep[k];
+        if (v != ~0ull) {
+            float z;
+            col = winner_color<PROG>(d, oi_key(d.resolve, v), (float)px, (float)py, z);
+            if (write_depth_mode) dep = z;
+        }
+        if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = col;
+        if (d.depth) d.depth[pix] = dep;
+        if (d.s.frag_px && sh.cnt[q]) d.s.frag_px[pix] += sh.cnt[q];
+    }
     frag_acc += f;
 }
 
@@ -972,7 +1153,10 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
             di = find_slot(b.tile_base, b.n, gt);
             tile = gt - b.tile_base[di];
         }
-        raster_tile<PROG, ORDERED, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, olist, olist_n, frag_acc);
+        if constexpr (ORDERED)
+            raster_tile<PROG, true, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, olist, olist_n, frag_acc);
+        else
+            raster_tile_oi<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, frag_acc);
     }
     for (uint32_t c = blockIdx.x; c < b.n_chunks; c += gridDim.x) clear_chunk<MULTI, TS, NT>(d0, b, c);
 
@@ -1032,6 +1216,55 @@ static hipError_t launch_raster(const DrawParams &d, const BatchArgs &b, uint32_
     default: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 16, 256>), grid, 256, d, b); break;
     }
     return hipSuccess;
+}
+
+// Workgroups of a raster instance that fit on one CU at once (registers,
+// LDS, waves); the persistent grid is sized to what can be resident, so that
+// the static schedule never waits for a second round of workgroups.
+template <int PROG, bool ORDERED, bool MULTI, int TS, int NT>
+static uint32_t raster_occupancy()
+{
+    static uint32_t cached = 0;
+    if (!cached) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_raster<PROG, ORDERED, MULTI, TS, NT>, NT, 0) !=
+                hipSuccess ||
+            n < 1)
+            n = 1;
+        cached = (uint32_t)n;
+    }
+    return cached;
+}
+
+template <int PROG, bool MULTI>
+static uint32_t raster_occupancy_cfg(bool ordered, uint32_t cfg)
+{
+    if (ordered) return raster_occupancy<PROG, true, MULTI, 16, 256>();
+    switch (cfg) {
+    case RASTER_8x8_64: return raster_occupancy<PROG, false, MULTI, 8, 64>();
+    case RASTER_16x16_64: return raster_occupancy<PROG, false, MULTI, 16, 64>();
+    case RASTER_32x32_256: return raster_occupancy<PROG, false, MULTI, 32, 256>();
+    default: return raster_occupancy<PROG, false, MULTI, 16, 256>();
+    }
+}
+
+uint32_t raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32_t cfg)
+{
+    switch (program) {
+    case PROG_MVP_COLOR:
+        return multi ? raster_occupancy_cfg<PROG_MVP_COLOR, true>(ordered, cfg)
+                     : raster_occupancy_cfg<PROG_MVP_COLOR, false>(ordered, cfg);
+    case PROG_MVP_TEXTURE:
+        return multi ? raster_occupancy_cfg<PROG_MVP_TEXTURE, true>(ordered, cfg)
+                     : raster_occupancy_cfg<PROG_MVP_TEXTURE, false>(ordered, cfg);
+    case PROG_COLORED_TRIANGLE:
+        return multi ? raster_occupancy_cfg<PROG_COLORED_TRIANGLE, true>(ordered, cfg)
+                     : raster_occupancy_cfg<PROG_COLORED_TRIANGLE, false>(ordered, cfg);
+    case PROG_CLIP_COLOR:
+        return multi ? raster_occupancy_cfg<PROG_CLIP_COLOR, true>(ordered, cfg)
+                     : raster_occupancy_cfg<PROG_CLIP_COLOR, false>(ordered, cfg);
+    default: return 1;
+    }
 }
 
 template <int PROG, bool MULTI>

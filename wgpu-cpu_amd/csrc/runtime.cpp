@@ -446,6 +446,7 @@ struct WgcDevice_T : Object {
     std::map<std::string, std::pair<double, uint64_t>> ktimes;
     uint64_t last_prims = 0, last_launches = 0;
     uint64_t arena_gen = 0;        // bumped whenever memory a cached plan points at moves
+    uint32_t cu_count = 256;        // compute units of the device
     uint32_t raster_cfg = RASTER_16x16_256;
     bool batching = true;          // merge independent single-draw passes into launch groups
     bool use_graphs = true;        // replay cached plans as hipGraphs
@@ -568,6 +569,11 @@ extern "C" int32_t wgc_device_create(const WgcDeviceDescriptor *desc, WgcDevice 
     d->semantics = desc->semantics;
     hipError_t e = hipSetDevice(d->hip_device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+        int cus = 0;
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->hip_device);
+        if (e == hipSuccess && cus > 0) d->cu_count = (uint32_t)cus;
+    }
     if (e == hipSuccess) e = hipMalloc(&d->d_counters, sizeof(DevCounters));
     if (e == hipSuccess) e = hipMalloc(&d->d_srgb, 256 * sizeof(float));
     if (e == hipSuccess) e = stream_zero(d->stream, d->d_counters, sizeof(DevCounters));
@@ -1541,13 +1547,16 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
     return WGC_OK;
 }
 
-// Persistent raster grid: enough workgroups to fill every CU a few times
-// over, never more than there are work items.
-static uint32_t raster_grid(uint32_t cfg, bool ordered, uint64_t items)
+// Persistent raster grid: every workgroup resident at once (CUs x the
+// kernel's occupancy), never more than there are work items, a multiple of
+// 8 (the raster splits its work over the 8 XCDs).
+static uint32_t raster_grid(WgcDevice_T *dev, uint32_t program, uint32_t cfg, bool ordered, bool multi,
+                            uint64_t items)
 {
-    uint64_t cap = (!ordered && (cfg == RASTER_8x8_64 || cfg == RASTER_16x16_64)) ? 4096u : 2048u;
-    uint64_t g = std::max<uint64_t>(1u, std::min<uint64_t>(items, cap));
-    return (uint32_t)align_up(g, 8);  // the raster splits its work over the 8 XCDs
+    uint64_t cap = (uint64_t)dev->cu_count * raster_blocks_per_cu(program, ordered, multi, cfg);
+    cap = std::max<uint64_t>(8u, cap / 8 * 8);
+    uint64_t g = align_up(std::max<uint64_t>(1u, std::min<uint64_t>(items, cap)), 8);
+    return (uint32_t)std::min(g, cap);
 }
 
 // Lays a group's scratch out at the start of the arena (groups run one after
@@ -1628,7 +1637,7 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
         b.tile_base = nullptr;
         b.total_prims = st.draws[0].total_prims;
     }
-    gl.raster_grid = raster_grid(gl.raster_cfg, gl.ordered != 0, total_tiles + b.n_chunks);
+    gl.raster_grid = raster_grid(dev, gl.program, gl.raster_cfg, gl.ordered != 0, st.multi, total_tiles + b.n_chunks);
     uint64_t tile_base = 0;
     for (uint32_t i = 0; i < n; ++i) {
         DrawParams &d = st.draws[i];

@@ -96,6 +96,9 @@ hipError_t launch_draw(const DrawParams &d, const BatchArgs &b, const GroupLaunc
                        KernelTimer *kt);
 hipError_t launch_batch(const DrawParams &d0, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
                         KernelTimer *kt);
+// Workgroups of the raster kernel for (program, ordered, batched, tile
+// shape) that fit on one CU at once.
+uint32_t raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32_t cfg);
 // Zeroes the submission counters (fragment slots and error bits).
 hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t st);
 hipError_t launch_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, uint64_t n,
