@@ -25,7 +25,7 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "wgpu-cpu_amd"))
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["k_raster", "k_setup", "k_bin_fill", "k_scan_blocks", "k_scan_totals", "k_scan_tiles", "k_clip",
+KERNELS = ["k_raster", "k_visibility", "k_shade", "k_setup", "k_bin_fill", "k_scan_blocks", "k_scan_totals", "k_scan_tiles", "k_clip",
            "k_zero", "k_clear"]
 
 

@@ -177,7 +177,9 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
     if constexpr (MULTI) {
         // draw ranges are padded to whole waves, so a wave never spans draws
         if (g >= b.total_prims) return;
-        di = find_slot(b.prim_base, b.n, g);
+        // every lane of the wave is in the same draw: look it up once, in
+        // scalar registers, so that the draw's parameters are scalar loads
+        di = __builtin_amdgcn_readfirstlane(find_slot(b.prim_base, b.n, __builtin_amdgcn_readfirstlane(g)));
         p = g - b.prim_base[di];
     }
     const DrawParams &d = draw_at<MULTI>(d0, b, di);
@@ -195,7 +197,8 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
             if (!face_and_cull(d, v, ff, culled)) {
                 atomicOr(d.s.error, (uint32_t)ERR_W_ZERO);
             } else if (!culled) {
-                if (fully_inside(v)) {
+                int cc = clip_class(v);
+                if (cc == 0) {
                     float pos[3][4];
 #pragma unroll
                     for (int k = 0; k < 3; ++k)
@@ -215,7 +218,7 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
                             write_record(d.s.tri + (size_t)p * kRecWords, r, v, p << 7);
                         }
                     }
-                } else {
+                } else if (cc == 1) {
                     needs_clip = true;
                 }
             }
@@ -243,34 +246,49 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
 // k_clip
 // ------------------------------------------------------------------
 
-// Depth-first walk over the plane-by-plane splits; leaves come out in the
-// order of the reference's queue (lexicographic in the split choices).  At
-// most one pending sibling per level: 7 stack entries.
+// Selects out[c] without indexing (keeps the triangles in registers).
+__device__ __forceinline__ void pick(CTri &t, const CTri out[2], uint32_t c)
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t.p[k][j] = c ? out[1].p[k][j] : out[0].p[k][j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) t.b[k][j] = c ? out[1].b[k][j] : out[0].b[k][j];
+    }
+}
+
+// The clipper's output in the order of the reference's plane-by-plane queue
+// (clipper.rs:756-775), which is lexicographic in the split choices (plane 0
+// most significant).  Every leaf is named by its 6-bit path and re-clipped
+// from the root along that path, so nothing but the current triangle lives in
+// registers (no stack in scratch memory); a path that leaves a node's output
+// count skips that node's whole subtree.
 template <class F>
 __device__ int clip_walk(const CTri &root, F &&emit)
 {
-    CTri stack[8];
-    int lvl[8];
-    int sp = 1;
-    stack[0] = root;
-    lvl[0] = 0;
     int leaf = 0;
-    while (sp > 0) {
-        --sp;
-        CTri t = stack[sp];
-        int L = lvl[sp];
-        if (L == 6) {
+    uint32_t code = 0;  // choice of plane L in bit 5 - L
+    while (code < 64u) {
+        CTri t = root;
+        int fail = -1;
+#pragma unroll
+        for (int L = 0; L < 6; ++L) {
+            if (fail < 0) {
+                CTri out[2];
+                uint32_t n = (uint32_t)clip_against_plane(t, L, out);
+                uint32_t c = (code >> (5 - L)) & 1u;
+                if (c >= n) fail = L;
+                else pick(t, out, c);
+            }
+        }
+        if (fail < 0) {
             emit(t, leaf);
             ++leaf;
-            continue;
-        }
-        CTri out[2];
-        int n = clip_against_plane(t, L, out);
-        if (n == 2) {
-            stack[sp] = out[1]; lvl[sp] = L + 1; ++sp;
-            stack[sp] = out[0]; lvl[sp] = L + 1; ++sp;
-        } else if (n == 1) {
-            stack[sp] = out[0]; lvl[sp] = L + 1; ++sp;
+            ++code;
+        } else {
+            uint32_t sh = 5u - (uint32_t)fail;
+            code = ((code >> sh) + 1u) << sh;  // next sibling subtree at that level
         }
     }
     return leaf;
@@ -529,10 +547,6 @@ struct RasterShared {
     uint32_t span[kBatch][TS];      // ordered path: per-row spans of the staged triangles
     uint32_t keys[kBatch];
     uint32_t nkeys;
-    // order-independent path: per pixel of the tile
-    unsigned long long best[TS * TS];  // encoded winner (see oi_value), ~0 = none
-    float idep[TS * TS];               // depth before the draw
-    uint32_t cnt[TS * TS];             // emitted fragments (debug counts only)
 };
 
 // Loads one list entry's triangle record into LDS.
@@ -964,37 +978,181 @@ __device__ __forceinline__ void raster_tile(const DrawParams &d, uint32_t tile, 
     frag_acc += f;
 }
 
-// Stages sh.keys[0..n) and merges their fragments into the tile's winners:
-// one lane per (triangle, row) computes the row's span (the reference's
-// integer scanline rule, clipped to the scissor and the tile) and walks its
-// pixels, so the work follows the fragments rather than triangles x pixels.
+// What the visibility pass needs of a triangle: its record without the
+// varyings (those are read again when the winner is shaded).
+struct VisTri {
+    float vpx[3], vpy[3];
+    float k0, k1, k2, total;
+    float z[3];
+    float cb[9];
+    uint32_t key;
+    uint32_t pad;
+};
+
+// Triangles staged per visibility batch.
+template <int NT>
+struct VisBatch {
+    static constexpr int value = NT >= 128 ? 128 : NT;
+};
+
 template <int TS, int NT>
-__device__ __forceinline__ uint32_t oi_batch(const DrawParams &d, RasterShared<TS> &sh, uint32_t n, uint32_t x0,
-                                             uint32_t y0)
+struct VisShared {
+    static constexpr int NB = VisBatch<NT>::value;
+    VisTri tri[NB];
+    ScanTri scan[NB];
+    uint32_t rows[NB];          // first covered tile row | count << 8 | wide << 16
+    uint32_t base[NB];          // exclusive prefix of the row counts
+    uint16_t owner[NB * TS];    // per covered row: triangle | row << 8
+    uint32_t keys[NB];
+    unsigned long long best[TS * TS];  // encoded winner (see oi_value), ~0 = none
+    float idep[TS * TS];               // depth before the draw
+    uint32_t cnt[TS * TS];             // emitted fragments (debug counts only)
+    uint32_t wsum[NT / 64];
+    uint32_t nkeys, total_rows;
+};
+
+__device__ __forceinline__ void stage_vis(const DrawParams &d, uint32_t key, VisTri &T, ScanTri &S, uint32_t &wide)
+{
+    uint32_t p = key >> 7, sub = (key >> 1) & 63u;
+    bool clipped = key & 1u;
+    uint32_t slot = clipped ? d.s.clip_base[p] + sub : 0u;
+    const uint4 *r = reinterpret_cast<const uint4 *>(clipped ? d.s.clip_tri + (size_t)slot * kRecWords
+                                                             : d.s.tri + (size_t)p * kRecWords);
+    static_assert(R_VAR == 24, "the visibility fields are the record's first 24 words");
+    uint32_t w[24];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        uint4 v = r[q];
+        w[4 * q] = v.x; w[4 * q + 1] = v.y; w[4 * q + 2] = v.z; w[4 * q + 3] = v.w;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        T.vpx[k] = __uint_as_float(w[R_VP + 2 * k]);
+        T.vpy[k] = __uint_as_float(w[R_VP + 2 * k + 1]);
+        T.z[k] = __uint_as_float(w[R_Z + k]);
+    }
+    T.k0 = __uint_as_float(w[R_K + 0]);
+    T.k1 = __uint_as_float(w[R_K + 1]);
+    T.k2 = __uint_as_float(w[R_K + 2]);
+    T.total = __uint_as_float(w[R_K + 3]);
+    T.key = key;
+    T.pad = 0u;
+    if (clipped) {
+        const float *ct = d.s.clip_tris + (size_t)slot * kClipFloats;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) T.cb[k * 3 + c] = ct[k * 7 + 4 + c];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) T.cb[k * 3 + c] = (c == k) ? 1.0f : 0.0f;
+    }
+    S.ax = (int32_t)w[R_SCAN + 0]; S.ay = (int32_t)w[R_SCAN + 1];
+    S.bx = (int32_t)w[R_SCAN + 2]; S.by = (int32_t)w[R_SCAN + 3];
+    S.cx = (int32_t)w[R_SCAN + 4]; S.cy = (int32_t)w[R_SCAN + 5];
+    S.mode = (int32_t)(w[R_MODE] & 0xFFu);
+    S.pad = 0;
+    wide = (w[R_MODE] >> 8) & 1u;
+}
+
+// Same arithmetic as bary_depth, on the visibility data.
+__device__ __forceinline__ float vis_depth(const VisTri &T, float pxf, float pyf)
+{
+    float l0 = (((T.vpy[1] - pyf) * (T.vpx[1] + pxf) + T.k0) + (pyf - T.vpy[2]) * (pxf + T.vpx[2])) / T.total;
+    float l1 = (((pyf - T.vpy[0]) * (pxf + T.vpx[0]) + (T.vpy[2] - pyf) * (T.vpx[2] + pxf)) + T.k1) / T.total;
+    float l2 = ((T.k2 + (pyf - T.vpy[1]) * (pxf + T.vpx[1])) + (T.vpy[0] - pyf) * (T.vpx[0] + pxf)) / T.total;
+    float lc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float acc = T.cb[0 * 3 + c] * l0;
+        acc = acc + T.cb[1 * 3 + c] * l1;
+        acc = acc + T.cb[2 * 3 + c] * l2;
+        lc[c] = acc;
+    }
+    float z = T.z[0] * lc[0];
+    z = z + T.z[1] * lc[1];
+    z = z + T.z[2] * lc[2];
+    return z;
+}
+
+// Stages sh.keys[0..n) and merges their fragments into the tile's winners.
+// Each staged triangle's covered rows (the reference's scanline rows, within
+// the tile and the scissor) are compacted into one list, so the lanes walk
+// only rows that hold fragments: one lane per (triangle, row) computes the
+// row's integer span and visits its pixels.
+template <int TS, int NT>
+__device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS, NT> &sh, uint32_t n, uint32_t x0,
+                                              uint32_t y0)
 {
     uint32_t tid = threadIdx.x;
-    if (tid < n) stage_record(d, sh.keys[tid], sh.tri[tid], sh.scan[tid], sh.wide[tid]);
-    __syncthreads();
     int64_t sx0 = d.scissor[0], sy0 = d.scissor[1];
     int64_t sx1 = sx0 + d.scissor[2], sy1 = sy0 + d.scissor[3];
+    uint32_t cnt_rows = 0, first = 0;
+    if (tid < n) {
+        uint32_t wide;
+        stage_vis(d, sh.keys[tid], sh.tri[tid], sh.scan[tid], wide);
+        int32_t r0, r1;
+        scan_rows(sh.scan[tid], r0, r1);
+        int64_t lo = max((int64_t)r0, max((int64_t)y0, sy0));
+        int64_t hi = min((int64_t)r1, min((int64_t)y0 + TS - 1, sy1 - 1));
+        if (lo <= hi) {
+            cnt_rows = (uint32_t)(hi - lo + 1);
+            first = (uint32_t)(lo - y0);
+        }
+        sh.rows[tid] = first | (cnt_rows << 8) | (wide << 16);
+    }
+    // exclusive scan of the row counts: within each wave, then across waves
+    uint32_t lane = lane_id(), wave = tid >> 6;
+    uint32_t v = cnt_rows;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t t = (uint32_t)__shfl_up((int)v, off);
+        if (lane >= (uint32_t)off) v += t;
+    }
+    if (lane == 63u) sh.wsum[wave] = v;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) {
+        uint32_t x = sh.wsum[w];
+        if ((uint32_t)w < wave) before += x;
+        total += x;
+    }
+    if (tid < n) {
+        uint32_t b = before + v - cnt_rows;
+        for (uint32_t j = 0; j < cnt_rows; ++j) sh.owner[b + j] = (uint16_t)(tid | ((first + j) << 8));
+    }
+    __syncthreads();
+
     bool counts = d.s.frag_px != nullptr;
     uint32_t f = 0;
-    for (uint32_t idx = tid; idx < n * (uint32_t)TS; idx += NT) {
-        uint32_t e = idx / TS, row = idx % TS;
+    for (uint32_t idx = tid; idx < total; idx += NT) {
+        uint32_t o = sh.owner[idx];
+        uint32_t e = o & 0xFFu, row = o >> 8;
         int32_t y = (int32_t)(y0 + row);
         int32_t x1, x2;
-        if ((int64_t)y < sy0 || (int64_t)y >= sy1 || !scan_span(sh.scan[e], y, x1, x2, sh.wide[e] != 0u)) continue;
+        if (!scan_span(sh.scan[e], y, x1, x2, ((sh.rows[e] >> 16) & 1u) != 0u)) continue;
         int64_t lo = max((int64_t)x1, max(sx0, (int64_t)x0));
         int64_t hi = min((int64_t)x2, min(sx1 - 1, (int64_t)x0 + TS - 1));
         if (lo > hi) continue;
-        const TriLds &T = sh.tri[e];
+        const VisTri &T = sh.tri[e];
+        uint32_t q0 = row * TS + (uint32_t)(lo - x0);
+        uint32_t len = (uint32_t)(hi - lo) + 1u;
+        f += len;
+        if (counts)
+            for (uint32_t i = 0; i < len; ++i) atomicAdd(&sh.cnt[q0 + i], 1u);
+        if (d.resolve == RES_LAST) {
+            // no depth test: the last fragment in API order wins, depth unused
+            unsigned long long val = (unsigned long long)(~T.key);
+            for (uint32_t i = 0; i < len; ++i) atomicMin(&sh.best[q0 + i], val);
+            continue;
+        }
         float pyf = (float)y;
-        for (int64_t x = lo; x <= hi; ++x) {
-            uint32_t q = row * TS + (uint32_t)(x - x0);
-            float lc[3];
-            float z = bary_depth(T, (float)x, pyf, lc);
-            ++f;
-            if (counts) atomicAdd(&sh.cnt[q], 1u);
+        for (uint32_t i = 0; i < len; ++i) {
+            uint32_t q = q0 + i;
+            float z = vis_depth(T, (float)(lo + i), pyf);
             if (oi_pass(d, z, sh.idep[q])) atomicMin(&sh.best[q], oi_value(d.resolve, z, T.key));
         }
     }
@@ -1002,14 +1160,16 @@ __device__ __forceinline__ uint32_t oi_batch(const DrawParams &d, RasterShared<T
     return f;
 }
 
-// One active tile of an order-independent draw: load or clear, merge the
-// winners of its bin segment and of the overlapping big triangles, shade
-// each pixel's winner once, store colour and depth.
-template <int PROG, int TS, int NT>
-__device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t tile, RasterShared<TS> &sh,
-                                               uint32_t &frag_acc)
+// One active tile of an order-independent draw, visibility only: the depth
+// the pixels hold before the draw (cleared or loaded), the winners of the
+// tile's bin segment and of the overlapping big triangles, stored per pixel
+// (encoded as in oi_value) into the visibility buffer at the tile's item slot.
+template <int TS, int NT>
+__device__ __forceinline__ void raster_tile_vis(const DrawParams &d, uint32_t tile, VisShared<TS, NT> &sh,
+                                                unsigned long long *vis, uint32_t &frag_acc)
 {
     constexpr int PPT = TS * TS / NT;
+    constexpr uint32_t NB = VisShared<TS, NT>::NB;
     uint32_t tx = tile % d.tiles_x, ty = tile / d.tiles_x;
     uint32_t x0 = tx * TS, y0 = ty * TS;
     uint32_t tid = threadIdx.x;
@@ -1017,22 +1177,13 @@ __device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t til
     uint32_t seg_n = d.s.tile_offset[tile + 1] - seg0;
     uint32_t nbig = min(d.s.counters[C_BIG], d.s.big_cap);
 
-    uint32_t init_col[PPT];
-    float init_dep[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         uint32_t q = tid + (uint32_t)k * NT;
         uint32_t px = x0 + q % TS, py = y0 + q / TS;
-        size_t pix = (size_t)py * d.width + px;
-        uint32_t c = 0u;
         float z = 0.0f;
-        if (px < d.width && py < d.height) {
-            if (d.color)
-                c = (d.init_clear && d.clear_color_on) ? d.clear_rgba : reinterpret_cast<const uint32_t *>(d.color)[pix];
-            if (d.depth) z = (d.init_clear && d.clear_depth_on) ? d.clear_depth : d.depth[pix];
-        }
-        init_col[k] = c;
-        init_dep[k] = z;
+        if (d.depth && px < d.width && py < d.height)
+            z = (d.init_clear && d.clear_depth_on) ? d.clear_depth : d.depth[(size_t)py * d.width + px];
         sh.idep[q] = z;
         sh.best[q] = ~0ull;
         sh.cnt[q] = 0u;
@@ -1043,7 +1194,7 @@ __device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t til
     for (;;) {
         uint32_t n;
         if (fpos < seg_n) {
-            n = min((uint32_t)kBatch, seg_n - fpos);
+            n = min(NB, seg_n - fpos);
             if (tid < n) sh.keys[tid] = d.s.bins[seg0 + fpos + tid];
             fpos += n;
         } else if (bpos < nbig) {
@@ -1070,31 +1221,52 @@ __device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t til
         } else {
             break;
         }
-        f += oi_batch<TS, NT>(d, sh, n, x0, y0);
+        f += vis_batch<TS, NT>(d, sh, n, x0, y0);
     }
 
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        uint32_t q = tid + (uint32_t)k * NT;
+        vis[q] = sh.best[q];
+        if (d.s.frag_px && sh.cnt[q]) {
+            uint32_t px = x0 + q % TS, py = y0 + q / TS;
+            if (px < d.width && py < d.height) d.s.frag_px[(size_t)py * d.width + px] += sh.cnt[q];
+        }
+    }
+    frag_acc += f;
+}
+
+// One active tile of an order-independent draw, shading: each pixel's
+// winner (from the visibility buffer) is shaded once; colour and depth are
+// stored where the pass changes them (a winner, or the LoadOp::Clear values).
+template <int PROG, int TS, int NT>
+__device__ __forceinline__ void shade_tile(const DrawParams &d, uint32_t tile, const unsigned long long *vis)
+{
+    constexpr int PPT = TS * TS / NT;
+    uint32_t x0 = (tile % d.tiles_x) * TS, y0 = (tile / d.tiles_x) * TS;
     bool write_depth_mode = d.depth_test && d.depth_write &&
                             (d.resolve != RES_PASS_LAST || d.depth_compare == CMP_EQUAL ||
                              d.depth_compare == CMP_ALWAYS);
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
-        uint32_t q = tid + (uint32_t)k * NT;
+        uint32_t q = threadIdx.x + (uint32_t)k * NT;
         uint32_t px = x0 + q % TS, py = y0 + q / TS;
         if (px >= d.width || py >= d.height) continue;
         size_t pix = (size_t)py * d.width + px;
-        unsigned long long v = sh.best[q];
-        uint32_t col = init_col[k];
-        float dep = init_dep[k];
+        unsigned long long v = vis[q];
         if (v != ~0ull) {
             float z;
-            col = winner_color<PROG>(d, oi_key(d.resolve, v), (float)px, (float)py, z);
-            if (write_depth_mode) dep = z;
+            uint32_t col = winner_color<PROG>(d, oi_key(d.resolve, v), (float)px, (float)py, z);
+            if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = col;
+            if (d.depth) {
+                if (write_depth_mode) d.depth[pix] = z;
+                else if (d.init_clear && d.clear_depth_on) d.depth[pix] = d.clear_depth;
+            }
+        } else if (d.init_clear) {
+            if (d.color && d.clear_color_on) reinterpret_cast<uint32_t *>(d.color)[pix] = d.clear_rgba;
+            if (d.depth && d.clear_depth_on) d.depth[pix] = d.clear_depth;
         }
-        if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = col;
-        if (d.depth) d.depth[pix] = dep;
-        if (d.s.frag_px && sh.cnt[q]) d.s.frag_px[pix] += sh.cnt[q];
     }
-    frag_acc += f;
 }
 
 // Stores the clear values into the untouched tiles of a chunk.
@@ -1107,7 +1279,7 @@ __device__ __forceinline__ void clear_chunk(const DrawParams &d0, const BatchArg
         if (b.tile_active[gt]) continue;
         uint32_t di = 0, tile = gt;
         if constexpr (MULTI) {
-            di = find_slot(b.tile_base, b.n, gt);
+            di = __builtin_amdgcn_readfirstlane(find_slot(b.tile_base, b.n, gt));
             tile = gt - b.tile_base[di];
         }
         const DrawParams &d = draw_at<MULTI>(d0, b, di);
@@ -1125,41 +1297,38 @@ __device__ __forceinline__ void clear_chunk(const DrawParams &d0, const BatchArg
     }
 }
 
-// Persistent, statically scheduled (no shared work counter): the active
-// tiles are cut into 8 contiguous ranges, one per XCD (workgroup i runs on
-// XCD i mod 8 and the grid is a multiple of 8), so that neighbouring tiles
-// share an L2; the clear chunks are strided over the whole grid.  The
-// fragment count is reduced per workgroup into one of kFragSlots counters.
-template <int PROG, bool ORDERED, bool MULTI, int TS, int NT>
-__global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
-{
-    constexpr int PPT = TS * TS / NT;
-    static_assert(NT >= kBatch && PPT >= 1 && TS * TS == PPT * NT, "tile shape");
-    __shared__ RasterShared<TS> sh;
-    __shared__ uint32_t olist[ORDERED ? kOrderedCap : 1];
-    __shared__ uint32_t olist_n;
-    __shared__ uint32_t wave_frags[NT / 64];
-
-    uint32_t n_active = b.header[H_ACTIVE];
-    uint32_t per_xcd = (n_active + 7u) / 8u;
-    uint32_t xcd = blockIdx.x & 7u, lw = blockIdx.x >> 3, nw = gridDim.x >> 3;
-    uint32_t frag_acc = 0;
-    for (uint32_t i = lw; i < per_xcd; i += nw) {
-        uint32_t item = xcd * per_xcd + i;
-        if (item >= n_active) break;
-        uint32_t gt = b.active_list[item];
-        uint32_t di = 0, tile = gt;
-        if constexpr (MULTI) {
-            di = find_slot(b.tile_base, b.n, gt);
-            tile = gt - b.tile_base[di];
-        }
-        if constexpr (ORDERED)
-            raster_tile<PROG, true, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, olist, olist_n, frag_acc);
-        else
-            raster_tile_oi<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, frag_acc);
+// Splits the active tiles into 8 contiguous ranges, one per XCD (workgroup i
+// runs on XCD i mod 8 and the grids are multiples of 8), so that neighbouring
+// tiles share an L2; yields the items of this workgroup.
+struct XcdSchedule {
+    uint32_t per_xcd, xcd, lw, nw, n;
+    __device__ XcdSchedule(uint32_t n_active)
+    {
+        n = n_active;
+        per_xcd = (n_active + 7u) / 8u;
+        xcd = blockIdx.x & 7u;
+        lw = blockIdx.x >> 3;
+        nw = gridDim.x >> 3;
     }
-    for (uint32_t c = blockIdx.x; c < b.n_chunks; c += gridDim.x) clear_chunk<MULTI, TS, NT>(d0, b, c);
+};
 
+template <bool MULTI>
+__device__ __forceinline__ void locate_tile(const BatchArgs &b, uint32_t item, uint32_t &di, uint32_t &tile)
+{
+    uint32_t gt = __builtin_amdgcn_readfirstlane(b.active_list[item]);
+    di = 0;
+    tile = gt;
+    if constexpr (MULTI) {
+        di = __builtin_amdgcn_readfirstlane(find_slot(b.tile_base, b.n, gt));
+        tile = gt - b.tile_base[di];
+    }
+}
+
+// Reduces the workgroup's fragment count into one of kFragSlots counters.
+template <bool MULTI, int NT>
+__device__ __forceinline__ void count_fragments(const DrawParams &d0, const BatchArgs &b, uint32_t frag_acc,
+                                                uint32_t *wave_frags)
+{
     for (int off = 32; off > 0; off >>= 1) frag_acc += __shfl_xor(frag_acc, off);
     if (lane_id() == 0) wave_frags[threadIdx.x >> 6] = frag_acc;
     __syncthreads();
@@ -1167,6 +1336,72 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
         uint32_t f = 0;
         for (int w = 0; w < NT / 64; ++w) f += wave_frags[w];
         if (f) atomicAdd(draw_at<MULTI>(d0, b, 0).s.frags + (blockIdx.x % kFragSlots), (unsigned long long)f);
+    }
+}
+
+// Ordered draws (blending, NotEqual + write): persistent and statically
+// scheduled (no shared work counter); each tile's fragments are applied in
+// API order.  Then the clear chunks, strided over the whole grid.
+template <int PROG, bool MULTI, int TS, int NT>
+__global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
+{
+    constexpr int PPT = TS * TS / NT;
+    static_assert(NT >= kBatch && PPT >= 1 && TS * TS == PPT * NT, "tile shape");
+    __shared__ RasterShared<TS> sh;
+    __shared__ uint32_t olist[kOrderedCap];
+    __shared__ uint32_t olist_n;
+    __shared__ uint32_t wave_frags[NT / 64];
+
+    XcdSchedule w(b.header[H_ACTIVE]);
+    uint32_t frag_acc = 0;
+    for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
+        uint32_t item = w.xcd * w.per_xcd + i;
+        if (item >= w.n) break;
+        uint32_t di, tile;
+        locate_tile<MULTI>(b, item, di, tile);
+        raster_tile<PROG, true, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, olist, olist_n, frag_acc);
+    }
+    for (uint32_t c = blockIdx.x; c < b.n_chunks; c += gridDim.x) clear_chunk<MULTI, TS, NT>(d0, b, c);
+    count_fragments<MULTI, NT>(d0, b, frag_acc, wave_frags);
+}
+
+// Order-independent draws, pass 1: visibility of every active tile into the
+// visibility buffer (TS*TS values per active-tile slot), then the clear
+// chunks.  Independent of the shader program.
+template <bool MULTI, int TS, int NT>
+__global__ __launch_bounds__(NT) void k_visibility(DrawParams d0, BatchArgs b, unsigned long long *vis)
+{
+    constexpr int PPT = TS * TS / NT;
+    static_assert(NT >= kBatch && PPT >= 1 && TS * TS == PPT * NT, "tile shape");
+    static_assert(VisShared<TS, NT>::NB <= 256 && TS <= 32, "owner entries hold 8-bit triangle and row indices");
+    __shared__ VisShared<TS, NT> sh;
+    __shared__ uint32_t wave_frags[NT / 64];
+
+    XcdSchedule w(b.header[H_ACTIVE]);
+    uint32_t frag_acc = 0;
+    for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
+        uint32_t item = w.xcd * w.per_xcd + i;
+        if (item >= w.n) break;
+        uint32_t di, tile;
+        locate_tile<MULTI>(b, item, di, tile);
+        raster_tile_vis<TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, vis + (size_t)item * (TS * TS), frag_acc);
+    }
+    for (uint32_t c = blockIdx.x; c < b.n_chunks; c += gridDim.x) clear_chunk<MULTI, TS, NT>(d0, b, c);
+    count_fragments<MULTI, NT>(d0, b, frag_acc, wave_frags);
+}
+
+// Order-independent draws, pass 2: shade each active tile's winners, with
+// the same XCD partition as pass 1 (its visibility values are in that L2).
+template <int PROG, bool MULTI, int TS, int NT>
+__global__ __launch_bounds__(NT) void k_shade(DrawParams d0, BatchArgs b, const unsigned long long *vis)
+{
+    XcdSchedule w(b.header[H_ACTIVE]);
+    for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
+        uint32_t item = w.xcd * w.per_xcd + i;
+        if (item >= w.n) break;
+        uint32_t di, tile;
+        locate_tile<MULTI>(b, item, di, tile);
+        shade_tile<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, vis + (size_t)item * (TS * TS));
     }
 }
 
@@ -1201,69 +1436,93 @@ __global__ void k_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, 
         if (e_ != hipSuccess) return e_;                                                \
     } while (0)
 
-template <int PROG, bool ORDERED, bool MULTI>
-static hipError_t launch_raster(const DrawParams &d, const BatchArgs &b, uint32_t grid, uint32_t cfg,
-                                hipStream_t st, KernelTimer *kt)
+template <int PROG, bool MULTI, int TS, int NT>
+static hipError_t launch_oi(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
+                            KernelTimer *kt)
 {
-    if (ORDERED) {
-        WGC_LAUNCH("k_raster", (k_raster<PROG, true, MULTI, 16, 256>), grid, 256, d, b);
-        return hipSuccess;
-    }
-    switch (cfg) {
-    case RASTER_8x8_64: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 8, 64>), grid, 64, d, b); break;
-    case RASTER_16x16_64: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 16, 64>), grid, 64, d, b); break;
-    case RASTER_32x32_256: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 32, 256>), grid, 256, d, b); break;
-    default: WGC_LAUNCH("k_raster", (k_raster<PROG, false, MULTI, 16, 256>), grid, 256, d, b); break;
-    }
+    WGC_LAUNCH("k_visibility", (k_visibility<MULTI, TS, NT>), gl.raster_grid, NT, d, b, gl.vis);
+    WGC_LAUNCH("k_shade", (k_shade<PROG, MULTI, TS, NT>), gl.shade_grid, NT, d, b, gl.vis);
     return hipSuccess;
 }
 
-// Workgroups of a raster instance that fit on one CU at once (registers,
-// LDS, waves); the persistent grid is sized to what can be resident, so that
-// the static schedule never waits for a second round of workgroups.
-template <int PROG, bool ORDERED, bool MULTI, int TS, int NT>
-static uint32_t raster_occupancy()
+template <int PROG, bool MULTI>
+static hipError_t launch_raster(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
+                                KernelTimer *kt)
 {
-    static uint32_t cached = 0;
-    if (!cached) {
-        int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_raster<PROG, ORDERED, MULTI, TS, NT>, NT, 0) !=
-                hipSuccess ||
-            n < 1)
-            n = 1;
-        cached = (uint32_t)n;
+    if (gl.ordered) {
+        WGC_LAUNCH("k_raster", (k_raster<PROG, MULTI, 16, 256>), gl.raster_grid, 256, d, b);
+        return hipSuccess;
     }
-    return cached;
+    switch (gl.raster_cfg) {
+    case RASTER_8x8_64: return launch_oi<PROG, MULTI, 8, 64>(d, b, gl, st, kt);
+    case RASTER_16x16_64: return launch_oi<PROG, MULTI, 16, 64>(d, b, gl, st, kt);
+    case RASTER_32x32_256: return launch_oi<PROG, MULTI, 32, 256>(d, b, gl, st, kt);
+    default: return launch_oi<PROG, MULTI, 16, 256>(d, b, gl, st, kt);
+    }
+}
+
+// Workgroups of a kernel that fit on one CU at once (registers, LDS, waves);
+// the persistent grids are sized to what can be resident, so that the static
+// schedule never waits for a second round of workgroups.
+template <class K>
+static uint32_t occupancy_of(K kernel, int threads)
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1) n = 1;
+    return (uint32_t)n;
+}
+
+template <int PROG, bool MULTI, int TS, int NT>
+static void occupancy_oi(uint32_t &raster, uint32_t &shade)
+{
+    static uint32_t r = 0, s = 0;
+    if (!r) {
+        r = occupancy_of(k_visibility<MULTI, TS, NT>, NT);
+        s = occupancy_of(k_shade<PROG, MULTI, TS, NT>, NT);
+    }
+    raster = r;
+    shade = s;
 }
 
 template <int PROG, bool MULTI>
-static uint32_t raster_occupancy_cfg(bool ordered, uint32_t cfg)
+static void occupancy_cfg(bool ordered, uint32_t cfg, uint32_t &raster, uint32_t &shade)
 {
-    if (ordered) return raster_occupancy<PROG, true, MULTI, 16, 256>();
+    if (ordered) {
+        static uint32_t r = 0;
+        if (!r) r = occupancy_of(k_raster<PROG, MULTI, 16, 256>, 256);
+        raster = shade = r;
+        return;
+    }
     switch (cfg) {
-    case RASTER_8x8_64: return raster_occupancy<PROG, false, MULTI, 8, 64>();
-    case RASTER_16x16_64: return raster_occupancy<PROG, false, MULTI, 16, 64>();
-    case RASTER_32x32_256: return raster_occupancy<PROG, false, MULTI, 32, 256>();
-    default: return raster_occupancy<PROG, false, MULTI, 16, 256>();
+    case RASTER_8x8_64: occupancy_oi<PROG, MULTI, 8, 64>(raster, shade); break;
+    case RASTER_16x16_64: occupancy_oi<PROG, MULTI, 16, 64>(raster, shade); break;
+    case RASTER_32x32_256: occupancy_oi<PROG, MULTI, 32, 256>(raster, shade); break;
+    default: occupancy_oi<PROG, MULTI, 16, 256>(raster, shade); break;
     }
 }
 
-uint32_t raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32_t cfg)
+void raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32_t cfg, uint32_t &raster,
+                          uint32_t &shade)
 {
+    raster = shade = 1;
     switch (program) {
     case PROG_MVP_COLOR:
-        return multi ? raster_occupancy_cfg<PROG_MVP_COLOR, true>(ordered, cfg)
-                     : raster_occupancy_cfg<PROG_MVP_COLOR, false>(ordered, cfg);
+        if (multi) occupancy_cfg<PROG_MVP_COLOR, true>(ordered, cfg, raster, shade);
+        else occupancy_cfg<PROG_MVP_COLOR, false>(ordered, cfg, raster, shade);
+        break;
     case PROG_MVP_TEXTURE:
-        return multi ? raster_occupancy_cfg<PROG_MVP_TEXTURE, true>(ordered, cfg)
-                     : raster_occupancy_cfg<PROG_MVP_TEXTURE, false>(ordered, cfg);
+        if (multi) occupancy_cfg<PROG_MVP_TEXTURE, true>(ordered, cfg, raster, shade);
+        else occupancy_cfg<PROG_MVP_TEXTURE, false>(ordered, cfg, raster, shade);
+        break;
     case PROG_COLORED_TRIANGLE:
-        return multi ? raster_occupancy_cfg<PROG_COLORED_TRIANGLE, true>(ordered, cfg)
-                     : raster_occupancy_cfg<PROG_COLORED_TRIANGLE, false>(ordered, cfg);
+        if (multi) occupancy_cfg<PROG_COLORED_TRIANGLE, true>(ordered, cfg, raster, shade);
+        else occupancy_cfg<PROG_COLORED_TRIANGLE, false>(ordered, cfg, raster, shade);
+        break;
     case PROG_CLIP_COLOR:
-        return multi ? raster_occupancy_cfg<PROG_CLIP_COLOR, true>(ordered, cfg)
-                     : raster_occupancy_cfg<PROG_CLIP_COLOR, false>(ordered, cfg);
-    default: return 1;
+        if (multi) occupancy_cfg<PROG_CLIP_COLOR, true>(ordered, cfg, raster, shade);
+        else occupancy_cfg<PROG_CLIP_COLOR, false>(ordered, cfg, raster, shade);
+        break;
+    default: break;
     }
 }
 
@@ -1289,8 +1548,7 @@ static hipError_t launch_group(const DrawParams &d, const BatchArgs &b, const Gr
         WGC_LAUNCH("k_bin_fill", k_bin_fill<MULTI>, g, 256, d, b, gl.recs, gl.rec_slots, gl.rec_prims,
                    gl.rec_clip_cap, gl.bins);
     }
-    if (gl.ordered) return launch_raster<PROG, true, MULTI>(d, b, gl.raster_grid, gl.raster_cfg, st, kt);
-    return launch_raster<PROG, false, MULTI>(d, b, gl.raster_grid, gl.raster_cfg, st, kt);
+    return launch_raster<PROG, MULTI>(d, b, gl, st, kt);
 }
 
 template <bool MULTI>

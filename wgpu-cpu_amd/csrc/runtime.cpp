@@ -1547,13 +1547,12 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
     return WGC_OK;
 }
 
-// Persistent raster grid: every workgroup resident at once (CUs x the
-// kernel's occupancy), never more than there are work items, a multiple of
-// 8 (the raster splits its work over the 8 XCDs).
-static uint32_t raster_grid(WgcDevice_T *dev, uint32_t program, uint32_t cfg, bool ordered, bool multi,
-                            uint64_t items)
+// Persistent raster grids: every workgroup resident at once (CUs x the
+// kernel's occupancy), never more than there are work items, multiples of 8
+// (the kernels split their work over the 8 XCDs).
+static uint32_t resident_grid(WgcDevice_T *dev, uint32_t per_cu, uint64_t items)
 {
-    uint64_t cap = (uint64_t)dev->cu_count * raster_blocks_per_cu(program, ordered, multi, cfg);
+    uint64_t cap = (uint64_t)dev->cu_count * per_cu;
     cap = std::max<uint64_t>(8u, cap / 8 * 8);
     uint64_t g = align_up(std::max<uint64_t>(1u, std::min<uint64_t>(items, cap)), 8);
     return (uint32_t)std::min(g, cap);
@@ -1596,6 +1595,10 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
     size_t o_ta = take(4 * total_tiles), o_al = take(4 * total_tiles), o_bs = take(8 * n_blocks);
     size_t o_bins = take(4 * bins_cap), o_recs = take(16 * rec_cap), o_rs = take(4ull * kMaxBinTiles * rec_cap);
     size_t o_ctr = take(4ull * C_NUM * n), o_hdr = take(4ull * H_NUM), o_cl = take(8 * prims);
+    // visibility buffer of the order-independent raster: one 64-bit value per
+    // pixel of every tile that may be active
+    uint64_t tile_px = st.draws.empty() ? 0 : (1ull << (2 * st.draws[0].tile_shift));
+    size_t o_vis = take(st.gl.ordered ? 0 : 8ull * tile_px * total_tiles);
     std::vector<size_t> o_big(n), o_tri(n), o_ctri(n), o_ct(n), o_cb(n);
     for (uint32_t i = 0; i < n; ++i) {
         const DrawParams &d = st.draws[i];
@@ -1637,7 +1640,11 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
         b.tile_base = nullptr;
         b.total_prims = st.draws[0].total_prims;
     }
-    gl.raster_grid = raster_grid(dev, gl.program, gl.raster_cfg, gl.ordered != 0, st.multi, total_tiles + b.n_chunks);
+    uint32_t occ_raster = 1, occ_shade = 1;
+    raster_blocks_per_cu(gl.program, gl.ordered != 0, st.multi, gl.raster_cfg, occ_raster, occ_shade);
+    gl.raster_grid = resident_grid(dev, occ_raster, total_tiles + b.n_chunks);
+    gl.shade_grid = resident_grid(dev, occ_shade, total_tiles);
+    gl.vis = gl.ordered ? nullptr : (unsigned long long *)(base + o_vis);
     uint64_t tile_base = 0;
     for (uint32_t i = 0; i < n; ++i) {
         DrawParams &d = st.draws[i];
@@ -1858,7 +1865,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
             continue;
         }
         layout_group(s, dev->arena, dev->frag_px, dev->frag_px_size, dev);
-        plan.launches += 8;
+        plan.launches += s.gl.ordered ? 8 : 9;
         for (auto &d : s.draws) plan.prims += d.total_prims;
         if (!s.multi) continue;
         uint32_t n = (uint32_t)s.draws.size();

@@ -394,16 +394,28 @@ __device__ __forceinline__ float clip_distance(int plane, const float *p)
     return a + b;
 }
 
-// True when no vertex is outside any plane: the clipper then returns the
-// triangle unchanged with identity barycentrics.
-__device__ __forceinline__ bool fully_inside(const VsOut v[3])
+// Bit per clip plane that the vertex is outside of (distance < 0).
+__device__ __forceinline__ uint32_t outcode(const float *p)
 {
-    bool inside = true;
+    uint32_t c = 0;
 #pragma unroll
-    for (int pl = 0; pl < 6; ++pl)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) inside = inside && !(clip_distance(pl, v[k].clip) < 0.0f);
-    return inside;
+    for (int pl = 0; pl < 6; ++pl) c |= (clip_distance(pl, p) < 0.0f) ? (1u << pl) : 0u;
+    return c;
+}
+
+// 0: no vertex outside any plane -- the clipper returns the triangle
+// unchanged with identity barycentrics; 2: all three vertices outside the
+// first plane (in the clipper's order, clipper.rs:153-163) that any vertex
+// is outside of -- the planes before it leave the triangle untouched and
+// that plane drops it, so the clipper emits nothing; 1: the clipper must run.
+__device__ __forceinline__ int clip_class(const VsOut v[3])
+{
+    uint32_t c0 = outcode(v[0].clip), c1 = outcode(v[1].clip), c2 = outcode(v[2].clip);
+    uint32_t any = c0 | c1 | c2;
+    if (any == 0u) return 0;
+    uint32_t first = any & (0u - any);
+    if ((c0 & c1 & c2 & first) != 0u) return 2;
+    return 1;
 }
 
 struct CTri {

@@ -54,6 +54,8 @@ struct GroupLaunch {
     uint32_t rec_prims, rec_clip_cap;   // record regions: per primitive, appended
     uint2 *clip_list;            // (draw, primitive) pairs that cross a clip plane
     uint32_t clip_cap;
+    unsigned long long *vis;     // visibility buffer: tile side^2 values per active tile
+    uint32_t shade_grid;
 };
 
 // Brackets launches with HIP events on the launching stream; the runtime
@@ -96,9 +98,11 @@ hipError_t launch_draw(const DrawParams &d, const BatchArgs &b, const GroupLaunc
                        KernelTimer *kt);
 hipError_t launch_batch(const DrawParams &d0, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
                         KernelTimer *kt);
-// Workgroups of the raster kernel for (program, ordered, batched, tile
-// shape) that fit on one CU at once.
-uint32_t raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32_t cfg);
+// Workgroups of the raster kernels for (program, ordered, batched, tile
+// shape) that fit on one CU at once: the visibility (or ordered raster)
+// kernel and the shading kernel.
+void raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32_t cfg, uint32_t &raster,
+                          uint32_t &shade);
 // Zeroes the submission counters (fragment slots and error bits).
 hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t st);
 hipError_t launch_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, uint64_t n,
