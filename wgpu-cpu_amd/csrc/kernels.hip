@@ -82,6 +82,46 @@ __device__ __forceinline__ void add_tile_counts(const DrawParams &d, uint32_t re
         for (uint32_t tx = tx0; tx <= tx1; ++tx) slots[k++] = atomicAdd(&d.s.tile_count[ty * d.tiles_x + tx], 1u);
 }
 
+// add_tile_counts for a whole wave (every lane calls it).  For the k-th
+// tile of every lane's footprint (row-major, k up to the largest footprint
+// in the wave), lanes on the same tile form a group first (up to 8 groups
+// per round, in registers: neighbouring triangles of a mesh mostly share
+// tiles, and one tile may collect thousands of them); then every group's
+// first lane, and every lane left alone, takes its positions with one atomic,
+// all of them issued together.
+__device__ __forceinline__ void add_tile_counts_wave(const DrawParams &d, bool has, uint32_t rec, uint32_t tx0,
+                                                     uint32_t ty0, uint32_t tx1, uint32_t ty1)
+{
+    uint32_t w = has ? tx1 - tx0 + 1u : 0u;
+    uint32_t ntl = has ? w * (ty1 - ty0 + 1u) : 0u;
+    uint32_t *slots = d.s.rec_slots + (size_t)rec * kMaxBinTiles;
+    uint32_t lane = lane_id();
+    uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (uint32_t k = 0; k < (uint32_t)kMaxBinTiles; ++k) {
+        bool act = k < ntl;
+        uint64_t pending = __ballot(act);
+        if (!pending) break;
+        uint32_t t = act ? (ty0 + k / w) * d.tiles_x + (tx0 + k % w) : 0xFFFFFFFFu;
+        uint32_t lead = lane, cnt = 1u, rank = 0u;
+        for (int it = 0; it < 8 && pending; ++it) {
+            int leader = __ffsll((unsigned long long)pending) - 1;
+            uint32_t tl = (uint32_t)__shfl((int)t, leader);
+            bool mine = act && t == tl;
+            uint64_t m = __ballot(mine);
+            if (mine) {
+                lead = (uint32_t)leader;
+                rank = (uint32_t)__popcll(m & lt);
+                cnt = (uint32_t)__popcll(m);
+            }
+            pending &= ~m;
+        }
+        uint32_t base = 0u;
+        if (act && lead == lane) base = atomicAdd(&d.s.tile_count[t], cnt);
+        base = (uint32_t)__shfl((int)base, (int)lead);
+        if (act) slots[k] = base + rank;
+    }
+}
+
 __device__ __forceinline__ uint32_t wave_max(uint32_t v)
 {
     for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
@@ -233,7 +273,7 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
     }
     // the per-primitive record slot: an empty footprint unless binned here
     d.s.recs[g] = small ? rec : make_uint4(0u, 0xFFFFFFFFu, 0u, di);
-    if (small) add_tile_counts(d, g, rec.y & 0xFFFFu, rec.y >> 16, rec.z & 0xFFFFu, rec.z >> 16);
+    add_tile_counts_wave(d, small, g, rec.y & 0xFFFFu, rec.y >> 16, rec.z & 0xFFFFu, rec.z >> 16);
     slot = wave_append(&d.s.counters[C_BIG], big);
     if (big) {
         if (slot < d.s.big_cap) d.s.big[slot] = rec;
@@ -294,33 +334,80 @@ __device__ int clip_walk(const CTri &root, F &&emit)
     return leaf;
 }
 
+// Wave-aggregated atomicAdd of v to *addr: lanes (with act) that pass the
+// same address share one atomic and receive consecutive ranges in lane
+// order.  Every lane of the wave must call it.
+__device__ __forceinline__ uint32_t wave_add(uint32_t *addr, uint32_t v, bool act)
+{
+    uint32_t lane = lane_id();
+    act = act && v > 0u;
+    uint64_t pending = __ballot(act);
+    uint32_t res = 0;
+    unsigned long long a = (unsigned long long)addr;
+    while (pending) {
+        int leader = __ffsll((unsigned long long)pending) - 1;
+        uint32_t lo = (uint32_t)__shfl((int)(uint32_t)a, leader);
+        uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(a >> 32), leader);
+        bool mine = act && a == (((unsigned long long)hi << 32) | lo);
+        uint64_t m = __ballot(mine);
+        // inclusive scan of v over this group's lanes
+        uint32_t x = mine ? v : 0u;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t t = (uint32_t)__shfl_up((int)x, off);
+            if (lane >= (uint32_t)off) x += t;
+        }
+        uint32_t total = (uint32_t)__shfl((int)x, 63);
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(addr, total);
+        base = (uint32_t)__shfl((int)base, leader);
+        if (mine) res = base + x - v;
+        pending &= ~m;
+    }
+    return res;
+}
+
+// One primitive that crosses a clip plane: the clipper's sub-triangles, in
+// its order, become clip records (positions and clip barycentrics), triangle
+// records and bin records.  Every lane of the wave calls it (with `act`
+// false for lanes without a primitive), so that the slot counters are taken
+// with one atomic per wave.  Sub-triangles that turn out empty or big leave
+// an empty bin record behind.
 template <int PROG>
-__device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p)
+__device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act)
 {
     VsOut v[3];
-    if (!shade_primitive<PROG>(d, p, v)) return;
     CTri root;
+    uint32_t nleaf = 0;
+    if (act && shade_primitive<PROG>(d, p, v)) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 3; ++k) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) root.p[k][c] = v[k].clip[c];
+            for (int c = 0; c < 4; ++c) root.p[k][c] = v[k].clip[c];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) root.b[k][c] = (c == k) ? 1.0f : 0.0f;
+            for (int c = 0; c < 3; ++c) root.b[k][c] = (c == k) ? 1.0f : 0.0f;
+        }
+        nleaf = (uint32_t)clip_walk(root, [](const CTri &, int) {});
+        if (nleaf > 63u) {  // the record key holds 6 bits of sub-triangle index
+            atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
+            nleaf = 0;
+        }
     }
-    int nleaf = clip_walk(root, [](const CTri &, int) {});
-    if (nleaf == 0) return;
-    if (nleaf > 63) {  // the record key holds 6 bits of sub-triangle index
+    uint32_t base = wave_add(&d.s.counters[C_CLIP_TRI], nleaf, act);
+    uint32_t rbase = wave_add(&d.s.header[H_REC], nleaf, act);
+    if (nleaf == 0u) return;
+    if (base + nleaf > d.s.clip_tri_cap || rbase + nleaf > d.s.rec_clip_cap) {
         atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
-        return;
-    }
-    uint32_t base = atomicAdd(&d.s.counters[C_CLIP_TRI], (uint32_t)nleaf);
-    if (base + (uint32_t)nleaf > d.s.clip_tri_cap) {
-        atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
+        // keep the reserved bin records that exist empty
+        for (uint32_t k = 0; k < nleaf && rbase + k < d.s.rec_clip_cap; ++k)
+            d.s.recs[d.s.rec_prims + rbase + k] = make_uint4(0u, 0xFFFFFFFFu, 0u, di);
         return;
     }
     d.s.clip_base[p] = base;
     clip_walk(root, [&](const CTri &t, int leaf) {
         uint32_t slot = base + (uint32_t)leaf;
+        uint32_t ri = d.s.rec_prims + rbase + (uint32_t)leaf;
+        uint4 empty = make_uint4(0u, 0xFFFFFFFFu, 0u, di);
         float *o = d.s.clip_tris + (size_t)slot * kClipFloats;
         for (int k = 0; k < 3; ++k) {
             for (int c = 0; c < 4; ++c) o[k * 7 + c] = t.p[k][c];
@@ -328,25 +415,26 @@ __device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p)
         }
         RasterTri r;
         raster_setup(d, t.p, r);
-        if (!coord_ok(r)) { atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE); return; }
-        if (r.total == 0.0f) return;
         ScanTri s = scan_setup(r);
         uint32_t tx0, ty0, tx1, ty1;
-        if (!tile_bbox(d, s, tx0, ty0, tx1, ty1)) return;
+        if (!coord_ok(r)) {
+            atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE);
+            d.s.recs[ri] = empty;
+            return;
+        }
+        if (r.total == 0.0f || !tile_bbox(d, s, tx0, ty0, tx1, ty1)) {
+            d.s.recs[ri] = empty;
+            return;
+        }
         uint32_t key = (p << 7) | ((uint32_t)leaf << 1) | 1u;
         write_record(d.s.clip_tri + (size_t)slot * kRecWords, r, v, key);
         uint4 rec = make_uint4(key, tx0 | (ty0 << 16), tx1 | (ty1 << 16), di);
         uint32_t nt = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
         if (nt <= (uint32_t)kMaxBinTiles) {
-            uint32_t rs = atomicAdd(&d.s.header[H_REC], 1u);
-            if (rs < d.s.rec_clip_cap) {
-                uint32_t ri = d.s.rec_prims + rs;
-                d.s.recs[ri] = rec;
-                add_tile_counts(d, ri, tx0, ty0, tx1, ty1);
-            } else {
-                atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
-            }
+            d.s.recs[ri] = rec;
+            add_tile_counts(d, ri, tx0, ty0, tx1, ty1);
         } else {
+            d.s.recs[ri] = empty;
             uint32_t bs = atomicAdd(&d.s.counters[C_BIG], 1u);
             if (bs < d.s.big_cap) {
                 d.s.big[bs] = rec;
@@ -361,13 +449,19 @@ __device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p)
     });
 }
 
+// Every lane runs the same number of iterations, so that the wave-wide slot
+// reservations in clip_primitive see the whole wave.
 template <int PROG, bool MULTI>
 __global__ __launch_bounds__(256) void k_clip(DrawParams d0, BatchArgs b, const uint2 *clip_list, uint32_t cap)
 {
     uint32_t n = min(b.header[H_CLIP], cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint2 e = clip_list[i];
-        clip_primitive<PROG>(draw_at<MULTI>(d0, b, e.x), e.x, e.y);
+    uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t first = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);  // the wave's first thread
+    for (uint32_t w0 = first; w0 < n; w0 += stride) {
+        uint32_t i = w0 + lane_id();
+        bool act = i < n;
+        uint2 e = act ? clip_list[i] : make_uint2(0u, 0u);
+        clip_primitive<PROG>(draw_at<MULTI>(d0, b, e.x), e.x, e.y, act);
     }
 }
 
