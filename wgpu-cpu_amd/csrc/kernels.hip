@@ -1073,15 +1073,13 @@ __device__ __forceinline__ void raster_tile(const DrawParams &d, uint32_t tile, 
 }
 
 // What the visibility pass needs of a triangle: its record without the
-// varyings (those are read again when the winner is shaded).
-struct VisTri {
-    float vpx[3], vpy[3];
-    float k0, k1, k2, total;
-    float z[3];
-    float cb[9];
-    uint32_t key;
-    uint32_t pad;
+// varyings (those are read again when the winner is shaded), kept in LDS as
+// structure-of-arrays (lanes that read different triangles hit different
+// banks).
+enum VisField {
+    V_VPX = 0, V_VPY = 3, V_K = 6, V_Z = 10, V_CB = 13, V_NUM = 22,
 };
+enum VisScan { S_AX = 0, S_AY, S_BX, S_BY, S_CX, S_CY, S_MODE, S_NUM };
 
 // Triangles staged per visibility batch.
 template <int NT>
@@ -1092,20 +1090,31 @@ struct VisBatch {
 template <int TS, int NT>
 struct VisShared {
     static constexpr int NB = VisBatch<NT>::value;
-    VisTri tri[NB];
-    ScanTri scan[NB];
+    float f[V_NUM][NB];
+    int32_t sc[S_NUM][NB];
+    uint32_t tkey[NB];
     uint32_t rows[NB];          // first covered tile row | count << 8 | wide << 16
-    uint32_t base[NB];          // exclusive prefix of the row counts
     uint16_t owner[NB * TS];    // per covered row: triangle | row << 8
     uint32_t keys[NB];
     unsigned long long best[TS * TS];  // encoded winner (see oi_value), ~0 = none
     float idep[TS * TS];               // depth before the draw
     uint32_t cnt[TS * TS];             // emitted fragments (debug counts only)
     uint32_t wsum[NT / 64];
-    uint32_t nkeys, total_rows;
+    uint32_t nkeys;
 };
 
-__device__ __forceinline__ void stage_vis(const DrawParams &d, uint32_t key, VisTri &T, ScanTri &S, uint32_t &wide)
+// Spreads consecutive items over the waves of the workgroup (item i goes to
+// wave i mod (NT/64)), so that a short batch keeps every SIMD busy.
+template <int NT>
+__device__ __forceinline__ uint32_t spread(uint32_t tid)
+{
+    constexpr uint32_t W = NT / 64;
+    return (tid & 63u) * W + (tid >> 6);
+}
+
+template <int TS, int NT>
+__device__ __forceinline__ void stage_vis(const DrawParams &d, uint32_t key, VisShared<TS, NT> &sh, uint32_t e,
+                                          ScanTri &S, uint32_t &wide)
 {
     uint32_t p = key >> 7, sub = (key >> 1) & 63u;
     bool clipped = key & 1u;
@@ -1121,53 +1130,58 @@ __device__ __forceinline__ void stage_vis(const DrawParams &d, uint32_t key, Vis
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        T.vpx[k] = __uint_as_float(w[R_VP + 2 * k]);
-        T.vpy[k] = __uint_as_float(w[R_VP + 2 * k + 1]);
-        T.z[k] = __uint_as_float(w[R_Z + k]);
+        sh.f[V_VPX + k][e] = __uint_as_float(w[R_VP + 2 * k]);
+        sh.f[V_VPY + k][e] = __uint_as_float(w[R_VP + 2 * k + 1]);
+        sh.f[V_Z + k][e] = __uint_as_float(w[R_Z + k]);
     }
-    T.k0 = __uint_as_float(w[R_K + 0]);
-    T.k1 = __uint_as_float(w[R_K + 1]);
-    T.k2 = __uint_as_float(w[R_K + 2]);
-    T.total = __uint_as_float(w[R_K + 3]);
-    T.key = key;
-    T.pad = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sh.f[V_K + k][e] = __uint_as_float(w[R_K + k]);
+    sh.tkey[e] = key;
     if (clipped) {
         const float *ct = d.s.clip_tris + (size_t)slot * kClipFloats;
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) T.cb[k * 3 + c] = ct[k * 7 + 4 + c];
+            for (int c = 0; c < 3; ++c) sh.f[V_CB + k * 3 + c][e] = ct[k * 7 + 4 + c];
     } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) T.cb[k * 3 + c] = (c == k) ? 1.0f : 0.0f;
+            for (int c = 0; c < 3; ++c) sh.f[V_CB + k * 3 + c][e] = (c == k) ? 1.0f : 0.0f;
     }
     S.ax = (int32_t)w[R_SCAN + 0]; S.ay = (int32_t)w[R_SCAN + 1];
     S.bx = (int32_t)w[R_SCAN + 2]; S.by = (int32_t)w[R_SCAN + 3];
     S.cx = (int32_t)w[R_SCAN + 4]; S.cy = (int32_t)w[R_SCAN + 5];
     S.mode = (int32_t)(w[R_MODE] & 0xFFu);
     S.pad = 0;
+    sh.sc[S_AX][e] = S.ax; sh.sc[S_AY][e] = S.ay;
+    sh.sc[S_BX][e] = S.bx; sh.sc[S_BY][e] = S.by;
+    sh.sc[S_CX][e] = S.cx; sh.sc[S_CY][e] = S.cy;
+    sh.sc[S_MODE][e] = S.mode;
     wide = (w[R_MODE] >> 8) & 1u;
 }
 
-// Same arithmetic as bary_depth, on the visibility data.
-__device__ __forceinline__ float vis_depth(const VisTri &T, float pxf, float pyf)
+// The same arithmetic as bary_depth, on a staged triangle.
+template <int TS, int NT>
+__device__ __forceinline__ float vis_depth(const VisShared<TS, NT> &sh, uint32_t e, float pxf, float pyf)
 {
-    float l0 = (((T.vpy[1] - pyf) * (T.vpx[1] + pxf) + T.k0) + (pyf - T.vpy[2]) * (pxf + T.vpx[2])) / T.total;
-    float l1 = (((pyf - T.vpy[0]) * (pxf + T.vpx[0]) + (T.vpy[2] - pyf) * (T.vpx[2] + pxf)) + T.k1) / T.total;
-    float l2 = ((T.k2 + (pyf - T.vpy[1]) * (pxf + T.vpx[1])) + (T.vpy[0] - pyf) * (T.vpx[0] + pxf)) / T.total;
+    float vx0 = sh.f[V_VPX][e], vx1 = sh.f[V_VPX + 1][e], vx2 = sh.f[V_VPX + 2][e];
+    float vy0 = sh.f[V_VPY][e], vy1 = sh.f[V_VPY + 1][e], vy2 = sh.f[V_VPY + 2][e];
+    float k0 = sh.f[V_K][e], k1 = sh.f[V_K + 1][e], k2 = sh.f[V_K + 2][e], total = sh.f[V_K + 3][e];
+    float l0 = (((vy1 - pyf) * (vx1 + pxf) + k0) + (pyf - vy2) * (pxf + vx2)) / total;
+    float l1 = (((pyf - vy0) * (pxf + vx0) + (vy2 - pyf) * (vx2 + pxf)) + k1) / total;
+    float l2 = ((k2 + (pyf - vy1) * (pxf + vx1)) + (vy0 - pyf) * (vx0 + pxf)) / total;
     float lc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        float acc = T.cb[0 * 3 + c] * l0;
-        acc = acc + T.cb[1 * 3 + c] * l1;
-        acc = acc + T.cb[2 * 3 + c] * l2;
+        float acc = sh.f[V_CB + 0 * 3 + c][e] * l0;
+        acc = acc + sh.f[V_CB + 1 * 3 + c][e] * l1;
+        acc = acc + sh.f[V_CB + 2 * 3 + c][e] * l2;
         lc[c] = acc;
     }
-    float z = T.z[0] * lc[0];
-    z = z + T.z[1] * lc[1];
-    z = z + T.z[2] * lc[2];
+    float z = sh.f[V_Z][e] * lc[0];
+    z = z + sh.f[V_Z + 1][e] * lc[1];
+    z = z + sh.f[V_Z + 2][e] * lc[2];
     return z;
 }
 
@@ -1181,23 +1195,26 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
                                               uint32_t y0)
 {
     uint32_t tid = threadIdx.x;
+    uint32_t e = spread<NT>(tid);  // the triangle this thread stages
     int64_t sx0 = d.scissor[0], sy0 = d.scissor[1];
     int64_t sx1 = sx0 + d.scissor[2], sy1 = sy0 + d.scissor[3];
     uint32_t cnt_rows = 0, first = 0;
-    if (tid < n) {
+    if (e < n) {
         uint32_t wide;
-        stage_vis(d, sh.keys[tid], sh.tri[tid], sh.scan[tid], wide);
+        ScanTri S;
+        stage_vis<TS, NT>(d, sh.keys[e], sh, e, S, wide);
         int32_t r0, r1;
-        scan_rows(sh.scan[tid], r0, r1);
+        scan_rows(S, r0, r1);
         int64_t lo = max((int64_t)r0, max((int64_t)y0, sy0));
         int64_t hi = min((int64_t)r1, min((int64_t)y0 + TS - 1, sy1 - 1));
         if (lo <= hi) {
             cnt_rows = (uint32_t)(hi - lo + 1);
             first = (uint32_t)(lo - y0);
         }
-        sh.rows[tid] = first | (cnt_rows << 8) | (wide << 16);
+        sh.rows[e] = first | (cnt_rows << 8) | (wide << 16);
     }
-    // exclusive scan of the row counts: within each wave, then across waves
+    // exclusive scan of the row counts in thread order: within each wave,
+    // then across waves
     uint32_t lane = lane_id(), wave = tid >> 6;
     uint32_t v = cnt_rows;
 #pragma unroll
@@ -1214,24 +1231,30 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
         if ((uint32_t)w < wave) before += x;
         total += x;
     }
-    if (tid < n) {
+    if (e < n) {
         uint32_t b = before + v - cnt_rows;
-        for (uint32_t j = 0; j < cnt_rows; ++j) sh.owner[b + j] = (uint16_t)(tid | ((first + j) << 8));
+        for (uint32_t j = 0; j < cnt_rows; ++j) sh.owner[b + j] = (uint16_t)(e | ((first + j) << 8));
     }
     __syncthreads();
 
     bool counts = d.s.frag_px != nullptr;
     uint32_t f = 0;
-    for (uint32_t idx = tid; idx < total; idx += NT) {
+    for (uint32_t idx = spread<NT>(tid); idx < total; idx += NT) {
         uint32_t o = sh.owner[idx];
-        uint32_t e = o & 0xFFu, row = o >> 8;
+        uint32_t te = o & 0xFFu, row = o >> 8;
         int32_t y = (int32_t)(y0 + row);
+        ScanTri S;
+        S.ax = sh.sc[S_AX][te]; S.ay = sh.sc[S_AY][te];
+        S.bx = sh.sc[S_BX][te]; S.by = sh.sc[S_BY][te];
+        S.cx = sh.sc[S_CX][te]; S.cy = sh.sc[S_CY][te];
+        S.mode = sh.sc[S_MODE][te];
+        S.pad = 0;
         int32_t x1, x2;
-        if (!scan_span(sh.scan[e], y, x1, x2, ((sh.rows[e] >> 16) & 1u) != 0u)) continue;
+        if (!scan_span(S, y, x1, x2, ((sh.rows[te] >> 16) & 1u) != 0u)) continue;
         int64_t lo = max((int64_t)x1, max(sx0, (int64_t)x0));
         int64_t hi = min((int64_t)x2, min(sx1 - 1, (int64_t)x0 + TS - 1));
         if (lo > hi) continue;
-        const VisTri &T = sh.tri[e];
+        uint32_t key = sh.tkey[te];
         uint32_t q0 = row * TS + (uint32_t)(lo - x0);
         uint32_t len = (uint32_t)(hi - lo) + 1u;
         f += len;
@@ -1239,15 +1262,15 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
             for (uint32_t i = 0; i < len; ++i) atomicAdd(&sh.cnt[q0 + i], 1u);
         if (d.resolve == RES_LAST) {
             // no depth test: the last fragment in API order wins, depth unused
-            unsigned long long val = (unsigned long long)(~T.key);
+            unsigned long long val = (unsigned long long)(~key);
             for (uint32_t i = 0; i < len; ++i) atomicMin(&sh.best[q0 + i], val);
             continue;
         }
         float pyf = (float)y;
         for (uint32_t i = 0; i < len; ++i) {
             uint32_t q = q0 + i;
-            float z = vis_depth(T, (float)(lo + i), pyf);
-            if (oi_pass(d, z, sh.idep[q])) atomicMin(&sh.best[q], oi_value(d.resolve, z, T.key));
+            float z = vis_depth<TS, NT>(sh, te, (float)(lo + i), pyf);
+            if (oi_pass(d, z, sh.idep[q])) atomicMin(&sh.best[q], oi_value(d.resolve, z, key));
         }
     }
     __syncthreads();
@@ -1290,6 +1313,7 @@ __device__ __forceinline__ void raster_tile_vis(const DrawParams &d, uint32_t ti
         if (fpos < seg_n) {
             n = min(NB, seg_n - fpos);
             if (tid < n) sh.keys[tid] = d.s.bins[seg0 + fpos + tid];
+            __syncthreads();  // staging reads the keys through another thread order
             fpos += n;
         } else if (bpos < nbig) {
             if (tid < kBatch) {  // wave 0 compacts the hits
