@@ -300,36 +300,48 @@ __device__ __forceinline__ void pick(CTri &t, const CTri out[2], uint32_t c)
 
 // The clipper's output in the order of the reference's plane-by-plane queue
 // (clipper.rs:756-775), which is lexicographic in the split choices (plane 0
-// most significant).  Every leaf is named by its 6-bit path and re-clipped
-// from the root along that path, so nothing but the current triangle lives in
-// registers (no stack in scratch memory); a path that leaves a node's output
-// count skips that node's whole subtree.
+// most significant): a depth-first walk that keeps the triangle entering each
+// plane in registers (levels are indexed only by unrolled loops, so nothing
+// goes to scratch memory).  Every node of the split tree is clipped once,
+// plus once more when its second piece is taken.
 template <class F>
 __device__ int clip_walk(const CTri &root, F &&emit)
 {
-    int leaf = 0;
-    uint32_t code = 0;  // choice of plane L in bit 5 - L
-    while (code < 64u) {
-        CTri t = root;
-        int fail = -1;
+    CTri t[7];
+    uint32_t nn[6], ch[6];
+    t[0] = root;
 #pragma unroll
-        for (int L = 0; L < 6; ++L) {
-            if (fail < 0) {
+    for (int k = 0; k < 6; ++k) { nn[k] = 0u; ch[k] = 0u; }
+    int leaf = 0;
+    int from = 0;  // first level to (re)compute, its choice already set in ch
+    for (;;) {
+        int dead = 6;  // level whose choice has no piece, 6 = reached a leaf
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            if (k >= from && dead == 6) {
                 CTri out[2];
-                uint32_t n = (uint32_t)clip_against_plane(t, L, out);
-                uint32_t c = (code >> (5 - L)) & 1u;
-                if (c >= n) fail = L;
-                else pick(t, out, c);
+                nn[k] = (uint32_t)clip_against_plane(t[k], k, out);
+                if (ch[k] >= nn[k]) dead = k;
+                else pick(t[k + 1], out, ch[k]);
             }
         }
-        if (fail < 0) {
-            emit(t, leaf);
+        if (dead == 6) {
+            emit(t[6], leaf);
             ++leaf;
-            ++code;
-        } else {
-            uint32_t sh = 5u - (uint32_t)fail;
-            code = ((code >> sh) + 1u) << sh;  // next sibling subtree at that level
         }
+        // backtrack: the deepest level above the end of this path whose first
+        // piece was taken and that has a second piece
+        int next = -1;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (k < dead && ch[k] == 0u && nn[k] == 2u) next = k;
+        if (next < 0) break;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            if (k == next) ch[k] = 1u;
+            if (k > next) ch[k] = 0u;
+        }
+        from = next;
     }
     return leaf;
 }
