@@ -335,6 +335,8 @@ struct Step {
     GroupLaunch gl{};
     BatchArgs batch{};
     size_t table_off = 0;            // offset of this group's tables in the plan's device block
+    uint32_t stage = 0;              // steps of one stage are independent and run concurrently
+    size_t arena_off = 0;            // this group's scratch within the arena
     // clear
     uint32_t *color = nullptr;
     uint32_t rgba = 0;
@@ -418,6 +420,8 @@ struct WgcDevice_T : Object {
     int hip_device = 0;
     uint32_t semantics = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;     // second stream for the independent groups of a stage
+    hipEvent_t fork = nullptr, join = nullptr;
     std::mutex mu;
     // per-draw scratch arena, reused by every draw (they run in stream order)
     uint8_t *arena = nullptr;
@@ -476,6 +480,9 @@ struct WgcDevice_T : Object {
         if (d_counters) (void)hipFree(d_counters);
         if (d_srgb) (void)hipFree(d_srgb);
         if (frag_px) (void)hipFree(frag_px);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -569,6 +576,9 @@ extern "C" int32_t wgc_device_create(const WgcDeviceDescriptor *desc, WgcDevice 
     d->semantics = desc->semantics;
     hipError_t e = hipSetDevice(d->hip_device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&d->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&d->join, hipEventDisableTiming);
     if (e == hipSuccess) {
         int cus = 0;
         e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->hip_device);
@@ -1748,9 +1758,10 @@ static int32_t collect_pass(WgcDevice_T *dev, const PassCommand &pc, uint32_t ti
     return WGC_OK;
 }
 
-static void add_group(std::vector<Step> &steps, std::vector<DrawParams> draws, uint32_t cfg)
+static void add_group(std::vector<Step> &steps, std::vector<DrawParams> draws, uint32_t cfg, uint32_t stage)
 {
     Step s;
+    s.stage = stage;
     s.kind = StepKind::Group;
     s.multi = draws.size() > 1;
     s.gl.program = draws[0].program;
@@ -1783,14 +1794,19 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         pass_textures(cb->passes[i], passes[i].writes, passes[i].reads);
     }
     // 2. steps: runs of independent single-draw passes become launch groups
-    //    (one per program and resolve kind); everything else runs in order
+    //    (one per program and resolve kind) of one stage; everything else
+    //    runs in order, one stage per step
     size_t i = 0;
+    uint32_t stage = 0;
     while (i < passes.size()) {
         PassWork &pw = passes[i];
         bool batchable = dev->batching && pw.draws.size() == 1;
         if (!batchable) {
-            for (auto &dw : pw.draws) add_group(plan.steps, {dw.d}, cfg);
-            if (pw.clear_only) plan.steps.push_back(pw.clear);
+            for (auto &dw : pw.draws) add_group(plan.steps, {dw.d}, cfg, stage++);
+            if (pw.clear_only) {
+                pw.clear.stage = stage++;
+                plan.steps.push_back(pw.clear);
+            }
             ++i;
             continue;
         }
@@ -1811,7 +1827,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
             ++j;
         }
         if (j - i == 1) {
-            add_group(plan.steps, {pw.draws[0].d}, cfg);
+            add_group(plan.steps, {pw.draws[0].d}, cfg, stage++);
         } else {
             // group by (program, ordered); ordered groups use the 16x16 raster
             std::map<std::pair<uint32_t, bool>, std::vector<DrawParams>> groups;
@@ -1819,16 +1835,27 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
                 const DrawParams &d = passes[k].draws[0].d;
                 groups[{d.program, d.resolve == RES_ORDERED}].push_back(d);
             }
-            for (auto &kv : groups) add_group(plan.steps, std::move(kv.second), cfg);
+            // the groups of one run of independent passes share a stage
+            for (auto &kv : groups) add_group(plan.steps, std::move(kv.second), cfg, stage);
+            ++stage;
         }
         i = j;
     }
-    // 3. scratch: the arena must hold the largest group
+    // 3. scratch: the groups of a stage side by side, stages reusing the
+    //    arena (they run one after another)
     size_t need = 0;
     size_t tables = 0;
+    size_t stage_off = 0;
+    uint32_t cur_stage = UINT32_MAX;
     for (Step &s : plan.steps) {
         if (s.kind != StepKind::Group) continue;
-        need = std::max(need, layout_group(s, nullptr, nullptr, 0, dev));
+        if (s.stage != cur_stage) {
+            cur_stage = s.stage;
+            stage_off = 0;
+        }
+        s.arena_off = stage_off;
+        stage_off = align_up(stage_off + layout_group(s, nullptr, nullptr, 0, dev), 256);
+        need = std::max(need, stage_off);
         if (s.multi) {
             s.table_off = tables;
             size_t n = s.draws.size();
@@ -1864,7 +1891,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
             plan.launches += 1;
             continue;
         }
-        layout_group(s, dev->arena, dev->frag_px, dev->frag_px_size, dev);
+        layout_group(s, dev->arena + s.arena_off, dev->frag_px, dev->frag_px_size, dev);
         plan.launches += s.gl.ordered ? 8 : 9;
         for (auto &d : s.draws) plan.prims += d.total_prims;
         if (!s.multi) continue;
@@ -1899,14 +1926,38 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
     return WGC_OK;
 }
 
+static hipError_t launch_step(WgcDevice_T *dev, const Step &s, hipStream_t st, KernelTimer *kt)
+{
+    if (s.kind == StepKind::Clear) return launch_clear(s.color, s.rgba, s.depth, s.dv, s.n, st, kt);
+    if (s.multi) return launch_batch(s.draws[0], s.batch, s.gl, st, kt);
+    return launch_draw(s.draws[0], s.batch, s.gl, st, kt);
+}
+
+// Steps run in order on the device stream, except that the steps of one
+// stage (independent launch groups) alternate between the device stream and
+// a second stream, forked from and joined back into the device stream with
+// events (the same under hipGraph capture).
 static hipError_t launch_plan(WgcDevice_T *dev, const Plan &plan, KernelTimer *kt)
 {
     hipError_t e = hipSuccess;
-    for (const Step &s : plan.steps) {
-        if (s.kind == StepKind::Clear) e = launch_clear(s.color, s.rgba, s.depth, s.dv, s.n, dev->stream, kt);
-        else if (s.multi) e = launch_batch(s.draws[0], s.batch, s.gl, dev->stream, kt);
-        else e = launch_draw(s.draws[0], s.batch, s.gl, dev->stream, kt);
+    size_t i = 0;
+    while (i < plan.steps.size()) {
+        size_t j = i + 1;
+        while (j < plan.steps.size() && plan.steps[j].stage == plan.steps[i].stage) ++j;
+        if (j - i == 1) {
+            e = launch_step(dev, plan.steps[i], dev->stream, kt);
+            if (e != hipSuccess) return e;
+            i = j;
+            continue;
+        }
+        e = hipEventRecord(dev->fork, dev->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(dev->side, dev->fork, 0);
+        for (size_t k = i; k < j && e == hipSuccess; ++k)
+            e = launch_step(dev, plan.steps[k], ((k - i) & 1u) ? dev->side : dev->stream, kt);
+        if (e == hipSuccess) e = hipEventRecord(dev->join, dev->side);
+        if (e == hipSuccess) e = hipStreamWaitEvent(dev->stream, dev->join, 0);
         if (e != hipSuccess) return e;
+        i = j;
     }
     return hipSuccess;
 }
