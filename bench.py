@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration for profiles)")
+    ap.add_argument("--profile-batch", type=int, default=0, metavar="N",
+                    help="only run N submissions of the batched workload (for counter profiles)")
     args = ap.parse_args()
 
     if args.calibrate_copy:
@@ -132,6 +134,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calibration_copy_bytes": n, "reads": n, "writes": n}), flush=True)
+        return
+
+    if args.profile_batch:
+        profile_batch(args.profile_batch)
         return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -157,7 +163,8 @@ def main():
         avg_s = prof[dominant]["avg_us"] * 1e-6
         achieved = alg_bytes / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(args.config, dominant),
                     "algorithmic_bytes_per_launch": alg_bytes}
     pass_bytes_rate = alg_bytes / (ms_per_step * 1e-3) / 1e9
 
@@ -244,8 +251,8 @@ def run_batched(devs, args):
     ms_plain = autotune.time_submissions(pp, max(2, reps // 2), warmup=1)
     dev.set_option(wgc.OPTION_BATCHING, 1)
     dev.set_option(wgc.OPTION_GRAPHS, 1)
-    alg = sum(scenes.algorithmic_bytes(s, count_texture=False) for s in specs) + 4 * 1024 * 1024
-    raster = prof.get("k_raster")
+    alg = batched_algorithmic_bytes(specs)
+    raster = prof.get("k_visibility")
     return {
         "workload": "256 passes, W,H log-uniform [64,1024], spheres of 100-50k triangles, half textured "
                     "(one shared 1024^2 texture), Rgba8Unorm + Depth32Float Less",
@@ -257,7 +264,7 @@ def run_batched(devs, args):
         "speedup_vs_plain": round(ms_plain / ms_batched, 2),
         "algorithmic_GBps": round(alg / (ms_batched * 1e-3) / 1e9, 1),
         "hbm_frac_of_submission": round(alg / (ms_batched * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "raster_kernel_hbm_frac": round(alg / (raster["avg_us"] * 1e-6 * raster["launches"] / 2) / 1e9 / HBM_PEAK_GBS, 4)
+        "visibility_kernel_hbm_frac": round(alg / (raster["avg_us"] * 1e-6 * raster["launches"] / 2) / 1e9 / HBM_PEAK_GBS, 4)
         if raster else None,
         "autotune_ms": {wgc.RASTER_TILES[k].__repr__(): round(v, 4) for k, v in tune.items()},
         "raster_tile": wgc.RASTER_TILES[best],
@@ -297,6 +304,55 @@ def run_serving(devs, args):
         out[mode] = srv.run(hi, batched=batched).summary()
         srv.heap.release()
     return out
+
+
+def batched_specs():
+    from wgc import scenes
+    seed = scenes.SEED_BASE + 4
+    return [scenes.batched_pass(seed, i, *a) for i, a in enumerate(scenes.batched_sizes(seed, 256))]
+
+
+def batched_algorithmic_bytes(specs) -> int:
+    """Stores of every pass plus vertices and indices, and the shared 4 MiB
+    texture once per submission."""
+    from wgc import scenes
+    return sum(scenes.algorithmic_bytes(s, count_texture=False) for s in specs) + 4 * 1024 * 1024
+
+
+def profile_batch(n: int):
+    """The batched workload alone: n submissions with the default raster tile
+    (16x16 px, 256 lanes), for kernel-trace and HBM-counter runs."""
+    import wgc
+    from wgc import runner
+    specs = batched_specs()
+    dev = wgc.Device(0, wgc.SEMANTICS_REFERENCE)
+    pp = runner.PreparedPasses(dev, specs)
+    for _ in range(2):
+        pp.submit()
+    dev.poll()
+    dev.timestamp(0)
+    for _ in range(n):
+        pp.submit()
+    dev.timestamp(1)
+    dev.poll()
+    ms = dev.elapsed_ms(0, 1) / n
+    print(json.dumps({"batched_submissions": n, "ms_per_submission": round(ms, 4),
+                      "fragments_per_submission": pp.total_fragments(),
+                      "algorithmic_bytes": batched_algorithmic_bytes(specs)}), flush=True)
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary (calibrated FETCH_SIZE + WRITE_SIZE), or None."""
+    path = ROOT / "profiles" / "hbm_traffic.txt"
+    try:
+        for line in path.read_text().splitlines():
+            f = line.split()
+            if len(f) >= 3 and not line.startswith("#") and f[0] == workload and f[1] == kernel:
+                return int(f[2])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_model() -> str:
