@@ -159,6 +159,11 @@ typedef struct WgcRenderPipelineDescriptor {
     WgcShaderModule fragment_module; /* NULL: no fragment stage */
     uint32_t target_format;  /* WgcTextureFormat */
     uint32_t blend;          /* WgcBlend */
+    /* Entry points of WGSL modules (wgpu::VertexState::entry_point,
+     * FragmentState::entry_point); NULL: the module's only entry point of
+     * that stage (state.rs via CompiledModule::find_entry_point). */
+    const char *vertex_entry;
+    const char *fragment_entry;
 } WgcRenderPipelineDescriptor;
 
 typedef struct WgcColorAttachment {
@@ -251,6 +256,44 @@ void wgc_bind_group_release(WgcBindGroup group);
 int32_t wgc_shader_module_create_builtin(WgcDevice device, uint32_t program,
                                          uint32_t interpolation, WgcShaderModule *out);
 void wgc_shader_module_release(WgcShaderModule module);
+
+/* A shader module from WGSL source (wgpu::Device::create_shader_module with
+ * ShaderSource::Wgsl; the reference's ShaderModule::new, shader.rs, runs
+ * naga's front end and validator).  The module is translated and checked
+ * here; pipelines compile their kernels from it.  Constructs the reference's
+ * shader compiler leaves as todo!() return WGC_ERR_UNSUPPORTED with the
+ * construct named in wgc_last_error(); invalid WGSL returns
+ * WGC_ERR_INVALID with line:column. */
+int32_t wgc_shader_module_create_wgsl(WgcDevice device, const char *source, size_t length,
+                                      WgcShaderModule *out_module);
+
+/* Runs one entry point of a WGSL module for `count` invocations on the
+ * device and returns their outputs (naga-cranelift EntryPoint::run,
+ * product.rs:282-300, the harness of its tests.rs).  Invocation i has
+ * vertex_index first_index + i (vertex) or position (first_index + i, 0, 0,
+ * 1) (fragment); location inputs read as zero; resources come from
+ * `groups[g]` for @group(g).  `results` receives WGC_RUN_WORDS words per
+ * invocation: the abort code (0, or WGC_ABORT_*), the clip position
+ * (vertex) or frag depth (fragment), then 16 locations x 4 output words. */
+#define WGC_RUN_WORDS 69
+enum { WGC_ABORT_KILL = 3, WGC_ABORT_OUT_OF_BOUNDS = 4, WGC_ABORT_DIVISION_BY_ZERO = 5, WGC_ABORT_OVERFLOW = 6 };
+int32_t wgc_shader_module_run(WgcDevice device, WgcShaderModule module, const char *entry_point,
+                              uint32_t first_index, uint32_t count, const WgcBindGroup *groups,
+                              uint32_t group_count, uint32_t *results, size_t results_words);
+
+/* Translation only, without a device (for tools and tests): what = 0 the
+ * emitted C++ functions, what = 1 the module's interface as JSON.  Writes at
+ * most `capacity` bytes (NUL-terminated) and the full size + 1 to *needed.
+ * No counterpart in the reference (its compiler output is machine code). */
+int32_t wgc_wgsl_translate(const char *source, size_t length, uint32_t what, char *out, size_t capacity,
+                           size_t *needed);
+
+/* Compiles the kernels of a pipeline made of the module's vertex and (if it
+ * has one) fragment entry point for gfx950, without a device: an offline
+ * check of the stage interface and of the generated code.  `semantics` is a
+ * WgcSemantics.  No counterpart in the reference. */
+int32_t wgc_wgsl_check_pipeline(const char *source, size_t length, const char *vertex_entry,
+                                const char *fragment_entry, uint32_t semantics);
 int32_t wgc_render_pipeline_create(WgcDevice device, const WgcRenderPipelineDescriptor *desc,
                                    WgcRenderPipeline *out);
 void wgc_render_pipeline_release(WgcRenderPipeline pipeline);

@@ -1,1557 +1,9 @@
-// kernels.hip -- HIP kernels of the render-pass executor for gfx950.
-//
-// Per launch group: k_zero -> k_setup -> k_clip -> k_scan_blocks ->
-// k_scan_totals -> k_scan_tiles -> k_bin_fill -> k_raster.  A group is one
-// draw (parameters passed by value) or a table of independent draws -- the
-// batched form, where hundreds of render passes share every launch (MULTI).
-//
-//  k_setup     one thread per primitive: vertex shader for its three indices
-//              (state.rs:519-535), winding and culling (state.rs:540-561),
-//              the clipper's pass-through test and raster setup; writes the
-//              triangle record the raster reads and the primitive's bin
-//              record, counts it into its tiles (keeping its bin positions),
-//              and appends (ballot + prefix count, one atomic per wave) to
-//              the big-triangle list and the clip list.
-//  k_clip      primitives that cross a clip plane: the 6-plane clipper as an
-//              in-order depth-first walk (same output order as the
-//              reference's plane-by-plane queue, clipper.rs:756-775).
-//  k_scan_*    per-tile bin offsets (exclusive scan over all tiles of the
-//              group) and the compacted list of tiles the raster must visit.
-//  k_bin_fill  scatters record keys into per-tile lists (no atomics).
-//  k_raster    persistent workgroups, statically scheduled: first
-//              the active tiles (TSxTS pixels, PPT pixels per lane; colour and
-//              depth in registers from load or clear to one coalesced store;
-//              triangles staged 64 at a time in LDS with per-row spans of the
-//              reference's integer scanline rule), then chunks of untouched
-//              tiles that only need the LoadOp::Clear values stored.
-#pragma clang fp contract(off)
-
-#include "wgc_core.h"
+// kernels.hip -- host-side launchers of the render-pass kernels for the
+// built-in programs (the kernels themselves are in kernels_dev.h).
+#include "kernels_dev.h"
 #include "wgc_launch.h"
 
 namespace wgc {
-
-// ------------------------------------------------------------------
-// helpers
-// ------------------------------------------------------------------
-
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-template <bool MULTI>
-__device__ __forceinline__ const DrawParams &draw_at(const DrawParams &d0, const BatchArgs &b, uint32_t i)
-{
-    if constexpr (MULTI) return b.draws[i];
-    else return d0;
-}
-
-// Largest i < n with base[i] <= g (empty ranges resolve to the last of them).
-__device__ __forceinline__ uint32_t find_slot(const uint32_t *base, uint32_t n, uint32_t g)
-{
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1u) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (base[mid] <= g) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// Wave-aggregated append: one atomic per wave, slots in lane order.  Every
-// lane of the wave must pass the same counter.
-__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
-{
-    uint64_t mask = __ballot(pred);
-    if (mask == 0) return 0xFFFFFFFFu;
-    uint32_t lane = lane_id();
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    uint64_t below = (lane == 0) ? 0ull : (mask & ((~0ull) >> (64 - lane)));
-    return pred ? base + (uint32_t)__popcll(below) : 0xFFFFFFFFu;
-}
-
-// Counts a record into the bins of its footprint (row-major) and keeps its
-// position in each, so that the bins can be filled without atomics.
-__device__ __forceinline__ void add_tile_counts(const DrawParams &d, uint32_t rec, uint32_t tx0, uint32_t ty0,
-                                                uint32_t tx1, uint32_t ty1)
-{
-    uint32_t *slots = d.s.rec_slots + (size_t)rec * kMaxBinTiles;
-    uint32_t k = 0;
-    for (uint32_t ty = ty0; ty <= ty1; ++ty)
-        for (uint32_t tx = tx0; tx <= tx1; ++tx) slots[k++] = atomicAdd(&d.s.tile_count[ty * d.tiles_x + tx], 1u);
-}
-
-// add_tile_counts for a whole wave (every lane calls it).  For the k-th
-// tile of every lane's footprint (row-major, k up to the largest footprint
-// in the wave), lanes on the same tile form a group first (up to 8 groups
-// per round, in registers: neighbouring triangles of a mesh mostly share
-// tiles, and one tile may collect thousands of them); then every group's
-// first lane, and every lane left alone, takes its positions with one atomic,
-// all of them issued together.
-__device__ __forceinline__ void add_tile_counts_wave(const DrawParams &d, bool has, uint32_t rec, uint32_t tx0,
-                                                     uint32_t ty0, uint32_t tx1, uint32_t ty1)
-{
-    uint32_t w = has ? tx1 - tx0 + 1u : 0u;
-    uint32_t ntl = has ? w * (ty1 - ty0 + 1u) : 0u;
-    uint32_t *slots = d.s.rec_slots + (size_t)rec * kMaxBinTiles;
-    uint32_t lane = lane_id();
-    uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    for (uint32_t k = 0; k < (uint32_t)kMaxBinTiles; ++k) {
-        bool act = k < ntl;
-        uint64_t pending = __ballot(act);
-        if (!pending) break;
-        uint32_t t = act ? (ty0 + k / w) * d.tiles_x + (tx0 + k % w) : 0xFFFFFFFFu;
-        uint32_t lead = lane, cnt = 1u, rank = 0u;
-        for (int it = 0; it < 8 && pending; ++it) {
-            int leader = __ffsll((unsigned long long)pending) - 1;
-            uint32_t tl = (uint32_t)__shfl((int)t, leader);
-            bool mine = act && t == tl;
-            uint64_t m = __ballot(mine);
-            if (mine) {
-                lead = (uint32_t)leader;
-                rank = (uint32_t)__popcll(m & lt);
-                cnt = (uint32_t)__popcll(m);
-            }
-            pending &= ~m;
-        }
-        uint32_t base = 0u;
-        if (act && lead == lane) base = atomicAdd(&d.s.tile_count[t], cnt);
-        base = (uint32_t)__shfl((int)base, (int)lead);
-        if (act) slots[k] = base + rank;
-    }
-}
-
-__device__ __forceinline__ uint32_t wave_max(uint32_t v)
-{
-    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
-    return v;
-}
-
-// Compile-time loop: f(IC<I>{}) for I in [B, E).
-template <int I>
-struct IC {
-    static constexpr int value = I;
-};
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F &&f)
-{
-    if constexpr (B < E) {
-        f(IC<B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
-// ------------------------------------------------------------------
-// triangle records
-// ------------------------------------------------------------------
-
-__device__ __forceinline__ void write_record(uint32_t *rec, const RasterTri &r, const VsOut v[3], uint32_t key)
-{
-    ScanTri s = scan_setup(r);
-    uint32_t w[kRecWords];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        w[R_VP + 2 * k] = __float_as_uint(r.vpx[k]);
-        w[R_VP + 2 * k + 1] = __float_as_uint(r.vpy[k]);
-        w[R_Z + k] = __float_as_uint(r.z[k]);
-        w[R_IW + k] = __float_as_uint(1.0f / v[k].clip[3]);
-#pragma unroll
-        for (int c = 0; c < kMaxVar; ++c) w[R_VAR + k * kMaxVar + c] = __float_as_uint(v[k].var[c]);
-    }
-    // constant terms of the three shoelace sums (the same products the
-    // per-pixel evaluation would form)
-    w[R_K + 0] = __float_as_uint((r.vpy[2] - r.vpy[1]) * (r.vpx[2] + r.vpx[1]));
-    w[R_K + 1] = __float_as_uint((r.vpy[0] - r.vpy[2]) * (r.vpx[0] + r.vpx[2]));
-    w[R_K + 2] = __float_as_uint((r.vpy[1] - r.vpy[0]) * (r.vpx[1] + r.vpx[0]));
-    w[R_K + 3] = __float_as_uint(r.total);
-    w[R_SCAN + 0] = (uint32_t)s.ax; w[R_SCAN + 1] = (uint32_t)s.ay;
-    w[R_SCAN + 2] = (uint32_t)s.bx; w[R_SCAN + 3] = (uint32_t)s.by;
-    w[R_SCAN + 4] = (uint32_t)s.cx; w[R_SCAN + 5] = (uint32_t)s.cy;
-    w[R_MODE] = (uint32_t)s.mode | ((scan_wide(r) ? 1u : 0u) << 8);
-    w[R_KEY] = key;
-    uint4 *o = reinterpret_cast<uint4 *>(rec);
-#pragma unroll
-    for (int q = 0; q < kRecWords / 4; ++q) o[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
-}
-
-// ------------------------------------------------------------------
-// k_zero: tile counts of the group, per-draw counters, group header
-// ------------------------------------------------------------------
-
-__global__ void k_zero(uint32_t *tile_count, uint32_t n_tiles, uint32_t *counters, uint32_t n_counters,
-                       uint32_t *header)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_tiles) tile_count[i] = 0u;
-    if (i < n_counters) counters[i] = 0u;
-    if (i < H_NUM) header[i] = 0u;
-}
-
-// ------------------------------------------------------------------
-// k_setup
-// ------------------------------------------------------------------
-
-// Grows the draw's big-triangle footprint union; kept as maxima of
-// (65535 - min) and max.  Every lane of the wave calls it (same draw).
-__device__ __forceinline__ void note_big_wave(const DrawParams &d, bool big, const uint4 &rec)
-{
-    if (!__ballot(big)) return;
-    uint32_t nx0 = wave_max(big ? 65535u - (rec.y & 0xFFFFu) : 0u);
-    uint32_t ny0 = wave_max(big ? 65535u - (rec.y >> 16) : 0u);
-    uint32_t x1 = wave_max(big ? (rec.z & 0xFFFFu) : 0u);
-    uint32_t y1 = wave_max(big ? (rec.z >> 16) : 0u);
-    if (lane_id() == 0) {
-        atomicMax(&d.s.counters[C_BIG_NX0], nx0);
-        atomicMax(&d.s.counters[C_BIG_NY0], ny0);
-        atomicMax(&d.s.counters[C_BIG_X1], x1);
-        atomicMax(&d.s.counters[C_BIG_Y1], y1);
-    }
-}
-
-template <int PROG, bool MULTI>
-__global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
-{
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t di = 0, p = g;
-    if constexpr (MULTI) {
-        // draw ranges are padded to whole waves, so a wave never spans draws
-        if (g >= b.total_prims) return;
-        // every lane of the wave is in the same draw: look it up once, in
-        // scalar registers, so that the draw's parameters are scalar loads
-        di = __builtin_amdgcn_readfirstlane(find_slot(b.prim_base, b.n, __builtin_amdgcn_readfirstlane(g)));
-        p = g - b.prim_base[di];
-    }
-    const DrawParams &d = draw_at<MULTI>(d0, b, di);
-    if constexpr (!MULTI) {
-        if (g >= d.total_prims) return;  // whole waves past the end only matter in the batched form
-    }
-    bool active = p < d.total_prims;
-    bool has_rec = false, needs_clip = false;
-    uint4 rec = make_uint4(0, 0, 0, 0);
-    uint32_t ntiles = 0;
-    if (active) {
-        VsOut v[3];
-        if (shade_primitive<PROG>(d, p, v)) {
-            bool ff, culled;
-            if (!face_and_cull(d, v, ff, culled)) {
-                atomicOr(d.s.error, (uint32_t)ERR_W_ZERO);
-            } else if (!culled) {
-                int cc = clip_class(v);
-                if (cc == 0) {
-                    float pos[3][4];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k)
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) pos[k][c] = v[k].clip[c];
-                    RasterTri r;
-                    raster_setup(d, pos, r);
-                    if (!coord_ok(r)) {
-                        atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE);
-                    } else if (r.total != 0.0f) {
-                        ScanTri s = scan_setup(r);
-                        uint32_t tx0, ty0, tx1, ty1;
-                        if (tile_bbox(d, s, tx0, ty0, tx1, ty1)) {
-                            has_rec = true;
-                            rec = make_uint4(p << 7, tx0 | (ty0 << 16), tx1 | (ty1 << 16), di);
-                            ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-                            write_record(d.s.tri + (size_t)p * kRecWords, r, v, p << 7);
-                        }
-                    }
-                } else if (cc == 1) {
-                    needs_clip = true;
-                }
-            }
-        }
-    }
-    bool small = has_rec && ntiles <= (uint32_t)kMaxBinTiles;
-    bool big = has_rec && !small;
-    uint32_t slot = wave_append(&d.s.header[H_CLIP], needs_clip);
-    if (needs_clip) {
-        if (slot < d.s.clip_list_cap) d.s.clip_list[slot] = make_uint2(di, p);
-        else atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
-    }
-    // the per-primitive record slot: an empty footprint unless binned here
-    d.s.recs[g] = small ? rec : make_uint4(0u, 0xFFFFFFFFu, 0u, di);
-    add_tile_counts_wave(d, small, g, rec.y & 0xFFFFu, rec.y >> 16, rec.z & 0xFFFFu, rec.z >> 16);
-    slot = wave_append(&d.s.counters[C_BIG], big);
-    if (big) {
-        if (slot < d.s.big_cap) d.s.big[slot] = rec;
-        else atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
-    }
-    note_big_wave(d, big && slot < d.s.big_cap, rec);
-}
-
-// ------------------------------------------------------------------
-// k_clip
-// ------------------------------------------------------------------
-
-// Selects out[c] without indexing (keeps the triangles in registers).
-__device__ __forceinline__ void pick(CTri &t, const CTri out[2], uint32_t c)
-{
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) t.p[k][j] = c ? out[1].p[k][j] : out[0].p[k][j];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) t.b[k][j] = c ? out[1].b[k][j] : out[0].b[k][j];
-    }
-}
-
-// The clipper's output in the order of the reference's plane-by-plane queue
-// (clipper.rs:756-775), which is lexicographic in the split choices (plane 0
-// most significant): a depth-first walk that keeps the triangle entering each
-// plane in registers (levels are indexed only by unrolled loops, so nothing
-// goes to scratch memory).  Every node of the split tree is clipped once,
-// plus once more when its second piece is taken.
-template <class F>
-__device__ int clip_walk(const CTri &root, F &&emit)
-{
-    CTri t[7];
-    uint32_t nn[6], ch[6];
-    t[0] = root;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) { nn[k] = 0u; ch[k] = 0u; }
-    int leaf = 0;
-    int from = 0;  // first level to (re)compute, its choice already set in ch
-    for (;;) {
-        int dead = 6;  // level whose choice has no piece, 6 = reached a leaf
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            if (k >= from && dead == 6) {
-                CTri out[2];
-                nn[k] = (uint32_t)clip_against_plane(t[k], k, out);
-                if (ch[k] >= nn[k]) dead = k;
-                else pick(t[k + 1], out, ch[k]);
-            }
-        }
-        if (dead == 6) {
-            emit(t[6], leaf);
-            ++leaf;
-        }
-        // backtrack: the deepest level above the end of this path whose first
-        // piece was taken and that has a second piece
-        int next = -1;
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-            if (k < dead && ch[k] == 0u && nn[k] == 2u) next = k;
-        if (next < 0) break;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            if (k == next) ch[k] = 1u;
-            if (k > next) ch[k] = 0u;
-        }
-        from = next;
-    }
-    return leaf;
-}
-
-// Wave-aggregated atomicAdd of v to *addr: lanes (with act) that pass the
-// same address share one atomic and receive consecutive ranges in lane
-// order.  Every lane of the wave must call it.
-__device__ __forceinline__ uint32_t wave_add(uint32_t *addr, uint32_t v, bool act)
-{
-    uint32_t lane = lane_id();
-    act = act && v > 0u;
-    uint64_t pending = __ballot(act);
-    uint32_t res = 0;
-    unsigned long long a = (unsigned long long)addr;
-    while (pending) {
-        int leader = __ffsll((unsigned long long)pending) - 1;
-        uint32_t lo = (uint32_t)__shfl((int)(uint32_t)a, leader);
-        uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(a >> 32), leader);
-        bool mine = act && a == (((unsigned long long)hi << 32) | lo);
-        uint64_t m = __ballot(mine);
-        // inclusive scan of v over this group's lanes
-        uint32_t x = mine ? v : 0u;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            uint32_t t = (uint32_t)__shfl_up((int)x, off);
-            if (lane >= (uint32_t)off) x += t;
-        }
-        uint32_t total = (uint32_t)__shfl((int)x, 63);
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(addr, total);
-        base = (uint32_t)__shfl((int)base, leader);
-        if (mine) res = base + x - v;
-        pending &= ~m;
-    }
-    return res;
-}
-
-// One primitive that crosses a clip plane: the clipper's sub-triangles, in
-// its order, become clip records (positions and clip barycentrics), triangle
-// records and bin records.  Every lane of the wave calls it (with `act`
-// false for lanes without a primitive), so that the slot counters are taken
-// with one atomic per wave.  Sub-triangles that turn out empty or big leave
-// an empty bin record behind.
-template <int PROG>
-__device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act)
-{
-    VsOut v[3];
-    CTri root;
-    uint32_t nleaf = 0;
-    if (act && shade_primitive<PROG>(d, p, v)) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) root.p[k][c] = v[k].clip[c];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) root.b[k][c] = (c == k) ? 1.0f : 0.0f;
-        }
-        nleaf = (uint32_t)clip_walk(root, [](const CTri &, int) {});
-        if (nleaf > 63u) {  // the record key holds 6 bits of sub-triangle index
-            atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
-            nleaf = 0;
-        }
-    }
-    uint32_t base = wave_add(&d.s.counters[C_CLIP_TRI], nleaf, act);
-    uint32_t rbase = wave_add(&d.s.header[H_REC], nleaf, act);
-    if (nleaf == 0u) return;
-    if (base + nleaf > d.s.clip_tri_cap || rbase + nleaf > d.s.rec_clip_cap) {
-        atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
-        // keep the reserved bin records that exist empty
-        for (uint32_t k = 0; k < nleaf && rbase + k < d.s.rec_clip_cap; ++k)
-            d.s.recs[d.s.rec_prims + rbase + k] = make_uint4(0u, 0xFFFFFFFFu, 0u, di);
-        return;
-    }
-    d.s.clip_base[p] = base;
-    clip_walk(root, [&](const CTri &t, int leaf) {
-        uint32_t slot = base + (uint32_t)leaf;
-        uint32_t ri = d.s.rec_prims + rbase + (uint32_t)leaf;
-        uint4 empty = make_uint4(0u, 0xFFFFFFFFu, 0u, di);
-        float *o = d.s.clip_tris + (size_t)slot * kClipFloats;
-        for (int k = 0; k < 3; ++k) {
-            for (int c = 0; c < 4; ++c) o[k * 7 + c] = t.p[k][c];
-            for (int c = 0; c < 3; ++c) o[k * 7 + 4 + c] = t.b[k][c];
-        }
-        RasterTri r;
-        raster_setup(d, t.p, r);
-        ScanTri s = scan_setup(r);
-        uint32_t tx0, ty0, tx1, ty1;
-        if (!coord_ok(r)) {
-            atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE);
-            d.s.recs[ri] = empty;
-            return;
-        }
-        if (r.total == 0.0f || !tile_bbox(d, s, tx0, ty0, tx1, ty1)) {
-            d.s.recs[ri] = empty;
-            return;
-        }
-        uint32_t key = (p << 7) | ((uint32_t)leaf << 1) | 1u;
-        write_record(d.s.clip_tri + (size_t)slot * kRecWords, r, v, key);
-        uint4 rec = make_uint4(key, tx0 | (ty0 << 16), tx1 | (ty1 << 16), di);
-        uint32_t nt = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-        if (nt <= (uint32_t)kMaxBinTiles) {
-            d.s.recs[ri] = rec;
-            add_tile_counts(d, ri, tx0, ty0, tx1, ty1);
-        } else {
-            d.s.recs[ri] = empty;
-            uint32_t bs = atomicAdd(&d.s.counters[C_BIG], 1u);
-            if (bs < d.s.big_cap) {
-                d.s.big[bs] = rec;
-                atomicMax(&d.s.counters[C_BIG_NX0], 65535u - tx0);
-                atomicMax(&d.s.counters[C_BIG_NY0], 65535u - ty0);
-                atomicMax(&d.s.counters[C_BIG_X1], tx1);
-                atomicMax(&d.s.counters[C_BIG_Y1], ty1);
-            } else {
-                atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
-            }
-        }
-    });
-}
-
-// Every lane runs the same number of iterations, so that the wave-wide slot
-// reservations in clip_primitive see the whole wave.
-template <int PROG, bool MULTI>
-__global__ __launch_bounds__(256) void k_clip(DrawParams d0, BatchArgs b, const uint2 *clip_list, uint32_t cap)
-{
-    uint32_t n = min(b.header[H_CLIP], cap);
-    uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t first = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);  // the wave's first thread
-    for (uint32_t w0 = first; w0 < n; w0 += stride) {
-        uint32_t i = w0 + lane_id();
-        bool act = i < n;
-        uint2 e = act ? clip_list[i] : make_uint2(0u, 0u);
-        clip_primitive<PROG>(draw_at<MULTI>(d0, b, e.x), e.x, e.y, act);
-    }
-}
-
-// ------------------------------------------------------------------
-// k_scan_*: bin offsets and the active-tile list
-// ------------------------------------------------------------------
-
-// Whether the raster must visit a tile: binned triangles, or inside the
-// footprint union of the draw's big triangles.
-template <bool MULTI>
-__device__ __forceinline__ bool tile_is_active(const DrawParams &d0, const BatchArgs &b, uint32_t gt, uint32_t count)
-{
-    if (count) return true;
-    uint32_t di = 0, t = gt;
-    if constexpr (MULTI) {
-        di = find_slot(b.tile_base, b.n, gt);
-        t = gt - b.tile_base[di];
-    }
-    const DrawParams &d = draw_at<MULTI>(d0, b, di);
-    if (d.s.counters[C_BIG] == 0) return false;
-    uint32_t tx = t % d.tiles_x, ty = t / d.tiles_x;
-    return tx >= 65535u - d.s.counters[C_BIG_NX0] && tx <= d.s.counters[C_BIG_X1] &&
-           ty >= 65535u - d.s.counters[C_BIG_NY0] && ty <= d.s.counters[C_BIG_Y1];
-}
-
-// Block-wide exclusive scan of two values per thread (256 threads).
-__device__ __forceinline__ void block_scan2(uint32_t &a, uint32_t &c, uint32_t &tot_a, uint32_t &tot_c)
-{
-    __shared__ uint32_t sa[256], sc[256];
-    uint32_t t = threadIdx.x;
-    sa[t] = a;
-    sc[t] = c;
-    __syncthreads();
-    for (uint32_t off = 1; off < 256u; off <<= 1) {
-        uint32_t va = (t >= off) ? sa[t - off] : 0u;
-        uint32_t vc = (t >= off) ? sc[t - off] : 0u;
-        __syncthreads();
-        sa[t] += va;
-        sc[t] += vc;
-        __syncthreads();
-    }
-    tot_a = sa[255];
-    tot_c = sc[255];
-    a = sa[t] - a;
-    c = sc[t] - c;
-    __syncthreads();
-}
-
-// Phase 1: per block of kScanBlock tiles, the bin count and active count.
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_scan_blocks(DrawParams d0, BatchArgs b)
-{
-    uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * (kScanBlock / 256);
-    uint32_t cnt = 0, act = 0;
-    for (uint32_t k = 0; k < kScanBlock / 256; ++k) {
-        uint32_t gt = base + k;
-        if (gt >= b.total_tiles) break;
-        uint32_t c = b.tile_count[gt];
-        cnt += c;
-        act += tile_is_active<MULTI>(d0, b, gt, c) ? 1u : 0u;
-    }
-    uint32_t ta, tc;
-    block_scan2(cnt, act, ta, tc);
-    if (threadIdx.x == 0) {
-        b.block_sums[2 * blockIdx.x] = ta;
-        b.block_sums[2 * blockIdx.x + 1] = tc;
-    }
-}
-
-// Phase 2: one workgroup scans the block sums.
-__global__ __launch_bounds__(256) void k_scan_totals(BatchArgs b)
-{
-    uint32_t run_c = 0, run_a = 0;
-    for (uint32_t base = 0; base < b.n_blocks; base += 256) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t c = i < b.n_blocks ? b.block_sums[2 * i] : 0u;
-        uint32_t a = i < b.n_blocks ? b.block_sums[2 * i + 1] : 0u;
-        uint32_t tc, ta;
-        block_scan2(c, a, tc, ta);
-        if (i < b.n_blocks) {
-            b.block_sums[2 * i] = run_c + c;
-            b.block_sums[2 * i + 1] = run_a + a;
-        }
-        run_c += tc;
-        run_a += ta;
-    }
-    if (threadIdx.x == 0) {
-        b.tile_offset[b.total_tiles] = run_c;
-        b.header[H_ACTIVE] = run_a;
-    }
-}
-
-// Phase 3: per tile, its bin offset, the active flag and list slot.
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_scan_tiles(DrawParams d0, BatchArgs b)
-{
-    constexpr uint32_t PER = kScanBlock / 256;
-    uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * PER;
-    uint32_t c[PER];
-    bool a[PER];
-    uint32_t cnt = 0, act = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) {
-        uint32_t gt = base + k;
-        c[k] = 0u;
-        a[k] = false;
-        if (gt < b.total_tiles) {
-            c[k] = b.tile_count[gt];
-            a[k] = tile_is_active<MULTI>(d0, b, gt, c[k]);
-        }
-        cnt += c[k];
-        act += a[k] ? 1u : 0u;
-    }
-    uint32_t ta, tc;
-    block_scan2(cnt, act, ta, tc);
-    uint32_t off = b.block_sums[2 * blockIdx.x] + cnt;
-    uint32_t aoff = b.block_sums[2 * blockIdx.x + 1] + act;
-#pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) {
-        uint32_t gt = base + k;
-        if (gt < b.total_tiles) {
-            b.tile_offset[gt] = off;
-            b.tile_active[gt] = a[k] ? 1u : 0u;
-            off += c[k];
-            if (a[k]) b.active_list[aoff++] = gt;
-        }
-    }
-}
-
-// ------------------------------------------------------------------
-// k_bin_fill
-// ------------------------------------------------------------------
-
-// Scatters every record's key into the bins of its footprint, at the
-// positions taken when the tiles were counted.
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_bin_fill(DrawParams d0, BatchArgs b, const uint4 *recs,
-                                                  const uint32_t *rec_slots, uint32_t rec_prims,
-                                                  uint32_t rec_clip_cap, uint32_t *bins)
-{
-    uint32_t n = rec_prims + min(b.header[H_REC], rec_clip_cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint4 r = recs[i];
-        uint32_t tx0 = r.y & 0xFFFFu, ty0 = r.y >> 16, tx1 = r.z & 0xFFFFu, ty1 = r.z >> 16;
-        if (tx0 > tx1) continue;
-        const DrawParams &d = draw_at<MULTI>(d0, b, r.w);
-        const uint32_t *slots = rec_slots + (size_t)i * kMaxBinTiles;
-        uint32_t k = 0;
-        for (uint32_t ty = ty0; ty <= ty1; ++ty)
-            for (uint32_t tx = tx0; tx <= tx1; ++tx) {
-                uint32_t t = ty * d.tiles_x + tx;
-                bins[d.s.tile_offset[t] + slots[k++]] = r.x;
-            }
-    }
-}
-
-// ------------------------------------------------------------------
-// k_raster
-// ------------------------------------------------------------------
-
-struct TriLds {
-    float vpx[3], vpy[3];
-    float k0, k1, k2, total;
-    float z[3];
-    float cb[9];
-    float var[3][kMaxVar];
-    float iw[3];
-    uint32_t key;
-    uint32_t pad;
-};
-
-template <int TS>
-struct RasterShared {
-    TriLds tri[kBatch];
-    ScanTri scan[kBatch];
-    uint32_t wide[kBatch];
-    uint32_t span[kBatch][TS];      // ordered path: per-row spans of the staged triangles
-    uint32_t keys[kBatch];
-    uint32_t nkeys;
-};
-
-// Loads one list entry's triangle record into LDS.
-__device__ __forceinline__ void stage_record(const DrawParams &d, uint32_t key, TriLds &T, ScanTri &S, uint32_t &wide)
-{
-    uint32_t p = key >> 7, sub = (key >> 1) & 63u;
-    bool clipped = key & 1u;
-    uint32_t slot = clipped ? d.s.clip_base[p] + sub : 0u;
-    const uint4 *r = reinterpret_cast<const uint4 *>(clipped ? d.s.clip_tri + (size_t)slot * kRecWords
-                                                             : d.s.tri + (size_t)p * kRecWords);
-    uint32_t w[kRecWords];
-#pragma unroll
-    for (int q = 0; q < kRecWords / 4; ++q) {
-        uint4 v = r[q];
-        w[4 * q] = v.x; w[4 * q + 1] = v.y; w[4 * q + 2] = v.z; w[4 * q + 3] = v.w;
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        T.vpx[k] = __uint_as_float(w[R_VP + 2 * k]);
-        T.vpy[k] = __uint_as_float(w[R_VP + 2 * k + 1]);
-        T.z[k] = __uint_as_float(w[R_Z + k]);
-        T.iw[k] = __uint_as_float(w[R_IW + k]);
-#pragma unroll
-        for (int c = 0; c < kMaxVar; ++c) T.var[k][c] = __uint_as_float(w[R_VAR + k * kMaxVar + c]);
-    }
-    T.k0 = __uint_as_float(w[R_K + 0]);
-    T.k1 = __uint_as_float(w[R_K + 1]);
-    T.k2 = __uint_as_float(w[R_K + 2]);
-    T.total = __uint_as_float(w[R_K + 3]);
-    T.key = key;
-    if (clipped) {
-        const float *ct = d.s.clip_tris + (size_t)slot * kClipFloats;
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) T.cb[k * 3 + c] = ct[k * 7 + 4 + c];
-    } else {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) T.cb[k * 3 + c] = (c == k) ? 1.0f : 0.0f;
-    }
-    S.ax = (int32_t)w[R_SCAN + 0]; S.ay = (int32_t)w[R_SCAN + 1];
-    S.bx = (int32_t)w[R_SCAN + 2]; S.by = (int32_t)w[R_SCAN + 3];
-    S.cx = (int32_t)w[R_SCAN + 4]; S.cy = (int32_t)w[R_SCAN + 5];
-    S.mode = (int32_t)(w[R_MODE] & 0xFFu);
-    S.pad = 0;
-    wide = (w[R_MODE] >> 8) & 1u;
-}
-
-// Pins a value to a register: keeps the compiler from turning a select of
-// two fields of the per-pixel state into a load through a selected address,
-// which would move that state to scratch memory.
-__device__ __forceinline__ float in_vgpr(float x)
-{
-    asm volatile("" : "+v"(x));
-    return x;
-}
-
-struct PixelState {
-    uint32_t col;
-    float dep;
-    float init_dep;
-    uint32_t key;
-    bool has;
-};
-
-// Barycentric at the pixel corner (raster.rs:313-336) composed with the
-// clip barycentrics (raster.rs:337-343); returns the interpolated depth
-// (raster.rs:345-346).
-__device__ __forceinline__ float bary_depth(const TriLds &T, float pxf, float pyf, float lc[3])
-{
-    float l0 = (((T.vpy[1] - pyf) * (T.vpx[1] + pxf) + T.k0) + (pyf - T.vpy[2]) * (pxf + T.vpx[2])) / T.total;
-    float l1 = (((pyf - T.vpy[0]) * (pxf + T.vpx[0]) + (T.vpy[2] - pyf) * (T.vpx[2] + pxf)) + T.k1) / T.total;
-    float l2 = ((T.k2 + (pyf - T.vpy[1]) * (pxf + T.vpx[1])) + (T.vpy[0] - pyf) * (T.vpx[0] + pxf)) / T.total;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        float acc = T.cb[0 * 3 + c] * l0;
-        acc = acc + T.cb[1 * 3 + c] * l1;
-        acc = acc + T.cb[2 * 3 + c] * l2;
-        lc[c] = acc;
-    }
-    float z = T.z[0] * lc[0];
-    z = z + T.z[1] * lc[1];
-    z = z + T.z[2] * lc[2];
-    return z;
-}
-
-// Varyings (fragment.rs:319-377) and the fragment shader.
-template <int PROG>
-__device__ __forceinline__ void fragment_color(const DrawParams &d, const TriLds &T, const float lc[3],
-                                               float color[4])
-{
-    float var[kMaxVar];
-    if (d.interp == INTERP_FLAT) {
-#pragma unroll
-        for (int c = 0; c < kMaxVar; ++c) var[c] = T.var[0][c];
-    } else if (d.interp == INTERP_PERSPECTIVE) {
-        float q0 = lc[0] * T.iw[0];
-        float q1 = lc[1] * T.iw[1];
-        float q2 = lc[2] * T.iw[2];
-        float qs = (q0 + q1) + q2;
-#pragma unroll
-        for (int c = 0; c < kMaxVar; ++c) {
-            float acc = T.var[0][c] * q0;
-            acc = acc + T.var[1][c] * q1;
-            acc = acc + T.var[2][c] * q2;
-            var[c] = acc / qs;
-        }
-    } else {
-#pragma unroll
-        for (int c = 0; c < kMaxVar; ++c) {
-            float acc = T.var[0][c] * lc[0];
-            acc = acc + T.var[1][c] * lc[1];
-            acc = acc + T.var[2][c] * lc[2];
-            var[c] = acc;
-        }
-    }
-    run_fs<PROG>(d, var, color);
-}
-
-// One fragment of triangle T at pixel (pxf, pyf).  Ordered: the depth test
-// against the running depth, the fragment shader and the colour write or
-// blend (fragment.rs:427-488, texture.rs:213-218), in API order.  Otherwise
-// only visibility: the order-independent restatement of the depth test picks
-// the winning (depth, key); its colour is shaded once after the tile's last
-// triangle (the fragment shader has no side effects, so only the winner's
-// output can reach the target).
-template <int PROG, bool ORDERED>
-__device__ __forceinline__ void shade_fragment(const DrawParams &d, const TriLds &T, float pxf, float pyf,
-                                               PixelState &ps)
-{
-    float lc[3];
-    float z = bary_depth(T, pxf, pyf, lc);
-    if (ORDERED) {
-        if (d.depth_test && !compare(d.depth_compare, z, ps.dep)) return;
-        float color[4];
-        fragment_color<PROG>(d, T, lc, color);
-        if (d.depth_test && d.depth_write) ps.dep = z;
-        ps.col = d.blend ? blend_over(color, ps.col, d.color_bgra) : pack_color(color, d.color_bgra);
-        ps.has = true;
-        return;
-    }
-    bool win;
-    switch (d.resolve) {
-    case RES_LAST: win = !ps.has || T.key > ps.key; break;
-    case RES_MIN_FIRST:
-        win = (z < ps.init_dep) && (!ps.has || z < ps.dep || (z == ps.dep && T.key < ps.key));
-        break;
-    case RES_MIN_LAST:
-        win = (z <= ps.init_dep) && (!ps.has || z < ps.dep || (z == ps.dep && T.key > ps.key));
-        break;
-    case RES_MAX_FIRST:
-        win = (z > ps.init_dep) && (!ps.has || z > ps.dep || (z == ps.dep && T.key < ps.key));
-        break;
-    case RES_MAX_LAST:
-        win = (z >= ps.init_dep) && (!ps.has || z > ps.dep || (z == ps.dep && T.key > ps.key));
-        break;
-    default:  // RES_PASS_LAST
-        win = compare(d.depth_compare, z, ps.init_dep) && (!ps.has || T.key > ps.key);
-        break;
-    }
-    if (win) {
-        ps.dep = z;
-        ps.key = T.key;
-        ps.has = true;
-    }
-}
-
-// Colour and depth of a pixel's winning fragment: its triangle record is
-// read back (from L2, shared by the neighbouring pixels it won) and the same
-// barycentrics are evaluated again, bit for bit.
-template <int PROG>
-__device__ __forceinline__ uint32_t winner_color(const DrawParams &d, uint32_t key, float pxf, float pyf, float &z)
-{
-    TriLds T;
-    ScanTri S;
-    uint32_t wide;
-    stage_record(d, key, T, S, wide);
-    float lc[3];
-    z = bary_depth(T, pxf, pyf, lc);
-    float color[4];
-    fragment_color<PROG>(d, T, lc, color);
-    return pack_color(color, d.color_bgra);
-}
-
-// Order-independent resolve as a minimum over 64-bit values: the depth in
-// an unsigned encoding that keeps IEEE order (with -0 equal to +0, as the
-// comparison treats them) in the high word, the primitive key (or its
-// complement, to prefer the later fragment) in the low word.  ~0 means no
-// fragment; no candidate encodes to it (NaN depths never pass the test).
-__device__ __forceinline__ uint32_t depth_order(float z)
-{
-    uint32_t u = __float_as_uint(z);
-    if ((u << 1) == 0u) u = 0u;
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-
-__device__ __forceinline__ unsigned long long oi_value(uint32_t resolve, float z, uint32_t key)
-{
-    switch (resolve) {
-    case RES_MIN_FIRST: return ((unsigned long long)depth_order(z) << 32) | key;
-    case RES_MIN_LAST: return ((unsigned long long)depth_order(z) << 32) | (~key);
-    case RES_MAX_FIRST: return ((unsigned long long)(~depth_order(z)) << 32) | key;
-    case RES_MAX_LAST: return ((unsigned long long)(~depth_order(z)) << 32) | (~key);
-    default: return (unsigned long long)(~key);  // last in API order
-    }
-}
-
-__device__ __forceinline__ uint32_t oi_key(uint32_t resolve, unsigned long long v)
-{
-    uint32_t lo = (uint32_t)v;
-    return (resolve == RES_MIN_FIRST || resolve == RES_MAX_FIRST) ? lo : ~lo;
-}
-
-// The depth test of the order-independent modes, against the depth the
-// pixel held before the draw.
-__device__ __forceinline__ bool oi_pass(const DrawParams &d, float z, float init)
-{
-    switch (d.resolve) {
-    case RES_LAST: return true;
-    case RES_MIN_FIRST: return z < init;
-    case RES_MIN_LAST: return z <= init;
-    case RES_MAX_FIRST: return z > init;
-    case RES_MAX_LAST: return z >= init;
-    default: return compare(d.depth_compare, z, init);  // RES_PASS_LAST
-    }
-}
-
-// Stages sh.keys[0..n) into LDS and applies them to this lane's pixels.
-template <int PROG, bool ORDERED, int TS, int NT>
-__device__ __forceinline__ void raster_batch(const DrawParams &d, RasterShared<TS> &sh, uint32_t n, uint32_t x0,
-                                             uint32_t y0, PixelState (&ps)[TS * TS / NT],
-                                             uint32_t (&nfrag)[TS * TS / NT])
-{
-    constexpr int PPT = TS * TS / NT;
-    uint32_t tid = threadIdx.x;
-    if (tid < n) stage_record(d, sh.keys[tid], sh.tri[tid], sh.scan[tid], sh.wide[tid]);
-    __syncthreads();
-    // per-row spans inside the scissor and the tile, tile-relative
-    int64_t sx0 = d.scissor[0], sy0 = d.scissor[1];
-    int64_t sx1 = sx0 + d.scissor[2], sy1 = sy0 + d.scissor[3];
-    for (uint32_t idx = tid; idx < n * (uint32_t)TS; idx += NT) {
-        uint32_t e = idx / TS, row = idx % TS;
-        int32_t y = (int32_t)(y0 + row);
-        uint32_t span = 1u;  // empty: lo = 1, hi = 0
-        int32_t x1, x2;
-        if ((int64_t)y >= sy0 && (int64_t)y < sy1 && scan_span(sh.scan[e], y, x1, x2, sh.wide[e] != 0u)) {
-            int64_t lo = max((int64_t)x1, max(sx0, (int64_t)x0));
-            int64_t hi = min((int64_t)x2, min(sx1 - 1, (int64_t)x0 + TS - 1));
-            if (lo <= hi) span = (uint32_t)(lo - x0) | ((uint32_t)(hi - x0) << 16);
-        }
-        sh.span[e][row] = span;
-    }
-    __syncthreads();
-    // pixels outer (compile-time, so the per-pixel state stays in registers),
-    // triangles inner in list order
-    static_for<0, PPT>([&](auto kc) __attribute__((always_inline)) {
-        constexpr int k = decltype(kc)::value;
-        uint32_t q = tid + (uint32_t)k * NT;
-        uint32_t lx = q % TS, ly = q / TS;
-        float pxf = (float)(x0 + lx), pyf = (float)(y0 + ly);
-        for (uint32_t e = 0; e < n; ++e) {
-            uint32_t sp = sh.span[e][ly];
-            if (lx < (sp & 0xFFFFu) || lx > (sp >> 16)) continue;
-            nfrag[k]++;
-            shade_fragment<PROG, ORDERED>(d, sh.tri[e], pxf, pyf, ps[k]);
-        }
-    });
-    __syncthreads();
-}
-
-// One active tile: load or clear, apply its triangles, store.
-template <int PROG, bool ORDERED, int TS, int NT>
-__device__ __forceinline__ void raster_tile(const DrawParams &d, uint32_t tile, RasterShared<TS> &sh,
-                                            uint32_t *olist, uint32_t &olist_n, uint32_t &frag_acc)
-{
-    constexpr int PPT = TS * TS / NT;
-    uint32_t tx = tile % d.tiles_x, ty = tile / d.tiles_x;
-    uint32_t x0 = tx * TS, y0 = ty * TS;
-    uint32_t tid = threadIdx.x;
-    uint32_t seg0 = d.s.tile_offset[tile];
-    uint32_t seg_n = d.s.tile_offset[tile + 1] - seg0;
-    uint32_t nbig = min(d.s.counters[C_BIG], d.s.big_cap);
-
-    PixelState ps[PPT];
-    uint32_t init_col[PPT];
-    uint32_t nfrag[PPT];
-    static_for<0, PPT>([&](auto kc) __attribute__((always_inline)) {
-        constexpr int k = decltype(kc)::value;
-        nfrag[k] = 0u;
-        uint32_t q = tid + (uint32_t)k * NT;
-        uint32_t px = x0 + q % TS, py = y0 + q / TS;
-        bool inside = px < d.width && py < d.height;
-        size_t pix = (size_t)py * d.width + px;
-        ps[k].has = false;
-        ps[k].key = 0u;
-        ps[k].col = 0u;
-        ps[k].dep = 0.0f;
-        if (inside) {
-            if (d.color)
-                ps[k].col = (d.init_clear && d.clear_color_on) ? d.clear_rgba
-                                                               : reinterpret_cast<const uint32_t *>(d.color)[pix];
-            if (d.depth) ps[k].dep = (d.init_clear && d.clear_depth_on) ? d.clear_depth : d.depth[pix];
-        }
-        ps[k].init_dep = ps[k].dep;
-        init_col[k] = ps[k].col;
-    });
-
-    uint32_t total = 0;  // ordered: length of the sorted list
-    if (ORDERED) {
-        // gather the tile's entries, sort them into API order
-        if (tid == 0) olist_n = 0u;
-        __syncthreads();
-        bool overflow = seg_n > (uint32_t)kOrderedCap;
-        if (!overflow) {
-            for (uint32_t i = tid; i < seg_n; i += NT) olist[i] = d.s.bins[seg0 + i];
-            if (tid == 0) olist_n = seg_n;
-        }
-        __syncthreads();
-        for (uint32_t bb = 0; bb < nbig && !overflow; bb += kBatch) {
-            uint32_t base = olist_n;
-            __syncthreads();  // every thread has read olist_n before thread 0 moves it
-            if (tid < kBatch) {
-                bool hit = false;
-                uint32_t key = 0u;
-                if (bb + tid < nbig) {
-                    uint4 r = d.s.big[bb + tid];
-                    hit = tx >= (r.y & 0xFFFFu) && tx <= (r.z & 0xFFFFu) && ty >= (r.y >> 16) && ty <= (r.z >> 16);
-                    key = r.x;
-                }
-                uint64_t mask = __ballot(hit);
-                uint64_t below = (tid == 0) ? 0ull : (mask & ((~0ull) >> (64 - tid)));
-                uint32_t pos = base + (uint32_t)__popcll(below);
-                if (hit && pos < (uint32_t)kOrderedCap) olist[pos] = key;
-                if (tid == 0) olist_n = base + (uint32_t)__popcll(mask);
-            }
-            __syncthreads();
-            overflow = olist_n > (uint32_t)kOrderedCap;
-        }
-        total = olist_n;
-        if (overflow || total > (uint32_t)kOrderedCap) {
-            if (tid == 0) atomicOr(d.s.error, (uint32_t)ERR_ORDERED_OVERFLOW);
-            total = 0u;
-        }
-        // bitonic sort of olist[0..P), padded with UINT32_MAX
-        uint32_t P = 1u;
-        while (P < total) P <<= 1;
-        for (uint32_t i = total + tid; i < P; i += NT) olist[i] = 0xFFFFFFFFu;
-        __syncthreads();
-        for (uint32_t k = 2; k <= P; k <<= 1) {
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = tid; i < P; i += NT) {
-                    uint32_t ixj = i ^ j;
-                    if (ixj > i) {
-                        uint32_t a = olist[i], c = olist[ixj];
-                        bool up = (i & k) == 0u;
-                        if ((a > c) == up) { olist[i] = c; olist[ixj] = a; }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-    }
-
-    // One loop, one call site: ordered -> the sorted list; otherwise the
-    // tile's bin segment, then the big-triangle list filtered by overlap.
-    uint32_t fpos = 0, bpos = 0;
-    for (;;) {
-        uint32_t n;
-        if (ORDERED) {
-            if (fpos >= total) break;
-            n = min((uint32_t)kBatch, total - fpos);
-            if (tid < n) sh.keys[tid] = olist[fpos + tid];
-            fpos += n;
-        } else if (fpos < seg_n) {
-            n = min((uint32_t)kBatch, seg_n - fpos);
-            if (tid < n) sh.keys[tid] = d.s.bins[seg0 + fpos + tid];
-            fpos += n;
-        } else if (bpos < nbig) {
-            if (tid < kBatch) {  // wave 0 compacts the hits
-                bool hit = false;
-                uint32_t key = 0u;
-                if (bpos + tid < nbig) {
-                    uint4 r = d.s.big[bpos + tid];
-                    hit = tx >= (r.y & 0xFFFFu) && tx <= (r.z & 0xFFFFu) && ty >= (r.y >> 16) && ty <= (r.z >> 16);
-                    key = r.x;
-                }
-                uint64_t mask = __ballot(hit);
-                uint64_t below = (tid == 0) ? 0ull : (mask & ((~0ull) >> (64 - tid)));
-                if (hit) sh.keys[__popcll(below)] = key;
-                if (tid == 0) sh.nkeys = (uint32_t)__popcll(mask);
-            }
-            __syncthreads();
-            n = sh.nkeys;
-            bpos += kBatch;
-            if (n == 0) {
-                __syncthreads();  // sh.nkeys is rewritten next round
-                continue;
-            }
-        } else {
-            break;
-        }
-        raster_batch<PROG, ORDERED, TS, NT>(d, sh, n, x0, y0, ps, nfrag);
-    }
-
-    uint32_t f = 0;
-    static_for<0, PPT>([&](auto kc) __attribute__((always_inline)) {
-        constexpr int k = decltype(kc)::value;
-        f += nfrag[k];
-        uint32_t q = tid + (uint32_t)k * NT;
-        uint32_t px = x0 + q % TS, py = y0 + q / TS;
-        if (px >= d.width || py >= d.height) return;
-        size_t pix = (size_t)py * d.width + px;
-        bool write_depth;
-        if (ORDERED) {
-            write_depth = d.depth_test && d.depth_write;
-        } else {
-            write_depth = ps[k].has && d.depth_test && d.depth_write &&
-                          (d.resolve != RES_PASS_LAST || d.depth_compare == CMP_EQUAL ||
-                           d.depth_compare == CMP_ALWAYS);
-            float zw;
-            ps[k].col = ps[k].has ? winner_color<PROG>(d, ps[k].key, (float)px, (float)py, zw) : init_col[k];
-        }
-        if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = ps[k].col;
-        if (d.depth) d.depth[pix] = write_depth ? in_vgpr(ps[k].dep) : in_vgpr(ps[k].init_dep);
-        if (d.s.frag_px && nfrag[k]) d.s.frag_px[pix] += nfrag[k];
-    });
-    frag_acc += f;
-}
-
-// What the visibility pass needs of a triangle: its record without the
-// varyings (those are read again when the winner is shaded), kept in LDS as
-// structure-of-arrays (lanes that read different triangles hit different
-// banks).
-enum VisField {
-    V_VPX = 0, V_VPY = 3, V_K = 6, V_Z = 10, V_CB = 13, V_NUM = 22,
-};
-enum VisScan { S_AX = 0, S_AY, S_BX, S_BY, S_CX, S_CY, S_MODE, S_NUM };
-
-// Triangles staged per visibility batch.
-template <int NT>
-struct VisBatch {
-    static constexpr int value = NT >= 128 ? 128 : NT;
-};
-
-template <int TS, int NT>
-struct VisShared {
-    static constexpr int NB = VisBatch<NT>::value;
-    float f[V_NUM][NB];
-    int32_t sc[S_NUM][NB];
-    uint32_t tkey[NB];
-    uint32_t rows[NB];          // first covered tile row | count << 8 | wide << 16
-    uint16_t owner[NB * TS];    // per covered row: triangle | row << 8
-    uint32_t keys[NB];
-    unsigned long long best[TS * TS];  // encoded winner (see oi_value), ~0 = none
-    float idep[TS * TS];               // depth before the draw
-    uint32_t cnt[TS * TS];             // emitted fragments (debug counts only)
-    uint32_t wsum[NT / 64];
-    uint32_t nkeys;
-};
-
-// Spreads consecutive items over the waves of the workgroup (item i goes to
-// wave i mod (NT/64)), so that a short batch keeps every SIMD busy.
-template <int NT>
-__device__ __forceinline__ uint32_t spread(uint32_t tid)
-{
-    constexpr uint32_t W = NT / 64;
-    return (tid & 63u) * W + (tid >> 6);
-}
-
-template <int TS, int NT>
-__device__ __forceinline__ void stage_vis(const DrawParams &d, uint32_t key, VisShared<TS, NT> &sh, uint32_t e,
-                                          ScanTri &S, uint32_t &wide)
-{
-    uint32_t p = key >> 7, sub = (key >> 1) & 63u;
-    bool clipped = key & 1u;
-    uint32_t slot = clipped ? d.s.clip_base[p] + sub : 0u;
-    const uint4 *r = reinterpret_cast<const uint4 *>(clipped ? d.s.clip_tri + (size_t)slot * kRecWords
-                                                             : d.s.tri + (size_t)p * kRecWords);
-    static_assert(R_VAR == 24, "the visibility fields are the record's first 24 words");
-    uint32_t w[24];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        uint4 v = r[q];
-        w[4 * q] = v.x; w[4 * q + 1] = v.y; w[4 * q + 2] = v.z; w[4 * q + 3] = v.w;
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        sh.f[V_VPX + k][e] = __uint_as_float(w[R_VP + 2 * k]);
-        sh.f[V_VPY + k][e] = __uint_as_float(w[R_VP + 2 * k + 1]);
-        sh.f[V_Z + k][e] = __uint_as_float(w[R_Z + k]);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) sh.f[V_K + k][e] = __uint_as_float(w[R_K + k]);
-    sh.tkey[e] = key;
-    if (clipped) {
-        const float *ct = d.s.clip_tris + (size_t)slot * kClipFloats;
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) sh.f[V_CB + k * 3 + c][e] = ct[k * 7 + 4 + c];
-    } else {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) sh.f[V_CB + k * 3 + c][e] = (c == k) ? 1.0f : 0.0f;
-    }
-    S.ax = (int32_t)w[R_SCAN + 0]; S.ay = (int32_t)w[R_SCAN + 1];
-    S.bx = (int32_t)w[R_SCAN + 2]; S.by = (int32_t)w[R_SCAN + 3];
-    S.cx = (int32_t)w[R_SCAN + 4]; S.cy = (int32_t)w[R_SCAN + 5];
-    S.mode = (int32_t)(w[R_MODE] & 0xFFu);
-    S.pad = 0;
-    sh.sc[S_AX][e] = S.ax; sh.sc[S_AY][e] = S.ay;
-    sh.sc[S_BX][e] = S.bx; sh.sc[S_BY][e] = S.by;
-    sh.sc[S_CX][e] = S.cx; sh.sc[S_CY][e] = S.cy;
-    sh.sc[S_MODE][e] = S.mode;
-    wide = (w[R_MODE] >> 8) & 1u;
-}
-
-// The same arithmetic as bary_depth, on a staged triangle.
-template <int TS, int NT>
-__device__ __forceinline__ float vis_depth(const VisShared<TS, NT> &sh, uint32_t e, float pxf, float pyf)
-{
-    float vx0 = sh.f[V_VPX][e], vx1 = sh.f[V_VPX + 1][e], vx2 = sh.f[V_VPX + 2][e];
-    float vy0 = sh.f[V_VPY][e], vy1 = sh.f[V_VPY + 1][e], vy2 = sh.f[V_VPY + 2][e];
-    float k0 = sh.f[V_K][e], k1 = sh.f[V_K + 1][e], k2 = sh.f[V_K + 2][e], total = sh.f[V_K + 3][e];
-    float l0 = (((vy1 - pyf) * (vx1 + pxf) + k0) + (pyf - vy2) * (pxf + vx2)) / total;
-    float l1 = (((pyf - vy0) * (pxf + vx0) + (vy2 - pyf) * (vx2 + pxf)) + k1) / total;
-    float l2 = ((k2 + (pyf - vy1) * (pxf + vx1)) + (vy0 - pyf) * (vx0 + pxf)) / total;
-    float lc[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        float acc = sh.f[V_CB + 0 * 3 + c][e] * l0;
-        acc = acc + sh.f[V_CB + 1 * 3 + c][e] * l1;
-        acc = acc + sh.f[V_CB + 2 * 3 + c][e] * l2;
-        lc[c] = acc;
-    }
-    float z = sh.f[V_Z][e] * lc[0];
-    z = z + sh.f[V_Z + 1][e] * lc[1];
-    z = z + sh.f[V_Z + 2][e] * lc[2];
-    return z;
-}
-
-// Stages sh.keys[0..n) and merges their fragments into the tile's winners.
-// Each staged triangle's covered rows (the reference's scanline rows, within
-// the tile and the scissor) are compacted into one list, so the lanes walk
-// only rows that hold fragments: one lane per (triangle, row) computes the
-// row's integer span and visits its pixels.
-template <int TS, int NT>
-__device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS, NT> &sh, uint32_t n, uint32_t x0,
-                                              uint32_t y0)
-{
-    uint32_t tid = threadIdx.x;
-    uint32_t e = spread<NT>(tid);  // the triangle this thread stages
-    int64_t sx0 = d.scissor[0], sy0 = d.scissor[1];
-    int64_t sx1 = sx0 + d.scissor[2], sy1 = sy0 + d.scissor[3];
-    uint32_t cnt_rows = 0, first = 0;
-    if (e < n) {
-        uint32_t wide;
-        ScanTri S;
-        stage_vis<TS, NT>(d, sh.keys[e], sh, e, S, wide);
-        int32_t r0, r1;
-        scan_rows(S, r0, r1);
-        int64_t lo = max((int64_t)r0, max((int64_t)y0, sy0));
-        int64_t hi = min((int64_t)r1, min((int64_t)y0 + TS - 1, sy1 - 1));
-        if (lo <= hi) {
-            cnt_rows = (uint32_t)(hi - lo + 1);
-            first = (uint32_t)(lo - y0);
-        }
-        sh.rows[e] = first | (cnt_rows << 8) | (wide << 16);
-    }
-    // exclusive scan of the row counts in thread order: within each wave,
-    // then across waves
-    uint32_t lane = lane_id(), wave = tid >> 6;
-    uint32_t v = cnt_rows;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t t = (uint32_t)__shfl_up((int)v, off);
-        if (lane >= (uint32_t)off) v += t;
-    }
-    if (lane == 63u) sh.wsum[wave] = v;
-    __syncthreads();
-    uint32_t before = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < NT / 64; ++w) {
-        uint32_t x = sh.wsum[w];
-        if ((uint32_t)w < wave) before += x;
-        total += x;
-    }
-    if (e < n) {
-        uint32_t b = before + v - cnt_rows;
-        for (uint32_t j = 0; j < cnt_rows; ++j) sh.owner[b + j] = (uint16_t)(e | ((first + j) << 8));
-    }
-    __syncthreads();
-
-    bool counts = d.s.frag_px != nullptr;
-    uint32_t f = 0;
-    for (uint32_t idx = spread<NT>(tid); idx < total; idx += NT) {
-        uint32_t o = sh.owner[idx];
-        uint32_t te = o & 0xFFu, row = o >> 8;
-        int32_t y = (int32_t)(y0 + row);
-        ScanTri S;
-        S.ax = sh.sc[S_AX][te]; S.ay = sh.sc[S_AY][te];
-        S.bx = sh.sc[S_BX][te]; S.by = sh.sc[S_BY][te];
-        S.cx = sh.sc[S_CX][te]; S.cy = sh.sc[S_CY][te];
-        S.mode = sh.sc[S_MODE][te];
-        S.pad = 0;
-        int32_t x1, x2;
-        if (!scan_span(S, y, x1, x2, ((sh.rows[te] >> 16) & 1u) != 0u)) continue;
-        int64_t lo = max((int64_t)x1, max(sx0, (int64_t)x0));
-        int64_t hi = min((int64_t)x2, min(sx1 - 1, (int64_t)x0 + TS - 1));
-        if (lo > hi) continue;
-        uint32_t key = sh.tkey[te];
-        uint32_t q0 = row * TS + (uint32_t)(lo - x0);
-        uint32_t len = (uint32_t)(hi - lo) + 1u;
-        f += len;
-        if (counts)
-            for (uint32_t i = 0; i < len; ++i) atomicAdd(&sh.cnt[q0 + i], 1u);
-        if (d.resolve == RES_LAST) {
-            // no depth test: the last fragment in API order wins, depth unused
-            unsigned long long val = (unsigned long long)(~key);
-            for (uint32_t i = 0; i < len; ++i) atomicMin(&sh.best[q0 + i], val);
-            continue;
-        }
-        float pyf = (float)y;
-        for (uint32_t i = 0; i < len; ++i) {
-            uint32_t q = q0 + i;
-            float z = vis_depth<TS, NT>(sh, te, (float)(lo + i), pyf);
-            if (oi_pass(d, z, sh.idep[q])) atomicMin(&sh.best[q], oi_value(d.resolve, z, key));
-        }
-    }
-    __syncthreads();
-    return f;
-}
-
-// One active tile of an order-independent draw, visibility only: the depth
-// the pixels hold before the draw (cleared or loaded), the winners of the
-// tile's bin segment and of the overlapping big triangles, stored per pixel
-// (encoded as in oi_value) into the visibility buffer at the tile's item slot.
-template <int TS, int NT>
-__device__ __forceinline__ void raster_tile_vis(const DrawParams &d, uint32_t tile, VisShared<TS, NT> &sh,
-                                                unsigned long long *vis, uint32_t &frag_acc)
-{
-    constexpr int PPT = TS * TS / NT;
-    constexpr uint32_t NB = VisShared<TS, NT>::NB;
-    uint32_t tx = tile % d.tiles_x, ty = tile / d.tiles_x;
-    uint32_t x0 = tx * TS, y0 = ty * TS;
-    uint32_t tid = threadIdx.x;
-    uint32_t seg0 = d.s.tile_offset[tile];
-    uint32_t seg_n = d.s.tile_offset[tile + 1] - seg0;
-    uint32_t nbig = min(d.s.counters[C_BIG], d.s.big_cap);
-
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        uint32_t q = tid + (uint32_t)k * NT;
-        uint32_t px = x0 + q % TS, py = y0 + q / TS;
-        float z = 0.0f;
-        if (d.depth && px < d.width && py < d.height)
-            z = (d.init_clear && d.clear_depth_on) ? d.clear_depth : d.depth[(size_t)py * d.width + px];
-        sh.idep[q] = z;
-        sh.best[q] = ~0ull;
-        sh.cnt[q] = 0u;
-    }
-    __syncthreads();
-
-    uint32_t f = 0, fpos = 0, bpos = 0;
-    for (;;) {
-        uint32_t n;
-        if (fpos < seg_n) {
-            n = min(NB, seg_n - fpos);
-            if (tid < n) sh.keys[tid] = d.s.bins[seg0 + fpos + tid];
-            __syncthreads();  // staging reads the keys through another thread order
-            fpos += n;
-        } else if (bpos < nbig) {
-            if (tid < kBatch) {  // wave 0 compacts the hits
-                bool hit = false;
-                uint32_t key = 0u;
-                if (bpos + tid < nbig) {
-                    uint4 r = d.s.big[bpos + tid];
-                    hit = tx >= (r.y & 0xFFFFu) && tx <= (r.z & 0xFFFFu) && ty >= (r.y >> 16) && ty <= (r.z >> 16);
-                    key = r.x;
-                }
-                uint64_t mask = __ballot(hit);
-                uint64_t below = (tid == 0) ? 0ull : (mask & ((~0ull) >> (64 - tid)));
-                if (hit) sh.keys[__popcll(below)] = key;
-                if (tid == 0) sh.nkeys = (uint32_t)__popcll(mask);
-            }
-            __syncthreads();
-            n = sh.nkeys;
-            bpos += kBatch;
-            if (n == 0) {
-                __syncthreads();  // sh.nkeys is rewritten next round
-                continue;
-            }
-        } else {
-            break;
-        }
-        f += vis_batch<TS, NT>(d, sh, n, x0, y0);
-    }
-
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        uint32_t q = tid + (uint32_t)k * NT;
-        vis[q] = sh.best[q];
-        if (d.s.frag_px && sh.cnt[q]) {
-            uint32_t px = x0 + q % TS, py = y0 + q / TS;
-            if (px < d.width && py < d.height) d.s.frag_px[(size_t)py * d.width + px] += sh.cnt[q];
-        }
-    }
-    frag_acc += f;
-}
-
-// One active tile of an order-independent draw, shading: each pixel's
-// winner (from the visibility buffer) is shaded once; colour and depth are
-// stored where the pass changes them (a winner, or the LoadOp::Clear values).
-template <int PROG, int TS, int NT>
-__device__ __forceinline__ void shade_tile(const DrawParams &d, uint32_t tile, const unsigned long long *vis)
-{
-    constexpr int PPT = TS * TS / NT;
-    uint32_t x0 = (tile % d.tiles_x) * TS, y0 = (tile / d.tiles_x) * TS;
-    bool write_depth_mode = d.depth_test && d.depth_write &&
-                            (d.resolve != RES_PASS_LAST || d.depth_compare == CMP_EQUAL ||
-                             d.depth_compare == CMP_ALWAYS);
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        uint32_t q = threadIdx.x + (uint32_t)k * NT;
-        uint32_t px = x0 + q % TS, py = y0 + q / TS;
-        if (px >= d.width || py >= d.height) continue;
-        size_t pix = (size_t)py * d.width + px;
-        unsigned long long v = vis[q];
-        if (v != ~0ull) {
-            float z;
-            uint32_t col = winner_color<PROG>(d, oi_key(d.resolve, v), (float)px, (float)py, z);
-            if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = col;
-            if (d.depth) {
-                if (write_depth_mode) d.depth[pix] = z;
-                else if (d.init_clear && d.clear_depth_on) d.depth[pix] = d.clear_depth;
-            }
-        } else if (d.init_clear) {
-            if (d.color && d.clear_color_on) reinterpret_cast<uint32_t *>(d.color)[pix] = d.clear_rgba;
-            if (d.depth && d.clear_depth_on) d.depth[pix] = d.clear_depth;
-        }
-    }
-}
-
-// Stores the clear values into the untouched tiles of a chunk.
-template <bool MULTI, int TS, int NT>
-__device__ __forceinline__ void clear_chunk(const DrawParams &d0, const BatchArgs &b, uint32_t chunk)
-{
-    constexpr int PPT = TS * TS / NT;
-    uint32_t g0 = chunk * kClearChunk, g1 = min(g0 + kClearChunk, b.total_tiles);
-    for (uint32_t gt = g0; gt < g1; ++gt) {
-        if (b.tile_active[gt]) continue;
-        uint32_t di = 0, tile = gt;
-        if constexpr (MULTI) {
-            di = __builtin_amdgcn_readfirstlane(find_slot(b.tile_base, b.n, gt));
-            tile = gt - b.tile_base[di];
-        }
-        const DrawParams &d = draw_at<MULTI>(d0, b, di);
-        if (!d.init_clear) continue;
-        uint32_t x0 = (tile % d.tiles_x) * TS, y0 = (tile / d.tiles_x) * TS;
-#pragma unroll
-        for (int k = 0; k < PPT; ++k) {
-            uint32_t q = threadIdx.x + (uint32_t)k * NT;
-            uint32_t px = x0 + q % TS, py = y0 + q / TS;
-            if (px >= d.width || py >= d.height) continue;
-            size_t pix = (size_t)py * d.width + px;
-            if (d.color && d.clear_color_on) reinterpret_cast<uint32_t *>(d.color)[pix] = d.clear_rgba;
-            if (d.depth && d.clear_depth_on) d.depth[pix] = d.clear_depth;
-        }
-    }
-}
-
-// Splits the active tiles into 8 contiguous ranges, one per XCD (workgroup i
-// runs on XCD i mod 8 and the grids are multiples of 8), so that neighbouring
-// tiles share an L2; yields the items of this workgroup.
-struct XcdSchedule {
-    uint32_t per_xcd, xcd, lw, nw, n;
-    __device__ XcdSchedule(uint32_t n_active)
-    {
-        n = n_active;
-        per_xcd = (n_active + 7u) / 8u;
-        xcd = blockIdx.x & 7u;
-        lw = blockIdx.x >> 3;
-        nw = gridDim.x >> 3;
-    }
-};
-
-template <bool MULTI>
-__device__ __forceinline__ void locate_tile(const BatchArgs &b, uint32_t item, uint32_t &di, uint32_t &tile)
-{
-    uint32_t gt = __builtin_amdgcn_readfirstlane(b.active_list[item]);
-    di = 0;
-    tile = gt;
-    if constexpr (MULTI) {
-        di = __builtin_amdgcn_readfirstlane(find_slot(b.tile_base, b.n, gt));
-        tile = gt - b.tile_base[di];
-    }
-}
-
-// Reduces the workgroup's fragment count into one of kFragSlots counters.
-template <bool MULTI, int NT>
-__device__ __forceinline__ void count_fragments(const DrawParams &d0, const BatchArgs &b, uint32_t frag_acc,
-                                                uint32_t *wave_frags)
-{
-    for (int off = 32; off > 0; off >>= 1) frag_acc += __shfl_xor(frag_acc, off);
-    if (lane_id() == 0) wave_frags[threadIdx.x >> 6] = frag_acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t f = 0;
-        for (int w = 0; w < NT / 64; ++w) f += wave_frags[w];
-        if (f) atomicAdd(draw_at<MULTI>(d0, b, 0).s.frags + (blockIdx.x % kFragSlots), (unsigned long long)f);
-    }
-}
-
-// Ordered draws (blending, NotEqual + write): persistent and statically
-// scheduled (no shared work counter); each tile's fragments are applied in
-// API order.  Then the clear chunks, strided over the whole grid.
-template <int PROG, bool MULTI, int TS, int NT>
-__global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
-{
-    constexpr int PPT = TS * TS / NT;
-    static_assert(NT >= kBatch && PPT >= 1 && TS * TS == PPT * NT, "tile shape");
-    __shared__ RasterShared<TS> sh;
-    __shared__ uint32_t olist[kOrderedCap];
-    __shared__ uint32_t olist_n;
-    __shared__ uint32_t wave_frags[NT / 64];
-
-    XcdSchedule w(b.header[H_ACTIVE]);
-    uint32_t frag_acc = 0;
-    for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
-        uint32_t item = w.xcd * w.per_xcd + i;
-        if (item >= w.n) break;
-        uint32_t di, tile;
-        locate_tile<MULTI>(b, item, di, tile);
-        raster_tile<PROG, true, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, olist, olist_n, frag_acc);
-    }
-    for (uint32_t c = blockIdx.x; c < b.n_chunks; c += gridDim.x) clear_chunk<MULTI, TS, NT>(d0, b, c);
-    count_fragments<MULTI, NT>(d0, b, frag_acc, wave_frags);
-}
-
-// Order-independent draws, pass 1: visibility of every active tile into the
-// visibility buffer (TS*TS values per active-tile slot), then the clear
-// chunks.  Independent of the shader program.
-template <bool MULTI, int TS, int NT>
-__global__ __launch_bounds__(NT) void k_visibility(DrawParams d0, BatchArgs b, unsigned long long *vis)
-{
-    constexpr int PPT = TS * TS / NT;
-    static_assert(NT >= kBatch && PPT >= 1 && TS * TS == PPT * NT, "tile shape");
-    static_assert(VisShared<TS, NT>::NB <= 256 && TS <= 32, "owner entries hold 8-bit triangle and row indices");
-    __shared__ VisShared<TS, NT> sh;
-    __shared__ uint32_t wave_frags[NT / 64];
-
-    XcdSchedule w(b.header[H_ACTIVE]);
-    uint32_t frag_acc = 0;
-    for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
-        uint32_t item = w.xcd * w.per_xcd + i;
-        if (item >= w.n) break;
-        uint32_t di, tile;
-        locate_tile<MULTI>(b, item, di, tile);
-        raster_tile_vis<TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, vis + (size_t)item * (TS * TS), frag_acc);
-    }
-    for (uint32_t c = blockIdx.x; c < b.n_chunks; c += gridDim.x) clear_chunk<MULTI, TS, NT>(d0, b, c);
-    count_fragments<MULTI, NT>(d0, b, frag_acc, wave_frags);
-}
-
-// Order-independent draws, pass 2: shade each active tile's winners, with
-// the same XCD partition as pass 1 (its visibility values are in that L2).
-template <int PROG, bool MULTI, int TS, int NT>
-__global__ __launch_bounds__(NT) void k_shade(DrawParams d0, BatchArgs b, const unsigned long long *vis)
-{
-    XcdSchedule w(b.header[H_ACTIVE]);
-    for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
-        uint32_t item = w.xcd * w.per_xcd + i;
-        if (item >= w.n) break;
-        uint32_t di, tile;
-        locate_tile<MULTI>(b, item, di, tile);
-        shade_tile<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, vis + (size_t)item * (TS * TS));
-    }
-}
-
-// ------------------------------------------------------------------
-// k_clear / k_reset
-// ------------------------------------------------------------------
-
-__global__ void k_reset(unsigned long long *frags, uint32_t *error)
-{
-    if (threadIdx.x < kFragSlots) frags[threadIdx.x] = 0ull;
-    if (threadIdx.x == 0) *error = 0u;
-}
-
-__global__ void k_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, uint64_t n)
-{
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (color) color[i] = rgba;
-    if (depth) depth[i] = dv;
-}
 
 // ------------------------------------------------------------------
 // launchers
@@ -1704,6 +156,67 @@ hipError_t launch_batch(const DrawParams &d0, const BatchArgs &b, const GroupLau
                         KernelTimer *kt)
 {
     return launch_group_prog<true>(d0, b, gl, st, kt);
+}
+
+// ---- WGSL pipelines: the shader kernels are module functions ----
+
+#define WGC_MLAUNCH(name, fn, grid, block, ...)                                                   \
+    do {                                                                                          \
+        void *args_[] = {__VA_ARGS__};                                                            \
+        if (kt) kt->begin(name, st);                                                              \
+        hipError_t e_ = hipModuleLaunchKernel(fn, (grid), 1, 1, (block), 1, 1, 0, st, args_, nullptr); \
+        if (kt) kt->end(st);                                                                      \
+        if (e_ != hipSuccess) return e_;                                                          \
+    } while (0)
+
+template <bool MULTI>
+static hipError_t launch_user_t(const UserKernels &k, const DrawParams &d0, const BatchArgs &b0,
+                                const GroupLaunch &gl, hipStream_t st, KernelTimer *kt)
+{
+    DrawParams d = d0;
+    BatchArgs b = b0;
+    const uint2 *clip_list = gl.clip_list;
+    uint32_t clip_cap = gl.clip_cap;
+    unsigned long long *vis = gl.vis;
+    const int m = MULTI ? 1 : 0;
+    uint32_t zero_n = max(max(gl.total_tiles, gl.n_counters), (uint32_t)H_NUM);
+    WGC_LAUNCH("k_zero", k_zero, (zero_n + 255) / 256, 256, gl.tile_count, gl.total_tiles, gl.counters,
+               gl.n_counters, gl.header);
+    if (gl.total_prims > 0) {
+        uint32_t g = (gl.total_prims + 255u) / 256u;
+        WGC_MLAUNCH("k_setup", k.setup[m], g, 256, &d, &b);
+        uint32_t gc = min(max(g / 4u, 1u), 1024u);
+        WGC_MLAUNCH("k_clip", k.clip[m], gc, 256, &d, &b, &clip_list, &clip_cap);
+    }
+    WGC_LAUNCH("k_scan_blocks", k_scan_blocks<MULTI>, b.n_blocks, 256, d, b);
+    WGC_LAUNCH("k_scan_totals", k_scan_totals, 1, 256, b);
+    WGC_LAUNCH("k_scan_tiles", k_scan_tiles<MULTI>, b.n_blocks, 256, d, b);
+    if (gl.total_prims > 0) {
+        uint32_t g = min((gl.rec_prims + gl.rec_clip_cap + 255u) / 256u, 4096u);
+        WGC_LAUNCH("k_bin_fill", k_bin_fill<MULTI>, g, 256, d, b, gl.recs, gl.rec_slots, gl.rec_prims,
+                   gl.rec_clip_cap, gl.bins);
+    }
+    if (gl.ordered) {
+        WGC_MLAUNCH("k_raster", k.raster[m], gl.raster_grid, 256, &d, &b);
+        return hipSuccess;
+    }
+    WGC_LAUNCH("k_visibility", (k_visibility<MULTI, 16, 256>), gl.raster_grid, 256, d, b, gl.vis);
+    WGC_MLAUNCH("k_shade", k.shade[m], gl.shade_grid, 256, &d, &b, &vis);
+    return hipSuccess;
+}
+
+hipError_t launch_user(const UserKernels &k, const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl,
+                       bool multi, hipStream_t st, KernelTimer *kt)
+{
+    return multi ? launch_user_t<true>(k, d, b, gl, st, kt) : launch_user_t<false>(k, d, b, gl, st, kt);
+}
+
+uint32_t visibility_blocks_per_cu(bool multi)
+{
+    static uint32_t r[2] = {0, 0};
+    uint32_t &v = r[multi ? 1 : 0];
+    if (!v) v = multi ? occupancy_of(k_visibility<true, 16, 256>, 256) : occupancy_of(k_visibility<false, 16, 256>, 256);
+    return v;
 }
 
 hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t st)

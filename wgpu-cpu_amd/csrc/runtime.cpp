@@ -19,11 +19,14 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/wgc.h"
+#include "user_program.h"
 #include "wgc_core.h"
 #include "wgc_launch.h"
+#include "wgsl.h"
 
 using namespace wgc;
 
@@ -270,12 +273,16 @@ struct WgcBindGroup_T : Object {
 
 struct WgcShaderModule_T : Object {
     uint32_t program = 0, interpolation = 0;
+    std::shared_ptr<wgsl::Module> wgsl;               // PROG_USER
+    std::map<std::string, std::shared_ptr<EntryRun>> runs;
+    std::mutex mu;
 };
 
 struct WgcRenderPipeline_T : Object {
     WgcRenderPipelineDescriptor desc{};
     uint32_t program = 0, interp = 0, nvar = 0;
     bool has_fragment = false;
+    std::shared_ptr<UserProgram> user;                // PROG_USER
     std::vector<uint32_t> vb_stride, vb_step;
     AttrDev attr[kMaxAttr]{};
     uint32_t n_attr = 0;
@@ -715,6 +722,9 @@ static int32_t check_device_error(WgcDevice_T *d)
     if (err & ERR_ORDERED_OVERFLOW) msg += " more than 4096 triangles on one tile in ordered (blend) mode;";
     if (err & ERR_COORD_RANGE) msg += " framebuffer coordinate beyond 2^30;";
     if (err & ERR_PRIM_RANGE) msg += " more than 2^25 primitives in one draw;";
+    if (err & ERR_SHADER)
+        msg += " a shader invocation aborted: integer division by zero, overflow or an index out of bounds "
+               "(reference: panic);";
     return set_error(WGC_ERR_EXECUTION, msg);
 }
 
@@ -1132,6 +1142,128 @@ extern "C" int32_t wgc_shader_module_create_builtin(WgcDevice device, uint32_t p
 
 extern "C" void wgc_shader_module_release(WgcShaderModule module) { release(module); }
 
+static bool is_unsupported_message(const std::string &err)
+{
+    return err.find("todo!()") != std::string::npos || err.find("not supported") != std::string::npos ||
+           err.find("not implemented") != std::string::npos;
+}
+
+extern "C" int32_t wgc_shader_module_create_wgsl(WgcDevice device, const char *source, size_t length,
+                                                 WgcShaderModule *out)
+{
+    if (!device || !source || !out) return set_error(WGC_ERR_INVALID, "null argument");
+    auto m = std::make_shared<wgsl::Module>();
+    std::string err;
+    if (!wgsl::translate(std::string(source, length), *m, err))
+        return set_error(is_unsupported_message(err) ? WGC_ERR_UNSUPPORTED : WGC_ERR_INVALID, err);
+    auto *sm = new WgcShaderModule_T();
+    sm->program = PROG_USER;
+    sm->wgsl = m;
+    *out = sm;
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_shader_module_run(WgcDevice device, WgcShaderModule module, const char *entry_point,
+                                         uint32_t first_index, uint32_t count, const WgcBindGroup *groups,
+                                         uint32_t group_count, uint32_t *results, size_t results_words)
+{
+    if (!device || !module || (!results && count)) return set_error(WGC_ERR_INVALID, "null argument");
+    if (!module->wgsl) return set_error(WGC_ERR_INVALID, "not a WGSL module");
+    if (results_words < (size_t)count * kRunWords) return set_error(WGC_ERR_INVALID, "results buffer too small");
+    if (group_count > 4 || (group_count && !groups)) return set_error(WGC_ERR_INVALID, "bad bind groups");
+    std::string name = entry_point ? entry_point : "";
+    std::shared_ptr<EntryRun> run;
+    {
+        std::lock_guard<std::mutex> lk(module->mu);
+        auto it = module->runs.find(name);
+        if (it != module->runs.end()) {
+            run = it->second;
+        } else {
+            run = std::make_shared<EntryRun>();
+            std::string err;
+            if (!build_entry_run(device->hip_device, *module->wgsl, name, *run, err))
+                return set_error(err.compare(0, 3, "hip") == 0 ? WGC_ERR_DEVICE : WGC_ERR_INVALID, err);
+            module->runs[name] = run;
+        }
+    }
+    DrawParams d;
+    memset(&d, 0, sizeof d);
+    bool extended = device->semantics == WGC_SEMANTICS_EXTENDED;
+    d.srgb = extended ? 1u : 0u;
+    d.srgb_lut = device->d_srgb;
+    if (run->slots.size() > (size_t)kMaxRes) return set_error(WGC_ERR_UNSUPPORTED, "too many resource bindings");
+    for (size_t k = 0; k < run->slots.size(); ++k) {
+        const wgsl::Resource &r = run->slots[k];
+        const BindingSlot *b = r.group < group_count && groups[r.group] ? groups[r.group]->find(r.binding) : nullptr;
+        if (!b) continue;   // unbound: only an error if the entry point reads it (reference: panic there)
+        ResBind &rb = d.res[k];
+        if (b->kind == WGC_BINDING_BUFFER) {
+            if (b->size < r.min_size) return set_error(WGC_ERR_INVALID, "bound buffer smaller than " + r.name);
+            rb.ptr = b->buffer->ptr + b->offset;
+            rb.size = b->size;
+        } else if (b->kind == WGC_BINDING_TEXTURE) {
+            rb.ptr = b->texture->ptr;
+            rb.w = b->texture->width;
+            rb.h = b->texture->height;
+            rb.srgb = b->texture->format == WGC_TEXTURE_FORMAT_RGBA8_UNORM_SRGB;
+        } else {
+            const WgcSamplerDescriptor &sd = b->sampler->desc;
+            rb.addr_u = sd.address_mode_u;
+            rb.addr_v = sd.address_mode_v;
+            rb.bilinear = (extended && (sd.mag_filter == WGC_FILTER_LINEAR || sd.min_filter == WGC_FILTER_LINEAR)) ? 1u : 0u;
+        }
+    }
+    if (!count) return WGC_OK;
+    std::lock_guard<std::mutex> lk(device->mu);
+    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    int32_t rc = drain(device);
+    if (rc != WGC_OK) return rc;
+    uint32_t *d_out = nullptr;
+    size_t bytes = (size_t)count * kRunWords * 4u;
+    HIP_TRY(hipMalloc(&d_out, bytes), "run results");
+    void *args[] = {&d, &first_index, &count, &d_out};
+    hipError_t e = hipModuleLaunchKernel(run->fn, (count + 63u) / 64u, 1, 1, 64, 1, 1, 0, device->stream, args, nullptr);
+    if (e == hipSuccess) e = stream_copy(device->stream, results, d_out, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return hip_fail(e, "entry-point run");
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_wgsl_check_pipeline(const char *source, size_t length, const char *vertex_entry,
+                                           const char *fragment_entry, uint32_t semantics)
+{
+    if (!source) return set_error(WGC_ERR_INVALID, "null argument");
+    wgsl::Module m;
+    std::string err;
+    if (!wgsl::translate(std::string(source, length), m, err))
+        return set_error(is_unsupported_message(err) ? WGC_ERR_UNSUPPORTED : WGC_ERR_INVALID, err);
+    UserProgram up;
+    bool has_fs = false;
+    for (auto &e : m.entries) has_fs = has_fs || e.stage == wgsl::Stage::Fragment;
+    if (!build_user_program(-1, m, vertex_entry ? vertex_entry : "", has_fs ? &m : nullptr,
+                            fragment_entry ? fragment_entry : "", semantics == WGC_SEMANTICS_EXTENDED, up, err))
+        return set_error(err.compare(0, 5, "hipRT") == 0 ? WGC_ERR_DEVICE : WGC_ERR_INVALID, err);
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_wgsl_translate(const char *source, size_t length, uint32_t what, char *out, size_t capacity,
+                                      size_t *needed)
+{
+    if (!source || what > 1) return set_error(WGC_ERR_INVALID, "bad argument");
+    wgsl::Module m;
+    std::string err;
+    if (!wgsl::translate(std::string(source, length), m, err))
+        return set_error(is_unsupported_message(err) ? WGC_ERR_UNSUPPORTED : WGC_ERR_INVALID, err);
+    std::string text = what == 0 ? m.code : reflection_json(m);
+    if (needed) *needed = text.size() + 1;
+    if (out && capacity) {
+        size_t n = std::min(capacity - 1, text.size());
+        memcpy(out, text.data(), n);
+        out[n] = '\0';
+    }
+    return WGC_OK;
+}
+
 extern "C" int32_t wgc_render_pipeline_create(WgcDevice device, const WgcRenderPipelineDescriptor *desc,
                                               WgcRenderPipeline *out)
 {
@@ -1139,7 +1271,10 @@ extern "C" int32_t wgc_render_pipeline_create(WgcDevice device, const WgcRenderP
     if (desc->topology != WGC_TOPOLOGY_TRIANGLE_LIST)
         return set_error(WGC_ERR_UNSUPPORTED, "only TriangleList is implemented");
     if (desc->vertex_buffer_count > (uint32_t)kMaxVb) return set_error(WGC_ERR_UNSUPPORTED, "too many vertex buffers");
-    if (desc->fragment_module && desc->fragment_module->program != desc->vertex_module->program)
+    bool user = desc->vertex_module->program == PROG_USER;
+    if (desc->fragment_module && (desc->fragment_module->program == PROG_USER) != user)
+        return set_error(WGC_ERR_INVALID, "a WGSL stage cannot be paired with a built-in program");
+    if (!user && desc->fragment_module && desc->fragment_module->program != desc->vertex_module->program)
         return set_error(WGC_ERR_INVALID, "vertex and fragment entry points come from different programs");
     auto *p = new WgcRenderPipeline_T();
     p->desc = *desc;
@@ -1167,6 +1302,41 @@ extern "C" int32_t wgc_render_pipeline_create(WgcDevice device, const WgcRenderP
             p->attr[at.shader_location] = AttrDev{b, at.offset, at.format, 0u};
             p->n_attr = std::max(p->n_attr, at.shader_location + 1u);
         }
+    }
+    if (user) {
+        auto up = std::make_shared<UserProgram>();
+        std::string err;
+        const WgcShaderModule_T *fm = desc->fragment_module;
+        if (!build_user_program(device->hip_device, *desc->vertex_module->wgsl,
+                                desc->vertex_entry ? desc->vertex_entry : "", fm ? fm->wgsl.get() : nullptr,
+                                desc->fragment_entry ? desc->fragment_entry : "",
+                                device->semantics == WGC_SEMANTICS_EXTENDED, *up, err)) {
+            release(p);
+            return set_error(err.compare(0, 5, "hipRT") == 0 || err.compare(0, 5, "hipMo") == 0 ? WGC_ERR_DEVICE
+                                                                                             : WGC_ERR_INVALID,
+                             err);
+        }
+        // vertex.rs:139-152 copies attribute_size bytes into the input: the
+        // attribute must have exactly the input's size
+        for (const wgsl::IoVar &in : up->vertex_inputs) {
+            if (in.location >= p->n_attr || !seen[in.location] || p->attr[in.location].ncomp != in.ncomp) {
+                release(p);
+                return set_error(WGC_ERR_INVALID, "vertex input @location(" + std::to_string(in.location) +
+                                                      ") has no vertex attribute of its size");
+            }
+        }
+        p->nvar = up->nvar;
+        p->user = up;
+        if (desc->has_depth_stencil && desc->depth_format != WGC_TEXTURE_FORMAT_DEPTH32_FLOAT) {
+            release(p);
+            return set_error(WGC_ERR_UNSUPPORTED, "only Depth32Float depth is implemented");
+        }
+        if (desc->has_depth_stencil && (desc->depth_compare < 1 || desc->depth_compare > 8)) {
+            release(p);
+            return set_error(WGC_ERR_INVALID, "bad depth compare");
+        }
+        *out = p;
+        return WGC_OK;
     }
     // the built-in programs' vertex inputs and varyings
     auto need = [&](uint32_t loc, uint32_t lo, uint32_t hi) {
@@ -1483,6 +1653,7 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
     d.depth_write = pd.depth_write_enabled;
     d.blend = (extended && pd.blend == WGC_BLEND_ALPHA && pc.color) ? 1u : 0u;
     d.resolve = resolve_mode(pd, d.depth_test != 0, d.blend != 0);
+    if (pl->user && pl->user->discards) d.resolve = RES_ORDERED;   // every fragment must run its shader
     // vertex buffers (vertex.rs:261-272)
     uint32_t n_vb = (uint32_t)pl->vb_stride.size();
     for (uint32_t b = 0; b < n_vb; ++b)
@@ -1539,6 +1710,50 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
         d.srgb = extended ? 1u : 0u;
         d.srgb_lut = dev->d_srgb;
     }
+    if (pl->user) {
+        // resource slots of the WGSL program (binding.rs:16-164)
+        const UserProgram &up = *pl->user;
+        d.user = &up.kernels;
+        d.var_flat_mask = up.flat_mask;
+        d.var_persp_mask = up.persp_mask;
+        d.srgb = extended ? 1u : 0u;
+        d.srgb_lut = dev->d_srgb;
+        for (size_t k = 0; k < up.slots.size(); ++k) {
+            const UserProgram::Slot &sl = up.slots[k];
+            std::string where = "@group(" + std::to_string(sl.group) + ") @binding(" + std::to_string(sl.binding) + ")";
+            const BindingSlot *b = sl.group < 4 && st.groups[sl.group] ? st.groups[sl.group]->find(sl.binding) : nullptr;
+            ResBind &r = d.res[k];
+            switch (sl.kind) {
+            case wgsl::ResKind::Uniform:
+            case wgsl::ResKind::Storage:
+                if (!b || b->kind != WGC_BINDING_BUFFER) return set_error(WGC_ERR_INVALID, where + ": buffer not bound");
+                if (b->size < sl.min_size)
+                    return set_error(WGC_ERR_INVALID, where + ": the bound buffer range is smaller than the variable");
+                r.ptr = b->buffer->ptr + b->offset;
+                r.size = b->size;
+                break;
+            case wgsl::ResKind::Texture: {
+                if (!b || b->kind != WGC_BINDING_TEXTURE) return set_error(WGC_ERR_INVALID, where + ": texture not bound");
+                const WgcTexture_T *tex = b->texture;
+                if (tex->format != WGC_TEXTURE_FORMAT_RGBA8_UNORM && tex->format != WGC_TEXTURE_FORMAT_RGBA8_UNORM_SRGB)
+                    return set_error(WGC_ERR_UNSUPPORTED, "texture format (texture.rs:186 todo!())");
+                r.ptr = tex->ptr;
+                r.w = tex->width;
+                r.h = tex->height;
+                r.srgb = tex->format == WGC_TEXTURE_FORMAT_RGBA8_UNORM_SRGB;
+                break;
+            }
+            case wgsl::ResKind::Sampler: {
+                if (!b || b->kind != WGC_BINDING_SAMPLER) return set_error(WGC_ERR_INVALID, where + ": sampler not bound");
+                const WgcSamplerDescriptor &sd = b->sampler->desc;
+                r.addr_u = sd.address_mode_u;
+                r.addr_v = sd.address_mode_v;
+                r.bilinear = (extended && (sd.mag_filter == WGC_FILTER_LINEAR || sd.min_filter == WGC_FILTER_LINEAR)) ? 1u : 0u;
+                break;
+            }
+            }
+        }
+    }
     memcpy(d.viewport, st.viewport, sizeof d.viewport);
     memcpy(d.scissor, st.scissor, sizeof d.scissor);
     if ((uint64_t)d.scissor[0] + d.scissor[2] > w || (uint64_t)d.scissor[1] + d.scissor[3] > h)
@@ -1546,7 +1761,8 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
     d.width = w;
     d.height = h;
     // the ordered (sorted, sequential) raster exists for 16x16 tiles only
-    d.tile_shift = d.resolve == RES_ORDERED ? 4u : tile_shift;
+    // the ordered raster and WGSL programs use 16x16 tiles
+    d.tile_shift = (d.resolve == RES_ORDERED || pl->user) ? 4u : tile_shift;
     tile_shift = d.tile_shift;
     d.tiles_x = (w + (1u << tile_shift) - 1) >> tile_shift;
     d.tiles_y = (h + (1u << tile_shift) - 1) >> tile_shift;
@@ -1651,7 +1867,13 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
         b.total_prims = st.draws[0].total_prims;
     }
     uint32_t occ_raster = 1, occ_shade = 1;
-    raster_blocks_per_cu(gl.program, gl.ordered != 0, st.multi, gl.raster_cfg, occ_raster, occ_shade);
+    if (gl.user) {
+        int m = st.multi ? 1 : 0;
+        occ_raster = gl.ordered ? gl.user->occ_raster[m] : gl.user->occ_vis[m];
+        occ_shade = gl.ordered ? occ_raster : gl.user->occ_shade[m];
+    } else {
+        raster_blocks_per_cu(gl.program, gl.ordered != 0, st.multi, gl.raster_cfg, occ_raster, occ_shade);
+    }
     gl.raster_grid = resident_grid(dev, occ_raster, total_tiles + b.n_chunks);
     gl.shade_grid = resident_grid(dev, occ_shade, total_tiles);
     gl.vis = gl.ordered ? nullptr : (unsigned long long *)(base + o_vis);
@@ -1766,7 +1988,8 @@ static void add_group(std::vector<Step> &steps, std::vector<DrawParams> draws, u
     s.multi = draws.size() > 1;
     s.gl.program = draws[0].program;
     s.gl.ordered = draws[0].resolve == RES_ORDERED;
-    s.gl.raster_cfg = cfg;
+    s.gl.user = (const UserKernels *)draws[0].user;
+    s.gl.raster_cfg = s.gl.user ? (uint32_t)RASTER_16x16_256 : cfg;
     uint64_t prims = 0;
     for (auto &d : draws) prims += s.multi ? align_up(d.total_prims, 64) : d.total_prims;
     s.gl.total_prims = (uint32_t)prims;
@@ -1830,10 +2053,10 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
             add_group(plan.steps, {pw.draws[0].d}, cfg, stage++);
         } else {
             // group by (program, ordered); ordered groups use the 16x16 raster
-            std::map<std::pair<uint32_t, bool>, std::vector<DrawParams>> groups;
+            std::map<std::tuple<uint32_t, bool, const void *>, std::vector<DrawParams>> groups;
             for (size_t k = i; k < j; ++k) {
                 const DrawParams &d = passes[k].draws[0].d;
-                groups[{d.program, d.resolve == RES_ORDERED}].push_back(d);
+                groups[{d.program, d.resolve == RES_ORDERED, d.user}].push_back(d);
             }
             // the groups of one run of independent passes share a stage
             for (auto &kv : groups) add_group(plan.steps, std::move(kv.second), cfg, stage);
@@ -1929,6 +2152,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
 static hipError_t launch_step(WgcDevice_T *dev, const Step &s, hipStream_t st, KernelTimer *kt)
 {
     if (s.kind == StepKind::Clear) return launch_clear(s.color, s.rgba, s.depth, s.dv, s.n, st, kt);
+    if (s.gl.user) return launch_user(*s.gl.user, s.draws[0], s.batch, s.gl, s.multi, st, kt);
     if (s.multi) return launch_batch(s.draws[0], s.batch, s.gl, st, kt);
     return launch_draw(s.draws[0], s.batch, s.gl, st, kt);
 }

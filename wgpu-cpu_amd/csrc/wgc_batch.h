@@ -1,0 +1,46 @@
+// wgc_batch.h -- launch-group arguments shared by the kernels and the
+// host-side launchers (device-compilable: no host library headers).
+#pragma once
+#ifndef __HIPCC_RTC__
+#include <hip/hip_runtime.h>
+#endif
+#include "wgc_core.h"
+
+namespace wgc {
+
+struct DrawParams;
+
+// Group-level arguments of every kernel: the table of independent draws
+// that share the launches (batched form; empty for a single draw passed by
+// value) and the group's tile bookkeeping.
+struct BatchArgs {
+    const DrawParams *draws;     // device array of n draws
+    const uint32_t *prim_base;   // n + 1 prefix of primitives, padded to multiples of 64
+    const uint32_t *tile_base;   // n + 1 prefix of tiles
+    uint32_t n, total_prims, total_tiles, n_chunks;
+    uint32_t *header;            // H_* words
+    uint32_t *tile_count, *tile_offset;
+    uint32_t *tile_active;       // 1 when the raster must visit the tile
+    uint32_t *active_list;       // compacted active tiles
+    uint32_t *block_sums;        // 2 words per scan block
+    uint32_t n_blocks, pad;
+};
+
+constexpr uint32_t kScanBlock = 1024;   // tiles per scan block
+constexpr uint32_t kClearChunk = 8;     // tiles per clear work item
+
+// Raster tile shapes: tile side x threads per workgroup.
+enum RasterCfg : uint32_t {
+    RASTER_16x16_256 = 0,
+    RASTER_8x8_64 = 1,
+    RASTER_16x16_64 = 2,
+    RASTER_32x32_256 = 3,
+    RASTER_NUM_CFG = 4,
+};
+
+__host__ __device__ inline uint32_t raster_tile_shift(uint32_t cfg)
+{
+    return cfg == RASTER_8x8_64 ? 3u : (cfg == RASTER_32x32_256 ? 5u : 4u);
+}
+
+}  // namespace wgc

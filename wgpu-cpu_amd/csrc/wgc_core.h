@@ -8,8 +8,21 @@
 // f32 division.
 #pragma once
 
+#ifdef __HIPCC_RTC__
+// hipRTC provides the HIP device API itself, and the fixed-width integer
+// types in its own namespace.
+using __hip_internal::int8_t;
+using __hip_internal::int16_t;
+using __hip_internal::int32_t;
+using __hip_internal::int64_t;
+using __hip_internal::uint8_t;
+using __hip_internal::uint16_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
+#else
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 namespace wgc {
 
@@ -27,6 +40,7 @@ enum Program : uint32_t {
     PROG_MVP_TEXTURE = 2,
     PROG_COLORED_TRIANGLE = 3,
     PROG_CLIP_COLOR = 4,
+    PROG_USER = 5,     // shaders translated from WGSL, compiled with the kernels at pipeline creation
 };
 
 enum Interp : uint32_t { INTERP_LINEAR = 0, INTERP_PERSPECTIVE = 1, INTERP_FLAT = 2 };
@@ -59,6 +73,8 @@ enum ErrorBits : uint32_t {
     ERR_ORDERED_OVERFLOW = 1u << 5,
     ERR_COORD_RANGE = 1u << 6,    // framebuffer coordinate >= 2^30
     ERR_PRIM_RANGE = 1u << 7,     // more than 2^25 primitives in a draw
+    ERR_SHADER = 1u << 8,         // a shader invocation aborted (not discard): division by zero,
+                                  // overflow, index out of bounds (reference: panic)
 };
 
 // Pipeline-level attribute description (shader location -> buffer slot).
@@ -131,6 +147,16 @@ enum RecWord {
     R_VAR = 24,    // 3 x kMaxVar varyings of the unclipped vertices
 };
 
+// One resource slot of a WGSL program: a buffer (uniform / storage), a
+// texture or a sampler, by the slot the translator assigned.
+struct ResBind {
+    const uint8_t *ptr;      // buffer data or texels
+    uint64_t size;           // buffer bytes
+    uint32_t w, h, srgb;     // texture size; texels need sRGB decoding
+    uint32_t addr_u, addr_v, bilinear;   // sampler
+};
+constexpr int kMaxRes = 8;
+
 struct DrawParams {
     // pipeline
     uint32_t program, nvar, interp, blend, bilinear, srgb;
@@ -162,8 +188,26 @@ struct DrawParams {
     uint32_t clear_color_on, clear_depth_on;
     uint32_t clear_rgba;     // packed in memory byte order
     float clear_depth;
+    // WGSL programs: resource slots, and per varying word whether it is
+    // flat (first vertex) or perspective-corrected (else linear)
+    ResBind res[kMaxRes];
+    uint32_t var_flat_mask, var_persp_mask;
+    const void *user;        // host-side program identity (never read on the device)
     Scratch s;
 };
+
+struct VsOut;
+struct TriLds;
+#ifdef WGC_USER_PROGRAM
+// Defined by the pipeline's generated glue, compiled with the kernels:
+// returns 0 or the ErrorBits of a failed invocation.
+__device__ __forceinline__ uint32_t user_vertex(const DrawParams &d, uint32_t instance, uint32_t vertex, VsOut &o);
+// Fragment of staged triangle T at composed barycentrics lc with
+// interpolated varyings `var`.  Returns 0 (colour written to `out`), wgsl
+// ABORT_KILL (discard) or another abort code.
+__device__ __forceinline__ uint32_t user_fragment(const DrawParams &d, const TriLds &T, const float lc[3],
+                                                  const float *var, float out[4]);
+#endif
 
 // ------------------------------------------------------------------
 // Scalar helpers with Rust semantics
@@ -341,6 +385,16 @@ __device__ __forceinline__ bool shade_primitive(const DrawParams &d, uint32_t p,
             atomicOr(d.s.error, (uint32_t)ERR_INDEX);
             return false;
         }
+#ifdef WGC_USER_PROGRAM
+        if constexpr (PROG == PROG_USER) {
+            uint32_t e = user_vertex(d, inst, vertex, v[k]);
+            if (e) {
+                atomicOr(d.s.error, e);
+                return false;
+            }
+            continue;
+        }
+#endif
         if (!run_vs<PROG>(d, inst, vertex, v[k])) {
             atomicOr(d.s.error, (uint32_t)ERR_VERTEX);
             return false;
@@ -688,10 +742,22 @@ __constant__ constexpr U8Table kU8ToF = make_u8_table();
 
 __device__ __forceinline__ float u8f(uint32_t b) { return kU8ToF.v[b & 0xFFu]; }
 
-__device__ __forceinline__ void fetch_texel(const DrawParams &d, uint32_t u, uint32_t v, float out[4])
+// A bound texture with its sampler's addressing and filter.
+struct TexView {
+    const uint8_t *tex;
+    uint32_t w, h, srgb, addr_u, addr_v, bilinear;
+};
+
+__device__ __forceinline__ TexView builtin_texture(const DrawParams &d)
 {
-    uint32_t t = *reinterpret_cast<const uint32_t *>(d.tex + ((uint64_t)v * d.tex_w + u) * 4u);
-    if (d.srgb && d.tex_srgb) {
+    return TexView{d.tex, d.tex_w, d.tex_h, d.tex_srgb, d.addr_u, d.addr_v, d.bilinear};
+}
+
+__device__ __forceinline__ void fetch_texel(const DrawParams &d, const TexView &tv, uint32_t u, uint32_t v,
+                                            float out[4])
+{
+    uint32_t t = *reinterpret_cast<const uint32_t *>(tv.tex + ((uint64_t)v * tv.w + u) * 4u);
+    if (d.srgb && tv.srgb) {
         out[0] = d.srgb_lut[t & 0xFFu];
         out[1] = d.srgb_lut[(t >> 8) & 0xFFu];
         out[2] = d.srgb_lut[(t >> 16) & 0xFFu];
@@ -718,23 +784,24 @@ __device__ __forceinline__ int32_t wrap_index(int32_t i, uint32_t mode, uint32_t
 }
 
 // Extension: bilinear, WebGPU texel-centre convention.
-__device__ __forceinline__ void sample_bilinear(const DrawParams &d, float u, float v, float out[4])
+__device__ __forceinline__ void sample_bilinear(const DrawParams &d, const TexView &tv, float u, float v,
+                                                float out[4])
 {
-    float fu = u * (float)d.tex_w - 0.5f;
-    float fv = v * (float)d.tex_h - 0.5f;
+    float fu = u * (float)tv.w - 0.5f;
+    float fv = v * (float)tv.h - 0.5f;
     float u0f = floorf(fu), v0f = floorf(fv);
     float au = fu - u0f, av = fv - v0f;
     int32_t iu = (int32_t)fmaxf(fminf(u0f, 1073741824.0f), -1073741824.0f);
     int32_t iv = (int32_t)fmaxf(fminf(v0f, 1073741824.0f), -1073741824.0f);
-    uint32_t x0 = (uint32_t)wrap_index(iu, d.addr_u, d.tex_w);
-    uint32_t x1 = (uint32_t)wrap_index(iu + 1, d.addr_u, d.tex_w);
-    uint32_t y0 = (uint32_t)wrap_index(iv, d.addr_v, d.tex_h);
-    uint32_t y1 = (uint32_t)wrap_index(iv + 1, d.addr_v, d.tex_h);
+    uint32_t x0 = (uint32_t)wrap_index(iu, tv.addr_u, tv.w);
+    uint32_t x1 = (uint32_t)wrap_index(iu + 1, tv.addr_u, tv.w);
+    uint32_t y0 = (uint32_t)wrap_index(iv, tv.addr_v, tv.h);
+    uint32_t y1 = (uint32_t)wrap_index(iv + 1, tv.addr_v, tv.h);
     float t00[4], t10[4], t01[4], t11[4];
-    fetch_texel(d, x0, y0, t00);
-    fetch_texel(d, x1, y0, t10);
-    fetch_texel(d, x0, y1, t01);
-    fetch_texel(d, x1, y1, t11);
+    fetch_texel(d, tv, x0, y0, t00);
+    fetch_texel(d, tv, x1, y0, t10);
+    fetch_texel(d, tv, x0, y1, t01);
+    fetch_texel(d, tv, x1, y1, t11);
     float su = 1.0f - au, sv = 1.0f - av;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -744,17 +811,24 @@ __device__ __forceinline__ void sample_bilinear(const DrawParams &d, float u, fl
     }
 }
 
+// textureSample (binding.rs:16-164; bilinear is the extension).
+__device__ __forceinline__ void sample_texture(const DrawParams &d, const TexView &tv, float u, float v,
+                                               float out[4])
+{
+    if (tv.bilinear) {
+        sample_bilinear(d, tv, u, v, out);
+    } else {
+        uint32_t x = texel_coordinate(u, tv.addr_u, tv.w);
+        uint32_t y = texel_coordinate(v, tv.addr_v, tv.h);
+        fetch_texel(d, tv, x, y, out);
+    }
+}
+
 template <int PROG>
 __device__ __forceinline__ void run_fs(const DrawParams &d, const float *var, float out[4])
 {
     if constexpr (PROG == PROG_MVP_TEXTURE) {
-        if (d.bilinear) {
-            sample_bilinear(d, var[0], var[1], out);
-        } else {
-            uint32_t x = texel_coordinate(var[0], d.addr_u, d.tex_w);
-            uint32_t y = texel_coordinate(var[1], d.addr_v, d.tex_h);
-            fetch_texel(d, x, y, out);
-        }
+        sample_texture(d, builtin_texture(d), var[0], var[1], out);
     } else {
         out[0] = var[0];
         out[1] = var[1];

@@ -7,46 +7,16 @@
 #include <string>
 #include <vector>
 
+#include "wgc_batch.h"
+
 namespace wgc {
 
-struct DrawParams;
-
-// Group-level arguments of every kernel: the table of independent draws
-// that share the launches (batched form; empty for a single draw passed by
-// value) and the group's tile bookkeeping.
-struct BatchArgs {
-    const DrawParams *draws;     // device array of n draws
-    const uint32_t *prim_base;   // n + 1 prefix of primitives, padded to multiples of 64
-    const uint32_t *tile_base;   // n + 1 prefix of tiles
-    uint32_t n, total_prims, total_tiles, n_chunks;
-    uint32_t *header;            // H_* words
-    uint32_t *tile_count, *tile_offset;
-    uint32_t *tile_active;       // 1 when the raster must visit the tile
-    uint32_t *active_list;       // compacted active tiles
-    uint32_t *block_sums;        // 2 words per scan block
-    uint32_t n_blocks, pad;
-};
-
-constexpr uint32_t kScanBlock = 1024;   // tiles per scan block
-constexpr uint32_t kClearChunk = 8;     // tiles per clear work item
-
-// Raster tile shapes: tile side x threads per workgroup.
-enum RasterCfg : uint32_t {
-    RASTER_16x16_256 = 0,
-    RASTER_8x8_64 = 1,
-    RASTER_16x16_64 = 2,
-    RASTER_32x32_256 = 3,
-    RASTER_NUM_CFG = 4,
-};
-
-inline uint32_t raster_tile_shift(uint32_t cfg)
-{
-    return cfg == RASTER_8x8_64 ? 3u : (cfg == RASTER_32x32_256 ? 5u : 4u);
-}
+struct UserKernels;
 
 // Group-wide scratch and launch shape.
 struct GroupLaunch {
     uint32_t program, ordered, raster_cfg;
+    const UserKernels *user;     // PROG_USER: the pipeline's kernels
     uint32_t total_prims, total_tiles, n_counters, raster_grid;
     uint32_t *tile_count, *bins, *counters, *header;
     uint4 *recs;
@@ -91,6 +61,22 @@ struct KernelTimer {
         (void)hipEventRecord(pending.back().end, st);
     }
 };
+
+// Kernels of a pipeline whose shaders come from WGSL: the instantiations
+// for PROG_USER of the kernels that run shader code, compiled with hipRTC
+// (index: 0 single draw, 1 batched), and their resident workgroups per CU.
+struct UserKernels {
+    hipModule_t module = nullptr;
+    hipFunction_t setup[2]{}, clip[2]{}, shade[2]{}, raster[2]{};
+    uint32_t occ_shade[2]{}, occ_raster[2]{}, occ_vis[2]{};
+};
+
+// Launch group of a WGSL pipeline (16x16 tiles): the shader kernels from
+// `k`, the rest from the library.
+hipError_t launch_user(const UserKernels &k, const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl,
+                       bool multi, hipStream_t st, KernelTimer *kt);
+// Resident workgroups per CU of the library's visibility kernel (16x16).
+uint32_t visibility_blocks_per_cu(bool multi);
 
 // A single draw passes its parameters by value (b.draws unused, b.n == 1);
 // a batch passes the device table in b.draws (d0 unused).

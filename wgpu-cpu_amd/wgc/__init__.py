@@ -32,6 +32,14 @@ ERR_DEVICE = -4
 ERR_TIMEOUT = -5
 ERR_EXECUTION = -6
 
+# WGSL entry-point runs (wgc_shader_module_run)
+RUN_WORDS = 69
+ABORT_KILL = 3
+ABORT_OUT_OF_BOUNDS = 4
+ABORT_DIVISION_BY_ZERO = 5
+ABORT_OVERFLOW = 6
+PROGRAM_WGSL = 5
+
 RGBA8_UNORM = 0
 RGBA8_UNORM_SRGB = 1
 BGRA8_UNORM = 2
@@ -123,7 +131,7 @@ class RenderPipelineDescriptor(C.Structure):
                 ("has_depth_stencil", C.c_uint32), ("depth_format", C.c_uint32),
                 ("depth_write_enabled", C.c_uint32), ("depth_compare", C.c_uint32),
                 ("fragment_module", C.c_void_p), ("target_format", C.c_uint32),
-                ("blend", C.c_uint32)]
+                ("blend", C.c_uint32), ("vertex_entry", C.c_char_p), ("fragment_entry", C.c_char_p)]
 
 
 class ColorAttachment(C.Structure):
@@ -172,6 +180,12 @@ SIGNATURES = {
     "wgc_bind_group_release": (None, [_P]),
     "wgc_shader_module_create_builtin": (C.c_int32, [_P, C.c_uint32, C.c_uint32, _PP]),
     "wgc_shader_module_release": (None, [_P]),
+    "wgc_shader_module_create_wgsl": (C.c_int32, [_P, C.c_char_p, C.c_size_t, _PP]),
+    "wgc_shader_module_run": (C.c_int32, [_P, _P, C.c_char_p, C.c_uint32, C.c_uint32, _PP, C.c_uint32,
+                                          C.POINTER(C.c_uint32), C.c_size_t]),
+    "wgc_wgsl_translate": (C.c_int32, [C.c_char_p, C.c_size_t, C.c_uint32, C.c_char_p, C.c_size_t,
+                                       C.POINTER(C.c_size_t)]),
+    "wgc_wgsl_check_pipeline": (C.c_int32, [C.c_char_p, C.c_size_t, C.c_char_p, C.c_char_p, C.c_uint32]),
     "wgc_render_pipeline_create": (C.c_int32, [_P, C.POINTER(RenderPipelineDescriptor), _PP]),
     "wgc_render_pipeline_release": (None, [_P]),
     "wgc_command_encoder_create": (C.c_int32, [_P, _PP]),
@@ -204,6 +218,28 @@ SIGNATURES = {
 }
 
 _lib: Optional[C.CDLL] = None
+
+
+def wgsl_translate(source: str, what: int = 0) -> str:
+    """Device-less translation: what=0 the emitted C++, what=1 the interface
+    as JSON (raises WgcError for invalid or unsupported WGSL)."""
+    lib = load_library()
+    data = source.encode()
+    need = C.c_size_t()
+    _check(lib.wgc_wgsl_translate(data, len(data), what, None, 0, C.byref(need)))
+    buf = C.create_string_buffer(need.value)
+    _check(lib.wgc_wgsl_translate(data, len(data), what, buf, need.value, C.byref(need)))
+    return buf.value.decode()
+
+
+def wgsl_check_pipeline(source: str, vertex_entry: Optional[str] = None, fragment_entry: Optional[str] = None,
+                        semantics: int = SEMANTICS_REFERENCE) -> None:
+    """Compiles a pipeline's kernels from the module for gfx950 without a
+    device (raises WgcError on failure)."""
+    lib = load_library()
+    data = source.encode()
+    _check(lib.wgc_wgsl_check_pipeline(data, len(data), vertex_entry.encode() if vertex_entry else None,
+                                       fragment_entry.encode() if fragment_entry else None, semantics))
 
 
 def library_path() -> Path:
@@ -513,11 +549,35 @@ class Device:
         _check(self._lib.wgc_shader_module_create_builtin(self._ptr, program, interpolation, C.byref(out)))
         return ShaderModule(out)
 
+    def create_shader_module_wgsl(self, source: str) -> ShaderModule:
+        """A module from WGSL source (raises WgcError on invalid or unsupported WGSL)."""
+        data = source.encode()
+        out = _out()
+        _check(self._lib.wgc_shader_module_create_wgsl(self._ptr, data, len(data), C.byref(out)))
+        return ShaderModule(out)
+
+    def run_entry_point(self, module: ShaderModule, entry: Optional[str] = None, first_index: int = 0,
+                        count: int = 1, groups: Sequence[Optional["BindGroup"]] = ()):
+        """Runs a WGSL entry point for `count` invocations on the device; returns
+        (count, RUN_WORDS) uint32: abort code, position / frag depth, 16 x 4
+        location words."""
+        import numpy as np
+        res = np.zeros((count, RUN_WORDS), np.uint32)
+        arr = (C.c_void_p * 4)(*[(g.ptr.value if g is not None else None) for g in groups] + [None] * (4 - len(groups)))
+        _check(self._lib.wgc_shader_module_run(self._ptr, module.ptr, entry.encode() if entry else None, first_index,
+                                               count, arr, len(groups),
+                                               res.ctypes.data_as(C.POINTER(C.c_uint32)), res.size))
+        return res
+
     def create_render_pipeline(self, module: ShaderModule, vertex_buffers: Sequence[tuple] = (),
                                front_face=FRONT_CCW, cull_mode=CULL_NONE, depth: Optional[tuple] = None,
-                               target_format=RGBA8_UNORM, blend=BLEND_REPLACE, fragment=True) -> RenderPipeline:
+                               target_format=RGBA8_UNORM, blend=BLEND_REPLACE, fragment=True,
+                               vertex_entry: Optional[str] = None, fragment_entry: Optional[str] = None,
+                               fragment_module: Optional[ShaderModule] = None) -> RenderPipeline:
         """vertex_buffers: (array_stride, step_mode, [(format, offset, location), ...]);
-        depth: (compare, write_enabled) or None."""
+        depth: (compare, write_enabled) or None.  WGSL modules: entry point
+        names (None: the module's only one of that stage); `fragment_module`
+        defaults to `module`."""
         layouts = (VertexBufferLayout * max(1, len(vertex_buffers)))()
         keep = []
         for i, (stride, step, attrs) in enumerate(vertex_buffers):
@@ -541,9 +601,11 @@ class Device:
             desc.depth_format = DEPTH32_FLOAT
             desc.depth_compare = depth[0]
             desc.depth_write_enabled = 1 if depth[1] else 0
-        desc.fragment_module = module.ptr if fragment else None
+        desc.fragment_module = (fragment_module or module).ptr if fragment else None
         desc.target_format = target_format
         desc.blend = blend
+        desc.vertex_entry = vertex_entry.encode() if vertex_entry else None
+        desc.fragment_entry = fragment_entry.encode() if fragment_entry else None
         out = _out()
         _check(self._lib.wgc_render_pipeline_create(self._ptr, C.byref(desc), C.byref(out)))
         return RenderPipeline(out)

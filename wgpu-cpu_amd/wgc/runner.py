@@ -59,10 +59,14 @@ def record_pass(device: Device, enc, spec: PassSpec, res: _PassResources, keep: 
         return cache[key]
 
     for d in spec.draws:
-        module = cached(("module", d.program, d.interp), lambda: device.create_shader_module_builtin(d.program, d.interp))
+        if d.wgsl is not None:
+            module = cached(("wgsl", d.wgsl), lambda: device.create_shader_module_wgsl(d.wgsl))
+        else:
+            module = cached(("module", d.program, d.interp),
+                            lambda: device.create_shader_module_builtin(d.program, d.interp))
         layouts = [(d.vertex_stride, STEP_VERTEX, d.attributes)] if d.vertex_data else []
-        pkey = ("pipeline", d.program, d.interp, d.vertex_stride, tuple(d.attributes), d.front_face, d.cull_mode,
-                d.depth, spec.color_format, d.blend)
+        pkey = ("pipeline", d.program, d.interp, d.wgsl, d.vertex_stride, tuple(d.attributes), d.front_face,
+                d.cull_mode, d.depth, spec.color_format, d.blend)
         pipeline = cached(pkey, lambda: device.create_render_pipeline(
             module, layouts, front_face=d.front_face, cull_mode=d.cull_mode, depth=d.depth,
             target_format=spec.color_format, blend=d.blend))

@@ -23,6 +23,43 @@ from . import (ALWAYS, BLEND_ALPHA, BLEND_REPLACE, CLAMP_TO_EDGE, CULL_BACK, CUL
                PROGRAM_MVP_TEXTURE, REPEAT, RGBA8_UNORM, RGBA8_UNORM_SRGB, SEMANTICS_EXTENDED,
                SEMANTICS_REFERENCE, STEP_VERTEX)
 
+SHADER_DIR = __import__("pathlib").Path(__file__).resolve().parent.parent / "shaders"
+_SHADER_FILES = {PROGRAM_MVP_COLOR: "mvp_color.wgsl", PROGRAM_MVP_TEXTURE: "mvp_texture.wgsl",
+                 PROGRAM_COLORED_TRIANGLE: "colored_triangle.wgsl", PROGRAM_CLIP_COLOR: "clip_color.wgsl"}
+
+
+def wgsl_source(program: int, interp: int = INTERP_LINEAR, color_components: int = 0) -> str:
+    """The WGSL module of a built-in program (wgpu-cpu_amd/shaders), with the
+    @interpolate attribute for `interp` and a vec3 / vec4 colour input."""
+    src = (SHADER_DIR / _SHADER_FILES[program]).read_text()
+    mode = {INTERP_LINEAR: "linear", INTERP_PERSPECTIVE: "perspective"}.get(interp, "flat")
+    src = src.replace("@interpolate(linear)", f"@interpolate({mode})")
+    if program in (PROGRAM_MVP_COLOR, PROGRAM_CLIP_COLOR) and color_components in (3, 4):
+        if color_components == 4 and program == PROGRAM_MVP_COLOR:
+            src = src.replace("@location(1) color: vec3f", "@location(1) color: vec4f")
+            src = src.replace("@interpolate(" + mode + ") color: vec3f", "@interpolate(" + mode + ") color: vec4f")
+            src = src.replace("vec4f(f.color.x, f.color.y, f.color.z, 1.0)", "f.color")
+        if color_components == 3 and program == PROGRAM_CLIP_COLOR:
+            src = src.replace("@location(1) color: vec4f", "@location(1) color: vec3f")
+            src = src.replace("@interpolate(" + mode + ") color: vec4f", "@interpolate(" + mode + ") color: vec3f")
+            src = src.replace("return f.color;", "return vec4f(f.color.x, f.color.y, f.color.z, 1.0);")
+    return src
+
+
+def as_wgsl(spec: "PassSpec") -> "PassSpec":
+    """A copy of `spec` whose draws run the WGSL modules of their programs."""
+    import copy
+    out = copy.deepcopy(spec)
+    for d in out.draws:
+        ncol = 0
+        for fmt, _off, loc in d.attributes:
+            if loc == 1:
+                ncol = {FLOAT32X3: 3, FLOAT32X4: 4}.get(fmt, 0)
+        d.wgsl = wgsl_source(d.program, d.interp, ncol)
+    out.name = (spec.name or "scene") + " (wgsl)"
+    return out
+
+
 GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 SEED_BASE = 0x5EED_0000
 
@@ -52,6 +89,7 @@ class DrawSpec:
     base_vertex: int = 0
     first_instance: int = 0
     instance_count: int = 1
+    wgsl: Optional[str] = None    # WGSL source replacing the built-in program (same interface)
 
     @property
     def indexed(self) -> bool:
