@@ -89,10 +89,15 @@ def cpu_baseline(spec, budget_s: float, max_reps: int):
     return med, frags, len(times)
 
 
-def run_config(dev_cache, name, steps, warmup):
+def run_config(dev_cache, name, steps, warmup, wgsl=False):
     import wgc
-    from wgc import runner
+    from wgc import runner, scenes
     spec, desc = make_spec(name)
+    if wgsl:
+        # the same pass with the workload's WGSL shaders, compiled into the
+        # kernels at pipeline creation, in place of the built-in programs
+        spec = scenes.as_wgsl(spec)
+        desc += "; WGSL shaders (wgpu-cpu_amd/shaders) through the WGSL front end"
     sem = spec.semantics
     if sem not in dev_cache:
         dev_cache[sem] = wgc.Device(0, sem)
@@ -111,6 +116,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="c2", choices=["c2", "c2ref", "c1", "c3", "c3blend"])
+    ap.add_argument("--shaders", default="builtin", choices=["builtin", "wgsl"],
+                    help="built-in programs or the same shaders as WGSL modules")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -151,7 +158,8 @@ def main():
     wgc.load_library()  # fails loudly when libwgc.so is missing
 
     devs = {}
-    spec, desc, dev, pp, frags, ev_ms, wall_ms = run_config(devs, args.config, args.steps, args.warmup)
+    spec, desc, dev, pp, frags, ev_ms, wall_ms = run_config(devs, args.config, args.steps, args.warmup,
+                                                            wgsl=args.shaders == "wgsl")
     ms_per_step = ev_ms / args.steps
     value = frags * args.steps / (ev_ms * 1e-3) / 1e6
 
@@ -188,6 +196,12 @@ def main():
             n2 = max(10, args.steps // 4)
             extra[name] = {"workload": d2, "Mfragments_per_s": round(f2 * n2 / (e2 * 1e-3) / 1e6, 2),
                            "frame_ms": round(e2 / n2, 4), "fragments": f2}
+        # the headline pass with the other shader form
+        other = args.shaders != "wgsl"
+        s2, d2, dv2, pp2, f2, e2, w2 = run_config(devs, args.config, args.steps, 3, wgsl=other)
+        extra[args.config + ("_wgsl" if other else "_builtin")] = {
+            "workload": d2, "Mfragments_per_s": round(f2 * args.steps / (e2 * 1e-3) / 1e6, 2),
+            "frame_ms": round(e2 / args.steps, 4), "fragments": f2}
 
     batched = None
     if not args.no_batch:
