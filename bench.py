@@ -203,6 +203,10 @@ def main():
             "workload": d2, "Mfragments_per_s": round(f2 * args.steps / (e2 * 1e-3) / 1e6, 2),
             "frame_ms": round(e2 / args.steps, 4), "fragments": f2}
 
+    readback = None
+    if not args.no_extra:
+        readback = run_readback(dev, args.steps)
+
     batched = None
     if not args.no_batch:
         batched = run_batched(devs, args)
@@ -233,10 +237,37 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "extra": extra,
+        "readback": readback,
         "batched": batched,
         "serving": serve,
     }
     print(json.dumps(line), flush=True)
+
+
+def run_readback(dev, steps):
+    """wgpu-style readback of a 1080p RGBA8 target: copy_texture_to_buffer
+    (7680-byte rows, already 256-aligned) submitted `steps` times; device
+    time per copy and the bytes it moves (read + write)."""
+    import wgc
+    w, h = 1920, 1080
+    tex = dev.create_texture(w, h, wgc.RGBA8_UNORM)
+    buf = dev.create_buffer(w * h * 4)
+    enc = dev.create_command_encoder()
+    for _ in range(10):
+        enc.copy_texture_to_buffer(tex, (0, 0), buf, 0, w * 4, w, h)
+    cb = enc.finish()
+    dev.submit([cb])
+    dev.poll()
+    reps = max(2, steps // 10)
+    dev.timestamp(0)
+    for _ in range(reps):
+        dev.submit([cb])
+    dev.timestamp(1)
+    dev.poll()
+    ms = dev.elapsed_ms(0, 1) / (reps * 10)
+    nbytes = 2 * w * h * 4
+    return {"op": "copy_texture_to_buffer 1920x1080 RGBA8", "us_per_copy": round(ms * 1e3, 2),
+            "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1), "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}
 
 
 def run_batched(devs, args):

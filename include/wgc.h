@@ -212,6 +212,22 @@ int32_t wgc_queue_write_buffer(WgcDevice device, WgcBuffer buffer, uint64_t offs
 int32_t wgc_buffer_read(WgcBuffer buffer, uint64_t offset, void *dst, uint64_t size);
 void wgc_buffer_release(WgcBuffer buffer);
 
+/* Mapping (BufferInterface::map_async / get_mapped_range / unmap,
+ * buffer.rs:112-167).  map_async copies [offset, offset + size) into host
+ * memory after the work submitted so far; the callback runs from
+ * wgc_device_poll once the range is mapped (status WGC_OK), or with an error
+ * status if the buffer is unmapped or released first.  A write mapping is
+ * copied back to the buffer by wgc_buffer_unmap.  Mapping a mapped buffer is
+ * WGC_ERR_INVALID (reference: panic "Buffer already mapped"); offset must be
+ * a multiple of 8 and size of 4 (wgpu's MAP_ALIGNMENT / COPY_BUFFER_ALIGNMENT). */
+typedef enum WgcMapMode { WGC_MAP_READ = 1, WGC_MAP_WRITE = 2 } WgcMapMode;
+typedef void (*WgcBufferMapCallback)(int32_t status, void *userdata);
+int32_t wgc_buffer_map_async(WgcBuffer buffer, uint32_t mode, uint64_t offset, uint64_t size,
+                             WgcBufferMapCallback callback, void *userdata);
+/* Host pointer to [offset, offset + size) of the mapped range. */
+int32_t wgc_buffer_get_mapped_range(WgcBuffer buffer, uint64_t offset, uint64_t size, void **data);
+int32_t wgc_buffer_unmap(WgcBuffer buffer);
+
 /* ---- textures (DeviceInterface::create_texture device.rs:165-176,
  *      QueueInterface::write_texture device.rs:371-434) ---- */
 int32_t wgc_texture_create(WgcDevice device, uint32_t width, uint32_t height,
@@ -322,6 +338,36 @@ int32_t wgc_render_pass_draw_indexed(WgcRenderPass pass, uint32_t first_index,
                                      uint32_t first_instance, uint32_t instance_count);
 /* RenderPassInterface::end (render_pass/mod.rs:309-321); releases `pass`. */
 int32_t wgc_render_pass_end(WgcRenderPass pass);
+/* Copies and clears recorded between passes, executed in order with them
+ * (CommandEncoderInterface::copy_* and clear_*, command.rs:37-73 and
+ * 102-116, all todo!() in the reference; wgpu's validation rules apply:
+ * 4-byte buffer offsets and sizes, bytes_per_row a multiple of 256 for
+ * buffer <-> texture copies, regions inside the resources, no copy within
+ * one resource).  Textures are 2D with 4-byte texels; texel origins and the
+ * extent are in texels. */
+typedef struct WgcTexelCopyBuffer {
+    WgcBuffer buffer;
+    uint64_t offset;
+    uint32_t bytes_per_row;   /* may be 0 when height == 1 */
+} WgcTexelCopyBuffer;
+typedef struct WgcTexelCopyTexture {
+    WgcTexture texture;
+    uint32_t origin_x, origin_y;
+} WgcTexelCopyTexture;
+int32_t wgc_command_encoder_copy_buffer_to_buffer(WgcCommandEncoder encoder, WgcBuffer source, uint64_t source_offset,
+                                                  WgcBuffer destination, uint64_t destination_offset, uint64_t size);
+int32_t wgc_command_encoder_copy_texture_to_buffer(WgcCommandEncoder encoder, const WgcTexelCopyTexture *source,
+                                                   const WgcTexelCopyBuffer *destination, uint32_t width,
+                                                   uint32_t height);
+int32_t wgc_command_encoder_copy_buffer_to_texture(WgcCommandEncoder encoder, const WgcTexelCopyBuffer *source,
+                                                   const WgcTexelCopyTexture *destination, uint32_t width,
+                                                   uint32_t height);
+int32_t wgc_command_encoder_copy_texture_to_texture(WgcCommandEncoder encoder, const WgcTexelCopyTexture *source,
+                                                    const WgcTexelCopyTexture *destination, uint32_t width,
+                                                    uint32_t height);
+/* size 0: to the end of the buffer */
+int32_t wgc_command_encoder_clear_buffer(WgcCommandEncoder encoder, WgcBuffer buffer, uint64_t offset, uint64_t size);
+int32_t wgc_command_encoder_clear_texture(WgcCommandEncoder encoder, WgcTexture texture);
 int32_t wgc_command_encoder_finish(WgcCommandEncoder encoder, WgcCommandBuffer *out);
 void wgc_command_encoder_release(WgcCommandEncoder encoder);
 void wgc_command_buffer_release(WgcCommandBuffer buffer);

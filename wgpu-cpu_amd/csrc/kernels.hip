@@ -1,5 +1,7 @@
 // kernels.hip -- host-side launchers of the render-pass kernels for the
 // built-in programs (the kernels themselves are in kernels_dev.h).
+#include <algorithm>
+
 #include "kernels_dev.h"
 #include "wgc_launch.h"
 
@@ -217,6 +219,60 @@ uint32_t visibility_blocks_per_cu(bool multi)
     uint32_t &v = r[multi ? 1 : 0];
     if (!v) v = multi ? occupancy_of(k_visibility<true, 16, 256>, 256) : occupancy_of(k_visibility<false, 16, 256>, 256);
     return v;
+}
+
+// ---- copies (command encoder copy_* / clear_*) ----
+
+// `rows` rows of `row_words` 32-bit words from src (pitch src_pitch words)
+// to dst (pitch dst_pitch words); 16-byte accesses when every address and
+// pitch allows them.  Grid-stride over (row, chunk).
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_copy_rows(const uint32_t *src, uint64_t src_pitch, uint32_t *dst,
+                                                   uint64_t dst_pitch, uint32_t row_words, uint32_t rows)
+{
+    constexpr uint32_t W = VEC ? 4u : 1u;
+    uint32_t chunks = row_words / W;
+    uint64_t total = (uint64_t)chunks * rows;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t r = (uint32_t)(i / chunks), c = (uint32_t)(i - (uint64_t)r * chunks);
+        const uint32_t *s = src + r * src_pitch + (uint64_t)c * W;
+        uint32_t *d = dst + r * dst_pitch + (uint64_t)c * W;
+        if constexpr (VEC) *reinterpret_cast<uint4 *>(d) = *reinterpret_cast<const uint4 *>(s);
+        else *d = *s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_words(uint32_t *dst, uint64_t n, uint32_t value)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = value;
+}
+
+hipError_t launch_copy_rows(const void *src, uint64_t src_pitch, void *dst, uint64_t dst_pitch, uint64_t row_bytes,
+                            uint32_t rows, hipStream_t st, KernelTimer *kt)
+{
+    if (!rows || !row_bytes) return hipSuccess;
+    bool vec = ((uintptr_t)src % 16u) == 0 && ((uintptr_t)dst % 16u) == 0 && src_pitch % 16u == 0 &&
+               dst_pitch % 16u == 0 && row_bytes % 16u == 0;
+    uint64_t units = (row_bytes / (vec ? 16u : 4u)) * rows;
+    uint32_t grid = (uint32_t)std::min<uint64_t>((units + 255) / 256, 65536);
+    const uint32_t *s = (const uint32_t *)src;
+    uint32_t *d = (uint32_t *)dst;
+    if (vec) WGC_LAUNCH("k_copy", k_copy_rows<true>, grid, 256, s, src_pitch / 4, d, dst_pitch / 4,
+                        (uint32_t)(row_bytes / 4), rows);
+    else WGC_LAUNCH("k_copy", k_copy_rows<false>, grid, 256, s, src_pitch / 4, d, dst_pitch / 4,
+                    (uint32_t)(row_bytes / 4), rows);
+    return hipSuccess;
+}
+
+hipError_t launch_fill(void *dst, uint64_t bytes, uint32_t value, hipStream_t st, KernelTimer *kt)
+{
+    uint64_t n = bytes / 4;
+    if (!n) return hipSuccess;
+    uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 65536);
+    WGC_LAUNCH("k_fill", k_fill_words, grid, 256, (uint32_t *)dst, n, value);
+    return hipSuccess;
 }
 
 hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t st)

@@ -224,11 +224,25 @@ struct WgcHeap_T : Object {
     ~WgcHeap_T() override;
 };
 
+// Mapping of a buffer into host memory (buffer.rs:112-167): the range is
+// copied into pinned memory on the device stream after the work submitted
+// before map_async, and is mapped once that copy has completed.
+struct MapState {
+    uint32_t mode = 0;           // WgcMapMode; 0: not mapped
+    bool pending = false;        // requested, copy not yet complete
+    uint64_t offset = 0, size = 0;
+    uint8_t *host = nullptr;     // pinned
+    hipEvent_t done = nullptr;
+    WgcBufferMapCallback callback = nullptr;
+    void *userdata = nullptr;
+};
+
 struct WgcBuffer_T : Object {
     WgcDevice_T *dev = nullptr;
     WgcHeap_T *heap = nullptr;   // set when sub-allocated from a heap
     uint8_t *ptr = nullptr;
     uint64_t size = 0;
+    MapState map;
     ~WgcBuffer_T() override;
 };
 
@@ -308,7 +322,18 @@ struct SubCommand {
     int32_t base_vertex = 0;
 };
 
+enum CopyKind : uint32_t {
+    COPY_NONE = 0, COPY_B2B, COPY_T2B, COPY_B2T, COPY_T2T, CLEAR_BUFFER, CLEAR_TEXTURE,
+};
+
+// A render pass, or (copy_kind != COPY_NONE) a copy or clear recorded on the
+// encoder between passes.
 struct PassCommand {
+    uint32_t copy_kind = COPY_NONE;
+    Object *copy_src = nullptr, *copy_dst = nullptr;
+    uint64_t src_off = 0, dst_off = 0, copy_bytes = 0;   // byte offsets; B2B / clear size
+    uint32_t src_pitch = 0, dst_pitch = 0;                // buffer bytes per row (texel copies)
+    uint32_t sx = 0, sy = 0, dx = 0, dy = 0, cw = 0, ch = 0;   // texel origins and extent
     WgcTexture_T *color = nullptr;
     uint32_t color_load = 0;
     double clear_color[4]{};
@@ -319,6 +344,9 @@ struct PassCommand {
 
     void drop()
     {
+        release(copy_src);
+        release(copy_dst);
+        copy_src = copy_dst = nullptr;
         release(color);
         release(depth);
         for (auto &c : cmds) release(c.obj);
@@ -333,7 +361,7 @@ struct DrawWork {
     const PassCommand *pass;
 };
 
-enum class StepKind { Group, Clear };
+enum class StepKind { Group, Clear, Copy };
 
 struct Step {
     StepKind kind = StepKind::Group;
@@ -350,6 +378,11 @@ struct Step {
     float *depth = nullptr;
     float dv = 0.0f;
     uint64_t n = 0;
+    // copy: rows of row_bytes (or a zero fill of row_bytes when src is null)
+    const uint8_t *copy_src = nullptr;
+    uint8_t *copy_dst = nullptr;
+    uint64_t src_pitch = 0, dst_pitch = 0, row_bytes = 0;
+    uint32_t rows = 0;
 };
 
 struct Plan {
@@ -447,6 +480,7 @@ struct WgcDevice_T : Object {
     BlockPool table_pool;                   // device draw tables of plans
     BlockPool staging_pool;                 // pinned staging of their uploads
     std::vector<Deferred> deferred;
+    std::vector<WgcBuffer_T *> pending_maps;   // held until their copies complete
     uint32_t pending_error = 0;             // error bits of retired submissions not yet reported
     uint64_t last_frags = 0;                // fragments of the last retired submission
     uint64_t total_frags = 0, total_submissions = 0;
@@ -487,6 +521,7 @@ struct WgcDevice_T : Object {
         if (d_counters) (void)hipFree(d_counters);
         if (d_srgb) (void)hipFree(d_srgb);
         if (frag_px) (void)hipFree(frag_px);
+        for (auto *b : pending_maps) release(b);
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
         if (side) (void)hipStreamDestroy(side);
@@ -507,6 +542,14 @@ static void heap_give_back(WgcHeap_T *heap, uint8_t *ptr, uint64_t size)
 
 WgcBuffer_T::~WgcBuffer_T()
 {
+    if (map.host || map.done) {
+        (void)hipSetDevice(dev->hip_device);
+        if (map.done) {
+            (void)hipEventSynchronize(map.done);
+            (void)hipEventDestroy(map.done);
+        }
+        if (map.host) (void)hipHostFree(map.host);
+    }
     if (heap) {
         heap_give_back(heap, ptr, size);
     } else if (ptr) {
@@ -728,18 +771,29 @@ static int32_t check_device_error(WgcDevice_T *d)
     return set_error(WGC_ERR_EXECUTION, msg);
 }
 
+struct MapCall {
+    WgcBufferMapCallback fn;
+    void *userdata;
+    int32_t status;
+};
+static std::vector<MapCall> finish_maps(WgcDevice_T *d, bool wait);
+
 extern "C" int32_t wgc_device_poll(WgcDevice device, uint64_t index, uint64_t timeout_ns)
 {
     if (!device) return set_error(WGC_ERR_INVALID, "null device");
-    std::lock_guard<std::mutex> lk(device->mu);
-    HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
-    int32_t r = retire(device, index, true, timeout_ns);
-    if (r != WGC_OK) return r;
-    if (device->timing) {
-        r = resolve_timers(device);
-        if (r != WGC_OK) return r;
+    std::vector<MapCall> calls;
+    int32_t r;
+    {
+        std::lock_guard<std::mutex> lk(device->mu);
+        HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+        r = retire(device, index, true, timeout_ns);
+        if (r == WGC_OK && device->timing) r = resolve_timers(device);
+        // mapping requests: a blocking poll waits for them like for submissions
+        if (!device->pending_maps.empty()) calls = finish_maps(device, r == WGC_OK && timeout_ns != 1);
+        if (r == WGC_OK) r = check_device_error(device);
     }
-    return check_device_error(device);
+    for (auto &c : calls) c.fn(c.status, c.userdata);   // the map callbacks, outside the lock
+    return r;
 }
 
 static int32_t drain(WgcDevice_T *d)
@@ -883,6 +937,111 @@ extern "C" int32_t wgc_buffer_read(WgcBuffer buffer, uint64_t offset, void *dst,
     if (r != WGC_OK) return r;
     HIP_TRY(stream_copy(d->stream, dst, buffer->ptr + offset, size, hipMemcpyDeviceToHost), "buffer read");
     return WGC_OK;
+}
+
+// ---- mapping (buffer.rs:112-167) ----
+
+extern "C" int32_t wgc_buffer_map_async(WgcBuffer buffer, uint32_t mode, uint64_t offset, uint64_t size,
+                                        WgcBufferMapCallback callback, void *userdata)
+{
+    if (!buffer) return set_error(WGC_ERR_INVALID, "null buffer");
+    if (mode != WGC_MAP_READ && mode != WGC_MAP_WRITE) return set_error(WGC_ERR_INVALID, "bad map mode");
+    if (buffer->map.mode || buffer->map.pending)
+        return set_error(WGC_ERR_INVALID, "Buffer already mapped (buffer.rs:119-124)");
+    if (offset % 8u || size % 4u) return set_error(WGC_ERR_INVALID, "map offset must be a multiple of 8, size of 4");
+    if (offset > buffer->size || size > buffer->size - offset) return set_error(WGC_ERR_INVALID, "map range outside the buffer");
+    WgcDevice_T *d = buffer->dev;
+    std::lock_guard<std::mutex> lk(d->mu);
+    HIP_TRY(hipSetDevice(d->hip_device), "hipSetDevice");
+    MapState &m = buffer->map;
+    if (size) HIP_TRY(hipHostMalloc((void **)&m.host, size, hipHostMallocDefault), "map staging");
+    hipError_t e = hipEventCreateWithFlags(&m.done, hipEventDisableTiming);
+    // a write mapping starts from the buffer's contents as well
+    if (e == hipSuccess && size) e = hipMemcpyAsync(m.host, buffer->ptr + offset, size, hipMemcpyDeviceToHost, d->stream);
+    if (e == hipSuccess) e = hipEventRecord(m.done, d->stream);
+    if (e != hipSuccess) {
+        if (m.done) (void)hipEventDestroy(m.done);
+        if (m.host) (void)hipHostFree(m.host);
+        m = MapState{};
+        return hip_fail(e, "map_async");
+    }
+    m.pending = true;
+    m.mode = mode;
+    m.offset = offset;
+    m.size = size;
+    m.callback = callback;
+    m.userdata = userdata;
+    retain(buffer);
+    d->pending_maps.push_back(buffer);
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_buffer_get_mapped_range(WgcBuffer buffer, uint64_t offset, uint64_t size, void **data)
+{
+    if (!buffer || !data) return set_error(WGC_ERR_INVALID, "null argument");
+    const MapState &m = buffer->map;
+    if (!m.mode || m.pending) return set_error(WGC_ERR_INVALID, "Buffer is not mapped (buffer.rs:162)");
+    if (offset < m.offset || offset - m.offset > m.size || size > m.size - (offset - m.offset))
+        return set_error(WGC_ERR_INVALID, "range outside the mapped range");
+    *data = m.host + (offset - m.offset);
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_buffer_unmap(WgcBuffer buffer)
+{
+    if (!buffer) return set_error(WGC_ERR_INVALID, "null buffer");
+    MapState &m = buffer->map;
+    if (!m.mode) return set_error(WGC_ERR_INVALID, "Buffer is not mapped (buffer.rs:171)");
+    WgcDevice_T *d = buffer->dev;
+    WgcBufferMapCallback cb = nullptr;
+    void *ud = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(d->mu);
+        HIP_TRY(hipSetDevice(d->hip_device), "hipSetDevice");
+        if (m.pending) {
+            // unmapped before the mapping completed: the request is aborted
+            cb = m.callback;
+            ud = m.userdata;
+            for (size_t i = 0; i < d->pending_maps.size(); ++i)
+                if (d->pending_maps[i] == buffer) {
+                    d->pending_maps.erase(d->pending_maps.begin() + i);
+                    release(buffer);   // the pending list's reference (the caller holds its own)
+                    break;
+                }
+            (void)hipEventSynchronize(m.done);
+        } else if (m.mode == WGC_MAP_WRITE && m.size) {
+            // the written bytes go back to the buffer, ordered before later submissions
+            HIP_TRY(stream_copy(d->stream, buffer->ptr + m.offset, m.host, m.size, hipMemcpyHostToDevice), "unmap");
+        }
+        (void)hipEventDestroy(m.done);
+        if (m.host) (void)hipHostFree(m.host);
+        m = MapState{};
+    }
+    if (cb) cb(WGC_ERR_INVALID, ud);
+    return WGC_OK;
+}
+
+// Completes the mapping requests whose copies have finished (waiting for
+// them when `wait`); returns the callbacks to run once the device lock is
+// released.
+static std::vector<MapCall> finish_maps(WgcDevice_T *d, bool wait)
+{
+    std::vector<MapCall> calls;
+    std::vector<WgcBuffer_T *> still;
+    for (WgcBuffer_T *b : d->pending_maps) {
+        hipError_t e = wait ? hipEventSynchronize(b->map.done) : hipEventQuery(b->map.done);
+        if (e == hipErrorNotReady) {
+            still.push_back(b);
+            continue;
+        }
+        b->map.pending = false;
+        if (b->map.callback)
+            calls.push_back(MapCall{b->map.callback, b->map.userdata, e == hipSuccess ? WGC_OK : WGC_ERR_DEVICE});
+        b->map.callback = nullptr;
+        release(b);
+    }
+    d->pending_maps.swap(still);
+    return calls;
 }
 
 static uint32_t texel_bytes(uint32_t format)
@@ -1567,6 +1726,171 @@ extern "C" int32_t wgc_render_pass_end(WgcRenderPass pass)
     return WGC_OK;
 }
 
+// ---- copies and clears between passes ----
+
+static int32_t push_copy(WgcCommandEncoder_T *enc, PassCommand &&pc)
+{
+    if (enc->pass_open) {
+        pc.drop();
+        return set_error(WGC_ERR_INVALID, "a render pass is open on this encoder");
+    }
+    enc->passes.push_back(std::move(pc));
+    return WGC_OK;
+}
+
+// The buffer side of a texel copy: bytes_per_row a multiple of 256 covering
+// the row (may be 0 for one row), 4-byte offset, the last row inside.
+static int32_t check_texel_buffer(const WgcTexelCopyBuffer &b, uint32_t w, uint32_t h, uint32_t &pitch)
+{
+    if (!b.buffer) return set_error(WGC_ERR_INVALID, "null buffer");
+    uint64_t row = (uint64_t)w * 4u;
+    pitch = b.bytes_per_row ? b.bytes_per_row : (uint32_t)row;
+    if (h > 1 && b.bytes_per_row == 0) return set_error(WGC_ERR_INVALID, "bytes_per_row is required for more than one row");
+    if (b.bytes_per_row % 256u) return set_error(WGC_ERR_INVALID, "bytes_per_row must be a multiple of 256");
+    if (b.bytes_per_row && b.bytes_per_row < row) return set_error(WGC_ERR_INVALID, "bytes_per_row is shorter than a row");
+    if (b.offset % 4u) return set_error(WGC_ERR_INVALID, "buffer offset must be a multiple of the texel size (4)");
+    uint64_t need = h ? (uint64_t)pitch * (h - 1) + row : 0;
+    if (b.offset > b.buffer->size || need > b.buffer->size - b.offset)
+        return set_error(WGC_ERR_INVALID, "copy range outside the buffer");
+    return WGC_OK;
+}
+
+static int32_t check_texel_texture(const WgcTexelCopyTexture &t, uint32_t w, uint32_t h)
+{
+    if (!t.texture) return set_error(WGC_ERR_INVALID, "null texture");
+    if ((uint64_t)t.origin_x + w > t.texture->width || (uint64_t)t.origin_y + h > t.texture->height)
+        return set_error(WGC_ERR_INVALID, "copy region outside the texture");
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_command_encoder_copy_buffer_to_buffer(WgcCommandEncoder enc, WgcBuffer src, uint64_t src_off,
+                                                             WgcBuffer dst, uint64_t dst_off, uint64_t size)
+{
+    if (!enc || !src || !dst) return set_error(WGC_ERR_INVALID, "null argument");
+    if (src == dst) return set_error(WGC_ERR_INVALID, "source and destination are the same buffer");
+    if (src_off % 4u || dst_off % 4u || size % 4u)
+        return set_error(WGC_ERR_INVALID, "copy offsets and size must be multiples of 4");
+    if (src_off > src->size || size > src->size - src_off || dst_off > dst->size || size > dst->size - dst_off)
+        return set_error(WGC_ERR_INVALID, "copy range outside a buffer");
+    PassCommand pc;
+    pc.copy_kind = COPY_B2B;
+    retain(src);
+    retain(dst);
+    pc.copy_src = src;
+    pc.copy_dst = dst;
+    pc.src_off = src_off;
+    pc.dst_off = dst_off;
+    pc.copy_bytes = size;
+    return push_copy(enc, std::move(pc));
+}
+
+extern "C" int32_t wgc_command_encoder_copy_texture_to_buffer(WgcCommandEncoder enc, const WgcTexelCopyTexture *src,
+                                                              const WgcTexelCopyBuffer *dst, uint32_t w, uint32_t h)
+{
+    if (!enc || !src || !dst) return set_error(WGC_ERR_INVALID, "null argument");
+    uint32_t pitch = 0;
+    int32_t r = check_texel_texture(*src, w, h);
+    if (r == WGC_OK) r = check_texel_buffer(*dst, w, h, pitch);
+    if (r != WGC_OK) return r;
+    PassCommand pc;
+    pc.copy_kind = COPY_T2B;
+    retain(src->texture);
+    retain(dst->buffer);
+    pc.copy_src = src->texture;
+    pc.copy_dst = dst->buffer;
+    pc.sx = src->origin_x;
+    pc.sy = src->origin_y;
+    pc.dst_off = dst->offset;
+    pc.dst_pitch = pitch;
+    pc.cw = w;
+    pc.ch = h;
+    return push_copy(enc, std::move(pc));
+}
+
+extern "C" int32_t wgc_command_encoder_copy_buffer_to_texture(WgcCommandEncoder enc, const WgcTexelCopyBuffer *src,
+                                                              const WgcTexelCopyTexture *dst, uint32_t w, uint32_t h)
+{
+    if (!enc || !src || !dst) return set_error(WGC_ERR_INVALID, "null argument");
+    uint32_t pitch = 0;
+    int32_t r = check_texel_texture(*dst, w, h);
+    if (r == WGC_OK) r = check_texel_buffer(*src, w, h, pitch);
+    if (r != WGC_OK) return r;
+    PassCommand pc;
+    pc.copy_kind = COPY_B2T;
+    retain(src->buffer);
+    retain(dst->texture);
+    pc.copy_src = src->buffer;
+    pc.copy_dst = dst->texture;
+    pc.src_off = src->offset;
+    pc.src_pitch = pitch;
+    pc.dx = dst->origin_x;
+    pc.dy = dst->origin_y;
+    pc.cw = w;
+    pc.ch = h;
+    return push_copy(enc, std::move(pc));
+}
+
+static bool copy_compatible(uint32_t a, uint32_t b)
+{
+    auto base = [](uint32_t f) {
+        return f == WGC_TEXTURE_FORMAT_RGBA8_UNORM_SRGB ? (uint32_t)WGC_TEXTURE_FORMAT_RGBA8_UNORM
+             : f == WGC_TEXTURE_FORMAT_BGRA8_UNORM_SRGB ? (uint32_t)WGC_TEXTURE_FORMAT_BGRA8_UNORM : f;
+    };
+    return base(a) == base(b);
+}
+
+extern "C" int32_t wgc_command_encoder_copy_texture_to_texture(WgcCommandEncoder enc, const WgcTexelCopyTexture *src,
+                                                               const WgcTexelCopyTexture *dst, uint32_t w, uint32_t h)
+{
+    if (!enc || !src || !dst) return set_error(WGC_ERR_INVALID, "null argument");
+    int32_t r = check_texel_texture(*src, w, h);
+    if (r == WGC_OK) r = check_texel_texture(*dst, w, h);
+    if (r != WGC_OK) return r;
+    if (src->texture == dst->texture) return set_error(WGC_ERR_INVALID, "source and destination are the same texture");
+    if (!copy_compatible(src->texture->format, dst->texture->format))
+        return set_error(WGC_ERR_INVALID, "texture formats are not copy-compatible");
+    PassCommand pc;
+    pc.copy_kind = COPY_T2T;
+    retain(src->texture);
+    retain(dst->texture);
+    pc.copy_src = src->texture;
+    pc.copy_dst = dst->texture;
+    pc.sx = src->origin_x;
+    pc.sy = src->origin_y;
+    pc.dx = dst->origin_x;
+    pc.dy = dst->origin_y;
+    pc.cw = w;
+    pc.ch = h;
+    return push_copy(enc, std::move(pc));
+}
+
+extern "C" int32_t wgc_command_encoder_clear_buffer(WgcCommandEncoder enc, WgcBuffer buf, uint64_t offset, uint64_t size)
+{
+    if (!enc || !buf) return set_error(WGC_ERR_INVALID, "null argument");
+    if (offset > buf->size) return set_error(WGC_ERR_INVALID, "clear range outside the buffer");
+    if (size == 0) size = buf->size - offset;
+    if (offset % 4u || size % 4u) return set_error(WGC_ERR_INVALID, "clear offset and size must be multiples of 4");
+    if (size > buf->size - offset) return set_error(WGC_ERR_INVALID, "clear range outside the buffer");
+    PassCommand pc;
+    pc.copy_kind = CLEAR_BUFFER;
+    retain(buf);
+    pc.copy_dst = buf;
+    pc.dst_off = offset;
+    pc.copy_bytes = size;
+    return push_copy(enc, std::move(pc));
+}
+
+extern "C" int32_t wgc_command_encoder_clear_texture(WgcCommandEncoder enc, WgcTexture tex)
+{
+    if (!enc || !tex) return set_error(WGC_ERR_INVALID, "null argument");
+    PassCommand pc;
+    pc.copy_kind = CLEAR_TEXTURE;
+    retain(tex);
+    pc.copy_dst = tex;
+    pc.copy_bytes = tex->size;
+    return push_copy(enc, std::move(pc));
+}
+
 extern "C" int32_t wgc_command_encoder_finish(WgcCommandEncoder enc, WgcCommandBuffer *out)
 {
     if (!enc || !out) return set_error(WGC_ERR_INVALID, "null argument");
@@ -1997,6 +2321,62 @@ static void add_group(std::vector<Step> &steps, std::vector<DrawParams> draws, u
     steps.push_back(std::move(s));
 }
 
+// The step of a copy or clear: rows of bytes between device addresses.
+static void copy_step(const PassCommand &pc, Step &s)
+{
+    s.kind = StepKind::Copy;
+    switch (pc.copy_kind) {
+    case COPY_B2B: {
+        auto *a = (WgcBuffer_T *)pc.copy_src, *b = (WgcBuffer_T *)pc.copy_dst;
+        s.copy_src = a->ptr + pc.src_off;
+        s.copy_dst = b->ptr + pc.dst_off;
+        s.row_bytes = pc.copy_bytes;
+        s.rows = pc.copy_bytes ? 1u : 0u;
+        break;
+    }
+    case COPY_T2B: {
+        auto *t = (WgcTexture_T *)pc.copy_src;
+        auto *b = (WgcBuffer_T *)pc.copy_dst;
+        s.copy_src = t->ptr + ((uint64_t)pc.sy * t->width + pc.sx) * 4u;
+        s.src_pitch = (uint64_t)t->width * 4u;
+        s.copy_dst = b->ptr + pc.dst_off;
+        s.dst_pitch = pc.dst_pitch;
+        s.row_bytes = (uint64_t)pc.cw * 4u;
+        s.rows = pc.ch;
+        break;
+    }
+    case COPY_B2T: {
+        auto *b = (WgcBuffer_T *)pc.copy_src;
+        auto *t = (WgcTexture_T *)pc.copy_dst;
+        s.copy_src = b->ptr + pc.src_off;
+        s.src_pitch = pc.src_pitch;
+        s.copy_dst = t->ptr + ((uint64_t)pc.dy * t->width + pc.dx) * 4u;
+        s.dst_pitch = (uint64_t)t->width * 4u;
+        s.row_bytes = (uint64_t)pc.cw * 4u;
+        s.rows = pc.ch;
+        break;
+    }
+    case COPY_T2T: {
+        auto *a = (WgcTexture_T *)pc.copy_src, *b = (WgcTexture_T *)pc.copy_dst;
+        s.copy_src = a->ptr + ((uint64_t)pc.sy * a->width + pc.sx) * 4u;
+        s.src_pitch = (uint64_t)a->width * 4u;
+        s.copy_dst = b->ptr + ((uint64_t)pc.dy * b->width + pc.dx) * 4u;
+        s.dst_pitch = (uint64_t)b->width * 4u;
+        s.row_bytes = (uint64_t)pc.cw * 4u;
+        s.rows = pc.ch;
+        break;
+    }
+    case CLEAR_BUFFER:
+        s.copy_dst = ((WgcBuffer_T *)pc.copy_dst)->ptr + pc.dst_off;
+        s.row_bytes = pc.copy_bytes;
+        break;
+    default:   // CLEAR_TEXTURE
+        s.copy_dst = ((WgcTexture_T *)pc.copy_dst)->ptr;
+        s.row_bytes = pc.copy_bytes;
+        break;
+    }
+}
+
 static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
 {
     plan.reset();
@@ -2008,10 +2388,17 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         std::vector<DrawWork> draws;
         bool clear_only = false;
         Step clear;
+        bool copy = false;
+        Step copy_step;
         std::vector<const void *> writes, reads;
     };
     std::vector<PassWork> passes(cb->passes.size());
     for (size_t i = 0; i < cb->passes.size(); ++i) {
+        if (cb->passes[i].copy_kind != COPY_NONE) {
+            passes[i].copy = true;
+            copy_step(cb->passes[i], passes[i].copy_step);
+            continue;
+        }
         int32_t r = collect_pass(dev, cb->passes[i], shift, passes[i].draws, passes[i].clear_only, passes[i].clear);
         if (r != WGC_OK) return r;
         pass_textures(cb->passes[i], passes[i].writes, passes[i].reads);
@@ -2025,6 +2412,10 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         PassWork &pw = passes[i];
         bool batchable = dev->batching && pw.draws.size() == 1;
         if (!batchable) {
+            if (pw.copy) {
+                pw.copy_step.stage = stage++;
+                plan.steps.push_back(pw.copy_step);
+            }
             for (auto &dw : pw.draws) add_group(plan.steps, {dw.d}, cfg, stage++);
             if (pw.clear_only) {
                 pw.clear.stage = stage++;
@@ -2152,6 +2543,10 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
 static hipError_t launch_step(WgcDevice_T *dev, const Step &s, hipStream_t st, KernelTimer *kt)
 {
     if (s.kind == StepKind::Clear) return launch_clear(s.color, s.rgba, s.depth, s.dv, s.n, st, kt);
+    if (s.kind == StepKind::Copy) {
+        if (!s.copy_src) return launch_fill(s.copy_dst, s.row_bytes, 0u, st, kt);
+        return launch_copy_rows(s.copy_src, s.src_pitch, s.copy_dst, s.dst_pitch, s.row_bytes, s.rows, st, kt);
+    }
     if (s.gl.user) return launch_user(*s.gl.user, s.draws[0], s.batch, s.gl, s.multi, st, kt);
     if (s.multi) return launch_batch(s.draws[0], s.batch, s.gl, st, kt);
     return launch_draw(s.draws[0], s.batch, s.gl, st, kt);
@@ -2214,6 +2609,30 @@ static int32_t submit_plan(WgcDevice_T *dev, Plan &plan)
     return WGC_OK;
 }
 
+static bool is_mapped(const WgcBuffer_T *b) { return b && (b->map.mode || b->map.pending); }
+
+// wgpu validation: a submission must not use a mapped buffer.
+static bool uses_mapped_buffer(const WgcCommandBuffer_T *cb)
+{
+    for (const PassCommand &pc : cb->passes) {
+        if (pc.copy_kind == COPY_B2B || pc.copy_kind == COPY_B2T) {
+            if (is_mapped((const WgcBuffer_T *)pc.copy_src)) return true;
+        }
+        if (pc.copy_kind == COPY_B2B || pc.copy_kind == COPY_T2B || pc.copy_kind == CLEAR_BUFFER) {
+            if (is_mapped((const WgcBuffer_T *)pc.copy_dst)) return true;
+        }
+        for (const SubCommand &c : pc.cmds) {
+            if (c.kind == CmdKind::SetVertexBuffer || c.kind == CmdKind::SetIndexBuffer) {
+                if (is_mapped((const WgcBuffer_T *)c.obj)) return true;
+            } else if (c.kind == CmdKind::SetBindGroup && c.obj) {
+                for (const BindingSlot &e : ((const WgcBindGroup_T *)c.obj)->entries)
+                    if (is_mapped(e.buffer)) return true;
+            }
+        }
+    }
+    return false;
+}
+
 extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcCommandBuffer *buffers,
                                     uint64_t *submission_index)
 {
@@ -2228,6 +2647,11 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
         device->slot_pool.push_back(f.slot);
         return hip_fail(e0, "counter reset");
     }
+    for (uint32_t i = 0; i < count; ++i)
+        if (buffers[i] && uses_mapped_buffer(buffers[i])) {
+            device->slot_pool.push_back(f.slot);
+            return set_error(WGC_ERR_INVALID, "a buffer used by the command buffer is mapped");
+        }
     f.index = device->next_index++;
     for (uint32_t i = 0; i < count; ++i) {
         WgcCommandBuffer_T *cb = buffers[i];

@@ -91,6 +91,11 @@ void raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32_t c
                           uint32_t &shade);
 // Zeroes the submission counters (fragment slots and error bits).
 hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t st);
+// Row copy (4-byte multiples; 16-byte accesses where aligned) and word fill,
+// ordered on the stream like the render launches.
+hipError_t launch_copy_rows(const void *src, uint64_t src_pitch, void *dst, uint64_t dst_pitch, uint64_t row_bytes,
+                            uint32_t rows, hipStream_t st, KernelTimer *kt);
+hipError_t launch_fill(void *dst, uint64_t bytes, uint32_t value, hipStream_t st, KernelTimer *kt);
 hipError_t launch_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, uint64_t n,
                         hipStream_t st, KernelTimer *kt);
 

@@ -149,6 +149,20 @@ class RenderPassDescriptor(C.Structure):
                 ("depth_stencil_attachment", C.POINTER(DepthStencilAttachment))]
 
 
+class TexelCopyBuffer(C.Structure):
+    _fields_ = [("buffer", C.c_void_p), ("offset", C.c_uint64), ("bytes_per_row", C.c_uint32)]
+
+
+class TexelCopyTexture(C.Structure):
+    _fields_ = [("texture", C.c_void_p), ("origin_x", C.c_uint32), ("origin_y", C.c_uint32)]
+
+
+MAP_CALLBACK = C.CFUNCTYPE(None, C.c_int32, C.c_void_p)
+_MAP_THUNKS: dict = {}   # callbacks of pending mapping requests
+MAP_READ = 1
+MAP_WRITE = 2
+
+
 class PassStats(C.Structure):
     _fields_ = [("fragments", C.c_uint64), ("primitives", C.c_uint64), ("kernel_launches", C.c_uint64)]
 
@@ -165,6 +179,18 @@ SIGNATURES = {
     "wgc_queue_write_buffer": (C.c_int32, [_P, _P, C.c_uint64, C.c_void_p, C.c_uint64]),
     "wgc_buffer_read": (C.c_int32, [_P, C.c_uint64, C.c_void_p, C.c_uint64]),
     "wgc_buffer_release": (None, [_P]),
+    "wgc_buffer_map_async": (C.c_int32, [_P, C.c_uint32, C.c_uint64, C.c_uint64, MAP_CALLBACK, C.c_void_p]),
+    "wgc_buffer_get_mapped_range": (C.c_int32, [_P, C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "wgc_buffer_unmap": (C.c_int32, [_P]),
+    "wgc_command_encoder_copy_buffer_to_buffer": (C.c_int32, [_P, _P, C.c_uint64, _P, C.c_uint64, C.c_uint64]),
+    "wgc_command_encoder_copy_texture_to_buffer": (C.c_int32, [_P, C.POINTER(TexelCopyTexture),
+                                                               C.POINTER(TexelCopyBuffer), C.c_uint32, C.c_uint32]),
+    "wgc_command_encoder_copy_buffer_to_texture": (C.c_int32, [_P, C.POINTER(TexelCopyBuffer),
+                                                               C.POINTER(TexelCopyTexture), C.c_uint32, C.c_uint32]),
+    "wgc_command_encoder_copy_texture_to_texture": (C.c_int32, [_P, C.POINTER(TexelCopyTexture),
+                                                                C.POINTER(TexelCopyTexture), C.c_uint32, C.c_uint32]),
+    "wgc_command_encoder_clear_buffer": (C.c_int32, [_P, _P, C.c_uint64, C.c_uint64]),
+    "wgc_command_encoder_clear_texture": (C.c_int32, [_P, _P]),
     "wgc_texture_create": (C.c_int32, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _PP]),
     "wgc_queue_write_texture": (C.c_int32, [_P, _P, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32]),
     "wgc_texture_read": (C.c_int32, [_P, C.c_void_p, C.c_uint64]),
@@ -305,6 +331,33 @@ class Buffer(_Handle):
         buf = (C.c_uint8 * n)()
         _check(load_library().wgc_buffer_read(self._ptr, offset, buf, n))
         return bytes(buf)
+
+    # mapping (BufferInterface::map_async / get_mapped_range / unmap)
+    def map_async(self, mode: int, offset: int = 0, size: Optional[int] = None, callback=None):
+        """callback(status) runs from Device.poll once the range is mapped."""
+        n = self.size - offset if size is None else size
+
+        key = object()
+
+        def trampoline(status, _ud):
+            _MAP_THUNKS.pop(key, None)
+            if callback is not None:
+                callback(status)
+        thunk = MAP_CALLBACK(trampoline)
+        _check(load_library().wgc_buffer_map_async(self._ptr, mode, offset, n, thunk, None))
+        # the library calls the thunk exactly once (mapped or aborted), maybe
+        # after this Buffer object is gone: keep it alive until then
+        _MAP_THUNKS[key] = thunk
+
+    def get_mapped_range(self, offset: int = 0, size: Optional[int] = None):
+        """A writable view (ctypes array) of the mapped bytes."""
+        n = self.size - offset if size is None else size
+        p = C.c_void_p()
+        _check(load_library().wgc_buffer_get_mapped_range(self._ptr, offset, n, C.byref(p)))
+        return (C.c_uint8 * n).from_address(p.value)
+
+    def unmap(self):
+        _check(load_library().wgc_buffer_unmap(self._ptr))
 
 
 class Texture(_Handle):
@@ -450,6 +503,37 @@ class CommandEncoder:
         out = _out()
         _check(self._lib.wgc_command_encoder_finish(self._ptr, C.byref(out)))
         return CommandBuffer(out)
+
+    # copies and clears, executed in order with the passes
+    def copy_buffer_to_buffer(self, src: Buffer, src_offset: int, dst: Buffer, dst_offset: int, size: int):
+        _check(self._lib.wgc_command_encoder_copy_buffer_to_buffer(self._ptr, src.ptr, src_offset, dst.ptr,
+                                                                   dst_offset, size))
+
+    def copy_texture_to_buffer(self, src: Texture, origin, dst: Buffer, offset: int, bytes_per_row: int,
+                               width: int, height: int):
+        t = TexelCopyTexture(src.ptr, origin[0], origin[1])
+        b = TexelCopyBuffer(dst.ptr, offset, bytes_per_row)
+        _check(self._lib.wgc_command_encoder_copy_texture_to_buffer(self._ptr, C.byref(t), C.byref(b), width,
+                                                                    height))
+
+    def copy_buffer_to_texture(self, src: Buffer, offset: int, bytes_per_row: int, dst: Texture, origin,
+                               width: int, height: int):
+        b = TexelCopyBuffer(src.ptr, offset, bytes_per_row)
+        t = TexelCopyTexture(dst.ptr, origin[0], origin[1])
+        _check(self._lib.wgc_command_encoder_copy_buffer_to_texture(self._ptr, C.byref(b), C.byref(t), width,
+                                                                    height))
+
+    def copy_texture_to_texture(self, src: Texture, src_origin, dst: Texture, dst_origin, width: int, height: int):
+        a = TexelCopyTexture(src.ptr, src_origin[0], src_origin[1])
+        b = TexelCopyTexture(dst.ptr, dst_origin[0], dst_origin[1])
+        _check(self._lib.wgc_command_encoder_copy_texture_to_texture(self._ptr, C.byref(a), C.byref(b), width,
+                                                                     height))
+
+    def clear_buffer(self, buffer: Buffer, offset: int = 0, size: int = 0):
+        _check(self._lib.wgc_command_encoder_clear_buffer(self._ptr, buffer.ptr, offset, size))
+
+    def clear_texture(self, texture: Texture):
+        _check(self._lib.wgc_command_encoder_clear_texture(self._ptr, texture.ptr))
 
     def __del__(self):
         try:
