@@ -44,6 +44,7 @@ class OrDraw(C.Structure):
         ("viewport", C.c_float * 6), ("scissor", C.c_uint32 * 4),
         ("indexed", C.c_uint32), ("first", C.c_uint32), ("count", C.c_uint32),
         ("base_vertex", C.c_int32), ("first_instance", C.c_uint32), ("instance_count", C.c_uint32),
+        ("topology", C.c_uint32), ("strip_restart", C.c_uint32),
     ]
 
 
@@ -91,6 +92,15 @@ def lib() -> C.CDLL:
         L.oracle_srgb_table.restype = C.POINTER(C.c_float)
         L.oracle_mat4_mul_vec4.argtypes = [C.POINTER(C.c_float)] * 3
         L.oracle_mat4_mul_vec4.restype = None
+        L.oracle_bresenham.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_void_p, C.c_void_p,
+                                       C.c_int64]
+        L.oracle_bresenham.restype = C.c_int64
+        L.oracle_clip_line.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                       C.POINTER(C.c_float)]
+        L.oracle_clip_line.restype = C.c_int
+        L.oracle_assemble.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                      C.c_int64]
+        L.oracle_assemble.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -179,6 +189,12 @@ def render(spec, frag_counts: bool = False) -> OracleResult:
         for k in range(4):
             o.scissor[k] = cur_sc[k]
         o.first, o.count = d.first, d.count
+        o.topology = d.topology
+        if d.strip_index_format:
+            # state.rs:304-312: strip_index_format must equal the bound index format
+            if d.index_data is not None and d.strip_index_format != d.index_format:
+                raise OracleError("strip_index_format differs from the index format")
+            o.strip_restart = 1
         o.base_vertex = d.base_vertex
         o.first_instance, o.instance_count = d.first_instance, d.instance_count
     p = OrPass()
@@ -259,3 +275,41 @@ def mat4_mul_vec4(m_cols, v):
 def srgb_table() -> np.ndarray:
     p = lib().oracle_srgb_table()
     return np.ctypeslib.as_array(p, shape=(256,)).copy()
+
+
+def bresenham(start, end):
+    """util/bresenham.rs: [(x, y), ...] and [t, ...] from start (inclusive) to end (exclusive)."""
+    a = (C.c_uint32 * 2)(*start)
+    b = (C.c_uint32 * 2)(*end)
+    n = lib().oracle_bresenham(a, b, None, None, 0)
+    xy = np.zeros(2 * max(n, 1), np.uint32)
+    t = np.zeros(max(n, 1), np.float32)
+    lib().oracle_bresenham(a, b, xy.ctypes.data, t.ctypes.data, n)
+    return [tuple(int(v) for v in xy[2 * i:2 * i + 2]) for i in range(n)], t[:n]
+
+
+def clip_line(a, b):
+    """clipper.rs Cohen-Sutherland: None or ((p0, p1), (alpha0, alpha1))."""
+    fa = (C.c_float * 4)(*a)
+    fb = (C.c_float * 4)(*b)
+    out = (C.c_float * 8)()
+    al = (C.c_float * 2)()
+    if not lib().oracle_clip_line(fa, fb, out, al):
+        return None
+    return (list(out[:4]), list(out[4:])), (al[0], al[1])
+
+
+def assemble(topology, count, indices=None, index_format=4, restart=False):
+    """primitive.rs assembly: index positions of every primitive."""
+    raw = None
+    keep = None
+    if indices is not None:
+        keep = np.asarray(indices, np.uint16 if index_format == 2 else np.uint32).tobytes()
+        raw = C.c_char_p(keep)
+    n = lib().oracle_assemble(topology, 1 if restart else 0, index_format, raw, count, None, 0)
+    if n < 0:
+        raise OracleError("strip_index_format set, but not a strip topology")
+    nv = 1 if topology == 0 else (2 if topology in (1, 2) else 3)
+    out = np.zeros(max(n, 1) * nv, np.uint32)
+    lib().oracle_assemble(topology, 1 if restart else 0, index_format, raw, count, out.ctypes.data, n)
+    return [list(int(v) for v in out[i * nv:(i + 1) * nv]) for i in range(n)]

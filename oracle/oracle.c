@@ -642,126 +642,543 @@ static int resolve_index(const OrDraw *d, uint32_t i, uint32_t *vertex)
     return 0;
 }
 
-static int draw(OrPass *p, const OrDraw *d)
+/* ------------------------------------------------------------------ */
+/* Primitive assembly (primitive.rs:206-487, index.rs:72-106)         */
+/* ------------------------------------------------------------------ */
+
+static int topo_vertices(uint32_t topology)
 {
-    if (d->program == OR_PROG_MVP_TEXTURE && !d->texture) return fail(p, "no texture bound");
-    /* ToRaster scissor box (raster.rs:130-143) */
+    switch (topology) {
+    case OR_TOPO_POINT_LIST: return 1;
+    case OR_TOPO_LINE_LIST:
+    case OR_TOPO_LINE_STRIP: return 2;
+    default: return 3;
+    }
+}
+
+/* TriStripGenerator::push (primitive.rs:349-365) */
+typedef struct {
+    uint32_t buf[2];
+    int odd;
+} strip_gen;
+
+static void strip_push(strip_gen *g, uint32_t v, uint32_t out[3])
+{
+    out[0] = g->buf[0];
+    out[1] = g->buf[1];
+    out[2] = v;
+    g->buf[g->odd] = v;
+    g->odd = !g->odd;
+}
+
+int64_t oracle_assemble(uint32_t topology, uint32_t restart, uint32_t index_format, const uint8_t *indices,
+                        uint32_t count, uint32_t *out, int64_t cap)
+{
+    int n = topo_vertices(topology);
+    int64_t np = 0;
+#define EMIT(...)                                                         \
+    do {                                                                  \
+        uint32_t v_[3] = {__VA_ARGS__};                                   \
+        if (np < cap)                                                     \
+            for (int k_ = 0; k_ < n; ++k_) out[np * n + k_] = v_[k_];     \
+        ++np;                                                             \
+    } while (0)
+    int strip = topology == OR_TOPO_LINE_STRIP || topology == OR_TOPO_TRIANGLE_STRIP;
+    if (restart && !strip) return -1;   /* state.rs:306-310: assert!(is_strip) */
+    if (!strip) {
+        /* List: array_chunks_::<n>, a remainder is dropped */
+        for (uint32_t p = 0; p < count / (uint32_t)n; ++p) EMIT(p * n, p * n + 1, p * n + 2);
+        return np;
+    }
+    if (!restart || !indices) {
+        if (topology == OR_TOPO_LINE_STRIP) {
+            /* tuple_windows (primitive.rs:270-282) */
+            for (uint32_t k = 0; k + 1 < count; ++k) EMIT(k, k + 1, 0);
+        } else if (count >= 2) {
+            /* TriStripIter (primitive.rs:367-398) */
+            strip_gen g = {{0, 1}, 0};
+            for (uint32_t k = 2; k < count; ++k) {
+                uint32_t t[3];
+                strip_push(&g, k, t);
+                EMIT(t[0], t[1], t[2]);
+            }
+        }
+        return np;
+    }
+    uint32_t sep = index_format == 2 ? 0xFFFFu : 0xFFFFFFFFu;
+#define IS_SEP(i) (index_format == 2 ? ((uint32_t)indices[2 * (i)] | ((uint32_t)indices[2 * (i) + 1] << 8)) == sep \
+                                     : ((uint32_t)indices[4 * (i)] | ((uint32_t)indices[4 * (i) + 1] << 8) |      \
+                                        ((uint32_t)indices[4 * (i) + 2] << 16) |                                  \
+                                        ((uint32_t)indices[4 * (i) + 3] << 24)) == sep)
+    if (topology == OR_TOPO_LINE_STRIP) {
+        /* Strip<2> with separators (primitive.rs:285-318) */
+        uint32_t buf[2];
+        int nb = 0;
+        for (uint32_t k = 0; k < count; ++k) {
+            if (IS_SEP(k)) {
+                nb = 0;
+                continue;
+            }
+            buf[nb++] = k;
+            if (nb == 2) {
+                EMIT(buf[0], buf[1], 0);
+                buf[0] = buf[1];
+                nb = 1;
+            }
+        }
+        return np;
+    }
+    /* ResetTriStripIter (primitive.rs:400-487) */
+    uint32_t k = 0;
+    for (;;) {
+        /* try_fill_buffer: two vertices, a separator empties the buffer */
+        uint32_t buf[2];
+        int nb = 0;
+        while (k < count && nb < 2) {
+            if (IS_SEP(k)) nb = 0;
+            else buf[nb++] = k;
+            ++k;
+        }
+        if (nb < 2) return np;
+        strip_gen g = {{buf[0], buf[1]}, 0};
+        int refill = 0;
+        while (k < count) {
+            uint32_t i = k++;
+            if (IS_SEP(i)) {
+                refill = 1;
+                break;
+            }
+            uint32_t t[3];
+            strip_push(&g, i, t);
+            EMIT(t[0], t[1], t[2]);
+        }
+        if (!refill) return np;
+    }
+#undef IS_SEP
+#undef EMIT
+}
+
+/* ------------------------------------------------------------------ */
+/* Lines and points (clipper.rs:207-407, raster.rs:184-265,           */
+/* util/bresenham.rs:78-153)                                          */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int64_t x, y;
+} p64;
+
+/* Octant::from_points (bresenham.rs:20-44) */
+static int octant_of(p64 s, p64 e)
+{
+    int64_t dx = e.x - s.x, dy = e.y - s.y;
+    int o = 0;
+    if (dy < 0) {
+        dx = -dx;
+        dy = -dy;
+        o += 4;
+    }
+    if (dx < 0) {
+        int64_t t = dx;
+        dx = dy;
+        dy = -t;
+        o += 2;
+    }
+    if (dx < dy) o += 1;
+    return o;
+}
+
+static p64 to_octant0(int o, p64 p)
+{
+    switch (o) {
+    case 0: return (p64){p.x, p.y};
+    case 1: return (p64){p.y, p.x};
+    case 2: return (p64){p.y, -p.x};
+    case 3: return (p64){-p.x, p.y};
+    case 4: return (p64){-p.x, -p.y};
+    case 5: return (p64){-p.y, -p.x};
+    case 6: return (p64){-p.y, p.x};
+    default: return (p64){p.x, -p.y};
+    }
+}
+
+static p64 from_octant0(int o, p64 p)
+{
+    switch (o) {
+    case 0: return (p64){p.x, p.y};
+    case 1: return (p64){p.y, p.x};
+    case 2: return (p64){-p.y, p.x};
+    case 3: return (p64){-p.x, p.y};
+    case 4: return (p64){-p.x, -p.y};
+    case 5: return (p64){-p.y, -p.x};
+    case 6: return (p64){p.y, -p.x};
+    default: return (p64){p.x, -p.y};
+    }
+}
+
+typedef struct {
+    p64 x, d;
+    int64_t x1, diff;
+    int octant;
+    float t, dt;
+} bres_t;
+
+static void bres_init(bres_t *b, const uint32_t start[2], const uint32_t end[2])
+{
+    p64 s = {(int64_t)start[0], (int64_t)start[1]}, e = {(int64_t)end[0], (int64_t)end[1]};
+    b->octant = octant_of(s, e);
+    s = to_octant0(b->octant, s);
+    e = to_octant0(b->octant, e);
+    b->d = (p64){e.x - s.x, e.y - s.y};
+    b->dt = 1.0f / (float)b->d.x;
+    b->x = s;
+    b->x1 = e.x;
+    b->diff = b->d.y - b->d.x;
+    b->t = 0.0f;
+}
+
+/* Bresenham::next: 1 and the point and its t, or 0 at the end */
+static int bres_next(bres_t *b, uint32_t xy[2], float *t)
+{
+    if (b->x.x >= b->x1) return 0;
+    p64 p = b->x;
+    if (b->diff >= 0) {
+        b->x.y += 1;
+        b->diff -= b->d.x;
+    }
+    b->diff += b->d.y;
+    *t = b->t;
+    b->t = b->t + b->dt;
+    b->x.x += 1;
+    p64 q = from_octant0(b->octant, p);
+    xy[0] = (uint32_t)q.x;
+    xy[1] = (uint32_t)q.y;
+    return 1;
+}
+
+int64_t oracle_bresenham(const uint32_t start[2], const uint32_t end[2], uint32_t *out_xy, float *out_t,
+                         int64_t cap)
+{
+    bres_t b;
+    bres_init(&b, start, end);
+    int64_t n = 0;
+    uint32_t xy[2];
+    float t;
+    while (bres_next(&b, xy, &t)) {
+        if (n < cap) {
+            out_xy[2 * n] = xy[0];
+            out_xy[2 * n + 1] = xy[1];
+            out_t[n] = t;
+        }
+        ++n;
+    }
+    return n;
+}
+
+/* lerp (interpolation.rs:84-89) on a Vector4: x0 * (1 - t) + x1 * t */
+static void lerp4(const float *x0, const float *x1, float t, float *out)
+{
+    float s = 1.0f - t;
+    for (int c = 0; c < 4; ++c) {
+        float a = x0[c] * s;
+        float b = x1[c] * t;
+        out[c] = a + b;
+    }
+}
+
+int oracle_clip_line(const float a[4], const float b[4], float out[8], float alphas[2])
+{
+    float dist[2][6];
+    unsigned oc[2] = {0, 0};
+    for (int k = 0; k < 2; ++k)
+        for (int pl = 0; pl < 6; ++pl) {
+            dist[k][pl] = clip_distance(planes[pl], k ? b : a);
+            if (dist[k][pl] < 0.0f) oc[k] |= 1u << pl;
+        }
+    unsigned any = oc[0] | oc[1], both = oc[0] & oc[1];
+    if (!any) {
+        memcpy(out, a, 16);
+        memcpy(out + 4, b, 16);
+        alphas[0] = 0.0f;
+        alphas[1] = 1.0f;
+        return 1;
+    }
+    if (both) return 0;
+    float al[2] = {0.0f, 1.0f};
+    for (int pl = 0; pl < 6; ++pl) {
+        unsigned bit = 1u << pl;
+        if (!(any & bit)) continue;
+        float da = dist[0][pl], db = dist[1][pl];
+        float alpha = da / (da - db);
+        if (oc[0] & bit) {
+            if (alpha > al[0]) al[0] = alpha;
+        } else {
+            if (alpha < al[1]) al[1] = alpha;
+        }
+        if (al[0] > al[1]) return 0;
+    }
+    /* update_point (clipper.rs:343-356): the second update reads the
+     * already updated first point */
+    float p[2][4];
+    memcpy(p[0], a, 16);
+    memcpy(p[1], b, 16);
+    if (oc[0]) {
+        float u[4];
+        lerp4(p[0], p[1], al[0], u);
+        memcpy(p[0], u, 16);
+    }
+    if (oc[1]) {
+        float u[4];
+        lerp4(p[0], p[1], al[1], u);
+        memcpy(p[1], u, 16);
+    }
+    memcpy(out, p[0], 16);
+    memcpy(out + 4, p[1], 16);
+    alphas[0] = al[0];
+    alphas[1] = al[1];
+    return 1;
+}
+
+/* Depth test and colour write of one fragment (fragment.rs:427-488). */
+static void write_fragment(OrPass *p, const OrDraw *d, uint32_t x, uint32_t y, float z, const float *var, int nvar)
+{
+    float color[4];
+    run_fs(d, var, nvar, color);
+    int pass = 1;
+    if (p->depth && d->depth_enabled) {
+        float *dp = p->depth + (uint64_t)y * p->width + x;
+        if (compare(d->depth_compare, z, *dp)) {
+            if (d->depth_write) *dp = z;
+        } else {
+            pass = 0;
+        }
+    }
+    if (pass && p->color) {
+        if (d->blend) blend_pixel(p, x, y, color);
+        else put_pixel(p, x, y, color);
+    }
+}
+
+static int in_scissor(const OrDraw *d, uint32_t x, uint32_t y)
+{
     uint64_t sx0 = d->scissor[0], sy0 = d->scissor[1];
     uint64_t sx1 = sx0 + d->scissor[2], sy1 = sy0 + d->scissor[3];
-    if (sx1 > p->width || sy1 > p->height) return fail(p, "scissor outside the framebuffer");
+    return x >= sx0 && x < sx1 && y >= sy0 && y < sy1;
+}
 
-    for (uint32_t inst = d->first_instance; inst < d->first_instance + d->instance_count; ++inst) {
-        uint32_t nprim = d->count / 3; /* array_chunks_::<3> drops a remainder */
-        for (uint32_t prim = 0; prim < nprim; ++prim) {
-            vout vo[3];
-            for (int k = 0; k < 3; ++k) {
-                uint32_t vertex;
-                if (resolve_index(d, d->first + prim * 3 + k, &vertex))
-                    return fail(p, "index out of range or vertex index overflow");
-                if (run_vs(d, inst, vertex, &vo[k])) return fail(p, "vertex fetch out of range");
-            }
-            /* TriFace::new (primitive.rs:169-184) */
-            float nd[3][3];
-            for (int k = 0; k < 3; ++k) {
-                if (vo[k].clip[3] == 0.0f) return fail(p, "clip position w = 0");
-                for (int c = 0; c < 3; ++c) nd[k][c] = vo[k].clip[c] / vo[k].clip[3];
-            }
-            float abx = nd[1][0] - nd[0][0], aby = nd[1][1] - nd[0][1];
-            float acx = nd[2][0] - nd[0][0], acy = nd[2][1] - nd[0][1];
-            float area_z = abx * acy - aby * acx;
-            int ccw = area_z > 0.0f;
-            int front_facing = ccw ? (d->front_face_cw == 0) : (d->front_face_cw == 1);
-            int cull = (d->cull_mode == OR_CULL_FRONT) ? front_facing
-                     : (d->cull_mode == OR_CULL_BACK) ? !front_facing : 0;
-            if (cull) continue;
-            (void)front_facing; /* the built-in programs do not read front_facing */
+/* PointRasterizer (raster.rs:198-215): no clipping; the vertex's own
+ * varyings (NoInterpolation) */
+static int draw_point(OrPass *p, const OrDraw *d, const vout *v)
+{
+    if (v->clip[3] == 0.0f) return fail(p, "clip position w = 0");
+    uint32_t fb[2];
+    float fr[4];
+    oracle_to_raster(d->viewport, v->clip, fb, fr);
+    if (!in_scissor(d, fb[0], fb[1])) return 0;
+    p->fragments++;
+    if (p->frag_count) p->frag_count[(uint64_t)fb[1] * p->width + fb[0]]++;
+    write_fragment(p, d, fb[0], fb[1], fr[2], v->var, v->nvar);
+    return 0;
+}
 
-            float pos[3][4];
-            for (int k = 0; k < 3; ++k) memcpy(pos[k], vo[k].clip, sizeof pos[k]);
-            ctri sub[CLIP_CAP];
-            int nsub = clip_triangle(pos, sub);
-            for (int t = 0; t < nsub; ++t) {
-                uint32_t fb[3][2];
-                float fr[3][4];
-                for (int k = 0; k < 3; ++k) oracle_to_raster(d->viewport, sub[t].v[k].p, fb[k], fr[k]);
-                float total = shoelace(fr[0][0], fr[0][1], fr[1][0], fr[1][1], fr[2][0], fr[2][1]);
-                if (total == 0.0f) continue;
-                scanlines_t sl;
-                scanline_t line;
-                scanlines_init(&sl, fb);
-                for (;;) {
-                    int r = scanlines_next(&sl, &line);
-                    if (r < 0) return fail(p, "scanline produced a negative coordinate");
-                    if (r == 0) break;
-                    for (int64_t xx = line.x1; xx <= line.x2; ++xx) {
-                        uint64_t x = (uint64_t)xx, y = (uint64_t)line.y;
-                        if (!(x >= sx0 && x < sx1 && y >= sy0 && y < sy1)) continue;
-                        p->fragments++;
-                        if (p->frag_count) p->frag_count[y * p->width + x]++;
-                        /* barycentric at the pixel corner (raster.rs:313-335) */
-                        float px = (float)x, py = (float)y;
-                        float l0 = shoelace(px, py, fr[1][0], fr[1][1], fr[2][0], fr[2][1]) / total;
-                        float l1 = shoelace(fr[0][0], fr[0][1], px, py, fr[2][0], fr[2][1]) / total;
-                        float l2 = shoelace(fr[0][0], fr[0][1], fr[1][0], fr[1][1], px, py) / total;
-                        /* compose with the clip barycentrics (raster.rs:338-343) */
-                        float lc[3];
-                        for (int c = 0; c < 3; ++c) {
-                            float acc = sub[t].v[0].b[c] * l0;
-                            acc = acc + sub[t].v[1].b[c] * l1;
-                            acc = acc + sub[t].v[2].b[c] * l2;
-                            lc[c] = acc;
-                        }
-                        /* fragment = lc . [a, b, c].fragment (raster.rs:345-346) */
-                        float frag_z = fr[0][2] * lc[0];
-                        frag_z = frag_z + fr[1][2] * lc[1];
-                        frag_z = frag_z + fr[2][2] * lc[2];
-                        /* varyings of the unclipped vertices (fragment.rs:150-152, 346-366) */
-                        float var[OR_MAX_VAR];
-                        int nvar = vo[0].nvar;
-                        if (d->interp == OR_INTERP_FLAT) {
-                            for (int c = 0; c < nvar; ++c) var[c] = vo[0].var[c];
-                        } else if (d->interp == OR_INTERP_PERSPECTIVE) {
-                            /* extension: perspective-correct weights q_i = lc_i * (1/w_i) */
-                            float q0 = lc[0] * (1.0f / vo[0].clip[3]);
-                            float q1 = lc[1] * (1.0f / vo[1].clip[3]);
-                            float q2 = lc[2] * (1.0f / vo[2].clip[3]);
-                            float qs = (q0 + q1) + q2;
-                            for (int c = 0; c < nvar; ++c) {
-                                float acc = vo[0].var[c] * q0;
-                                acc = acc + vo[1].var[c] * q1;
-                                acc = acc + vo[2].var[c] * q2;
-                                var[c] = acc / qs;
-                            }
-                        } else {
-                            for (int c = 0; c < nvar; ++c) {
-                                float acc = vo[0].var[c] * lc[0];
-                                acc = acc + vo[1].var[c] * lc[1];
-                                acc = acc + vo[2].var[c] * lc[2];
-                                var[c] = acc;
-                            }
-                        }
-                        float color[4];
-                        run_fs(d, var, nvar, color);
-                        /* late depth test on the @location write (fragment.rs:427-488) */
-                        int pass = 1;
-                        if (p->depth && d->depth_enabled) {
-                            float *dp = p->depth + y * p->width + x;
-                            if (compare(d->depth_compare, frag_z, *dp)) {
-                                if (d->depth_write) *dp = frag_z;
-                            } else {
-                                pass = 0;
-                            }
-                        }
-                        if (pass && p->color) {
-                            if (d->blend) blend_pixel(p, (uint32_t)x, (uint32_t)y, color);
-                            else put_pixel(p, (uint32_t)x, (uint32_t)y, color);
-                        }
+/* CohenSutherland + LineRasterizer (raster.rs:230-265): Bresenham between
+ * the clipped end points; L = lerp of the clip alphas by t; the fragment is
+ * L applied to the clipped end points' (x, y, z, 1/w), the varyings L
+ * applied to the unclipped vertices'. */
+static int draw_line(OrPass *p, const OrDraw *d, const vout v[2])
+{
+    float cp[8], al[2];
+    if (!oracle_clip_line(v[0].clip, v[1].clip, cp, al)) return 0;
+    if (cp[3] == 0.0f || cp[7] == 0.0f) return fail(p, "clip position w = 0");
+    uint32_t fb[2][2];
+    float fr[2][4];
+    oracle_to_raster(d->viewport, cp, fb[0], fr[0]);
+    oracle_to_raster(d->viewport, cp + 4, fb[1], fr[1]);
+    bres_t b;
+    bres_init(&b, fb[0], fb[1]);
+    uint32_t xy[2];
+    float t;
+    int nvar = v[0].nvar;
+    while (bres_next(&b, xy, &t)) {
+        if (!in_scissor(d, xy[0], xy[1])) continue;
+        p->fragments++;
+        if (p->frag_count) p->frag_count[(uint64_t)xy[1] * p->width + xy[0]]++;
+        float s = 1.0f - t;
+        float l0 = al[0] * s;
+        float l1 = al[1] * t;
+        float L = l0 + l1;
+        float m = 1.0f - L;
+        float za = fr[0][2] * m, zb = fr[1][2] * L;
+        float z = za + zb;
+        float var[OR_MAX_VAR];
+        if (d->interp == OR_INTERP_FLAT) {
+            for (int c = 0; c < nvar; ++c) var[c] = v[0].var[c];
+        } else if (d->interp == OR_INTERP_PERSPECTIVE) {
+            /* extension: perspective-correct weights q_i = w_i(L) * (1/w_i) */
+            float q0 = m * (1.0f / v[0].clip[3]);
+            float q1 = L * (1.0f / v[1].clip[3]);
+            float qs = q0 + q1;
+            for (int c = 0; c < nvar; ++c) {
+                float acc = v[0].var[c] * q0;
+                acc = acc + v[1].var[c] * q1;
+                var[c] = acc / qs;
+            }
+        } else {
+            for (int c = 0; c < nvar; ++c) {
+                float a = v[0].var[c] * m;
+                float bb = v[1].var[c] * L;
+                var[c] = a + bb;
+            }
+        }
+        write_fragment(p, d, xy[0], xy[1], z, var, nvar);
+    }
+    return 0;
+}
+
+static int draw_triangle(OrPass *p, const OrDraw *d, const vout vo[3])
+{
+    uint64_t sx0 = d->scissor[0], sy0 = d->scissor[1];
+    uint64_t sx1 = sx0 + d->scissor[2], sy1 = sy0 + d->scissor[3];
+    /* TriFace::new (primitive.rs:169-184) */
+    float nd[3][3];
+    for (int k = 0; k < 3; ++k) {
+        if (vo[k].clip[3] == 0.0f) return fail(p, "clip position w = 0");
+        for (int c = 0; c < 3; ++c) nd[k][c] = vo[k].clip[c] / vo[k].clip[3];
+    }
+    float abx = nd[1][0] - nd[0][0], aby = nd[1][1] - nd[0][1];
+    float acx = nd[2][0] - nd[0][0], acy = nd[2][1] - nd[0][1];
+    float area_z = abx * acy - aby * acx;
+    int ccw = area_z > 0.0f;
+    int front_facing = ccw ? (d->front_face_cw == 0) : (d->front_face_cw == 1);
+    int cull = (d->cull_mode == OR_CULL_FRONT) ? front_facing
+             : (d->cull_mode == OR_CULL_BACK) ? !front_facing : 0;
+    if (cull) return 0;
+    (void)front_facing; /* the built-in programs do not read front_facing */
+
+    float pos[3][4];
+    for (int k = 0; k < 3; ++k) memcpy(pos[k], vo[k].clip, sizeof pos[k]);
+    ctri sub[CLIP_CAP];
+    int nsub = clip_triangle(pos, sub);
+    for (int t = 0; t < nsub; ++t) {
+        uint32_t fb[3][2];
+        float fr[3][4];
+        for (int k = 0; k < 3; ++k) oracle_to_raster(d->viewport, sub[t].v[k].p, fb[k], fr[k]);
+        float total = shoelace(fr[0][0], fr[0][1], fr[1][0], fr[1][1], fr[2][0], fr[2][1]);
+        if (total == 0.0f) continue;
+        scanlines_t sl;
+        scanline_t line;
+        scanlines_init(&sl, fb);
+        for (;;) {
+            int r = scanlines_next(&sl, &line);
+            if (r < 0) return fail(p, "scanline produced a negative coordinate");
+            if (r == 0) break;
+            for (int64_t xx = line.x1; xx <= line.x2; ++xx) {
+                uint64_t x = (uint64_t)xx, y = (uint64_t)line.y;
+                if (!(x >= sx0 && x < sx1 && y >= sy0 && y < sy1)) continue;
+                p->fragments++;
+                if (p->frag_count) p->frag_count[y * p->width + x]++;
+                /* barycentric at the pixel corner (raster.rs:313-335) */
+                float px = (float)x, py = (float)y;
+                float l0 = shoelace(px, py, fr[1][0], fr[1][1], fr[2][0], fr[2][1]) / total;
+                float l1 = shoelace(fr[0][0], fr[0][1], px, py, fr[2][0], fr[2][1]) / total;
+                float l2 = shoelace(fr[0][0], fr[0][1], fr[1][0], fr[1][1], px, py) / total;
+                /* compose with the clip barycentrics (raster.rs:338-343) */
+                float lc[3];
+                for (int c = 0; c < 3; ++c) {
+                    float acc = sub[t].v[0].b[c] * l0;
+                    acc = acc + sub[t].v[1].b[c] * l1;
+                    acc = acc + sub[t].v[2].b[c] * l2;
+                    lc[c] = acc;
+                }
+                /* fragment = lc . [a, b, c].fragment (raster.rs:345-346) */
+                float frag_z = fr[0][2] * lc[0];
+                frag_z = frag_z + fr[1][2] * lc[1];
+                frag_z = frag_z + fr[2][2] * lc[2];
+                /* varyings of the unclipped vertices (fragment.rs:150-152, 346-366) */
+                float var[OR_MAX_VAR];
+                int nvar = vo[0].nvar;
+                if (d->interp == OR_INTERP_FLAT) {
+                    for (int c = 0; c < nvar; ++c) var[c] = vo[0].var[c];
+                } else if (d->interp == OR_INTERP_PERSPECTIVE) {
+                    /* extension: perspective-correct weights q_i = lc_i * (1/w_i) */
+                    float q0 = lc[0] * (1.0f / vo[0].clip[3]);
+                    float q1 = lc[1] * (1.0f / vo[1].clip[3]);
+                    float q2 = lc[2] * (1.0f / vo[2].clip[3]);
+                    float qs = (q0 + q1) + q2;
+                    for (int c = 0; c < nvar; ++c) {
+                        float acc = vo[0].var[c] * q0;
+                        acc = acc + vo[1].var[c] * q1;
+                        acc = acc + vo[2].var[c] * q2;
+                        var[c] = acc / qs;
                     }
+                } else {
+                    for (int c = 0; c < nvar; ++c) {
+                        float acc = vo[0].var[c] * lc[0];
+                        acc = acc + vo[1].var[c] * lc[1];
+                        acc = acc + vo[2].var[c] * lc[2];
+                        var[c] = acc;
+                    }
+                }
+                float color[4];
+                run_fs(d, var, nvar, color);
+                /* late depth test on the @location write (fragment.rs:427-488) */
+                int pass = 1;
+                if (p->depth && d->depth_enabled) {
+                    float *dp = p->depth + y * p->width + x;
+                    if (compare(d->depth_compare, frag_z, *dp)) {
+                        if (d->depth_write) *dp = frag_z;
+                    } else {
+                        pass = 0;
+                    }
+                }
+                if (pass && p->color) {
+                    if (d->blend) blend_pixel(p, (uint32_t)x, (uint32_t)y, color);
+                    else put_pixel(p, (uint32_t)x, (uint32_t)y, color);
                 }
             }
         }
     }
     return 0;
+}
+
+static int draw(OrPass *p, const OrDraw *d)
+{
+    if (d->program == OR_PROG_MVP_TEXTURE && !d->texture) return fail(p, "no texture bound");
+    /* ToRaster scissor box (raster.rs:130-143) */
+    uint64_t sx1 = (uint64_t)d->scissor[0] + d->scissor[2], sy1 = (uint64_t)d->scissor[1] + d->scissor[3];
+    if (sx1 > p->width || sy1 > p->height) return fail(p, "scissor outside the framebuffer");
+    int n = topo_vertices(d->topology);
+    int strip = d->topology == OR_TOPO_LINE_STRIP || d->topology == OR_TOPO_TRIANGLE_STRIP;
+    if (d->strip_restart && !strip) return fail(p, "strip_index_format set, but not a strip topology");
+    /* the separators are raw index values (index.rs:72-106): read them in range */
+    const uint8_t *raw = NULL;
+    if (d->indexed && d->strip_restart) {
+        uint64_t end = ((uint64_t)d->first + d->count) * d->index_format;
+        if (end > d->index_size) return fail(p, "index out of range or vertex index overflow");
+        raw = d->index_data + (uint64_t)d->first * d->index_format;
+    }
+    int64_t cap = (int64_t)d->count + 3;
+    uint32_t *pos = (uint32_t *)malloc(sizeof(uint32_t) * 3 * (size_t)cap);
+    if (!pos) return fail(p, "out of memory");
+    int64_t nprim = oracle_assemble(d->topology, raw != NULL, d->index_format, raw, d->count, pos, cap);
+    int rc = 0;
+    for (uint32_t inst = d->first_instance; inst < d->first_instance + d->instance_count && !rc; ++inst) {
+        for (int64_t prim = 0; prim < nprim && !rc; ++prim) {
+            vout vo[3];
+            for (int k = 0; k < n && !rc; ++k) {
+                uint32_t vertex;
+                if (resolve_index(d, d->first + pos[prim * n + k], &vertex))
+                    rc = fail(p, "index out of range or vertex index overflow");
+                else if (run_vs(d, inst, vertex, &vo[k]))
+                    rc = fail(p, "vertex fetch out of range");
+            }
+            if (rc) break;
+            if (n == 1) rc = draw_point(p, d, &vo[0]);
+            else if (n == 2) rc = draw_line(p, d, vo);
+            else rc = draw_triangle(p, d, vo);
+        }
+    }
+    free(pos);
+    return rc;
 }
 
 /* RenderPassCommand::execute (render_pass/mod.rs:345-394) */

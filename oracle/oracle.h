@@ -9,7 +9,12 @@
  * Every function follows a file of the reference (paths relative to the
  * reference checkout, jgraef/wgpu-cpu):
  *   wgpu-cpu/src/render_pass/state.rs:490-593   draw loop (VS per index, cull)
- *   wgpu-cpu/src/render_pass/index.rs:45-70     index resolution
+ *   wgpu-cpu/src/render_pass/index.rs:45-106    index resolution, restart separators
+ *   wgpu-cpu/src/render_pass/primitive.rs:206-487  list / strip assembly (TriStripIter,
+ *                                                  ResetTriStripIter, line windows)
+ *   wgpu-cpu/src/render_pass/clipper.rs:207-407  Cohen-Sutherland line clipper
+ *   wgpu-cpu/src/render_pass/raster.rs:184-265   point and line rasterizers
+ *   wgpu-cpu/src/util/bresenham.rs:78-153        Bresenham iterator (end exclusive)
  *   wgpu-cpu/src/render_pass/vertex.rs:128-196  vertex attribute fetch
  *   wgpu-cpu/src/render_pass/primitive.rs:159-197  TriFace / winding
  *   wgpu-cpu/src/render_pass/clipper.rs:150-179,625-802  6-plane triangle clipper
@@ -62,6 +67,9 @@ enum { OR_CMP_NEVER = 1, OR_CMP_LESS = 2, OR_CMP_EQUAL = 3, OR_CMP_LESS_EQUAL = 
 enum { OR_ADDR_CLAMP = 0, OR_ADDR_REPEAT = 1, OR_ADDR_MIRROR = 2 };
 enum { OR_INTERP_LINEAR = 0, OR_INTERP_PERSPECTIVE = 1, OR_INTERP_FLAT = 2 };
 enum { OR_CULL_NONE = 0, OR_CULL_FRONT = 1, OR_CULL_BACK = 2 };
+/* wgpu::PrimitiveTopology discriminants (state.rs:434-455) */
+enum { OR_TOPO_POINT_LIST = 0, OR_TOPO_LINE_LIST = 1, OR_TOPO_LINE_STRIP = 2, OR_TOPO_TRIANGLE_LIST = 3,
+       OR_TOPO_TRIANGLE_STRIP = 4 };
 
 typedef struct {
     const uint8_t *data;   /* buffer slice, offset already applied */
@@ -109,6 +117,9 @@ typedef struct {
     uint32_t first, count;   /* vertices or indices */
     int32_t base_vertex;
     uint32_t first_instance, instance_count;
+    /* primitive state */
+    uint32_t topology;       /* OR_TOPO_* */
+    uint32_t strip_restart;  /* strip_index_format set: index MAX separates strips */
 } OrDraw;
 
 typedef struct {
@@ -158,6 +169,23 @@ const float *oracle_srgb_table(void);
 
 /* naga-cranelift binary.rs:270-341: column-major mat4x4f * vec4f */
 void oracle_mat4_mul_vec4(const float m[16], const float v[4], float r[4]);
+
+/* util/bresenham.rs:78-153: the points from start (inclusive) to end
+ * (exclusive) and their t; returns the number of points (at most cap written) */
+int64_t oracle_bresenham(const uint32_t start[2], const uint32_t end[2], uint32_t *out_xy, float *out_t,
+                         int64_t cap);
+
+/* clipper.rs:266-366 (Cohen-Sutherland): returns 1 and the clipped points
+ * and their alphas, or 0 when the line is outside */
+int oracle_clip_line(const float a[4], const float b[4], float out[8], float alphas[2]);
+
+/* primitive.rs:206-487 with index.rs:72-106: the index positions (relative to
+ * the first index) of every primitive of `count` indices, `n` per primitive
+ * (1, 2, 3) in assembly order.  `indices` (NULL: non-indexed) are raw values
+ * of `index_format` bytes; with `restart` the maximum value separates strips.
+ * Returns the number of primitives (at most cap written). */
+int64_t oracle_assemble(uint32_t topology, uint32_t restart, uint32_t index_format, const uint8_t *indices,
+                        uint32_t count, uint32_t *out_positions, int64_t cap);
 
 #ifdef __cplusplus
 }

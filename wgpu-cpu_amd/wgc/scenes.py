@@ -90,6 +90,8 @@ class DrawSpec:
     first_instance: int = 0
     instance_count: int = 1
     wgsl: Optional[str] = None    # WGSL source replacing the built-in program (same interface)
+    topology: int = 3             # wgpu::PrimitiveTopology: 0 points, 1 lines, 2 line strip, 3 triangles, 4 strip
+    strip_index_format: int = 0   # 0: none; 2 / 4: the index MAX value restarts strips
 
     @property
     def indexed(self) -> bool:
