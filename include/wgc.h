@@ -83,7 +83,14 @@ typedef enum WgcVertexFormat {
 
 typedef enum WgcVertexStepMode { WGC_STEP_VERTEX = 0, WGC_STEP_INSTANCE = 1 } WgcVertexStepMode;
 typedef enum WgcIndexFormat { WGC_INDEX_UINT16 = 2, WGC_INDEX_UINT32 = 4 } WgcIndexFormat;
-typedef enum WgcTopology { WGC_TOPOLOGY_TRIANGLE_LIST = 3 } WgcTopology;
+/* wgpu::PrimitiveTopology (state.rs:434-455) */
+typedef enum WgcTopology {
+    WGC_TOPOLOGY_POINT_LIST = 0,
+    WGC_TOPOLOGY_LINE_LIST = 1,
+    WGC_TOPOLOGY_LINE_STRIP = 2,
+    WGC_TOPOLOGY_TRIANGLE_LIST = 3,
+    WGC_TOPOLOGY_TRIANGLE_STRIP = 4
+} WgcTopology;
 typedef enum WgcFrontFace { WGC_FRONT_FACE_CCW = 0, WGC_FRONT_FACE_CW = 1 } WgcFrontFace;
 typedef enum WgcCullMode { WGC_CULL_NONE = 0, WGC_CULL_FRONT = 1, WGC_CULL_BACK = 2 } WgcCullMode;
 typedef enum WgcCompare {
@@ -164,6 +171,9 @@ typedef struct WgcRenderPipelineDescriptor {
      * that stage (state.rs via CompiledModule::find_entry_point). */
     const char *vertex_entry;
     const char *fragment_entry;
+    /* PrimitiveState::strip_index_format: 0 none, else the WgcIndexFormat
+     * whose maximum value restarts strips (index.rs:72-106) */
+    uint32_t strip_index_format;
 } WgcRenderPipelineDescriptor;
 
 typedef struct WgcColorAttachment {

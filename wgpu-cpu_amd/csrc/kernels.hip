@@ -117,6 +117,11 @@ static hipError_t launch_group(const DrawParams &d, const BatchArgs &b, const Gr
     uint32_t zero_n = max(max(gl.total_tiles, gl.n_counters), (uint32_t)H_NUM);
     WGC_LAUNCH("k_zero", k_zero, (zero_n + 255) / 256, 256, gl.tile_count, gl.total_tiles, gl.counters,
                gl.n_counters, gl.header);
+    if (!MULTI && d.strip_restart && d.indexed && d.topology == TOPO_TRIANGLE_STRIP && d.count > 0) {
+        uint32_t chunks = (d.count + kStripChunk - 1) / kStripChunk;
+        WGC_LAUNCH("k_strip_chunks", k_strip_chunks, (chunks + 255) / 256, 256, d, d.s.strip_sep);
+        WGC_LAUNCH("k_strip_prefix", k_strip_prefix, 1, 64, d, d.s.strip_sep);
+    }
     if (gl.total_prims > 0) {
         uint32_t g = (gl.total_prims + 255u) / 256u;
         WGC_LAUNCH("k_setup", (k_setup<PROG, MULTI>), g, 256, d, b);
@@ -184,6 +189,11 @@ static hipError_t launch_user_t(const UserKernels &k, const DrawParams &d0, cons
     uint32_t zero_n = max(max(gl.total_tiles, gl.n_counters), (uint32_t)H_NUM);
     WGC_LAUNCH("k_zero", k_zero, (zero_n + 255) / 256, 256, gl.tile_count, gl.total_tiles, gl.counters,
                gl.n_counters, gl.header);
+    if (!MULTI && d.strip_restart && d.indexed && d.topology == TOPO_TRIANGLE_STRIP && d.count > 0) {
+        uint32_t chunks = (d.count + kStripChunk - 1) / kStripChunk;
+        WGC_LAUNCH("k_strip_chunks", k_strip_chunks, (chunks + 255) / 256, 256, d, d.s.strip_sep);
+        WGC_LAUNCH("k_strip_prefix", k_strip_prefix, 1, 64, d, d.s.strip_sep);
+    }
     if (gl.total_prims > 0) {
         uint32_t g = (gl.total_prims + 255u) / 256u;
         WGC_MLAUNCH("k_setup", k.setup[m], g, 256, &d, &b);
@@ -273,6 +283,61 @@ hipError_t launch_fill(void *dst, uint64_t bytes, uint32_t value, hipStream_t st
     uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 65536);
     WGC_LAUNCH("k_fill", k_fill_words, grid, 256, (uint32_t *)dst, n, value);
     return hipSuccess;
+}
+
+// ---- points and lines ----
+
+template <int PROG, int N>
+static hipError_t launch_lines_t(const DrawParams &d, const GroupLaunch &gl, hipStream_t st, KernelTimer *kt)
+{
+    uint32_t g = (d.total_prims + 255u) / 256u;
+    if (gl.ordered) {
+        WGC_LAUNCH("k_lp_ordered", (k_lp_ordered<PROG, N>), 1, 256, d);
+        return hipSuccess;
+    }
+    if (!g) return hipSuccess;
+    hipError_t e = launch_fill(gl.vis, 8ull * d.width * d.height, 0xFFFFFFFFu, st, kt);
+    if (e != hipSuccess) return e;
+    WGC_LAUNCH("k_lp_visibility", (k_lp_visibility<PROG, N>), g, 256, d, gl.vis);
+    WGC_LAUNCH("k_lp_shade", (k_lp_shade<PROG, N>), g, 256, d, (const unsigned long long *)gl.vis);
+    return hipSuccess;
+}
+
+template <int PROG>
+static hipError_t launch_lines_p(const DrawParams &d, const GroupLaunch &gl, hipStream_t st, KernelTimer *kt)
+{
+    return d.topology == TOPO_POINT_LIST ? launch_lines_t<PROG, 1>(d, gl, st, kt) : launch_lines_t<PROG, 2>(d, gl, st, kt);
+}
+
+static hipError_t launch_lines_user(const UserKernels &k, const DrawParams &d0, const GroupLaunch &gl, hipStream_t st,
+                                    KernelTimer *kt)
+{
+    DrawParams d = d0;
+    unsigned long long *vis = gl.vis;
+    const int n = d.topology == TOPO_POINT_LIST ? 0 : 1;
+    uint32_t g = (d.total_prims + 255u) / 256u;
+    if (gl.ordered) {
+        WGC_MLAUNCH("k_lp_ordered", k.lp_ordered[n], 1, 256, &d);
+        return hipSuccess;
+    }
+    if (!g) return hipSuccess;
+    hipError_t e = launch_fill(gl.vis, 8ull * d.width * d.height, 0xFFFFFFFFu, st, kt);
+    if (e != hipSuccess) return e;
+    WGC_MLAUNCH("k_lp_visibility", k.lp_vis[n], g, 256, &d, &vis);
+    WGC_MLAUNCH("k_lp_shade", k.lp_shade[n], g, 256, &d, &vis);
+    return hipSuccess;
+}
+
+hipError_t launch_lines(const DrawParams &d, const GroupLaunch &gl, hipStream_t st, KernelTimer *kt)
+{
+    if (gl.user) return launch_lines_user(*gl.user, d, gl, st, kt);
+    switch (d.program) {
+    case PROG_MVP_COLOR: return launch_lines_p<PROG_MVP_COLOR>(d, gl, st, kt);
+    case PROG_MVP_TEXTURE: return launch_lines_p<PROG_MVP_TEXTURE>(d, gl, st, kt);
+    case PROG_COLORED_TRIANGLE: return launch_lines_p<PROG_COLORED_TRIANGLE>(d, gl, st, kt);
+    case PROG_CLIP_COLOR: return launch_lines_p<PROG_CLIP_COLOR>(d, gl, st, kt);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t st)

@@ -781,6 +781,14 @@ __device__ __forceinline__ void frag_position(const DrawParams &d, const TriLds 
     p[3] = w + iw[2] * lc[2];
 }
 
+// Position provider of a triangle fragment for WGSL fragment shaders.
+struct TriPos {
+    const DrawParams &d;
+    const TriLds &T;
+    const float *lc;
+    __device__ void operator()(float p[4]) const { frag_position(d, T, lc, p); }
+};
+
 // Varyings (fragment.rs:319-377) and the fragment shader.  Returns false
 // when the fragment writes nothing (a WGSL shader discarded it or aborted).
 template <int PROG>
@@ -811,7 +819,7 @@ __device__ __forceinline__ bool fragment_color(const DrawParams &d, const TriLds
                 var[c] = acc;
             }
         }
-        uint32_t rc = user_fragment(d, T, lc, var, color);
+        uint32_t rc = user_fragment(d, TriPos{d, T, lc}, T.ff != 0u, var, color);
         if (rc == 0u) return true;
         if (rc != 3u) atomicOr(d.s.error, (uint32_t)ERR_SHADER);   // 3: discard
         return false;
@@ -1626,5 +1634,499 @@ __global__ void k_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, 
     if (depth) depth[i] = dv;
 }
 
+
+
+// ------------------------------------------------------------------
+// Restart triangle strips: separator summary per chunk of positions
+// ------------------------------------------------------------------
+
+// Last separator position inside each chunk (-1: none); one thread per chunk.
+__global__ __launch_bounds__(256) void k_strip_chunks(DrawParams d, int32_t *last)
+{
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n_chunks = (d.count + kStripChunk - 1) / kStripChunk;
+    if (c >= n_chunks) return;
+    uint32_t sep = d.index_format == 2u ? 0xFFFFu : 0xFFFFFFFFu;
+    int32_t found = -1;
+    uint32_t end = min(d.count, (c + 1) * kStripChunk);
+    for (uint32_t i = c * kStripChunk; i < end; ++i) {
+        uint32_t idx;
+        if (raw_index(d, d.first + i, idx) && idx == sep) found = (int32_t)i;
+    }
+    last[c] = found;
+}
+
+// Inclusive prefix maximum over the chunks (one thread: a few thousand
+// chunks at most per draw).
+__global__ void k_strip_prefix(DrawParams d, int32_t *last)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t n_chunks = (d.count + kStripChunk - 1) / kStripChunk;
+    int32_t m = -1;
+    for (uint32_t c = 0; c < n_chunks; ++c) {
+        m = max(m, last[c]);
+        last[c] = m;
+    }
+}
+
+// ------------------------------------------------------------------
+// Points and lines (raster.rs:184-265, clipper.rs:207-407,
+// util/bresenham.rs:78-153)
+// ------------------------------------------------------------------
+
+// Cohen-Sutherland (clipper.rs:266-366); false when nothing remains.  The
+// second end point is updated from the already updated first one, as the
+// reference does.
+__device__ __forceinline__ bool clip_line(const float *a, const float *b, float out[2][4], float al[2])
+{
+    float da[6], db[6];
+    uint32_t oa = 0, ob = 0;
+#pragma unroll
+    for (int pl = 0; pl < 6; ++pl) {
+        da[pl] = clip_distance(pl, a);
+        db[pl] = clip_distance(pl, b);
+        oa |= (da[pl] < 0.0f) ? (1u << pl) : 0u;
+        ob |= (db[pl] < 0.0f) ? (1u << pl) : 0u;
+    }
+    al[0] = 0.0f;
+    al[1] = 1.0f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        out[0][c] = a[c];
+        out[1][c] = b[c];
+    }
+    if ((oa | ob) == 0u) return true;
+    if (oa & ob) return false;
+    for (int pl = 0; pl < 6; ++pl) {
+        uint32_t bit = 1u << pl;
+        if (!((oa | ob) & bit)) continue;
+        float alpha = da[pl] / (da[pl] - db[pl]);
+        if (oa & bit) {
+            if (alpha > al[0]) al[0] = alpha;
+        } else {
+            if (alpha < al[1]) al[1] = alpha;
+        }
+        if (al[0] > al[1]) return false;
+    }
+    if (oa) {
+        float s = 1.0f - al[0];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float x = out[0][c] * s;
+            float y = out[1][c] * al[0];
+            out[0][c] = x + y;
+        }
+    }
+    if (ob) {
+        float s = 1.0f - al[1];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float x = out[0][c] * s;
+            float y = out[1][c] * al[1];
+            out[1][c] = x + y;
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool in_scissor(const DrawParams &d, uint32_t x, uint32_t y)
+{
+    return x >= d.scissor[0] && (uint64_t)x < (uint64_t)d.scissor[0] + d.scissor[2] && y >= d.scissor[1] &&
+           (uint64_t)y < (uint64_t)d.scissor[1] + d.scissor[3];
+}
+
+// One fragment of a point or line: pixel, step (key bits), depth, the lerp
+// coefficient of the second vertex (lines) and the fragment position.
+struct LpFrag {
+    uint32_t x, y, k;
+    float z, L;
+    float pos[4];
+};
+
+// End points of a line (or the point) after clipping and to_raster.
+struct LpSetup {
+    uint32_t fb[2][2];
+    float fr[2][4];    // viewport x, y, NDC z, 1/w of the (clipped) end points
+    float al[2];
+    bool visible;
+};
+
+// Clip + to_raster of primitive v[0..N).  False raises an error bit (w == 0).
+template <int N>
+__device__ __forceinline__ bool lp_setup(const DrawParams &d, const VsOut *v, LpSetup &s)
+{
+    float cp[2][4];
+    s.visible = true;
+    if (N == 1) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) cp[0][c] = v[0].clip[c];
+        s.al[0] = 0.0f;
+        s.al[1] = 1.0f;
+    } else {
+        s.visible = clip_line(v[0].clip, v[1].clip, cp, s.al);
+        if (!s.visible) return true;
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (cp[k][3] == 0.0f) {
+            atomicOr(d.s.error, (uint32_t)ERR_W_ZERO);
+            return false;
+        }
+        to_raster(d.viewport, cp[k], s.fb[k][0], s.fb[k][1], s.fr[k][0], s.fr[k][1], s.fr[k][2]);
+        s.fr[k][3] = 1.0f / cp[k][3];
+        if ((s.fb[k][0] | s.fb[k][1]) >= (1u << 30)) {
+            atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE);
+            return false;
+        }
+    }
+    return true;
+}
+
+// Bresenham iterator (bresenham.rs:78-153): start inclusive, end exclusive;
+// t is the running binary32 sum of 1/major.
+struct Bres {
+    int64_t x, y, dx, dy, x1, diff;
+    int oct;
+    float t, dt;
+    __device__ static void to0(int o, int64_t &x, int64_t &y)
+    {
+        int64_t a = x, b = y;
+        switch (o) {
+        case 0: x = a; y = b; break;
+        case 1: x = b; y = a; break;
+        case 2: x = b; y = -a; break;
+        case 3: x = -a; y = b; break;
+        case 4: x = -a; y = -b; break;
+        case 5: x = -b; y = -a; break;
+        case 6: x = -b; y = a; break;
+        default: x = a; y = -b; break;
+        }
+    }
+    __device__ static void from0(int o, int64_t &x, int64_t &y)
+    {
+        int64_t a = x, b = y;
+        switch (o) {
+        case 0: x = a; y = b; break;
+        case 1: x = b; y = a; break;
+        case 2: x = -b; y = a; break;
+        case 3: x = -a; y = b; break;
+        case 4: x = -a; y = -b; break;
+        case 5: x = -b; y = -a; break;
+        case 6: x = b; y = -a; break;
+        default: x = a; y = -b; break;
+        }
+    }
+    __device__ void init(const uint32_t s[2], const uint32_t e[2])
+    {
+        int64_t sx = s[0], sy = s[1], ex = e[0], ey = e[1];
+        int64_t ddx = ex - sx, ddy = ey - sy;
+        oct = 0;
+        if (ddy < 0) {
+            ddx = -ddx;
+            ddy = -ddy;
+            oct += 4;
+        }
+        if (ddx < 0) {
+            int64_t t0 = ddx;
+            ddx = ddy;
+            ddy = -t0;
+            oct += 2;
+        }
+        if (ddx < ddy) oct += 1;
+        to0(oct, sx, sy);
+        to0(oct, ex, ey);
+        dx = ex - sx;
+        dy = ey - sy;
+        dt = 1.0f / (float)dx;
+        x = sx;
+        y = sy;
+        x1 = ex;
+        diff = dy - dx;
+        t = 0.0f;
+    }
+    __device__ bool next(uint32_t &px, uint32_t &py, float &tt)
+    {
+        if (x >= x1) return false;
+        int64_t qx = x, qy = y;
+        if (diff >= 0) {
+            y += 1;
+            diff -= dx;
+        }
+        diff += dy;
+        tt = t;
+        t = t + dt;
+        x += 1;
+        from0(oct, qx, qy);
+        px = (uint32_t)qx;
+        py = (uint32_t)qy;
+        return true;
+    }
+};
+
+// Calls f(frag) for every in-scissor fragment of a set-up point or line, in
+// rasterizer order; false (error bit raised) when a line is too long for
+// the fragment key.
+template <int N, class F>
+__device__ __forceinline__ bool lp_walk(const DrawParams &d, const LpSetup &s, F &&f)
+{
+    if (!s.visible) return true;
+    LpFrag fr;
+    if (N == 1) {
+        if (!in_scissor(d, s.fb[0][0], s.fb[0][1])) return true;
+        fr.x = s.fb[0][0];
+        fr.y = s.fb[0][1];
+        fr.k = 0u;
+        fr.z = s.fr[0][2];
+        fr.L = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) fr.pos[c] = s.fr[0][c];
+        f(fr);
+        return true;
+    }
+    Bres b;
+    b.init(s.fb[0], s.fb[1]);
+    uint32_t k = 0;
+    float t;
+    while (b.next(fr.x, fr.y, t)) {
+        uint32_t step = k++;
+        if (step >= kLineSteps) {   // also bounds the walk of lines far outside the scissor
+            atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE);
+            return false;
+        }
+        if (!in_scissor(d, fr.x, fr.y)) continue;
+        // raster.rs:251-254: L = lerp of the clip alphas; the fragment is L
+        // applied to the clipped end points
+        float u = 1.0f - t;
+        float l0 = s.al[0] * u;
+        float l1 = s.al[1] * t;
+        float L = l0 + l1;
+        float m = 1.0f - L;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float a = s.fr[0][c] * m;
+            float bb = s.fr[1][c] * L;
+            fr.pos[c] = a + bb;
+        }
+        fr.z = fr.pos[2];
+        fr.L = L;
+        fr.k = step;
+        f(fr);
+    }
+    return true;
+}
+
+// Varyings of a point (the vertex's own) or line fragment (lerp by L of the
+// unclipped vertices; flat: the first; perspective: the extension).
+template <int PROG, int N>
+__device__ __forceinline__ void lp_varyings(const DrawParams &d, const VsOut *v, float L, float var[kMaxVar])
+{
+    if (N == 1) {
+#pragma unroll
+        for (int c = 0; c < kMaxVar; ++c) var[c] = v[0].var[c];
+        return;
+    }
+    float m = 1.0f - L;
+    float q0 = m * (1.0f / v[0].clip[3]);
+    float q1 = L * (1.0f / v[1].clip[3]);
+    float qs = q0 + q1;
+#pragma unroll
+    for (int c = 0; c < kMaxVar; ++c) {
+        bool flat, persp;
+        if constexpr (PROG == PROG_USER) {
+            flat = (d.var_flat_mask >> c) & 1u;
+            persp = (d.var_persp_mask >> c) & 1u;
+        } else {
+            flat = d.interp == INTERP_FLAT;
+            persp = d.interp == INTERP_PERSPECTIVE;
+        }
+        if (flat) {
+            var[c] = v[0].var[c];
+        } else if (persp) {
+            float acc = v[0].var[c] * q0;
+            acc = acc + v[1].var[c] * q1;
+            var[c] = acc / qs;
+        } else {
+            float a = v[0].var[c] * m;
+            float b = v[1].var[c] * L;
+            var[c] = a + b;
+        }
+    }
+}
+
+struct VecPos {
+    const float *p;
+    __device__ void operator()(float o[4]) const
+    {
+        for (int c = 0; c < 4; ++c) o[c] = p[c];
+    }
+};
+
+// Fragment shader of a point / line fragment (front_facing is true for
+// them, state.rs:560-561).  False: nothing is written.
+template <int PROG, int N>
+__device__ __forceinline__ bool lp_color(const DrawParams &d, const VsOut *v, const LpFrag &f, float color[4])
+{
+    float var[kMaxVar];
+    lp_varyings<PROG, N>(d, v, f.L, var);
+#ifdef WGC_USER_PROGRAM
+    if constexpr (PROG == PROG_USER) {
+        uint32_t rc = user_fragment(d, VecPos{f.pos}, true, var, color);
+        if (rc == 0u) return true;
+        if (rc != 3u) atomicOr(d.s.error, (uint32_t)ERR_SHADER);
+        return false;
+    }
+#endif
+    run_fs<PROG>(d, var, color);
+    return true;
+}
+
+__device__ __forceinline__ uint32_t lp_key(uint32_t p, uint32_t k, int n) { return n == 1 ? p : (p << 13) | k; }
+
+__device__ __forceinline__ void lp_count_fragment(const DrawParams &d, uint32_t pix)
+{
+    if (d.s.frag_px) atomicAdd(&d.s.frag_px[pix], 1u);
+}
+
+// Pass 1 of an order-independent point / line draw: every fragment that
+// passes the depth test against the depth before the draw proposes
+// (depth, key) for its pixel (the same encoding as the triangle path).
+template <int PROG, int N>
+__global__ __launch_bounds__(256) void k_lp_visibility(DrawParams d, unsigned long long *vis)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long frags = 0;
+    if (p < d.total_prims) {
+        VsOut v[N];
+        LpSetup s;
+        if (shade_primitive<PROG, N>(d, p, v) && lp_setup<N>(d, v, s)) {
+            lp_walk<N>(d, s, [&](const LpFrag &f) {
+                uint32_t pix = f.y * d.width + f.x;
+                ++frags;
+                lp_count_fragment(d, pix);
+                uint32_t key = lp_key(p, f.k, N);
+                if (d.resolve == RES_LAST) {
+                    atomicMin(&vis[pix], (unsigned long long)(~key));
+                } else if (oi_pass(d, f.z, d.depth[pix])) {
+                    atomicMin(&vis[pix], oi_value(d.resolve, f.z, key));
+                }
+            });
+        }
+    }
+    // fragment statistics: one atomic per wave
+    unsigned long long tot = frags;
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    if (lane_id() == 0 && tot) atomicAdd(&d.s.frags[(blockIdx.x * 4 + threadIdx.x / 64) % kFragSlots], tot);
+}
+
+// Pass 2: the winning fragment of each pixel runs the fragment shader and
+// writes colour (and depth).
+template <int PROG, int N>
+__global__ __launch_bounds__(256) void k_lp_shade(DrawParams d, const unsigned long long *vis)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.total_prims) return;
+    VsOut v[N];
+    LpSetup s;
+    if (!shade_primitive<PROG, N>(d, p, v) || !lp_setup<N>(d, v, s)) return;
+    lp_walk<N>(d, s, [&](const LpFrag &f) {
+        uint32_t pix = f.y * d.width + f.x;
+        uint32_t key = lp_key(p, f.k, N);
+        unsigned long long want = d.resolve == RES_LAST ? (unsigned long long)(~key) : oi_value(d.resolve, f.z, key);
+        if (vis[pix] != want) return;
+        float color[4];
+        (void)lp_color<PROG, N>(d, v, f, color);
+        if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = pack_color(color, d.color_bgra);
+        if (d.depth_test && d.depth_write) d.depth[pix] = f.z;
+    });
+}
+
+// Ordered points / lines (blending, NotEqual + write, or a fragment shader
+// that may discard): one workgroup takes the primitives in API order; thread
+// 0 sets up each primitive and lists its fragments, all threads shade them
+// (the fragments of one primitive are distinct pixels), in batches.
+constexpr int kLpBatch = 512;
+
+template <int PROG, int N>
+__global__ __launch_bounds__(256) void k_lp_ordered(DrawParams d)
+{
+    __shared__ VsOut v[N];
+    __shared__ LpSetup s;
+    __shared__ LpFrag fr[kLpBatch];
+    __shared__ int ok, nf, done;
+    __shared__ Bres b;
+    __shared__ uint32_t step;
+    unsigned long long frags = 0;
+    for (uint32_t p = 0; p < d.total_prims; ++p) {
+        if (threadIdx.x == 0) {
+            ok = shade_primitive<PROG, N>(d, p, v) && lp_setup<N>(d, v, s) && s.visible;
+            done = 0;
+            if (ok && N == 2) b.init(s.fb[0], s.fb[1]);
+            step = 0;
+        }
+        __syncthreads();
+        while (ok && !done) {
+            if (threadIdx.x == 0) {
+                nf = 0;
+                if (N == 1) {
+                    lp_walk<N>(d, s, [&](const LpFrag &f) { fr[nf++] = f; });
+                    done = 1;
+                } else {
+                    uint32_t x, y;
+                    float t;
+                    while (nf < kLpBatch) {
+                        if (!b.next(x, y, t)) {
+                            done = 1;
+                            break;
+                        }
+                        uint32_t k = step++;
+                        if (k >= kLineSteps) {
+                            atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE);
+                            done = 1;
+                            break;
+                        }
+                        if (!in_scissor(d, x, y)) continue;
+                        float u = 1.0f - t;
+                        float l0 = s.al[0] * u;
+                        float l1 = s.al[1] * t;
+                        float L = l0 + l1;
+                        float m = 1.0f - L;
+                        LpFrag f;
+                        f.x = x;
+                        f.y = y;
+                        f.k = k;
+                        for (int c = 0; c < 4; ++c) {
+                            float a = s.fr[0][c] * m;
+                            float bb = s.fr[1][c] * L;
+                            f.pos[c] = a + bb;
+                        }
+                        f.z = f.pos[2];
+                        f.L = L;
+                        fr[nf++] = f;
+                    }
+                }
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+                const LpFrag &f = fr[i];
+                uint32_t pix = f.y * d.width + f.x;
+                ++frags;
+                lp_count_fragment(d, pix);
+                if (d.depth_test && !compare(d.depth_compare, f.z, d.depth[pix])) continue;
+                float color[4];
+                if (!lp_color<PROG, N>(d, v, f, color)) continue;
+                if (d.depth_test && d.depth_write) d.depth[pix] = f.z;
+                if (d.color) {
+                    uint32_t *cp = reinterpret_cast<uint32_t *>(d.color) + pix;
+                    *cp = d.blend ? blend_over(color, *cp, d.color_bgra) : pack_color(color, d.color_bgra);
+                }
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+    }
+    unsigned long long tot = frags;
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    if (lane_id() == 0 && tot) atomicAdd(&d.s.frags[(threadIdx.x / 64) % kFragSlots], tot);
+}
 
 }  // namespace wgc

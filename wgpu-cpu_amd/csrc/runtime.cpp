@@ -361,7 +361,7 @@ struct DrawWork {
     const PassCommand *pass;
 };
 
-enum class StepKind { Group, Clear, Copy };
+enum class StepKind { Group, Clear, Copy, Lines };
 
 struct Step {
     StepKind kind = StepKind::Group;
@@ -1427,8 +1427,13 @@ extern "C" int32_t wgc_render_pipeline_create(WgcDevice device, const WgcRenderP
                                               WgcRenderPipeline *out)
 {
     if (!device || !desc || !out || !desc->vertex_module) return set_error(WGC_ERR_INVALID, "null argument");
-    if (desc->topology != WGC_TOPOLOGY_TRIANGLE_LIST)
-        return set_error(WGC_ERR_UNSUPPORTED, "only TriangleList is implemented");
+    if (desc->topology > WGC_TOPOLOGY_TRIANGLE_STRIP) return set_error(WGC_ERR_INVALID, "bad topology");
+    if (desc->strip_index_format != 0 && desc->strip_index_format != WGC_INDEX_UINT16 &&
+        desc->strip_index_format != WGC_INDEX_UINT32)
+        return set_error(WGC_ERR_INVALID, "bad strip_index_format");
+    if (desc->strip_index_format && desc->topology != WGC_TOPOLOGY_LINE_STRIP &&
+        desc->topology != WGC_TOPOLOGY_TRIANGLE_STRIP)
+        return set_error(WGC_ERR_INVALID, "strip_index_format set, but not a strip topology (state.rs:306-310)");
     if (desc->vertex_buffer_count > (uint32_t)kMaxVb) return set_error(WGC_ERR_UNSUPPORTED, "too many vertex buffers");
     bool user = desc->vertex_module->program == PROG_USER;
     if (desc->fragment_module && (desc->fragment_module->program == PROG_USER) != user)
@@ -2003,7 +2008,22 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
     d.count = c.count;
     d.first_instance = c.first_instance;
     d.instance_count = c.instance_count;
-    d.nprim = c.count / 3u;
+    // primitives per instance (primitive.rs:206-487); with restart an upper
+    // bound, the primitives that span a separator are skipped on the device
+    d.topology = pd.topology;
+    d.strip_restart = pd.strip_index_format ? 1u : 0u;
+    if (d.indexed && d.strip_restart && pd.strip_index_format != st.ib.format)
+        return set_error(WGC_ERR_INVALID, "strip_index_format differs from the index format (state.rs:311)");
+    switch (d.topology) {
+    case TOPO_POINT_LIST: d.nprim = c.count; break;
+    case TOPO_LINE_LIST: d.nprim = c.count / 2u; break;
+    case TOPO_LINE_STRIP: d.nprim = c.count >= 2u ? c.count - 1u : 0u; break;
+    case TOPO_TRIANGLE_STRIP: d.nprim = c.count >= 3u ? c.count - 2u : 0u; break;
+    default: d.nprim = c.count / 3u; break;
+    }
+    if (d.topology <= TOPO_LINE_STRIP && d.topology != TOPO_POINT_LIST &&
+        (uint64_t)d.nprim * c.instance_count > (1ull << 19))
+        return set_error(WGC_ERR_UNSUPPORTED, "more than 2^19 lines in one draw");
     uint64_t total = (uint64_t)d.nprim * c.instance_count;
     if (total >= (1ull << 25)) return set_error(WGC_ERR_UNSUPPORTED, "more than 2^25 primitives in one draw");
     d.total_prims = (uint32_t)total;
@@ -2087,6 +2107,9 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
     // the ordered (sorted, sequential) raster exists for 16x16 tiles only
     // the ordered raster and WGSL programs use 16x16 tiles
     d.tile_shift = (d.resolve == RES_ORDERED || pl->user) ? 4u : tile_shift;
+    // points and lines: unsigned pixel indices (their keys and visibility buffer)
+    if (d.topology <= TOPO_LINE_STRIP && (uint64_t)w * h > 0xFFFFFFFFull)
+        return set_error(WGC_ERR_UNSUPPORTED, "target too large for points and lines");
     tile_shift = d.tile_shift;
     d.tiles_x = (w + (1u << tile_shift) - 1) >> tile_shift;
     d.tiles_y = (h + (1u << tile_shift) - 1) >> tile_shift;
@@ -2117,6 +2140,28 @@ static uint32_t resident_grid(WgcDevice_T *dev, uint32_t per_cu, uint64_t items)
 // bins, the clip list, the header and per-draw counters.  Per draw: triangle
 // records by primitive, clipped sub-triangle records and their clip data,
 // the big-triangle list and the first clip slot per primitive.
+// A triangle strip drawn with restart separators: needs the separator
+// summary pre-pass and is never merged into a batched group.
+static bool restart_strip(const DrawParams &d)
+{
+    return d.topology == TOPO_TRIANGLE_STRIP && d.strip_restart && d.indexed;
+}
+
+static bool is_point_or_line(const DrawParams &d) { return d.topology <= TOPO_LINE_STRIP; }
+
+// Scratch of a point / line draw: the per-pixel visibility values.
+static size_t layout_lines(Step &st, uint8_t *base, uint32_t *frag_px, size_t frag_px_size, WgcDevice_T *dev)
+{
+    DrawParams &d = st.draws[0];
+    size_t bytes = d.resolve == RES_ORDERED ? 256 : align_up(8ull * d.width * d.height, 256);
+    if (!base) return bytes;
+    st.gl.vis = d.resolve == RES_ORDERED ? nullptr : (unsigned long long *)base;
+    d.s.frags = dev->d_frags;
+    d.s.error = dev->d_error;
+    d.s.frag_px = (frag_px && frag_px_size == (size_t)d.width * d.height * 4u) ? frag_px : nullptr;
+    return bytes;
+}
+
 static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t frag_px_size, WgcDevice_T *dev)
 {
     uint32_t n = (uint32_t)st.draws.size();
@@ -2149,10 +2194,11 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
     // pixel of every tile that may be active
     uint64_t tile_px = st.draws.empty() ? 0 : (1ull << (2 * st.draws[0].tile_shift));
     size_t o_vis = take(st.gl.ordered ? 0 : 8ull * tile_px * total_tiles);
-    std::vector<size_t> o_big(n), o_tri(n), o_ctri(n), o_ct(n), o_cb(n);
+    std::vector<size_t> o_big(n), o_tri(n), o_ctri(n), o_ct(n), o_cb(n), o_ss(n);
     for (uint32_t i = 0; i < n; ++i) {
         const DrawParams &d = st.draws[i];
         uint64_t rc = d.total_prims + clip_cap[i];
+        o_ss[i] = take(restart_strip(d) ? 4ull * ((d.count + kStripChunk - 1) / kStripChunk) : 0);
         o_big[i] = take(16 * rc);
         o_tri[i] = take(4ull * kRecWords * d.total_prims);
         o_ctri[i] = take(4ull * kRecWords * clip_cap[i]);
@@ -2217,6 +2263,7 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
         s.clip_tri = (uint32_t *)(base + o_ctri[i]);
         s.clip_tris = (float *)(base + o_ct[i]);
         s.clip_base = (uint32_t *)(base + o_cb[i]);
+        s.strip_sep = restart_strip(d) ? (int32_t *)(base + o_ss[i]) : nullptr;
         s.counters = gl.counters + C_NUM * i;
         s.frags = dev->d_frags;
         s.error = dev->d_error;
@@ -2410,13 +2457,42 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
     uint32_t stage = 0;
     while (i < passes.size()) {
         PassWork &pw = passes[i];
-        bool batchable = dev->batching && pw.draws.size() == 1;
+        auto mergeable = [](const PassWork &q) {
+            return q.draws.size() == 1 && !is_point_or_line(q.draws[0].d) && !restart_strip(q.draws[0].d);
+        };
+        bool batchable = dev->batching && mergeable(pw);
         if (!batchable) {
             if (pw.copy) {
                 pw.copy_step.stage = stage++;
                 plan.steps.push_back(pw.copy_step);
             }
-            for (auto &dw : pw.draws) add_group(plan.steps, {dw.d}, cfg, stage++);
+            for (auto &dw : pw.draws) {
+                if (!is_point_or_line(dw.d)) {
+                    add_group(plan.steps, {dw.d}, cfg, stage++);
+                    continue;
+                }
+                // points and lines: the pass's clear (if still pending) first
+                if (dw.d.init_clear) {
+                    Step c;
+                    c.kind = StepKind::Clear;
+                    c.color = dw.d.clear_color_on ? (uint32_t *)dw.d.color : nullptr;
+                    c.rgba = dw.d.clear_rgba;
+                    c.depth = dw.d.clear_depth_on ? dw.d.depth : nullptr;
+                    c.dv = dw.d.clear_depth;
+                    c.n = (uint64_t)dw.d.width * dw.d.height;
+                    c.stage = stage++;
+                    plan.steps.push_back(c);
+                    dw.d.init_clear = 0;
+                }
+                Step l;
+                l.kind = StepKind::Lines;
+                l.draws = {dw.d};
+                l.gl.program = dw.d.program;
+                l.gl.ordered = dw.d.resolve == RES_ORDERED;
+                l.gl.user = (const UserKernels *)dw.d.user;
+                l.stage = stage++;
+                plan.steps.push_back(std::move(l));
+            }
             if (pw.clear_only) {
                 pw.clear.stage = stage++;
                 plan.steps.push_back(pw.clear);
@@ -2426,7 +2502,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         }
         std::vector<const void *> written, read;
         size_t j = i;
-        while (j < passes.size() && passes[j].draws.size() == 1) {
+        while (j < passes.size() && mergeable(passes[j])) {
             PassWork &q = passes[j];
             bool clash = false;
             for (auto *t : q.writes) {
@@ -2462,13 +2538,15 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
     size_t stage_off = 0;
     uint32_t cur_stage = UINT32_MAX;
     for (Step &s : plan.steps) {
-        if (s.kind != StepKind::Group) continue;
+        if (s.kind != StepKind::Group && s.kind != StepKind::Lines) continue;
         if (s.stage != cur_stage) {
             cur_stage = s.stage;
             stage_off = 0;
         }
         s.arena_off = stage_off;
-        stage_off = align_up(stage_off + layout_group(s, nullptr, nullptr, 0, dev), 256);
+        size_t bytes = s.kind == StepKind::Lines ? layout_lines(s, nullptr, nullptr, 0, dev)
+                                                  : layout_group(s, nullptr, nullptr, 0, dev);
+        stage_off = align_up(stage_off + bytes, 256);
         need = std::max(need, stage_off);
         if (s.multi) {
             s.table_off = tables;
@@ -2501,6 +2579,12 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         retain(dev);
     }
     for (Step &s : plan.steps) {
+        if (s.kind == StepKind::Lines) {
+            layout_lines(s, dev->arena + s.arena_off, dev->frag_px, dev->frag_px_size, dev);
+            plan.launches += s.gl.ordered ? 1 : 3;
+            plan.prims += s.draws[0].total_prims;
+            continue;
+        }
         if (s.kind != StepKind::Group) {
             plan.launches += 1;
             continue;
@@ -2547,6 +2631,7 @@ static hipError_t launch_step(WgcDevice_T *dev, const Step &s, hipStream_t st, K
         if (!s.copy_src) return launch_fill(s.copy_dst, s.row_bytes, 0u, st, kt);
         return launch_copy_rows(s.copy_src, s.src_pitch, s.copy_dst, s.dst_pitch, s.row_bytes, s.rows, st, kt);
     }
+    if (s.kind == StepKind::Lines) return launch_lines(s.draws[0], s.gl, st, kt);
     if (s.gl.user) return launch_user(*s.gl.user, s.draws[0], s.batch, s.gl, s.multi, st, kt);
     if (s.multi) return launch_batch(s.draws[0], s.batch, s.gl, st, kt);
     return launch_draw(s.draws[0], s.batch, s.gl, st, kt);

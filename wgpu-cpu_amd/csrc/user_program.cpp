@@ -331,16 +331,17 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
          "    return rc ? (uint32_t)ERR_SHADER : 0u;\n"
          "}\n";
     if (fe) {
-        g += "struct FsEnv {\n"
+        g += "template <class Pos>\n"
+             "struct FsEnv {\n"
              "    const DrawParams &d;\n"
-             "    const TriLds &T;\n"
-             "    const float *lc;\n"
+             "    const Pos &pos;\n"
+             "    bool ff;\n"
              "    const float *var;\n"
              "    float *color;\n"
              "    __device__ uint32_t vertex_index() const { return 0u; }\n"
              "    __device__ uint32_t instance_index() const { return 0u; }\n"
-             "    __device__ bool front_facing() const { return T.ff != 0u; }\n"
-             "    __device__ void position(float *p) const { frag_position(d, T, lc, p); }\n"
+             "    __device__ bool front_facing() const { return ff; }\n"
+             "    __device__ void position(float *p) const { pos(p); }\n"
              "    __device__ void in_location(uint32_t l, uint32_t *w, uint32_t n) const\n"
              "    {\n"
              "        uint32_t b = var_offset(l);\n"
@@ -355,15 +356,17 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
              "    __device__ void out_frag_depth(float) {}\n" +
              env_resources("fs_slot") +
              "};\n"
-             "__device__ __forceinline__ uint32_t user_fragment(const DrawParams &d, const TriLds &T, const float lc[3],\n"
+             "template <class Pos>\n"
+             "__device__ __forceinline__ uint32_t user_fragment(const DrawParams &d, const Pos &pos, bool front_facing,\n"
              "                                                  const float *var, float out[4])\n"
              "{\n"
-             "    FsEnv env{d, T, lc, var, out};\n"
+             "    FsEnv<Pos> env{d, pos, front_facing, var, out};\n"
              "    return " + std::string(same ? "umv::" : "umf::") + fe->symbol + "(env);\n"
              "}\n";
     } else {
-        g += "__device__ __forceinline__ uint32_t user_fragment(const DrawParams &, const TriLds &, const float *,\n"
-             "                                                  const float *, float out[4])\n"
+        g += "template <class Pos>\n"
+             "__device__ __forceinline__ uint32_t user_fragment(const DrawParams &, const Pos &, bool, const float *,\n"
+             "                                                  float out[4])\n"
              "{\n    out[0] = out[1] = out[2] = 0.0f;\n    out[3] = 1.0f;\n    return 0u;\n}\n";
     }
     g += "}  // namespace wgc\n";
@@ -374,6 +377,9 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
         "wgc::k_clip<5, false>",           "wgc::k_clip<5, true>",
         "wgc::k_shade<5, false, 16, 256>", "wgc::k_shade<5, true, 16, 256>",
         "wgc::k_raster<5, false, 16, 256>", "wgc::k_raster<5, true, 16, 256>",
+        "wgc::k_lp_visibility<5, 1>",      "wgc::k_lp_visibility<5, 2>",
+        "wgc::k_lp_shade<5, 1>",           "wgc::k_lp_shade<5, 2>",
+        "wgc::k_lp_ordered<5, 1>",         "wgc::k_lp_ordered<5, 2>",
     };
     Compiled c;
     if (!rtc_build(hip_device, src, {{"wgc_user.h", g}}, names, c, err)) return false;
@@ -386,6 +392,9 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
         k.clip[m] = c.fns[2 + m];
         k.shade[m] = c.fns[4 + m];
         k.raster[m] = c.fns[6 + m];
+        k.lp_vis[m] = c.fns[8 + m];
+        k.lp_shade[m] = c.fns[10 + m];
+        k.lp_ordered[m] = c.fns[12 + m];
         k.occ_shade[m] = occupancy(k.shade[m], 256);
         k.occ_raster[m] = occupancy(k.raster[m], 256);
         k.occ_vis[m] = visibility_blocks_per_cu(m == 1);

@@ -44,6 +44,12 @@ enum Program : uint32_t {
 };
 
 enum Interp : uint32_t { INTERP_LINEAR = 0, INTERP_PERSPECTIVE = 1, INTERP_FLAT = 2 };
+// wgpu::PrimitiveTopology (state.rs:434-455)
+enum Topology : uint32_t {
+    TOPO_POINT_LIST = 0, TOPO_LINE_LIST = 1, TOPO_LINE_STRIP = 2, TOPO_TRIANGLE_LIST = 3, TOPO_TRIANGLE_STRIP = 4,
+};
+constexpr uint32_t kStripChunk = 256;   // index positions per separator summary (restart strips)
+constexpr uint32_t kLineSteps = 8192;   // Bresenham steps per line (the fragment key holds 13 bits)
 enum Compare : uint32_t {
     CMP_NEVER = 1, CMP_LESS = 2, CMP_EQUAL = 3, CMP_LESS_EQUAL = 4,
     CMP_GREATER = 5, CMP_NOT_EQUAL = 6, CMP_GREATER_EQUAL = 7, CMP_ALWAYS = 8,
@@ -121,6 +127,8 @@ struct Scratch {
     uint32_t rec_prims;      // size of the per-primitive record region (group-wide)
     uint32_t rec_clip_cap;   // capacity of the appended region
     uint32_t big_cap, clip_list_cap, clip_tri_cap;
+    int32_t *strip_sep;      // restart triangle strips: the last separator position up to the end of
+                             // each kStripChunk of index positions (-1: none)
 };
 
 constexpr int kFragSlots = 64;  // fragment counters, spread so that atomics rarely meet
@@ -171,6 +179,7 @@ struct DrawParams {
     int32_t base_vertex;
     uint32_t indexed, first, count, first_instance, instance_count;
     uint32_t nprim, total_prims;
+    uint32_t topology, strip_restart;   // Topology; strip_index_format set (index MAX restarts strips)
     // bindings
     const float *uniform;
     const uint8_t *tex;
@@ -202,10 +211,11 @@ struct TriLds;
 // Defined by the pipeline's generated glue, compiled with the kernels:
 // returns 0 or the ErrorBits of a failed invocation.
 __device__ __forceinline__ uint32_t user_vertex(const DrawParams &d, uint32_t instance, uint32_t vertex, VsOut &o);
-// Fragment of staged triangle T at composed barycentrics lc with
-// interpolated varyings `var`.  Returns 0 (colour written to `out`), wgsl
-// ABORT_KILL (discard) or another abort code.
-__device__ __forceinline__ uint32_t user_fragment(const DrawParams &d, const TriLds &T, const float lc[3],
+// A fragment with interpolated varyings `var`; `pos(p)` writes its
+// @builtin(position).  Returns 0 (colour written to `out`), wgsl ABORT_KILL
+// (discard) or another abort code.
+template <class Pos>
+__device__ __forceinline__ uint32_t user_fragment(const DrawParams &d, const Pos &pos, bool front_facing,
                                                   const float *var, float out[4]);
 #endif
 
@@ -371,17 +381,87 @@ __device__ __forceinline__ bool resolve_index(const DrawParams &d, uint32_t i, u
     return true;
 }
 
-// Runs the vertex shader for the three vertices of primitive `p` (global
-// across instances), returns false and raises an error bit on failure.
-template <int PROG>
-__device__ __forceinline__ bool shade_primitive(const DrawParams &d, uint32_t p, VsOut v[3])
+// The raw index value at position i of the draw (restart separators are
+// raw values, index.rs:72-106); false when outside the index buffer.
+__device__ __forceinline__ bool raw_index(const DrawParams &d, uint32_t i, uint32_t &idx)
+{
+    uint64_t off = (uint64_t)i * d.index_format;
+    if (off + d.index_format > d.index_size) return false;
+    if (d.index_format == 2u) idx = *reinterpret_cast<const uint16_t *>(d.index_data + off);
+    else idx = *reinterpret_cast<const uint32_t *>(d.index_data + off);
+    return true;
+}
+
+// Index positions (relative to `first`) of primitive `prim` of one instance
+// in assembly order (primitive.rs:206-487): lists take n consecutive
+// positions; strips start at `prim`, odd triangles of a strip swap their
+// first two vertices (TriStripGenerator); with restart a primitive that
+// would span a separator does not exist, and the parity counts from the
+// start of its segment (ResetTriStripIter).  Returns 1, 0 (no primitive) or
+// -1 (an index outside the index buffer).
+template <int N>
+__device__ __forceinline__ int prim_positions(const DrawParams &d, uint32_t prim, uint32_t pos[N])
+{
+    if (d.topology == TOPO_POINT_LIST || d.topology == TOPO_LINE_LIST || d.topology == TOPO_TRIANGLE_LIST) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) pos[k] = prim * (uint32_t)N + (uint32_t)k;
+        return 1;
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) pos[k] = prim + (uint32_t)k;
+    bool odd = false;
+    if (d.strip_restart && d.indexed) {
+        uint32_t sep = d.index_format == 2u ? 0xFFFFu : 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            uint32_t idx;
+            if (!raw_index(d, d.first + pos[k], idx)) return -1;
+            if (idx == sep) return 0;
+        }
+        if (N == 3) {
+            // segment start: the position after the last separator before prim
+            uint32_t c0 = prim / kStripChunk * kStripChunk;
+            int64_t last = -1;
+            for (int64_t j = (int64_t)prim - 1; j >= (int64_t)c0; --j) {
+                uint32_t idx;
+                if (!raw_index(d, d.first + (uint32_t)j, idx)) return -1;
+                if (idx == sep) {
+                    last = j;
+                    break;
+                }
+            }
+            if (last < 0 && c0 > 0) last = d.s.strip_sep[c0 / kStripChunk - 1];
+            odd = (((int64_t)prim - (last + 1)) & 1) != 0;
+        }
+    } else {
+        odd = (prim & 1u) != 0u;
+    }
+    if (N == 3 && odd) {
+        uint32_t t = pos[0];
+        pos[0] = pos[1];
+        pos[1] = t;
+    }
+    return 1;
+}
+
+// Runs the vertex shader for the N vertices of primitive `p` (global across
+// instances); false when there is no such primitive (a restart separator) or
+// on failure, which raises an error bit.
+template <int PROG, int N = 3>
+__device__ __forceinline__ bool shade_primitive(const DrawParams &d, uint32_t p, VsOut v[N])
 {
     uint32_t inst = d.first_instance + p / d.nprim;
     uint32_t prim = p % d.nprim;
+    uint32_t pos[N];
+    int pr = prim_positions<N>(d, prim, pos);
+    if (pr <= 0) {
+        if (pr < 0) atomicOr(d.s.error, (uint32_t)ERR_INDEX);
+        return false;
+    }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < N; ++k) {
         uint32_t vertex;
-        if (!resolve_index(d, d.first + prim * 3u + (uint32_t)k, vertex)) {
+        if (!resolve_index(d, d.first + pos[k], vertex)) {
             atomicOr(d.s.error, (uint32_t)ERR_INDEX);
             return false;
         }

@@ -68,6 +68,7 @@ struct KernelTimer {
 struct UserKernels {
     hipModule_t module = nullptr;
     hipFunction_t setup[2]{}, clip[2]{}, shade[2]{}, raster[2]{};
+    hipFunction_t lp_vis[2]{}, lp_shade[2]{}, lp_ordered[2]{};   // index: points 0, lines 1
     uint32_t occ_shade[2]{}, occ_raster[2]{}, occ_vis[2]{};
 };
 
@@ -75,6 +76,10 @@ struct UserKernels {
 // `k`, the rest from the library.
 hipError_t launch_user(const UserKernels &k, const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl,
                        bool multi, hipStream_t st, KernelTimer *kt);
+// A point or line draw: visibility + shading passes over its primitives
+// (or one ordered workgroup); gl.vis holds one 64-bit value per pixel.
+hipError_t launch_lines(const DrawParams &d, const GroupLaunch &gl, hipStream_t st, KernelTimer *kt);
+
 // Resident workgroups per CU of the library's visibility kernel (16x16).
 uint32_t visibility_blocks_per_cu(bool multi);
 

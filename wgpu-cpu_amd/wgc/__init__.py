@@ -55,7 +55,11 @@ STEP_VERTEX = 0
 STEP_INSTANCE = 1
 INDEX_UINT16 = 2
 INDEX_UINT32 = 4
+POINT_LIST = 0
+LINE_LIST = 1
+LINE_STRIP = 2
 TRIANGLE_LIST = 3
+TRIANGLE_STRIP = 4
 FRONT_CCW = 0
 FRONT_CW = 1
 CULL_NONE = 0
@@ -131,7 +135,8 @@ class RenderPipelineDescriptor(C.Structure):
                 ("has_depth_stencil", C.c_uint32), ("depth_format", C.c_uint32),
                 ("depth_write_enabled", C.c_uint32), ("depth_compare", C.c_uint32),
                 ("fragment_module", C.c_void_p), ("target_format", C.c_uint32),
-                ("blend", C.c_uint32), ("vertex_entry", C.c_char_p), ("fragment_entry", C.c_char_p)]
+                ("blend", C.c_uint32), ("vertex_entry", C.c_char_p), ("fragment_entry", C.c_char_p),
+                ("strip_index_format", C.c_uint32)]
 
 
 class ColorAttachment(C.Structure):
@@ -657,7 +662,8 @@ class Device:
                                front_face=FRONT_CCW, cull_mode=CULL_NONE, depth: Optional[tuple] = None,
                                target_format=RGBA8_UNORM, blend=BLEND_REPLACE, fragment=True,
                                vertex_entry: Optional[str] = None, fragment_entry: Optional[str] = None,
-                               fragment_module: Optional[ShaderModule] = None) -> RenderPipeline:
+                               fragment_module: Optional[ShaderModule] = None, topology: int = TRIANGLE_LIST,
+                               strip_index_format: int = 0) -> RenderPipeline:
         """vertex_buffers: (array_stride, step_mode, [(format, offset, location), ...]);
         depth: (compare, write_enabled) or None.  WGSL modules: entry point
         names (None: the module's only one of that stage); `fragment_module`
@@ -677,7 +683,8 @@ class Device:
         desc.vertex_module = module.ptr
         desc.vertex_buffer_count = len(vertex_buffers)
         desc.vertex_buffers = C.cast(layouts, C.POINTER(VertexBufferLayout))
-        desc.topology = TRIANGLE_LIST
+        desc.topology = topology
+        desc.strip_index_format = strip_index_format
         desc.front_face = front_face
         desc.cull_mode = cull_mode
         if depth is not None:

@@ -66,10 +66,11 @@ def record_pass(device: Device, enc, spec: PassSpec, res: _PassResources, keep: 
                             lambda: device.create_shader_module_builtin(d.program, d.interp))
         layouts = [(d.vertex_stride, STEP_VERTEX, d.attributes)] if d.vertex_data else []
         pkey = ("pipeline", d.program, d.interp, d.wgsl, d.vertex_stride, tuple(d.attributes), d.front_face,
-                d.cull_mode, d.depth, spec.color_format, d.blend)
+                d.cull_mode, d.depth, spec.color_format, d.blend, d.topology, d.strip_index_format)
         pipeline = cached(pkey, lambda: device.create_render_pipeline(
             module, layouts, front_face=d.front_face, cull_mode=d.cull_mode, depth=d.depth,
-            target_format=spec.color_format, blend=d.blend))
+            target_format=spec.color_format, blend=d.blend, topology=d.topology,
+            strip_index_format=d.strip_index_format))
         rp.set_pipeline(pipeline)
         if d.uniform is not None:
             ub = cached(("buf", d.uniform), lambda: device.create_buffer_init(d.uniform))

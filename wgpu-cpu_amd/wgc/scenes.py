@@ -360,12 +360,13 @@ def config3_overdraw(seed: int = SEED_BASE + 3, blend: bool = False, n: int = 20
                     name="c3_overdraw_" + ("blend" if blend else "ref"))
 
 
-def colored_triangle(front_face: int, cull_mode: int, vertices: int = 3) -> PassSpec:
+def colored_triangle(front_face: int, cull_mode: int, vertices: int = 3, topology: int = 3) -> PassSpec:
     """wgpu-cpu-tests/src/tests/colored_triangle.rs:11-138: 512x512
     Rgba8UnormSrgb, clear black, Depth32Float Less (clear 1.0), the
-    vertex_index-only shader."""
+    vertex_index-only shader.  colored_triangle_lines (:188-198) is
+    LineStrip over vertices 0..4."""
     draw = DrawSpec(program=PROGRAM_COLORED_TRIANGLE, front_face=front_face, cull_mode=cull_mode,
-                    depth=(LESS, True), count=vertices)
+                    depth=(LESS, True), count=vertices, topology=topology)
     return PassSpec(512, 512, RGBA8_UNORM_SRGB, clear_color=(0.0, 0.0, 0.0, 1.0), has_depth=True,
                     clear_depth=1.0, draws=[draw], name="colored_triangle")
 
