@@ -25,8 +25,7 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "wgpu-cpu_amd"))
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["k_raster", "k_visibility", "k_shade", "k_setup", "k_bin_fill", "k_scan_blocks", "k_scan_totals", "k_scan_tiles", "k_clip",
-           "k_zero", "k_clear"]
+KERNELS = ["k_raster", "k_tiles", "k_setup", "k_clip", "k_zero", "k_clear"]
 
 
 def make_spec(name: str):
@@ -297,7 +296,7 @@ def run_batched(devs, args):
     dev.set_option(wgc.OPTION_BATCHING, 1)
     dev.set_option(wgc.OPTION_GRAPHS, 1)
     alg = batched_algorithmic_bytes(specs)
-    raster = prof.get("k_visibility")
+    raster = prof.get("k_tiles")
     return {
         "workload": "256 passes, W,H log-uniform [64,1024], spheres of 100-50k triangles, half textured "
                     "(one shared 1024^2 texture), Rgba8Unorm + Depth32Float Less",
@@ -309,7 +308,7 @@ def run_batched(devs, args):
         "speedup_vs_plain": round(ms_plain / ms_batched, 2),
         "algorithmic_GBps": round(alg / (ms_batched * 1e-3) / 1e9, 1),
         "hbm_frac_of_submission": round(alg / (ms_batched * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "visibility_kernel_hbm_frac": round(alg / (raster["avg_us"] * 1e-6 * raster["launches"] / 2) / 1e9 / HBM_PEAK_GBS, 4)
+        "tile_kernel_hbm_frac": round(alg / (raster["avg_us"] * 1e-6 * raster["launches"] / 2) / 1e9 / HBM_PEAK_GBS, 4)
         if raster else None,
         "autotune_ms": {wgc.RASTER_TILES[k].__repr__(): round(v, 4) for k, v in tune.items()},
         "raster_tile": wgc.RASTER_TILES[best],

@@ -20,15 +20,6 @@ namespace wgc {
         if (e_ != hipSuccess) return e_;                                                \
     } while (0)
 
-template <int PROG, bool MULTI, int TS, int NT>
-static hipError_t launch_oi(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
-                            KernelTimer *kt)
-{
-    WGC_LAUNCH("k_visibility", (k_visibility<MULTI, TS, NT>), gl.raster_grid, NT, d, b, gl.vis);
-    WGC_LAUNCH("k_shade", (k_shade<PROG, MULTI, TS, NT>), gl.shade_grid, NT, d, b, gl.vis);
-    return hipSuccess;
-}
-
 template <int PROG, bool MULTI>
 static hipError_t launch_raster(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
                                 KernelTimer *kt)
@@ -38,11 +29,12 @@ static hipError_t launch_raster(const DrawParams &d, const BatchArgs &b, const G
         return hipSuccess;
     }
     switch (gl.raster_cfg) {
-    case RASTER_8x8_64: return launch_oi<PROG, MULTI, 8, 64>(d, b, gl, st, kt);
-    case RASTER_16x16_64: return launch_oi<PROG, MULTI, 16, 64>(d, b, gl, st, kt);
-    case RASTER_32x32_256: return launch_oi<PROG, MULTI, 32, 256>(d, b, gl, st, kt);
-    default: return launch_oi<PROG, MULTI, 16, 256>(d, b, gl, st, kt);
+    case RASTER_8x8_64: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 8, 64>), gl.raster_grid, 64, d, b); break;
+    case RASTER_16x16_64: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 16, 64>), gl.raster_grid, 64, d, b); break;
+    case RASTER_32x32_256: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 32, 256>), gl.raster_grid, 256, d, b); break;
+    default: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 16, 256>), gl.raster_grid, 256, d, b); break;
     }
+    return hipSuccess;
 }
 
 // Workgroups of a kernel that fit on one CU at once (registers, LDS, waves);
@@ -57,56 +49,44 @@ static uint32_t occupancy_of(K kernel, int threads)
 }
 
 template <int PROG, bool MULTI, int TS, int NT>
-static void occupancy_oi(uint32_t &raster, uint32_t &shade)
+static uint32_t occupancy_tiles()
 {
-    static uint32_t r = 0, s = 0;
-    if (!r) {
-        r = occupancy_of(k_visibility<MULTI, TS, NT>, NT);
-        s = occupancy_of(k_shade<PROG, MULTI, TS, NT>, NT);
-    }
-    raster = r;
-    shade = s;
+    static uint32_t r = 0;
+    if (!r) r = occupancy_of(k_tiles<PROG, MULTI, TS, NT>, NT);
+    return r;
 }
 
 template <int PROG, bool MULTI>
-static void occupancy_cfg(bool ordered, uint32_t cfg, uint32_t &raster, uint32_t &shade)
+static uint32_t occupancy_cfg(bool ordered, uint32_t cfg)
 {
     if (ordered) {
         static uint32_t r = 0;
         if (!r) r = occupancy_of(k_raster<PROG, MULTI, 16, 256>, 256);
-        raster = shade = r;
-        return;
+        return r;
     }
     switch (cfg) {
-    case RASTER_8x8_64: occupancy_oi<PROG, MULTI, 8, 64>(raster, shade); break;
-    case RASTER_16x16_64: occupancy_oi<PROG, MULTI, 16, 64>(raster, shade); break;
-    case RASTER_32x32_256: occupancy_oi<PROG, MULTI, 32, 256>(raster, shade); break;
-    default: occupancy_oi<PROG, MULTI, 16, 256>(raster, shade); break;
+    case RASTER_8x8_64: return occupancy_tiles<PROG, MULTI, 8, 64>();
+    case RASTER_16x16_64: return occupancy_tiles<PROG, MULTI, 16, 64>();
+    case RASTER_32x32_256: return occupancy_tiles<PROG, MULTI, 32, 256>();
+    default: return occupancy_tiles<PROG, MULTI, 16, 256>();
     }
 }
 
-void raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32_t cfg, uint32_t &raster,
-                          uint32_t &shade)
+uint32_t raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32_t cfg)
 {
-    raster = shade = 1;
     switch (program) {
     case PROG_MVP_COLOR:
-        if (multi) occupancy_cfg<PROG_MVP_COLOR, true>(ordered, cfg, raster, shade);
-        else occupancy_cfg<PROG_MVP_COLOR, false>(ordered, cfg, raster, shade);
-        break;
+        return multi ? occupancy_cfg<PROG_MVP_COLOR, true>(ordered, cfg) : occupancy_cfg<PROG_MVP_COLOR, false>(ordered, cfg);
     case PROG_MVP_TEXTURE:
-        if (multi) occupancy_cfg<PROG_MVP_TEXTURE, true>(ordered, cfg, raster, shade);
-        else occupancy_cfg<PROG_MVP_TEXTURE, false>(ordered, cfg, raster, shade);
-        break;
+        return multi ? occupancy_cfg<PROG_MVP_TEXTURE, true>(ordered, cfg)
+                     : occupancy_cfg<PROG_MVP_TEXTURE, false>(ordered, cfg);
     case PROG_COLORED_TRIANGLE:
-        if (multi) occupancy_cfg<PROG_COLORED_TRIANGLE, true>(ordered, cfg, raster, shade);
-        else occupancy_cfg<PROG_COLORED_TRIANGLE, false>(ordered, cfg, raster, shade);
-        break;
+        return multi ? occupancy_cfg<PROG_COLORED_TRIANGLE, true>(ordered, cfg)
+                     : occupancy_cfg<PROG_COLORED_TRIANGLE, false>(ordered, cfg);
     case PROG_CLIP_COLOR:
-        if (multi) occupancy_cfg<PROG_CLIP_COLOR, true>(ordered, cfg, raster, shade);
-        else occupancy_cfg<PROG_CLIP_COLOR, false>(ordered, cfg, raster, shade);
-        break;
-    default: break;
+        return multi ? occupancy_cfg<PROG_CLIP_COLOR, true>(ordered, cfg)
+                     : occupancy_cfg<PROG_CLIP_COLOR, false>(ordered, cfg);
+    default: return 1;
     }
 }
 
@@ -128,14 +108,6 @@ static hipError_t launch_group(const DrawParams &d, const BatchArgs &b, const Gr
         // one thread per clipped primitive at most; the count is only known on the device
         uint32_t gc = min(max(g / 4u, 1u), 1024u);
         WGC_LAUNCH("k_clip", (k_clip<PROG, MULTI>), gc, 256, d, b, gl.clip_list, gl.clip_cap);
-    }
-    WGC_LAUNCH("k_scan_blocks", k_scan_blocks<MULTI>, b.n_blocks, 256, d, b);
-    WGC_LAUNCH("k_scan_totals", k_scan_totals, 1, 256, b);
-    WGC_LAUNCH("k_scan_tiles", k_scan_tiles<MULTI>, b.n_blocks, 256, d, b);
-    if (gl.total_prims > 0) {
-        uint32_t g = min((gl.rec_prims + gl.rec_clip_cap + 255u) / 256u, 4096u);
-        WGC_LAUNCH("k_bin_fill", k_bin_fill<MULTI>, g, 256, d, b, gl.recs, gl.rec_slots, gl.rec_prims,
-                   gl.rec_clip_cap, gl.bins);
     }
     return launch_raster<PROG, MULTI>(d, b, gl, st, kt);
 }
@@ -184,7 +156,6 @@ static hipError_t launch_user_t(const UserKernels &k, const DrawParams &d0, cons
     BatchArgs b = b0;
     const uint2 *clip_list = gl.clip_list;
     uint32_t clip_cap = gl.clip_cap;
-    unsigned long long *vis = gl.vis;
     const int m = MULTI ? 1 : 0;
     uint32_t zero_n = max(max(gl.total_tiles, gl.n_counters), (uint32_t)H_NUM);
     WGC_LAUNCH("k_zero", k_zero, (zero_n + 255) / 256, 256, gl.tile_count, gl.total_tiles, gl.counters,
@@ -200,20 +171,8 @@ static hipError_t launch_user_t(const UserKernels &k, const DrawParams &d0, cons
         uint32_t gc = min(max(g / 4u, 1u), 1024u);
         WGC_MLAUNCH("k_clip", k.clip[m], gc, 256, &d, &b, &clip_list, &clip_cap);
     }
-    WGC_LAUNCH("k_scan_blocks", k_scan_blocks<MULTI>, b.n_blocks, 256, d, b);
-    WGC_LAUNCH("k_scan_totals", k_scan_totals, 1, 256, b);
-    WGC_LAUNCH("k_scan_tiles", k_scan_tiles<MULTI>, b.n_blocks, 256, d, b);
-    if (gl.total_prims > 0) {
-        uint32_t g = min((gl.rec_prims + gl.rec_clip_cap + 255u) / 256u, 4096u);
-        WGC_LAUNCH("k_bin_fill", k_bin_fill<MULTI>, g, 256, d, b, gl.recs, gl.rec_slots, gl.rec_prims,
-                   gl.rec_clip_cap, gl.bins);
-    }
-    if (gl.ordered) {
-        WGC_MLAUNCH("k_raster", k.raster[m], gl.raster_grid, 256, &d, &b);
-        return hipSuccess;
-    }
-    WGC_LAUNCH("k_visibility", (k_visibility<MULTI, 16, 256>), gl.raster_grid, 256, d, b, gl.vis);
-    WGC_MLAUNCH("k_shade", k.shade[m], gl.shade_grid, 256, &d, &b, &vis);
+    if (gl.ordered) WGC_MLAUNCH("k_raster", k.raster[m], gl.raster_grid, 256, &d, &b);
+    else WGC_MLAUNCH("k_tiles", k.tiles[m], gl.raster_grid, 256, &d, &b);
     return hipSuccess;
 }
 
@@ -221,14 +180,6 @@ hipError_t launch_user(const UserKernels &k, const DrawParams &d, const BatchArg
                        bool multi, hipStream_t st, KernelTimer *kt)
 {
     return multi ? launch_user_t<true>(k, d, b, gl, st, kt) : launch_user_t<false>(k, d, b, gl, st, kt);
-}
-
-uint32_t visibility_blocks_per_cu(bool multi)
-{
-    static uint32_t r[2] = {0, 0};
-    uint32_t &v = r[multi ? 1 : 0];
-    if (!v) v = multi ? occupancy_of(k_visibility<true, 16, 256>, 256) : occupancy_of(k_visibility<false, 16, 256>, 256);
-    return v;
 }
 
 // ---- copies (command encoder copy_* / clear_*) ----
@@ -343,6 +294,12 @@ hipError_t launch_lines(const DrawParams &d, const GroupLaunch &gl, hipStream_t 
 hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t st)
 {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, frags, error);
+    return hipGetLastError();
+}
+
+hipError_t launch_take_counters(unsigned long long *frags, uint32_t *error, void *host_out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_take_counters, dim3(1), dim3(64), 0, st, frags, error, (unsigned long long *)host_out);
     return hipGetLastError();
 }
 

@@ -1,29 +1,30 @@
 // kernels_dev.h -- device code of the render-pass executor (gfx950).
 //
-// Per launch group: k_zero -> k_setup -> k_clip -> k_scan_blocks ->
-// k_scan_totals -> k_scan_tiles -> k_bin_fill -> k_raster.  A group is one
-// draw (parameters passed by value) or a table of independent draws -- the
+// Per launch group: k_zero -> k_setup -> k_clip -> k_tiles (order-
+// independent draws) or k_raster (ordered draws).  A group is one draw
+// (parameters passed by value) or a table of independent draws -- the
 // batched form, where hundreds of render passes share every launch (MULTI).
 //
 //  k_setup     one thread per primitive: vertex shader for its three indices
 //              (state.rs:519-535), winding and culling (state.rs:540-561),
 //              the clipper's pass-through test and raster setup; writes the
-//              triangle record the raster reads and the primitive's bin
-//              record, counts it into its tiles (keeping its bin positions),
-//              and appends (ballot + prefix count, one atomic per wave) to
-//              the big-triangle list and the clip list.
+//              triangle record the raster reads and bins its key into the
+//              tiles of its footprint (positions from one atomic add per tile
+//              and group of lanes, the tile appended to the active list when
+//              it is binned first), and appends (ballot + prefix count, one
+//              atomic per wave) to the big-triangle list and the clip list.
 //  k_clip      primitives that cross a clip plane: the 6-plane clipper as an
 //              in-order depth-first walk (same output order as the
 //              reference's plane-by-plane queue, clipper.rs:756-775).
-//  k_scan_*    per-tile bin offsets (exclusive scan over all tiles of the
-//              group) and the compacted list of tiles the raster must visit.
-//  k_bin_fill  scatters record keys into per-tile lists (no atomics).
-//  k_raster    persistent workgroups, statically scheduled: first
-//              the active tiles (TSxTS pixels, PPT pixels per lane; colour and
-//              depth in registers from load or clear to one coalesced store;
-//              triangles staged 64 at a time in LDS with per-row spans of the
-//              reference's integer scanline rule), then chunks of untouched
-//              tiles that only need the LoadOp::Clear values stored.
+//  k_tiles     persistent workgroups, statically scheduled over the active
+//              tiles: the winners of each pixel (an order-independent minimum
+//              of depth and API-order key in LDS), then the winners shaded,
+//              one coalesced colour and depth store; then chunks of untouched
+//              tiles (the LoadOp::Clear values, or only big triangles).
+//  k_raster    the same schedule for ordered draws: each tile's keys sorted
+//              into API order, colour and depth in registers from load or
+//              clear to one store, triangles staged 64 at a time in LDS with
+//              per-row spans of the reference's integer scanline rule.
 //
 // Compiled twice: into the library for the built-in programs, and at
 // pipeline creation (hipRTC) together with the device functions emitted
@@ -76,37 +77,118 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
     return pred ? base + (uint32_t)__popcll(below) : 0xFFFFFFFFu;
 }
 
-// Counts a record into the bins of its footprint (row-major) and keeps its
-// position in each, so that the bins can be filled without atomics.
-__device__ __forceinline__ void add_tile_counts(const DrawParams &d, uint32_t rec, uint32_t tx0, uint32_t ty0,
-                                                uint32_t tx1, uint32_t ty1)
+// Lanes of the wave below this one.
+__device__ __forceinline__ uint64_t lanes_below()
 {
-    uint32_t *slots = d.s.rec_slots + (size_t)rec * kMaxBinTiles;
-    uint32_t k = 0;
-    for (uint32_t ty = ty0; ty <= ty1; ++ty)
-        for (uint32_t tx = tx0; tx <= tx1; ++tx) slots[k++] = atomicAdd(&d.s.tile_count[ty * d.tiles_x + tx], 1u);
+    uint32_t lane = lane_id();
+    return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
 }
 
-// add_tile_counts for a whole wave (every lane calls it).  For the k-th
-// tile of every lane's footprint (row-major, k up to the largest footprint
-// in the wave), lanes on the same tile form a group first (up to 8 groups
-// per round, in registers: neighbouring triangles of a mesh mostly share
-// tiles, and one tile may collect thousands of them); then every group's
-// first lane, and every lane left alone, takes its positions with one atomic,
-// all of them issued together.
-__device__ __forceinline__ void add_tile_counts_wave(const DrawParams &d, bool has, uint32_t rec, uint32_t tx0,
-                                                     uint32_t ty0, uint32_t tx1, uint32_t ty1)
+// Wave-aggregated atomicAdd of v (below 256) to *addr: lanes (with act)
+// that pass the same address share one atomic and receive consecutive
+// ranges in lane order.  Every active lane of the wave must call it.  The
+// prefix sums are formed bit by bit from ballots, so lanes that are not
+// active (or have left the kernel) contribute nothing.
+__device__ __forceinline__ uint32_t wave_add(uint32_t *addr, uint32_t v, bool act)
 {
-    uint32_t w = has ? tx1 - tx0 + 1u : 0u;
-    uint32_t ntl = has ? w * (ty1 - ty0 + 1u) : 0u;
-    uint32_t *slots = d.s.rec_slots + (size_t)rec * kMaxBinTiles;
     uint32_t lane = lane_id();
-    uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    for (uint32_t k = 0; k < (uint32_t)kMaxBinTiles; ++k) {
-        bool act = k < ntl;
+    uint64_t lt = lanes_below();
+    act = act && v > 0u;
+    uint64_t pending = __ballot(act);
+    uint32_t res = 0;
+    unsigned long long a = (unsigned long long)addr;
+    while (pending) {
+        int leader = __ffsll((unsigned long long)pending) - 1;
+        uint32_t lo = (uint32_t)__shfl((int)(uint32_t)a, leader);
+        uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(a >> 32), leader);
+        bool mine = act && a == (((unsigned long long)hi << 32) | lo);
+        uint64_t m = __ballot(mine);
+        uint32_t below = 0, total = 0;
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            uint64_t mb = __ballot(mine && ((v >> bit) & 1u));
+            below += (uint32_t)__popcll(mb & lt) << bit;
+            total += (uint32_t)__popcll(mb) << bit;
+        }
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(addr, total);
+        base = (uint32_t)__shfl((int)base, leader);
+        if (mine) res = base + below;
+        pending &= ~m;
+    }
+    return res;
+}
+
+// The k-th tile (row-major) of a footprint w tiles wide.
+__device__ __forceinline__ uint32_t footprint_tile(const DrawParams &d, uint32_t tx0, uint32_t ty0, uint32_t w,
+                                                   uint32_t k)
+{
+    return (ty0 + k / w) * d.tiles_x + (tx0 + k % w);
+}
+
+// One bin entry that did not fit its tile's bin: (global tile, key) to the
+// spill list.
+__device__ __forceinline__ void spill_entry(const DrawParams &d, uint32_t at, uint32_t tile, uint32_t key)
+{
+    if (at < d.s.spill_cap) d.s.spill[at] = make_uint2(d.s.tile0 + tile, key);
+    else atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
+}
+
+// A tile binned for the first time: to the active list (each tile is
+// appended once, by the record that took position 0 of its bin).
+__device__ __forceinline__ void activate_tile(const DrawParams &d, uint32_t at, uint32_t tile)
+{
+    if (at < d.s.active_cap) d.s.active_list[at] = d.s.tile0 + tile;
+    else atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
+}
+
+// Bins one record from one lane (no wave-wide operations: the clipper's
+// sub-triangles are emitted under divergent control flow).
+__device__ __forceinline__ void bin_lane(const DrawParams &d, uint32_t key, uint32_t tx0, uint32_t ty0, uint32_t tx1,
+                                         uint32_t ty1)
+{
+    for (uint32_t ty = ty0; ty <= ty1; ++ty)
+        for (uint32_t tx = tx0; tx <= tx1; ++tx) {
+            uint32_t t = ty * d.tiles_x + tx;
+            uint32_t slot = atomicAdd(&d.s.tile_count[t], 1u);
+            if (slot < d.s.bin_cap) d.s.bins[(size_t)t * d.s.bin_cap + slot] = key;
+            else spill_entry(d, atomicAdd(&d.s.header[H_SPILL], 1u), t, key);
+            if (slot == 0u) activate_tile(d, atomicAdd(&d.s.header[H_ACTIVE], 1u), t);
+        }
+}
+
+// Bins every lane's record (with `has`) into the tiles of its footprint
+// (row-major, at most kMaxBinTiles).  For the k-th tile of every lane's
+// footprint, lanes on the same tile form a group first (up to 8 groups per
+// round, in registers: neighbouring triangles of a mesh mostly share tiles,
+// and one tile may collect thousands of them); every group's first lane,
+// and every lane left alone, takes the group's bin positions with one atomic
+// add.  All the adds of the wave are issued before any result is used, so
+// the wave waits for them once.  Then the keys go to their positions, the
+// entries past a full bin to the spill list and the tiles binned for the
+// first time to the active list (one atomic per list per wave).  Every lane
+// of the wave must call it.
+__device__ __forceinline__ void bin_wave(const DrawParams &d, bool has, uint32_t key, uint32_t tx0, uint32_t ty0,
+                                         uint32_t tx1, uint32_t ty1)
+{
+    uint32_t w = has ? tx1 - tx0 + 1u : 1u;
+    uint32_t ntl = has ? w * (ty1 - ty0 + 1u) : 0u;
+    uint32_t lane = lane_id();
+    uint64_t lt = lanes_below();
+    uint32_t got[kMaxBinTiles], grp[kMaxBinTiles];
+    uint32_t nk = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxBinTiles; ++k) {
+        got[k] = 0u;
+        grp[k] = 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kMaxBinTiles; ++k) {
+        bool act = (uint32_t)k < ntl;
         uint64_t pending = __ballot(act);
         if (!pending) break;
-        uint32_t t = act ? (ty0 + k / w) * d.tiles_x + (tx0 + k % w) : 0xFFFFFFFFu;
+        nk = (uint32_t)k + 1u;
+        uint32_t t = act ? footprint_tile(d, tx0, ty0, w, (uint32_t)k) : 0xFFFFFFFFu;
         uint32_t lead = lane, cnt = 1u, rank = 0u;
         for (int it = 0; it < 8 && pending; ++it) {
             int leader = __ffsll((unsigned long long)pending) - 1;
@@ -120,10 +202,33 @@ __device__ __forceinline__ void add_tile_counts_wave(const DrawParams &d, bool h
             }
             pending &= ~m;
         }
-        uint32_t base = 0u;
-        if (act && lead == lane) base = atomicAdd(&d.s.tile_count[t], cnt);
-        base = (uint32_t)__shfl((int)base, (int)lead);
-        if (act) slots[k] = base + rank;
+        if (act && lead == lane) got[k] = atomicAdd(&d.s.tile_count[t], cnt);
+        grp[k] = act ? ((1u << 16) | (rank << 8) | lead) : 0u;
+    }
+    uint32_t first = 0u, spilled = 0u;   // bit k: the k-th tile
+#pragma unroll
+    for (int k = 0; k < kMaxBinTiles; ++k) {
+        if ((uint32_t)k >= nk) break;
+        uint32_t g = grp[k];
+        uint32_t lead = g & 0xFFu;
+        uint32_t base = (uint32_t)__shfl((int)got[k], (int)lead);
+        if (g >> 16) {
+            uint32_t slot = base + ((g >> 8) & 0xFFu);
+            uint32_t t = footprint_tile(d, tx0, ty0, w, (uint32_t)k);
+            if (slot < d.s.bin_cap) d.s.bins[(size_t)t * d.s.bin_cap + slot] = key;
+            else spilled |= 1u << k;
+            if (lead == lane && base == 0u) first |= 1u << k;
+        }
+    }
+    uint32_t nf = (uint32_t)__popc(first), ns = (uint32_t)__popc(spilled);
+    if (__ballot(nf != 0u)) {
+        uint32_t at = wave_add(&d.s.header[H_ACTIVE], nf, nf != 0u);
+        for (uint32_t m = first; m; m &= m - 1u) activate_tile(d, at++, footprint_tile(d, tx0, ty0, w, __ffs(m) - 1u));
+    }
+    if (__ballot(ns != 0u)) {
+        uint32_t at = wave_add(&d.s.header[H_SPILL], ns, ns != 0u);
+        for (uint32_t m = spilled; m; m &= m - 1u)
+            spill_entry(d, at++, footprint_tile(d, tx0, ty0, w, __ffs(m) - 1u), key);
     }
 }
 
@@ -208,6 +313,7 @@ __device__ __forceinline__ void note_big_wave(const DrawParams &d, bool big, con
     uint32_t x1 = wave_max(big ? (rec.z & 0xFFFFu) : 0u);
     uint32_t y1 = wave_max(big ? (rec.z >> 16) : 0u);
     if (lane_id() == 0) {
+        atomicOr(&d.s.header[H_BIG], 1u);
         atomicMax(&d.s.counters[C_BIG_NX0], nx0);
         atomicMax(&d.s.counters[C_BIG_NY0], ny0);
         atomicMax(&d.s.counters[C_BIG_X1], x1);
@@ -230,7 +336,9 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
     }
     const DrawParams &d = draw_at<MULTI>(d0, b, di);
     if constexpr (!MULTI) {
-        if (g >= d.total_prims) return;  // whole waves past the end only matter in the batched form
+        // whole waves past the end leave; the lanes of a partial wave stay
+        // (inactive) for the wave-wide steps
+        if (__builtin_amdgcn_readfirstlane(g) >= d.total_prims) return;
     }
     bool active = p < d.total_prims;
     bool has_rec = false, needs_clip = false;
@@ -277,9 +385,7 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
         if (slot < d.s.clip_list_cap) d.s.clip_list[slot] = make_uint2(di, p);
         else atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
     }
-    // the per-primitive record slot: an empty footprint unless binned here
-    d.s.recs[g] = small ? rec : make_uint4(0u, 0xFFFFFFFFu, 0u, di);
-    add_tile_counts_wave(d, small, g, rec.y & 0xFFFFu, rec.y >> 16, rec.z & 0xFFFFu, rec.z >> 16);
+    bin_wave(d, small, rec.x, rec.y & 0xFFFFu, rec.y >> 16, rec.z & 0xFFFFu, rec.z >> 16);
     slot = wave_append(&d.s.counters[C_BIG], big);
     if (big) {
         if (slot < d.s.big_cap) d.s.big[slot] = rec;
@@ -352,45 +458,11 @@ __device__ int clip_walk(const CTri &root, F &&emit)
     return leaf;
 }
 
-// Wave-aggregated atomicAdd of v to *addr: lanes (with act) that pass the
-// same address share one atomic and receive consecutive ranges in lane
-// order.  Every lane of the wave must call it.
-__device__ __forceinline__ uint32_t wave_add(uint32_t *addr, uint32_t v, bool act)
-{
-    uint32_t lane = lane_id();
-    act = act && v > 0u;
-    uint64_t pending = __ballot(act);
-    uint32_t res = 0;
-    unsigned long long a = (unsigned long long)addr;
-    while (pending) {
-        int leader = __ffsll((unsigned long long)pending) - 1;
-        uint32_t lo = (uint32_t)__shfl((int)(uint32_t)a, leader);
-        uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(a >> 32), leader);
-        bool mine = act && a == (((unsigned long long)hi << 32) | lo);
-        uint64_t m = __ballot(mine);
-        // inclusive scan of v over this group's lanes
-        uint32_t x = mine ? v : 0u;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            uint32_t t = (uint32_t)__shfl_up((int)x, off);
-            if (lane >= (uint32_t)off) x += t;
-        }
-        uint32_t total = (uint32_t)__shfl((int)x, 63);
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(addr, total);
-        base = (uint32_t)__shfl((int)base, leader);
-        if (mine) res = base + x - v;
-        pending &= ~m;
-    }
-    return res;
-}
-
 // One primitive that crosses a clip plane: the clipper's sub-triangles, in
-// its order, become clip records (positions and clip barycentrics), triangle
-// records and bin records.  Every lane of the wave calls it (with `act`
-// false for lanes without a primitive), so that the slot counters are taken
-// with one atomic per wave.  Sub-triangles that turn out empty or big leave
-// an empty bin record behind.
+// its order, become clip records (positions and clip barycentrics) and
+// triangle records, binned like the primitives of k_setup.  Every lane of
+// the wave calls it (with `act` false for lanes without a primitive), so
+// that the clip slots are taken with one atomic per wave.
 template <int PROG>
 __device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act)
 {
@@ -419,20 +491,14 @@ __device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, boo
         }
     }
     uint32_t base = wave_add(&d.s.counters[C_CLIP_TRI], nleaf, act);
-    uint32_t rbase = wave_add(&d.s.header[H_REC], nleaf, act);
     if (nleaf == 0u) return;
-    if (base + nleaf > d.s.clip_tri_cap || rbase + nleaf > d.s.rec_clip_cap) {
+    if (base + nleaf > d.s.clip_tri_cap) {
         atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
-        // keep the reserved bin records that exist empty
-        for (uint32_t k = 0; k < nleaf && rbase + k < d.s.rec_clip_cap; ++k)
-            d.s.recs[d.s.rec_prims + rbase + k] = make_uint4(0u, 0xFFFFFFFFu, 0u, di);
         return;
     }
     d.s.clip_base[p] = base;
     clip_walk(root, [&](const CTri &t, int leaf) {
         uint32_t slot = base + (uint32_t)leaf;
-        uint32_t ri = d.s.rec_prims + rbase + (uint32_t)leaf;
-        uint4 empty = make_uint4(0u, 0xFFFFFFFFu, 0u, di);
         float *o = d.s.clip_tris + (size_t)slot * kClipFloats;
         for (int k = 0; k < 3; ++k) {
             for (int c = 0; c < 4; ++c) o[k * 7 + c] = t.p[k][c];
@@ -444,25 +510,19 @@ __device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, boo
         uint32_t tx0, ty0, tx1, ty1;
         if (!coord_ok(r)) {
             atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE);
-            d.s.recs[ri] = empty;
             return;
         }
-        if (r.total == 0.0f || !tile_bbox(d, s, tx0, ty0, tx1, ty1)) {
-            d.s.recs[ri] = empty;
-            return;
-        }
+        if (r.total == 0.0f || !tile_bbox(d, s, tx0, ty0, tx1, ty1)) return;
         uint32_t key = (p << 7) | ((uint32_t)leaf << 1) | 1u;
         write_record(d.s.clip_tri + (size_t)slot * kRecWords, r, v, key, ff);
-        uint4 rec = make_uint4(key, tx0 | (ty0 << 16), tx1 | (ty1 << 16), di);
         uint32_t nt = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
         if (nt <= (uint32_t)kMaxBinTiles) {
-            d.s.recs[ri] = rec;
-            add_tile_counts(d, ri, tx0, ty0, tx1, ty1);
+            bin_lane(d, key, tx0, ty0, tx1, ty1);
         } else {
-            d.s.recs[ri] = empty;
             uint32_t bs = atomicAdd(&d.s.counters[C_BIG], 1u);
             if (bs < d.s.big_cap) {
-                d.s.big[bs] = rec;
+                d.s.big[bs] = make_uint4(key, tx0 | (ty0 << 16), tx1 | (ty1 << 16), di);
+                atomicOr(&d.s.header[H_BIG], 1u);
                 atomicMax(&d.s.counters[C_BIG_NX0], 65535u - tx0);
                 atomicMax(&d.s.counters[C_BIG_NY0], 65535u - ty0);
                 atomicMax(&d.s.counters[C_BIG_X1], tx1);
@@ -491,156 +551,95 @@ __global__ __launch_bounds__(256) void k_clip(DrawParams d0, BatchArgs b, const 
 }
 
 // ------------------------------------------------------------------
-// k_scan_*: bin offsets and the active-tile list
+// what a tile holds
 // ------------------------------------------------------------------
 
-// Whether the raster must visit a tile: binned triangles, or inside the
-// footprint union of the draw's big triangles.
-template <bool MULTI>
-__device__ __forceinline__ bool tile_is_active(const DrawParams &d0, const BatchArgs &b, uint32_t gt, uint32_t count)
+// Whether a tile lies in the footprint union of the draw's big triangles.
+__device__ __forceinline__ bool in_big_union(const DrawParams &d, uint32_t tile)
 {
-    if (count) return true;
-    uint32_t di = 0, t = gt;
-    if constexpr (MULTI) {
-        di = find_slot(b.tile_base, b.n, gt);
-        t = gt - b.tile_base[di];
-    }
-    const DrawParams &d = draw_at<MULTI>(d0, b, di);
-    if (d.s.counters[C_BIG] == 0) return false;
-    uint32_t tx = t % d.tiles_x, ty = t / d.tiles_x;
+    if (d.s.counters[C_BIG] == 0u) return false;
+    uint32_t tx = tile % d.tiles_x, ty = tile / d.tiles_x;
     return tx >= 65535u - d.s.counters[C_BIG_NX0] && tx <= d.s.counters[C_BIG_X1] &&
            ty >= 65535u - d.s.counters[C_BIG_NY0] && ty <= d.s.counters[C_BIG_Y1];
 }
 
-// Block-wide exclusive scan of two values per thread (256 threads).
-__device__ __forceinline__ void block_scan2(uint32_t &a, uint32_t &c, uint32_t &tot_a, uint32_t &tot_c)
+// The sources of a tile's triangles, in the order the raster reads them:
+// its bin, its spill entries (only when the bin overflowed: the spill list
+// is filtered by tile), the big triangles whose footprint holds the tile.
+struct TileFeed {
+    uint32_t bin0, nbin, pos;     // bin: first slot, entries, next entry
+    uint32_t nspill, spos;        // spill list: entries, next entry
+    uint32_t nbig, bpos;          // big list: entries, next entry
+    uint32_t gt, tx, ty;          // global tile, tile coordinates
+};
+
+__device__ __forceinline__ TileFeed tile_feed(const DrawParams &d, uint32_t tile)
 {
-    __shared__ uint32_t sa[256], sc[256];
-    uint32_t t = threadIdx.x;
-    sa[t] = a;
-    sc[t] = c;
-    __syncthreads();
-    for (uint32_t off = 1; off < 256u; off <<= 1) {
-        uint32_t va = (t >= off) ? sa[t - off] : 0u;
-        uint32_t vc = (t >= off) ? sc[t - off] : 0u;
-        __syncthreads();
-        sa[t] += va;
-        sc[t] += vc;
-        __syncthreads();
-    }
-    tot_a = sa[255];
-    tot_c = sc[255];
-    a = sa[t] - a;
-    c = sc[t] - c;
-    __syncthreads();
+    TileFeed f;
+    uint32_t cnt = d.s.tile_count[tile];
+    f.bin0 = tile * d.s.bin_cap;
+    f.nbin = min(cnt, d.s.bin_cap);
+    f.pos = 0u;
+    f.nspill = cnt > d.s.bin_cap ? min(d.s.header[H_SPILL], d.s.spill_cap) : 0u;
+    f.spos = 0u;
+    f.nbig = min(d.s.counters[C_BIG], d.s.big_cap);
+    f.bpos = 0u;
+    f.gt = d.s.tile0 + tile;
+    f.tx = tile % d.tiles_x;
+    f.ty = tile / d.tiles_x;
+    return f;
 }
 
-// Phase 1: per block of kScanBlock tiles, the bin count and active count.
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_scan_blocks(DrawParams d0, BatchArgs b)
+// Wave 0 compacts the next 64 entries of the spill or big list that belong
+// to the tile into keys[0..); returns how many (in *nkeys for the workgroup).
+__device__ __forceinline__ void feed_filtered(const DrawParams &d, TileFeed &f, bool spill, uint32_t *keys,
+                                              uint32_t *nkeys)
 {
-    uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * (kScanBlock / 256);
-    uint32_t cnt = 0, act = 0;
-    for (uint32_t k = 0; k < kScanBlock / 256; ++k) {
-        uint32_t gt = base + k;
-        if (gt >= b.total_tiles) break;
-        uint32_t c = b.tile_count[gt];
-        cnt += c;
-        act += tile_is_active<MULTI>(d0, b, gt, c) ? 1u : 0u;
-    }
-    uint32_t ta, tc;
-    block_scan2(cnt, act, ta, tc);
-    if (threadIdx.x == 0) {
-        b.block_sums[2 * blockIdx.x] = ta;
-        b.block_sums[2 * blockIdx.x + 1] = tc;
-    }
-}
-
-// Phase 2: one workgroup scans the block sums.
-__global__ __launch_bounds__(256) void k_scan_totals(BatchArgs b)
-{
-    uint32_t run_c = 0, run_a = 0;
-    for (uint32_t base = 0; base < b.n_blocks; base += 256) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t c = i < b.n_blocks ? b.block_sums[2 * i] : 0u;
-        uint32_t a = i < b.n_blocks ? b.block_sums[2 * i + 1] : 0u;
-        uint32_t tc, ta;
-        block_scan2(c, a, tc, ta);
-        if (i < b.n_blocks) {
-            b.block_sums[2 * i] = run_c + c;
-            b.block_sums[2 * i + 1] = run_a + a;
-        }
-        run_c += tc;
-        run_a += ta;
-    }
-    if (threadIdx.x == 0) {
-        b.tile_offset[b.total_tiles] = run_c;
-        b.header[H_ACTIVE] = run_a;
-    }
-}
-
-// Phase 3: per tile, its bin offset, the active flag and list slot.
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_scan_tiles(DrawParams d0, BatchArgs b)
-{
-    constexpr uint32_t PER = kScanBlock / 256;
-    uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * PER;
-    uint32_t c[PER];
-    bool a[PER];
-    uint32_t cnt = 0, act = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) {
-        uint32_t gt = base + k;
-        c[k] = 0u;
-        a[k] = false;
-        if (gt < b.total_tiles) {
-            c[k] = b.tile_count[gt];
-            a[k] = tile_is_active<MULTI>(d0, b, gt, c[k]);
-        }
-        cnt += c[k];
-        act += a[k] ? 1u : 0u;
-    }
-    uint32_t ta, tc;
-    block_scan2(cnt, act, ta, tc);
-    uint32_t off = b.block_sums[2 * blockIdx.x] + cnt;
-    uint32_t aoff = b.block_sums[2 * blockIdx.x + 1] + act;
-#pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) {
-        uint32_t gt = base + k;
-        if (gt < b.total_tiles) {
-            b.tile_offset[gt] = off;
-            b.tile_active[gt] = a[k] ? 1u : 0u;
-            off += c[k];
-            if (a[k]) b.active_list[aoff++] = gt;
-        }
-    }
-}
-
-// ------------------------------------------------------------------
-// k_bin_fill
-// ------------------------------------------------------------------
-
-// Scatters every record's key into the bins of its footprint, at the
-// positions taken when the tiles were counted.
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_bin_fill(DrawParams d0, BatchArgs b, const uint4 *recs,
-                                                  const uint32_t *rec_slots, uint32_t rec_prims,
-                                                  uint32_t rec_clip_cap, uint32_t *bins)
-{
-    uint32_t n = rec_prims + min(b.header[H_REC], rec_clip_cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint4 r = recs[i];
-        uint32_t tx0 = r.y & 0xFFFFu, ty0 = r.y >> 16, tx1 = r.z & 0xFFFFu, ty1 = r.z >> 16;
-        if (tx0 > tx1) continue;
-        const DrawParams &d = draw_at<MULTI>(d0, b, r.w);
-        const uint32_t *slots = rec_slots + (size_t)i * kMaxBinTiles;
-        uint32_t k = 0;
-        for (uint32_t ty = ty0; ty <= ty1; ++ty)
-            for (uint32_t tx = tx0; tx <= tx1; ++tx) {
-                uint32_t t = ty * d.tiles_x + tx;
-                bins[d.s.tile_offset[t] + slots[k++]] = r.x;
+    uint32_t tid = threadIdx.x;
+    if (tid < 64u) {
+        bool hit = false;
+        uint32_t key = 0u;
+        if (spill) {
+            if (f.spos + tid < f.nspill) {
+                uint2 e = d.s.spill[f.spos + tid];
+                hit = e.x == f.gt;
+                key = e.y;
             }
+        } else if (f.bpos + tid < f.nbig) {
+            uint4 r = d.s.big[f.bpos + tid];
+            hit = f.tx >= (r.y & 0xFFFFu) && f.tx <= (r.z & 0xFFFFu) && f.ty >= (r.y >> 16) && f.ty <= (r.z >> 16);
+            key = r.x;
+        }
+        uint64_t mask = __ballot(hit);
+        if (hit) keys[__popcll(mask & lanes_below())] = key;
+        if (tid == 0u) *nkeys = (uint32_t)__popcll(mask);
     }
+    if (spill) f.spos += 64u;
+    else f.bpos += 64u;
+}
+
+// The tile's next batch of at most nb keys (nb >= 64) into keys[]; 0 when
+// the tile has no more.  Every thread of the workgroup calls it; the keys
+// are visible to all of them on return, and the caller synchronises the
+// workgroup before the next call.
+__device__ __forceinline__ uint32_t next_batch(const DrawParams &d, TileFeed &f, uint32_t *keys, uint32_t *nkeys,
+                                               uint32_t nb)
+{
+    if (f.pos < f.nbin) {
+        uint32_t n = min(nb, f.nbin - f.pos);
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) keys[i] = d.s.bins[f.bin0 + f.pos + i];
+        f.pos += n;
+        __syncthreads();
+        return n;
+    }
+    while (f.spos < f.nspill || f.bpos < f.nbig) {
+        feed_filtered(d, f, f.spos < f.nspill, keys, nkeys);
+        __syncthreads();
+        uint32_t n = *nkeys;
+        if (n) return n;
+        __syncthreads();  // *nkeys is rewritten next round
+    }
+    return 0u;
 }
 
 // ------------------------------------------------------------------
@@ -1006,18 +1005,17 @@ __device__ __forceinline__ void raster_batch(const DrawParams &d, RasterShared<T
     __syncthreads();
 }
 
-// One active tile: load or clear, apply its triangles, store.
-template <int PROG, bool ORDERED, int TS, int NT>
+// One tile of an ordered draw: load or clear, apply its triangles in API
+// order, store.
+template <int PROG, int TS, int NT>
 __device__ __forceinline__ void raster_tile(const DrawParams &d, uint32_t tile, RasterShared<TS> &sh,
                                             uint32_t *olist, uint32_t &olist_n, uint32_t &frag_acc)
 {
+    constexpr bool ORDERED = true;
     constexpr int PPT = TS * TS / NT;
-    uint32_t tx = tile % d.tiles_x, ty = tile / d.tiles_x;
-    uint32_t x0 = tx * TS, y0 = ty * TS;
+    uint32_t x0 = (tile % d.tiles_x) * TS, y0 = (tile / d.tiles_x) * TS;
     uint32_t tid = threadIdx.x;
-    uint32_t seg0 = d.s.tile_offset[tile];
-    uint32_t seg_n = d.s.tile_offset[tile + 1] - seg0;
-    uint32_t nbig = min(d.s.counters[C_BIG], d.s.big_cap);
+    TileFeed feed = tile_feed(d, tile);
 
     PixelState ps[PPT];
     uint32_t init_col[PPT];
@@ -1043,100 +1041,54 @@ __device__ __forceinline__ void raster_tile(const DrawParams &d, uint32_t tile, 
         init_col[k] = ps[k].col;
     });
 
-    uint32_t total = 0;  // ordered: length of the sorted list
-    if (ORDERED) {
-        // gather the tile's entries, sort them into API order
-        if (tid == 0) olist_n = 0u;
+    // gather the tile's keys (bin, spill entries, overlapping big
+    // triangles), then sort them into API order
+    if (tid == 0) olist_n = 0u;
+    __syncthreads();
+    bool overflow = feed.nbin > (uint32_t)kOrderedCap;
+    if (!overflow) {
+        for (uint32_t i = tid; i < feed.nbin; i += NT) olist[i] = d.s.bins[feed.bin0 + i];
+        if (tid == 0) olist_n = feed.nbin;
+    }
+    __syncthreads();
+    while (!overflow && (feed.spos < feed.nspill || feed.bpos < feed.nbig)) {
+        feed_filtered(d, feed, feed.spos < feed.nspill, sh.keys, &sh.nkeys);
         __syncthreads();
-        bool overflow = seg_n > (uint32_t)kOrderedCap;
-        if (!overflow) {
-            for (uint32_t i = tid; i < seg_n; i += NT) olist[i] = d.s.bins[seg0 + i];
-            if (tid == 0) olist_n = seg_n;
-        }
+        uint32_t n = sh.nkeys, base = olist_n;
+        if (base + n > (uint32_t)kOrderedCap) overflow = true;
+        else if (tid < n) olist[base + tid] = sh.keys[tid];
+        __syncthreads();  // every thread has read olist_n and sh.nkeys
+        if (tid == 0 && !overflow) olist_n = base + n;
         __syncthreads();
-        for (uint32_t bb = 0; bb < nbig && !overflow; bb += kBatch) {
-            uint32_t base = olist_n;
-            __syncthreads();  // every thread has read olist_n before thread 0 moves it
-            if (tid < kBatch) {
-                bool hit = false;
-                uint32_t key = 0u;
-                if (bb + tid < nbig) {
-                    uint4 r = d.s.big[bb + tid];
-                    hit = tx >= (r.y & 0xFFFFu) && tx <= (r.z & 0xFFFFu) && ty >= (r.y >> 16) && ty <= (r.z >> 16);
-                    key = r.x;
+    }
+    uint32_t total = olist_n;
+    if (overflow) {
+        if (tid == 0) atomicOr(d.s.error, (uint32_t)ERR_ORDERED_OVERFLOW);
+        total = 0u;
+    }
+    // bitonic sort of olist[0..P), padded with UINT32_MAX
+    uint32_t P = 1u;
+    while (P < total) P <<= 1;
+    for (uint32_t i = total + tid; i < P; i += NT) olist[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < P; i += NT) {
+                uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    uint32_t a = olist[i], c = olist[ixj];
+                    bool up = (i & k) == 0u;
+                    if ((a > c) == up) { olist[i] = c; olist[ixj] = a; }
                 }
-                uint64_t mask = __ballot(hit);
-                uint64_t below = (tid == 0) ? 0ull : (mask & ((~0ull) >> (64 - tid)));
-                uint32_t pos = base + (uint32_t)__popcll(below);
-                if (hit && pos < (uint32_t)kOrderedCap) olist[pos] = key;
-                if (tid == 0) olist_n = base + (uint32_t)__popcll(mask);
             }
             __syncthreads();
-            overflow = olist_n > (uint32_t)kOrderedCap;
-        }
-        total = olist_n;
-        if (overflow || total > (uint32_t)kOrderedCap) {
-            if (tid == 0) atomicOr(d.s.error, (uint32_t)ERR_ORDERED_OVERFLOW);
-            total = 0u;
-        }
-        // bitonic sort of olist[0..P), padded with UINT32_MAX
-        uint32_t P = 1u;
-        while (P < total) P <<= 1;
-        for (uint32_t i = total + tid; i < P; i += NT) olist[i] = 0xFFFFFFFFu;
-        __syncthreads();
-        for (uint32_t k = 2; k <= P; k <<= 1) {
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = tid; i < P; i += NT) {
-                    uint32_t ixj = i ^ j;
-                    if (ixj > i) {
-                        uint32_t a = olist[i], c = olist[ixj];
-                        bool up = (i & k) == 0u;
-                        if ((a > c) == up) { olist[i] = c; olist[ixj] = a; }
-                    }
-                }
-                __syncthreads();
-            }
         }
     }
 
-    // One loop, one call site: ordered -> the sorted list; otherwise the
-    // tile's bin segment, then the big-triangle list filtered by overlap.
-    uint32_t fpos = 0, bpos = 0;
-    for (;;) {
-        uint32_t n;
-        if (ORDERED) {
-            if (fpos >= total) break;
-            n = min((uint32_t)kBatch, total - fpos);
-            if (tid < n) sh.keys[tid] = olist[fpos + tid];
-            fpos += n;
-        } else if (fpos < seg_n) {
-            n = min((uint32_t)kBatch, seg_n - fpos);
-            if (tid < n) sh.keys[tid] = d.s.bins[seg0 + fpos + tid];
-            fpos += n;
-        } else if (bpos < nbig) {
-            if (tid < kBatch) {  // wave 0 compacts the hits
-                bool hit = false;
-                uint32_t key = 0u;
-                if (bpos + tid < nbig) {
-                    uint4 r = d.s.big[bpos + tid];
-                    hit = tx >= (r.y & 0xFFFFu) && tx <= (r.z & 0xFFFFu) && ty >= (r.y >> 16) && ty <= (r.z >> 16);
-                    key = r.x;
-                }
-                uint64_t mask = __ballot(hit);
-                uint64_t below = (tid == 0) ? 0ull : (mask & ((~0ull) >> (64 - tid)));
-                if (hit) sh.keys[__popcll(below)] = key;
-                if (tid == 0) sh.nkeys = (uint32_t)__popcll(mask);
-            }
-            __syncthreads();
-            n = sh.nkeys;
-            bpos += kBatch;
-            if (n == 0) {
-                __syncthreads();  // sh.nkeys is rewritten next round
-                continue;
-            }
-        } else {
-            break;
-        }
+    for (uint32_t fpos = 0; fpos < total;) {
+        uint32_t n = min((uint32_t)kBatch, total - fpos);
+        if (tid < n) sh.keys[tid] = olist[fpos + tid];
+        fpos += n;
         raster_batch<PROG, ORDERED, TS, NT>(d, sh, n, x0, y0, ps, nfrag);
     }
 
@@ -1370,22 +1322,23 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
     return f;
 }
 
-// One active tile of an order-independent draw, visibility only: the depth
-// the pixels hold before the draw (cleared or loaded), the winners of the
-// tile's bin segment and of the overlapping big triangles, stored per pixel
-// (encoded as in oi_value) into the visibility buffer at the tile's item slot.
-template <int TS, int NT>
-__device__ __forceinline__ void raster_tile_vis(const DrawParams &d, uint32_t tile, VisShared<TS, NT> &sh,
-                                                unsigned long long *vis, uint32_t &frag_acc)
+// One tile of an order-independent draw: the depth the pixels hold before
+// the draw (cleared or loaded), the winners of the tile's triangles (its
+// bin, spill entries and overlapping big triangles) as a minimum per pixel
+// in LDS (encoded as in oi_value), then each winner shaded once: its
+// record is read back (from L2, shared by the neighbouring pixels it won)
+// and the same barycentrics are evaluated again, bit for bit.  Colour and
+// depth are stored where the pass changes them (a winner, or the
+// LoadOp::Clear values).
+template <int PROG, int TS, int NT>
+__device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t tile, VisShared<TS, NT> &sh,
+                                               uint32_t &frag_acc)
 {
     constexpr int PPT = TS * TS / NT;
     constexpr uint32_t NB = VisShared<TS, NT>::NB;
-    uint32_t tx = tile % d.tiles_x, ty = tile / d.tiles_x;
-    uint32_t x0 = tx * TS, y0 = ty * TS;
+    uint32_t x0 = (tile % d.tiles_x) * TS, y0 = (tile / d.tiles_x) * TS;
     uint32_t tid = threadIdx.x;
-    uint32_t seg0 = d.s.tile_offset[tile];
-    uint32_t seg_n = d.s.tile_offset[tile + 1] - seg0;
-    uint32_t nbig = min(d.s.counters[C_BIG], d.s.big_cap);
+    TileFeed feed = tile_feed(d, tile);
 
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
@@ -1400,71 +1353,23 @@ __device__ __forceinline__ void raster_tile_vis(const DrawParams &d, uint32_t ti
     }
     __syncthreads();
 
-    uint32_t f = 0, fpos = 0, bpos = 0;
+    uint32_t f = 0;
     for (;;) {
-        uint32_t n;
-        if (fpos < seg_n) {
-            n = min(NB, seg_n - fpos);
-            if (tid < n) sh.keys[tid] = d.s.bins[seg0 + fpos + tid];
-            __syncthreads();  // staging reads the keys through another thread order
-            fpos += n;
-        } else if (bpos < nbig) {
-            if (tid < kBatch) {  // wave 0 compacts the hits
-                bool hit = false;
-                uint32_t key = 0u;
-                if (bpos + tid < nbig) {
-                    uint4 r = d.s.big[bpos + tid];
-                    hit = tx >= (r.y & 0xFFFFu) && tx <= (r.z & 0xFFFFu) && ty >= (r.y >> 16) && ty <= (r.z >> 16);
-                    key = r.x;
-                }
-                uint64_t mask = __ballot(hit);
-                uint64_t below = (tid == 0) ? 0ull : (mask & ((~0ull) >> (64 - tid)));
-                if (hit) sh.keys[__popcll(below)] = key;
-                if (tid == 0) sh.nkeys = (uint32_t)__popcll(mask);
-            }
-            __syncthreads();
-            n = sh.nkeys;
-            bpos += kBatch;
-            if (n == 0) {
-                __syncthreads();  // sh.nkeys is rewritten next round
-                continue;
-            }
-        } else {
-            break;
-        }
+        uint32_t n = next_batch(d, feed, sh.keys, &sh.nkeys, NB);
+        if (n == 0u) break;
         f += vis_batch<TS, NT>(d, sh, n, x0, y0);
     }
 
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        uint32_t q = tid + (uint32_t)k * NT;
-        vis[q] = sh.best[q];
-        if (d.s.frag_px && sh.cnt[q]) {
-            uint32_t px = x0 + q % TS, py = y0 + q / TS;
-            if (px < d.width && py < d.height) d.s.frag_px[(size_t)py * d.width + px] += sh.cnt[q];
-        }
-    }
-    frag_acc += f;
-}
-
-// One active tile of an order-independent draw, shading: each pixel's
-// winner (from the visibility buffer) is shaded once; colour and depth are
-// stored where the pass changes them (a winner, or the LoadOp::Clear values).
-template <int PROG, int TS, int NT>
-__device__ __forceinline__ void shade_tile(const DrawParams &d, uint32_t tile, const unsigned long long *vis)
-{
-    constexpr int PPT = TS * TS / NT;
-    uint32_t x0 = (tile % d.tiles_x) * TS, y0 = (tile / d.tiles_x) * TS;
     bool write_depth_mode = d.depth_test && d.depth_write &&
                             (d.resolve != RES_PASS_LAST || d.depth_compare == CMP_EQUAL ||
                              d.depth_compare == CMP_ALWAYS);
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
-        uint32_t q = threadIdx.x + (uint32_t)k * NT;
+        uint32_t q = tid + (uint32_t)k * NT;
         uint32_t px = x0 + q % TS, py = y0 + q / TS;
         if (px >= d.width || py >= d.height) continue;
         size_t pix = (size_t)py * d.width + px;
-        unsigned long long v = vis[q];
+        unsigned long long v = sh.best[q];
         if (v != ~0ull) {
             float z;
             uint32_t col = winner_color<PROG>(d, oi_key(d.resolve, v), (float)px, (float)py, z);
@@ -1477,35 +1382,69 @@ __device__ __forceinline__ void shade_tile(const DrawParams &d, uint32_t tile, c
             if (d.color && d.clear_color_on) reinterpret_cast<uint32_t *>(d.color)[pix] = d.clear_rgba;
             if (d.depth && d.clear_depth_on) d.depth[pix] = d.clear_depth;
         }
+        if (d.s.frag_px && sh.cnt[q]) d.s.frag_px[pix] += sh.cnt[q];
+    }
+    __syncthreads();  // the next tile resets the per-pixel state
+    frag_acc += f;
+}
+
+// Stores the clear values into one tile.
+template <int TS, int NT>
+__device__ __forceinline__ void clear_tile(const DrawParams &d, uint32_t tile)
+{
+    constexpr int PPT = TS * TS / NT;
+    uint32_t x0 = (tile % d.tiles_x) * TS, y0 = (tile / d.tiles_x) * TS;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        uint32_t q = threadIdx.x + (uint32_t)k * NT;
+        uint32_t px = x0 + q % TS, py = y0 + q / TS;
+        if (px >= d.width || py >= d.height) continue;
+        size_t pix = (size_t)py * d.width + px;
+        if (d.color && d.clear_color_on) reinterpret_cast<uint32_t *>(d.color)[pix] = d.clear_rgba;
+        if (d.depth && d.clear_depth_on) d.depth[pix] = d.clear_depth;
     }
 }
 
-// Stores the clear values into the untouched tiles of a chunk.
-template <bool MULTI, int TS, int NT>
-__device__ __forceinline__ void clear_chunk(const DrawParams &d0, const BatchArgs &b, uint32_t chunk)
+// The tiles without binned records, kClearChunk per work item: those in
+// the footprint union of their draw's big triangles are rastered (`tile_fn`),
+// the others get the LoadOp::Clear values (when the draw clears).  The
+// chunk's bin counts are read together, one per lane.
+template <bool MULTI, int TS, int NT, class F>
+__device__ __forceinline__ void untouched_chunk(const DrawParams &d0, const BatchArgs &b, uint32_t chunk,
+                                                uint32_t *kinds, F &&tile_fn)
 {
-    constexpr int PPT = TS * TS / NT;
-    uint32_t g0 = chunk * kClearChunk, g1 = min(g0 + kClearChunk, b.total_tiles);
-    for (uint32_t gt = g0; gt < g1; ++gt) {
-        if (b.tile_active[gt]) continue;
-        uint32_t di = 0, tile = gt;
-        if constexpr (MULTI) {
-            di = __builtin_amdgcn_readfirstlane(find_slot(b.tile_base, b.n, gt));
-            tile = gt - b.tile_base[di];
+    uint32_t g0 = chunk * kClearChunk;
+    if (threadIdx.x < kClearChunk) {
+        uint32_t gt = g0 + threadIdx.x, kind = 0u, di = 0u;
+        if (gt < b.total_tiles && b.tile_count[gt] == 0u) {
+            uint32_t tile = gt;
+            if constexpr (MULTI) {
+                di = find_slot(b.tile_base, b.n, gt);
+                tile = gt - b.tile_base[di];
+            }
+            const DrawParams &d = draw_at<MULTI>(d0, b, di);
+            kind = in_big_union(d, tile) ? 2u : (d.init_clear ? 1u : 0u);
         }
-        const DrawParams &d = draw_at<MULTI>(d0, b, di);
-        if (!d.init_clear) continue;
-        uint32_t x0 = (tile % d.tiles_x) * TS, y0 = (tile / d.tiles_x) * TS;
-#pragma unroll
-        for (int k = 0; k < PPT; ++k) {
-            uint32_t q = threadIdx.x + (uint32_t)k * NT;
-            uint32_t px = x0 + q % TS, py = y0 + q / TS;
-            if (px >= d.width || py >= d.height) continue;
-            size_t pix = (size_t)py * d.width + px;
-            if (d.color && d.clear_color_on) reinterpret_cast<uint32_t *>(d.color)[pix] = d.clear_rgba;
-            if (d.depth && d.clear_depth_on) d.depth[pix] = d.clear_depth;
-        }
+        kinds[threadIdx.x] = kind | (di << 2);
     }
+    __syncthreads();
+    for (uint32_t i = 0; i < kClearChunk; ++i) {
+        uint32_t kd = __builtin_amdgcn_readfirstlane(kinds[i]);
+        if ((kd & 3u) == 0u) continue;
+        uint32_t di = kd >> 2, gt = g0 + i;
+        const DrawParams &d = draw_at<MULTI>(d0, b, di);
+        uint32_t tile = gt - d.s.tile0;
+        if ((kd & 3u) == 1u) clear_tile<TS, NT>(d, tile);
+        else tile_fn(d, tile);
+    }
+    __syncthreads();  // kinds is rewritten by the next chunk
+}
+
+// Whether the untouched tiles need a pass: some draw clears, or has big
+// triangles whose footprint may hold tiles without binned records.
+__device__ __forceinline__ bool untouched_pass(const BatchArgs &b)
+{
+    return b.any_clear != 0u || b.header[H_BIG] != 0u;
 }
 
 // Splits the active tiles into 8 contiguous ranges, one per XCD (workgroup i
@@ -1552,7 +1491,8 @@ __device__ __forceinline__ void count_fragments(const DrawParams &d0, const Batc
 
 // Ordered draws (blending, NotEqual + write): persistent and statically
 // scheduled (no shared work counter); each tile's fragments are applied in
-// API order.  Then the clear chunks, strided over the whole grid.
+// API order.  First the active tiles, then the untouched ones in chunks,
+// strided over the whole grid.
 template <int PROG, bool MULTI, int TS, int NT>
 __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
 {
@@ -1562,58 +1502,53 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
     __shared__ uint32_t olist[kOrderedCap];
     __shared__ uint32_t olist_n;
     __shared__ uint32_t wave_frags[NT / 64];
+    __shared__ uint32_t kinds[kClearChunk];
 
-    XcdSchedule w(b.header[H_ACTIVE]);
+    XcdSchedule w(min(b.header[H_ACTIVE], b.total_tiles));
     uint32_t frag_acc = 0;
     for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
         uint32_t item = w.xcd * w.per_xcd + i;
         if (item >= w.n) break;
         uint32_t di, tile;
         locate_tile<MULTI>(b, item, di, tile);
-        raster_tile<PROG, true, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, olist, olist_n, frag_acc);
+        raster_tile<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, olist, olist_n, frag_acc);
     }
-    for (uint32_t c = blockIdx.x; c < b.n_chunks; c += gridDim.x) clear_chunk<MULTI, TS, NT>(d0, b, c);
+    if (untouched_pass(b))
+        for (uint32_t c = blockIdx.x; c < b.n_chunks; c += gridDim.x)
+            untouched_chunk<MULTI, TS, NT>(d0, b, c, kinds, [&](const DrawParams &d, uint32_t tile) {
+                raster_tile<PROG, TS, NT>(d, tile, sh, olist, olist_n, frag_acc);
+            });
     count_fragments<MULTI, NT>(d0, b, frag_acc, wave_frags);
 }
 
-// Order-independent draws, pass 1: visibility of every active tile into the
-// visibility buffer (TS*TS values per active-tile slot), then the clear
-// chunks.  Independent of the shader program.
-template <bool MULTI, int TS, int NT>
-__global__ __launch_bounds__(NT) void k_visibility(DrawParams d0, BatchArgs b, unsigned long long *vis)
+// Order-independent draws: visibility and shading of every active tile
+// (raster_tile_oi), then the untouched tiles in chunks, strided over the
+// whole grid.
+template <int PROG, bool MULTI, int TS, int NT>
+__global__ __launch_bounds__(NT) void k_tiles(DrawParams d0, BatchArgs b)
 {
     constexpr int PPT = TS * TS / NT;
     static_assert(NT >= kBatch && PPT >= 1 && TS * TS == PPT * NT, "tile shape");
     static_assert(VisShared<TS, NT>::NB <= 256 && TS <= 32, "owner entries hold 8-bit triangle and row indices");
     __shared__ VisShared<TS, NT> sh;
     __shared__ uint32_t wave_frags[NT / 64];
+    __shared__ uint32_t kinds[kClearChunk];
 
-    XcdSchedule w(b.header[H_ACTIVE]);
+    XcdSchedule w(min(b.header[H_ACTIVE], b.total_tiles));
     uint32_t frag_acc = 0;
     for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
         uint32_t item = w.xcd * w.per_xcd + i;
         if (item >= w.n) break;
         uint32_t di, tile;
         locate_tile<MULTI>(b, item, di, tile);
-        raster_tile_vis<TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, vis + (size_t)item * (TS * TS), frag_acc);
+        raster_tile_oi<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, frag_acc);
     }
-    for (uint32_t c = blockIdx.x; c < b.n_chunks; c += gridDim.x) clear_chunk<MULTI, TS, NT>(d0, b, c);
+    if (untouched_pass(b))
+        for (uint32_t c = blockIdx.x; c < b.n_chunks; c += gridDim.x)
+            untouched_chunk<MULTI, TS, NT>(d0, b, c, kinds, [&](const DrawParams &d, uint32_t tile) {
+                raster_tile_oi<PROG, TS, NT>(d, tile, sh, frag_acc);
+            });
     count_fragments<MULTI, NT>(d0, b, frag_acc, wave_frags);
-}
-
-// Order-independent draws, pass 2: shade each active tile's winners, with
-// the same XCD partition as pass 1 (its visibility values are in that L2).
-template <int PROG, bool MULTI, int TS, int NT>
-__global__ __launch_bounds__(NT) void k_shade(DrawParams d0, BatchArgs b, const unsigned long long *vis)
-{
-    XcdSchedule w(b.header[H_ACTIVE]);
-    for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
-        uint32_t item = w.xcd * w.per_xcd + i;
-        if (item >= w.n) break;
-        uint32_t di, tile;
-        locate_tile<MULTI>(b, item, di, tile);
-        shade_tile<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, vis + (size_t)item * (TS * TS));
-    }
 }
 
 // ------------------------------------------------------------------
@@ -1624,6 +1559,20 @@ __global__ void k_reset(unsigned long long *frags, uint32_t *error)
 {
     if (threadIdx.x < kFragSlots) frags[threadIdx.x] = 0ull;
     if (threadIdx.x == 0) *error = 0u;
+}
+
+// The submission's counters to host memory (out: kFragSlots fragment
+// counters, then the error word), and zero for the next submission.
+__global__ void k_take_counters(unsigned long long *frags, uint32_t *error, unsigned long long *out)
+{
+    if (threadIdx.x < kFragSlots) {
+        out[threadIdx.x] = frags[threadIdx.x];
+        frags[threadIdx.x] = 0ull;
+    }
+    if (threadIdx.x == 0) {
+        reinterpret_cast<uint32_t *>(out + kFragSlots)[0] = *error;
+        *error = 0u;
+    }
 }
 
 __global__ void k_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, uint64_t n)

@@ -468,6 +468,7 @@ struct WgcDevice_T : Object {
     size_t arena_size = 0;
     // persistent device words
     DevCounters *d_counters = nullptr;
+    bool counters_dirty = false;            // a submission stopped before taking its counters
     uint32_t *d_error = nullptr;            // &d_counters->error
     unsigned long long *d_frags = nullptr;  // d_counters->frags
     float *d_srgb = nullptr;
@@ -2135,11 +2136,11 @@ static uint32_t resident_grid(WgcDevice_T *dev, uint32_t per_cu, uint64_t items)
 // another on the stream and reuse it).  Returns the bytes it needs; with a
 // base pointer, fills the pointers.
 //
-// Group-wide: tile counts / offsets / fill cursors / active flags / active
-// list (one word per tile of every draw), scan block sums, bin records and
-// bins, the clip list, the header and per-draw counters.  Per draw: triangle
-// records by primitive, clipped sub-triangle records and their clip data,
-// the big-triangle list and the first clip slot per primitive.
+// Group-wide: tile counts and the active list (one word per tile of every
+// draw), the bins (bin_cap keys per tile), the spill list, the clip list,
+// the header and per-draw counters.  Per draw: triangle records by
+// primitive, clipped sub-triangle records and their clip data, the
+// big-triangle list and the first clip slot per primitive.
 // A triangle strip drawn with restart separators: needs the separator
 // summary pre-pass and is never merged into a batched group.
 static bool restart_strip(const DrawParams &d)
@@ -2177,23 +2178,20 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
         prims_padded += st.multi ? align_up(d.total_prims, 64) : d.total_prims;
         any_clear = any_clear || d.init_clear;
     }
-    uint64_t rec_cap = prims_padded + clip_rec_cap;
-    uint64_t bins_cap = rec_cap * (uint64_t)kMaxBinTiles;
-    uint64_t n_blocks = (total_tiles + kScanBlock - 1) / kScanBlock;
+    // bins: twice the tile's pixels in slots (a bin rarely fills; the
+    // entries past it go to the spill list, which holds every entry at most)
+    uint64_t tile_px = st.draws.empty() ? 0 : (1ull << (2 * st.draws[0].tile_shift));
+    uint64_t bin_cap = std::min<uint64_t>(2 * tile_px, 0xFFFFFFFFull / std::max<uint64_t>(total_tiles, 1));
+    uint64_t spill_cap = (prims_padded + clip_rec_cap) * (uint64_t)kMaxBinTiles;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         size_t o = off;
         off = align_up(off + std::max<size_t>(bytes, 4), 256);
         return o;
     };
-    size_t o_tc = take(4 * total_tiles), o_to = take(4 * (total_tiles + 1));
-    size_t o_ta = take(4 * total_tiles), o_al = take(4 * total_tiles), o_bs = take(8 * n_blocks);
-    size_t o_bins = take(4 * bins_cap), o_recs = take(16 * rec_cap), o_rs = take(4ull * kMaxBinTiles * rec_cap);
+    size_t o_tc = take(4 * total_tiles), o_al = take(4 * total_tiles);
+    size_t o_bins = take(4 * bin_cap * total_tiles), o_sp = take(8 * spill_cap);
     size_t o_ctr = take(4ull * C_NUM * n), o_hdr = take(4ull * H_NUM), o_cl = take(8 * prims);
-    // visibility buffer of the order-independent raster: one 64-bit value per
-    // pixel of every tile that may be active
-    uint64_t tile_px = st.draws.empty() ? 0 : (1ull << (2 * st.draws[0].tile_shift));
-    size_t o_vis = take(st.gl.ordered ? 0 : 8ull * tile_px * total_tiles);
     std::vector<size_t> o_big(n), o_tri(n), o_ctri(n), o_ct(n), o_cb(n), o_ss(n);
     for (uint32_t i = 0; i < n; ++i) {
         const DrawParams &d = st.draws[i];
@@ -2211,51 +2209,41 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
     gl.total_tiles = (uint32_t)total_tiles;
     gl.n_counters = C_NUM * n;
     gl.tile_count = (uint32_t *)(base + o_tc);
-    gl.bins = (uint32_t *)(base + o_bins);
-    gl.recs = (uint4 *)(base + o_recs);
     gl.counters = (uint32_t *)(base + o_ctr);
     gl.header = (uint32_t *)(base + o_hdr);
-    gl.rec_slots = (uint32_t *)(base + o_rs);
-    gl.rec_prims = (uint32_t)prims_padded;
-    gl.rec_clip_cap = (uint32_t)clip_rec_cap;
     gl.clip_list = (uint2 *)(base + o_cl);
     gl.clip_cap = (uint32_t)prims;
+    gl.vis = nullptr;
     b.n = n;
     b.total_tiles = (uint32_t)total_tiles;
-    b.n_chunks = any_clear ? (uint32_t)((total_tiles + kClearChunk - 1) / kClearChunk) : 0u;
+    b.n_chunks = (uint32_t)((total_tiles + kClearChunk - 1) / kClearChunk);
     b.header = gl.header;
     b.tile_count = gl.tile_count;
-    b.tile_offset = (uint32_t *)(base + o_to);
-    b.tile_active = (uint32_t *)(base + o_ta);
     b.active_list = (uint32_t *)(base + o_al);
-    b.block_sums = (uint32_t *)(base + o_bs);
-    b.n_blocks = (uint32_t)n_blocks;
+    b.any_clear = any_clear ? 1u : 0u;
+    b.pad = 0;
     if (!st.multi) {
         b.draws = nullptr;
         b.prim_base = nullptr;
         b.tile_base = nullptr;
         b.total_prims = st.draws[0].total_prims;
     }
-    uint32_t occ_raster = 1, occ_shade = 1;
+    uint32_t occ = 1;
     if (gl.user) {
         int m = st.multi ? 1 : 0;
-        occ_raster = gl.ordered ? gl.user->occ_raster[m] : gl.user->occ_vis[m];
-        occ_shade = gl.ordered ? occ_raster : gl.user->occ_shade[m];
+        occ = gl.ordered ? gl.user->occ_raster[m] : gl.user->occ_tiles[m];
     } else {
-        raster_blocks_per_cu(gl.program, gl.ordered != 0, st.multi, gl.raster_cfg, occ_raster, occ_shade);
+        occ = raster_blocks_per_cu(gl.program, gl.ordered != 0, st.multi, gl.raster_cfg);
     }
-    gl.raster_grid = resident_grid(dev, occ_raster, total_tiles + b.n_chunks);
-    gl.shade_grid = resident_grid(dev, occ_shade, total_tiles);
-    gl.vis = gl.ordered ? nullptr : (unsigned long long *)(base + o_vis);
+    gl.raster_grid = resident_grid(dev, occ, total_tiles + b.n_chunks);
     uint64_t tile_base = 0;
     for (uint32_t i = 0; i < n; ++i) {
         DrawParams &d = st.draws[i];
         Scratch &s = d.s;
         s.tile_count = gl.tile_count + tile_base;
-        s.tile_offset = b.tile_offset + tile_base;
-        s.bins = gl.bins;
-        s.recs = gl.recs;
-        s.rec_slots = gl.rec_slots;
+        s.bins = (uint32_t *)(base + o_bins) + tile_base * bin_cap;
+        s.spill = (uint2 *)(base + o_sp);
+        s.active_list = b.active_list;
         s.header = gl.header;
         s.clip_list = gl.clip_list;
         s.big = (uint4 *)(base + o_big[i]);
@@ -2269,9 +2257,10 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
         s.error = dev->d_error;
         // debug counts are only written when they were sized for this pass
         s.frag_px = (frag_px && frag_px_size == (size_t)d.width * d.height * 4u) ? frag_px : nullptr;
-        s.bins_cap = (uint32_t)bins_cap;
-        s.rec_prims = gl.rec_prims;
-        s.rec_clip_cap = gl.rec_clip_cap;
+        s.tile0 = (uint32_t)tile_base;
+        s.bin_cap = (uint32_t)bin_cap;
+        s.spill_cap = (uint32_t)std::min<uint64_t>(spill_cap, 0xFFFFFFFFull);
+        s.active_cap = (uint32_t)total_tiles;
         s.big_cap = (uint32_t)(d.total_prims + clip_cap[i]);
         s.clip_list_cap = (uint32_t)prims;
         s.clip_tri_cap = (uint32_t)clip_cap[i];
@@ -2590,7 +2579,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
             continue;
         }
         layout_group(s, dev->arena + s.arena_off, dev->frag_px, dev->frag_px_size, dev);
-        plan.launches += s.gl.ordered ? 8 : 9;
+        plan.launches += (s.gl.total_prims > 0 ? 4 : 2) + (!s.multi && restart_strip(s.draws[0]) ? 2 : 0);
         for (auto &d : s.draws) plan.prims += d.total_prims;
         if (!s.multi) continue;
         uint32_t n = (uint32_t)s.draws.size();
@@ -2727,10 +2716,15 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
     Inflight f;
     f.prims = f.launches = 0;
     HIP_TRY(take_slot(device, f.slot), "pinned counter slot");
-    hipError_t e0 = launch_reset(device->d_frags, device->d_error, device->stream);
-    if (e0 != hipSuccess) {
-        device->slot_pool.push_back(f.slot);
-        return hip_fail(e0, "counter reset");
+    // the counters are zero here: every submission takes (and zeroes) them
+    // at its end, or is followed by this reset when it stopped before that
+    if (device->counters_dirty) {
+        hipError_t e0 = launch_reset(device->d_frags, device->d_error, device->stream);
+        if (e0 != hipSuccess) {
+            device->slot_pool.push_back(f.slot);
+            return hip_fail(e0, "counter reset");
+        }
+        device->counters_dirty = false;
     }
     for (uint32_t i = 0; i < count; ++i)
         if (buffers[i] && uses_mapped_buffer(buffers[i])) {
@@ -2738,6 +2732,7 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
             return set_error(WGC_ERR_INVALID, "a buffer used by the command buffer is mapped");
         }
     f.index = device->next_index++;
+    device->counters_dirty = true;
     for (uint32_t i = 0; i < count; ++i) {
         WgcCommandBuffer_T *cb = buffers[i];
         if (!cb) return set_error(WGC_ERR_INVALID, "null command buffer");
@@ -2763,8 +2758,8 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
         f.keep.push_back(cb);
     }
     // the submission's counters, read by the host once its event has passed
-    HIP_TRY(hipMemcpyAsync(f.slot, device->d_counters, offsetof(DevCounters, pad), hipMemcpyDeviceToHost,
-                           device->stream), "counter readback");
+    HIP_TRY(launch_take_counters(device->d_frags, device->d_error, f.slot, device->stream), "counter readback");
+    device->counters_dirty = false;
     if (!device->event_pool.empty()) {
         f.done = device->event_pool.back();
         device->event_pool.pop_back();

@@ -375,7 +375,7 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
     std::vector<std::string> names = {
         "wgc::k_setup<5, false>",          "wgc::k_setup<5, true>",
         "wgc::k_clip<5, false>",           "wgc::k_clip<5, true>",
-        "wgc::k_shade<5, false, 16, 256>", "wgc::k_shade<5, true, 16, 256>",
+        "wgc::k_tiles<5, false, 16, 256>", "wgc::k_tiles<5, true, 16, 256>",
         "wgc::k_raster<5, false, 16, 256>", "wgc::k_raster<5, true, 16, 256>",
         "wgc::k_lp_visibility<5, 1>",      "wgc::k_lp_visibility<5, 2>",
         "wgc::k_lp_shade<5, 1>",           "wgc::k_lp_shade<5, 2>",
@@ -390,14 +390,13 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
     for (int m = 0; m < 2; ++m) {
         k.setup[m] = c.fns[0 + m];
         k.clip[m] = c.fns[2 + m];
-        k.shade[m] = c.fns[4 + m];
+        k.tiles[m] = c.fns[4 + m];
         k.raster[m] = c.fns[6 + m];
         k.lp_vis[m] = c.fns[8 + m];
         k.lp_shade[m] = c.fns[10 + m];
         k.lp_ordered[m] = c.fns[12 + m];
-        k.occ_shade[m] = occupancy(k.shade[m], 256);
+        k.occ_tiles[m] = occupancy(k.tiles[m], 256);
         k.occ_raster[m] = occupancy(k.raster[m], 256);
-        k.occ_vis[m] = visibility_blocks_per_cu(m == 1);
     }
     return true;
 }

@@ -19,15 +19,13 @@ struct BatchArgs {
     const uint32_t *tile_base;   // n + 1 prefix of tiles
     uint32_t n, total_prims, total_tiles, n_chunks;
     uint32_t *header;            // H_* words
-    uint32_t *tile_count, *tile_offset;
-    uint32_t *tile_active;       // 1 when the raster must visit the tile
-    uint32_t *active_list;       // compacted active tiles
-    uint32_t *block_sums;        // 2 words per scan block
-    uint32_t n_blocks, pad;
+    uint32_t *tile_count;        // all tiles of the group
+    uint32_t *active_list;       // tiles with binned records, in the order they were first binned
+    uint32_t any_clear;          // some draw of the group stores LoadOp::Clear values
+    uint32_t pad;
 };
 
-constexpr uint32_t kScanBlock = 1024;   // tiles per scan block
-constexpr uint32_t kClearChunk = 8;     // tiles per clear work item
+constexpr uint32_t kClearChunk = 8;     // tiles per work item of the untouched-tile pass
 
 // Raster tile shapes: tile side x threads per workgroup.
 enum RasterCfg : uint32_t {

@@ -96,22 +96,21 @@ struct AttrBind {
     uint32_t stride, step_instance, offset, ncomp;
 };
 
-// Scratch of one draw.  The tile arrays, the bin records, the bins, the clip
-// list and the group header are slices of arrays shared by every draw of a
-// launch group (tile offsets and bin indices are global); the rest is per
-// draw.
+// Scratch of one draw.  The tile arrays, the bins, the spill list, the
+// active-tile list, the clip list and the group header are slices of arrays
+// shared by every draw of a launch group (tile indices in the spill and
+// active lists are global: tile0 + tile of the draw); the rest is per draw.
 //
-// Bin records: one slot per primitive of the group (slot = group primitive
-// index; an empty footprint marks "no record"), followed by a region that
-// clipped sub-triangles append to.  For every tile of a record's footprint
-// (row-major, at most kMaxBinTiles) rec_slots holds its position in that
-// tile's bin, taken from the tile count when it was incremented.
+// Bins: every tile owns bin_cap slots.  A small record (footprint of at most
+// kMaxBinTiles tiles) takes its position in each tile of its footprint from
+// the tile's count (one atomic add) and stores its key there; positions at
+// or past bin_cap go to the spill list as (global tile, key).  The record
+// that takes position 0 of a tile appends the tile to the active list.
 struct Scratch {
-    uint32_t *tile_count;    // n_tiles (this draw's slice)
-    uint32_t *tile_offset;   // n_tiles + 1 (global bin offsets)
-    uint32_t *bins;          // shared, bins_cap keys
-    uint4 *recs;             // shared bin records: (key, tx0|ty0<<16, tx1|ty1<<16, draw)
-    uint32_t *rec_slots;     // shared, kMaxBinTiles per bin record
+    uint32_t *tile_count;    // n_tiles (this draw's slice): records binned to the tile
+    uint32_t *bins;          // n_tiles x bin_cap keys (this draw's slice)
+    uint2 *spill;            // shared: (global tile, key) past a full bin
+    uint32_t *active_list;   // shared: global tiles with at least one binned record
     uint32_t *header;        // shared group header, see Header
     uint2 *clip_list;        // shared: (draw, primitive) that need the clipper
     uint4 *big;              // this draw's big-triangle bin records
@@ -123,9 +122,9 @@ struct Scratch {
     unsigned long long *frags;  // kFragSlots fragment counters of the submission (summed on read)
     uint32_t *error;
     uint32_t *frag_px;       // optional per-pixel fragment count
-    uint32_t bins_cap;
-    uint32_t rec_prims;      // size of the per-primitive record region (group-wide)
-    uint32_t rec_clip_cap;   // capacity of the appended region
+    uint32_t tile0;          // global index of the draw's first tile
+    uint32_t bin_cap;        // slots per tile bin
+    uint32_t spill_cap, active_cap;
     uint32_t big_cap, clip_list_cap, clip_tri_cap;
     int32_t *strip_sep;      // restart triangle strips: the last separator position up to the end of
                              // each kStripChunk of index positions (-1: none)
@@ -137,9 +136,9 @@ constexpr int kFragSlots = 64;  // fragment counters, spread so that atomics rar
 // (65535 - min) and max so that zero-initialised counters work.
 enum Counter { C_BIG = 0, C_CLIP_TRI = 1, C_BIG_NX0 = 2, C_BIG_NY0 = 3, C_BIG_X1 = 4, C_BIG_Y1 = 5, C_NUM = 8 };
 
-// Group header words.
-// H_REC counts records appended after the per-primitive region.
-enum Header { H_REC = 0, H_ACTIVE = 1, H_CLIP = 2, H_NUM = 4 };
+// Group header words: active tiles, primitives to clip, spilled bin
+// entries, big triangles of every draw of the group.
+enum Header { H_ACTIVE = 0, H_CLIP = 1, H_SPILL = 2, H_BIG = 3, H_NUM = 4 };
 
 // Triangle record: everything the raster needs about one (sub-)triangle,
 // computed once in setup (the same values the raster would compute).

@@ -18,14 +18,10 @@ struct GroupLaunch {
     uint32_t program, ordered, raster_cfg;
     const UserKernels *user;     // PROG_USER: the pipeline's kernels
     uint32_t total_prims, total_tiles, n_counters, raster_grid;
-    uint32_t *tile_count, *bins, *counters, *header;
-    uint4 *recs;
-    uint32_t *rec_slots;
-    uint32_t rec_prims, rec_clip_cap;   // record regions: per primitive, appended
+    uint32_t *tile_count, *counters, *header;
     uint2 *clip_list;            // (draw, primitive) pairs that cross a clip plane
     uint32_t clip_cap;
-    unsigned long long *vis;     // visibility buffer: tile side^2 values per active tile
-    uint32_t shade_grid;
+    unsigned long long *vis;     // points and lines: one visibility value per pixel
 };
 
 // Brackets launches with HIP events on the launching stream; the runtime
@@ -67,9 +63,9 @@ struct KernelTimer {
 // (index: 0 single draw, 1 batched), and their resident workgroups per CU.
 struct UserKernels {
     hipModule_t module = nullptr;
-    hipFunction_t setup[2]{}, clip[2]{}, shade[2]{}, raster[2]{};
+    hipFunction_t setup[2]{}, clip[2]{}, tiles[2]{}, raster[2]{};
     hipFunction_t lp_vis[2]{}, lp_shade[2]{}, lp_ordered[2]{};   // index: points 0, lines 1
-    uint32_t occ_shade[2]{}, occ_raster[2]{}, occ_vis[2]{};
+    uint32_t occ_tiles[2]{}, occ_raster[2]{};
 };
 
 // Launch group of a WGSL pipeline (16x16 tiles): the shader kernels from
@@ -80,22 +76,20 @@ hipError_t launch_user(const UserKernels &k, const DrawParams &d, const BatchArg
 // (or one ordered workgroup); gl.vis holds one 64-bit value per pixel.
 hipError_t launch_lines(const DrawParams &d, const GroupLaunch &gl, hipStream_t st, KernelTimer *kt);
 
-// Resident workgroups per CU of the library's visibility kernel (16x16).
-uint32_t visibility_blocks_per_cu(bool multi);
-
 // A single draw passes its parameters by value (b.draws unused, b.n == 1);
 // a batch passes the device table in b.draws (d0 unused).
 hipError_t launch_draw(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
                        KernelTimer *kt);
 hipError_t launch_batch(const DrawParams &d0, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
                         KernelTimer *kt);
-// Workgroups of the raster kernels for (program, ordered, batched, tile
-// shape) that fit on one CU at once: the visibility (or ordered raster)
-// kernel and the shading kernel.
-void raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32_t cfg, uint32_t &raster,
-                          uint32_t &shade);
+// Workgroups of the raster kernel for (program, ordered, batched, tile
+// shape) that fit on one CU at once.
+uint32_t raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32_t cfg);
 // Zeroes the submission counters (fragment slots and error bits).
 hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t st);
+// Copies the submission counters to host memory (kFragSlots 64-bit fragment
+// counters, then the 32-bit error word) and zeroes them.
+hipError_t launch_take_counters(unsigned long long *frags, uint32_t *error, void *host_out, hipStream_t st);
 // Row copy (4-byte multiples; 16-byte accesses where aligned) and word fill,
 // ordered on the stream like the render launches.
 hipError_t launch_copy_rows(const void *src, uint64_t src_pitch, void *dst, uint64_t dst_pitch, uint64_t row_bytes,
