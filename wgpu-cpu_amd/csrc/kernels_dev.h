@@ -1139,7 +1139,8 @@ struct VisShared {
     int32_t sc[S_NUM][NB];
     uint32_t tkey[NB];
     uint32_t rows[NB];          // first covered tile row | count << 8 | wide << 16
-    uint16_t owner[NB * TS];    // per covered row: triangle | row << 8
+    uint32_t owner[NB * TS];    // per covered row: triangle | row << 8 | span start << 16 | length << 24
+    uint32_t flist[2 * NT];     // a window of the batch's fragments: owner entry | pixel in span << 16
     uint32_t keys[NB];
     unsigned long long best[TS * TS];  // encoded winner (see oi_value), ~0 = none
     float idep[TS * TS];               // depth before the draw
@@ -1232,9 +1233,10 @@ __device__ __forceinline__ float vis_depth(const VisShared<TS, NT> &sh, uint32_t
 
 // Stages sh.keys[0..n) and merges their fragments into the tile's winners.
 // Each staged triangle's covered rows (the reference's scanline rows, within
-// the tile and the scissor) are compacted into one list, so the lanes walk
-// only rows that hold fragments: one lane per (triangle, row) computes the
-// row's integer span and visits its pixels.
+// the tile and the scissor) are compacted into one list; one lane per
+// (triangle, row) computes the row's integer span; the spans' fragments are
+// then listed and spread evenly over the lanes.  Returns the fragments of
+// this thread's entries (the workgroup's sum is the batch's fragments).
 template <int TS, int NT>
 __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS, NT> &sh, uint32_t n, uint32_t x0,
                                               uint32_t y0)
@@ -1278,16 +1280,17 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
     }
     if (e < n) {
         uint32_t b = before + v - cnt_rows;
-        for (uint32_t j = 0; j < cnt_rows; ++j) sh.owner[b + j] = (uint16_t)(e | ((first + j) << 8));
+        for (uint32_t j = 0; j < cnt_rows; ++j) sh.owner[b + j] = e | ((first + j) << 8);
     }
     __syncthreads();
 
+    // the span of every (triangle, row) entry, tile-relative; this thread's
+    // entries are tid, tid + NT, ... and its fragments follow each other
     bool counts = d.s.frag_px != nullptr;
-    uint32_t f = 0;
-    for (uint32_t idx = spread<NT>(tid); idx < total; idx += NT) {
+    uint32_t nfr = 0;
+    for (uint32_t idx = tid; idx < total; idx += NT) {
         uint32_t o = sh.owner[idx];
-        uint32_t te = o & 0xFFu, row = o >> 8;
-        int32_t y = (int32_t)(y0 + row);
+        uint32_t te = o & 0xFFu, row = (o >> 8) & 0xFFu;
         ScanTri S;
         S.ax = sh.sc[S_AX][te]; S.ay = sh.sc[S_AY][te];
         S.bx = sh.sc[S_BX][te]; S.by = sh.sc[S_BY][te];
@@ -1295,31 +1298,78 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
         S.mode = sh.sc[S_MODE][te];
         S.pad = 0;
         int32_t x1, x2;
-        if (!scan_span(S, y, x1, x2, ((sh.rows[te] >> 16) & 1u) != 0u)) continue;
-        int64_t lo = max((int64_t)x1, max(sx0, (int64_t)x0));
-        int64_t hi = min((int64_t)x2, min(sx1 - 1, (int64_t)x0 + TS - 1));
-        if (lo > hi) continue;
-        uint32_t key = sh.tkey[te];
-        uint32_t q0 = row * TS + (uint32_t)(lo - x0);
-        uint32_t len = (uint32_t)(hi - lo) + 1u;
-        f += len;
-        if (counts)
-            for (uint32_t i = 0; i < len; ++i) atomicAdd(&sh.cnt[q0 + i], 1u);
+        uint32_t start = 0, len = 0;
+        if (scan_span(S, (int32_t)(y0 + row), x1, x2, ((sh.rows[te] >> 16) & 1u) != 0u)) {
+            int64_t lo = max((int64_t)x1, max(sx0, (int64_t)x0));
+            int64_t hi = min((int64_t)x2, min(sx1 - 1, (int64_t)x0 + TS - 1));
+            if (lo <= hi) {
+                start = (uint32_t)(lo - x0);
+                len = (uint32_t)(hi - lo) + 1u;
+            }
+        }
+        nfr += len;
         if (d.resolve == RES_LAST) {
-            // no depth test: the last fragment in API order wins, depth unused
-            unsigned long long val = (unsigned long long)(~key);
-            for (uint32_t i = 0; i < len; ++i) atomicMin(&sh.best[q0 + i], val);
+            // no depth test: the last fragment in API order wins, depth
+            // unused; one atomic per fragment, applied right here
+            uint32_t q0 = row * TS + start;
+            unsigned long long val = (unsigned long long)(~sh.tkey[te]);
+            for (uint32_t i = 0; i < len; ++i) {
+                if (counts) atomicAdd(&sh.cnt[q0 + i], 1u);
+                atomicMin(&sh.best[q0 + i], val);
+            }
             continue;
         }
-        float pyf = (float)y;
-        for (uint32_t i = 0; i < len; ++i) {
-            uint32_t q = q0 + i;
-            float z = vis_depth<TS, NT>(sh, te, (float)(lo + i), pyf);
+        sh.owner[idx] = te | (row << 8) | (start << 16) | (len << 24);
+    }
+    if (d.resolve == RES_LAST) {
+        __syncthreads();
+        return nfr;
+    }
+    // exclusive scan of the fragment counts in thread order
+    uint32_t c = nfr;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t t = (uint32_t)__shfl_up((int)c, off);
+        if (lane >= (uint32_t)off) c += t;
+    }
+    if (lane == 63u) sh.wsum[wave] = c;  // every thread read the row-count sums before the last barrier
+    __syncthreads();
+    uint32_t fbase = c - nfr, nfrag = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) {
+        uint32_t x = sh.wsum[w];
+        if ((uint32_t)w < wave) fbase += x;
+        nfrag += x;
+    }
+
+    // the fragments, a window at a time: each entry lists its fragments in
+    // the window, then every lane takes fragments of the list (consecutive
+    // ones spread over the waves), so that long and short spans cost alike
+    constexpr uint32_t W = 2 * NT;
+    for (uint32_t w0 = 0; w0 < nfrag; w0 += W) {
+        uint32_t pos = fbase;
+        for (uint32_t idx = tid; idx < total; idx += NT) {
+            uint32_t len = sh.owner[idx] >> 24;
+            uint32_t a = max(pos, w0), bnd = min(pos + len, w0 + W);
+            for (uint32_t q = a; q < bnd; ++q) sh.flist[q - w0] = idx | ((q - pos) << 16);
+            pos += len;
+        }
+        __syncthreads();
+        uint32_t nw = min(W, nfrag - w0);
+        for (uint32_t k = spread<NT>(tid); k < nw; k += NT) {
+            uint32_t fl = sh.flist[k];
+            uint32_t o = sh.owner[fl & 0xFFFFu];
+            uint32_t te = o & 0xFFu, row = (o >> 8) & 0xFFu;
+            uint32_t px = ((o >> 16) & 0xFFu) + (fl >> 16);
+            uint32_t q = row * TS + px;
+            if (counts) atomicAdd(&sh.cnt[q], 1u);
+            uint32_t key = sh.tkey[te];
+            float z = vis_depth<TS, NT>(sh, te, (float)(x0 + px), (float)(y0 + row));
             if (oi_pass(d, z, sh.idep[q])) atomicMin(&sh.best[q], oi_value(d.resolve, z, key));
         }
+        __syncthreads();
     }
-    __syncthreads();
-    return f;
+    return nfr;
 }
 
 // One tile of an order-independent draw: the depth the pixels hold before
