@@ -50,18 +50,6 @@ __device__ __forceinline__ const DrawParams &draw_at(const DrawParams &d0, const
     else return d0;
 }
 
-// Largest i < n with base[i] <= g (empty ranges resolve to the last of them).
-__device__ __forceinline__ uint32_t find_slot(const uint32_t *base, uint32_t n, uint32_t g)
-{
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1u) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (base[mid] <= g) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // Wave-aggregated append: one atomic per wave, slots in lane order.  Every
 // lane of the wave must pass the same counter.
 __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
@@ -331,7 +319,7 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
         if (g >= b.total_prims) return;
         // every lane of the wave is in the same draw: look it up once, in
         // scalar registers, so that the draw's parameters are scalar loads
-        di = __builtin_amdgcn_readfirstlane(find_slot(b.prim_base, b.n, __builtin_amdgcn_readfirstlane(g)));
+        di = __builtin_amdgcn_readfirstlane(b.wave_draw[__builtin_amdgcn_readfirstlane(g) >> 6]);
         p = g - b.prim_base[di];
     }
     const DrawParams &d = draw_at<MULTI>(d0, b, di);
@@ -1126,15 +1114,16 @@ enum VisField {
 };
 enum VisScan { S_AX = 0, S_AY, S_BX, S_BY, S_CX, S_CY, S_MODE, S_NUM };
 
-// Triangles staged per visibility batch.
-template <int NT>
+// Triangles staged per visibility batch: 96 for 16x16 tiles of 256 threads
+// (so that six workgroups fit a CU's LDS), else 128 (64 for one wave).
+template <int TS, int NT>
 struct VisBatch {
-    static constexpr int value = NT >= 128 ? 128 : NT;
+    static constexpr int value = (TS == 16 && NT == 256) ? 96 : (NT >= 128 ? 128 : NT);
 };
 
 template <int TS, int NT>
 struct VisShared {
-    static constexpr int NB = VisBatch<NT>::value;
+    static constexpr int NB = VisBatch<TS, NT>::value;
     float f[V_NUM][NB];
     int32_t sc[S_NUM][NB];
     uint32_t tkey[NB];
@@ -1469,7 +1458,7 @@ __device__ __forceinline__ void untouched_chunk(const DrawParams &d0, const Batc
         if (gt < b.total_tiles && b.tile_count[gt] == 0u) {
             uint32_t tile = gt;
             if constexpr (MULTI) {
-                di = find_slot(b.tile_base, b.n, gt);
+                di = b.tile_draw[gt];
                 tile = gt - b.tile_base[di];
             }
             const DrawParams &d = draw_at<MULTI>(d0, b, di);
@@ -1519,7 +1508,7 @@ __device__ __forceinline__ void locate_tile(const BatchArgs &b, uint32_t item, u
     di = 0;
     tile = gt;
     if constexpr (MULTI) {
-        di = __builtin_amdgcn_readfirstlane(find_slot(b.tile_base, b.n, gt));
+        di = __builtin_amdgcn_readfirstlane(b.tile_draw[gt]);
         tile = gt - b.tile_base[di];
     }
 }
@@ -1563,8 +1552,9 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
         locate_tile<MULTI>(b, item, di, tile);
         raster_tile<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, olist, olist_n, frag_acc);
     }
+    // the last workgroups have the fewest active tiles: chunks from there
     if (untouched_pass(b))
-        for (uint32_t c = blockIdx.x; c < b.n_chunks; c += gridDim.x)
+        for (uint32_t c = gridDim.x - 1u - blockIdx.x; c < b.n_chunks; c += gridDim.x)
             untouched_chunk<MULTI, TS, NT>(d0, b, c, kinds, [&](const DrawParams &d, uint32_t tile) {
                 raster_tile<PROG, TS, NT>(d, tile, sh, olist, olist_n, frag_acc);
             });
@@ -1574,8 +1564,9 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
 // Order-independent draws: visibility and shading of every active tile
 // (raster_tile_oi), then the untouched tiles in chunks, strided over the
 // whole grid.
+// Six waves per SIMD for 256 threads (registers for six resident workgroups).
 template <int PROG, bool MULTI, int TS, int NT>
-__global__ __launch_bounds__(NT) void k_tiles(DrawParams d0, BatchArgs b)
+__global__ __launch_bounds__(NT, NT == 256 && TS == 16 ? 6 : 1) void k_tiles(DrawParams d0, BatchArgs b)
 {
     constexpr int PPT = TS * TS / NT;
     static_assert(NT >= kBatch && PPT >= 1 && TS * TS == PPT * NT, "tile shape");
@@ -1593,8 +1584,9 @@ __global__ __launch_bounds__(NT) void k_tiles(DrawParams d0, BatchArgs b)
         locate_tile<MULTI>(b, item, di, tile);
         raster_tile_oi<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, frag_acc);
     }
+    // the last workgroups have the fewest active tiles: chunks from there
     if (untouched_pass(b))
-        for (uint32_t c = blockIdx.x; c < b.n_chunks; c += gridDim.x)
+        for (uint32_t c = gridDim.x - 1u - blockIdx.x; c < b.n_chunks; c += gridDim.x)
             untouched_chunk<MULTI, TS, NT>(d0, b, c, kinds, [&](const DrawParams &d, uint32_t tile) {
                 raster_tile_oi<PROG, TS, NT>(d, tile, sh, frag_acc);
             });

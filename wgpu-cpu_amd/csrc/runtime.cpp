@@ -2178,10 +2178,22 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
         prims_padded += st.multi ? align_up(d.total_prims, 64) : d.total_prims;
         any_clear = any_clear || d.init_clear;
     }
-    // bins: twice the tile's pixels in slots (a bin rarely fills; the
+    // bins, per draw: slots for twice the tile's pixels, or four times the
+    // draw's primitives per tile when that is more (a bin rarely fills; the
     // entries past it go to the spill list, which holds every entry at most)
     uint64_t tile_px = st.draws.empty() ? 0 : (1ull << (2 * st.draws[0].tile_shift));
-    uint64_t bin_cap = std::min<uint64_t>(2 * tile_px, 0xFFFFFFFFull / std::max<uint64_t>(total_tiles, 1));
+    std::vector<uint64_t> bin_cap(n);
+    uint64_t bins_total = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const DrawParams &d = st.draws[i];
+        uint64_t tiles = std::max<uint64_t>((uint64_t)d.tiles_x * d.tiles_y, 1);
+        uint64_t per_tile = (4ull * d.total_prims + tiles - 1) / tiles;
+        uint64_t cap = 2 * tile_px;
+        while (cap < per_tile) cap <<= 1;
+        cap = std::min<uint64_t>(cap, d.total_prims + clip_cap[i]);   // no tile holds more
+        bin_cap[i] = std::max<uint64_t>(std::min<uint64_t>(cap, 0xFFFFFFFFull / tiles), 1);
+        bins_total += bin_cap[i] * tiles;
+    }
     uint64_t spill_cap = (prims_padded + clip_rec_cap) * (uint64_t)kMaxBinTiles;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -2190,7 +2202,7 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
         return o;
     };
     size_t o_tc = take(4 * total_tiles), o_al = take(4 * total_tiles);
-    size_t o_bins = take(4 * bin_cap * total_tiles), o_sp = take(8 * spill_cap);
+    size_t o_bins = take(4 * bins_total), o_sp = take(8 * spill_cap);
     size_t o_ctr = take(4ull * C_NUM * n), o_hdr = take(4ull * H_NUM), o_cl = take(8 * prims);
     std::vector<size_t> o_big(n), o_tri(n), o_ctri(n), o_ct(n), o_cb(n), o_ss(n);
     for (uint32_t i = 0; i < n; ++i) {
@@ -2221,11 +2233,12 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
     b.tile_count = gl.tile_count;
     b.active_list = (uint32_t *)(base + o_al);
     b.any_clear = any_clear ? 1u : 0u;
-    b.pad = 0;
     if (!st.multi) {
         b.draws = nullptr;
         b.prim_base = nullptr;
         b.tile_base = nullptr;
+        b.wave_draw = nullptr;
+        b.tile_draw = nullptr;
         b.total_prims = st.draws[0].total_prims;
     }
     uint32_t occ = 1;
@@ -2236,12 +2249,13 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
         occ = raster_blocks_per_cu(gl.program, gl.ordered != 0, st.multi, gl.raster_cfg);
     }
     gl.raster_grid = resident_grid(dev, occ, total_tiles + b.n_chunks);
-    uint64_t tile_base = 0;
+    uint64_t tile_base = 0, bin_base = 0;
     for (uint32_t i = 0; i < n; ++i) {
         DrawParams &d = st.draws[i];
         Scratch &s = d.s;
         s.tile_count = gl.tile_count + tile_base;
-        s.bins = (uint32_t *)(base + o_bins) + tile_base * bin_cap;
+        s.bins = (uint32_t *)(base + o_bins) + bin_base;
+        bin_base += bin_cap[i] * std::max<uint64_t>((uint64_t)d.tiles_x * d.tiles_y, 1);
         s.spill = (uint2 *)(base + o_sp);
         s.active_list = b.active_list;
         s.header = gl.header;
@@ -2258,7 +2272,7 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
         // debug counts are only written when they were sized for this pass
         s.frag_px = (frag_px && frag_px_size == (size_t)d.width * d.height * 4u) ? frag_px : nullptr;
         s.tile0 = (uint32_t)tile_base;
-        s.bin_cap = (uint32_t)bin_cap;
+        s.bin_cap = (uint32_t)bin_cap[i];
         s.spill_cap = (uint32_t)std::min<uint64_t>(spill_cap, 0xFFFFFFFFull);
         s.active_cap = (uint32_t)total_tiles;
         s.big_cap = (uint32_t)(d.total_prims + clip_cap[i]);
@@ -2538,9 +2552,15 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         stage_off = align_up(stage_off + bytes, 256);
         need = std::max(need, stage_off);
         if (s.multi) {
+            // draws, primitive and tile prefixes, the draw of every wave of
+            // primitives and of every tile
             s.table_off = tables;
-            size_t n = s.draws.size();
-            tables = align_up(tables + n * sizeof(DrawParams) + 8 * (n + 1), 256);
+            size_t n = s.draws.size(), waves = 0, tiles = 0;
+            for (const DrawParams &d : s.draws) {
+                waves += align_up(d.total_prims, 64) / 64;
+                tiles += (size_t)d.tiles_x * d.tiles_y;
+            }
+            tables = align_up(tables + n * sizeof(DrawParams) + 8 * (n + 1) + 4 * (waves + tiles), 256);
         }
     }
     if (need > dev->arena_size) {
@@ -2597,9 +2617,16 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         }
         pb[n] = acc_p;
         tb[n] = acc_t;
+        uint32_t *wd = tb + (n + 1), *td = wd + acc_p / 64;
+        for (uint32_t k = 0; k < n; ++k) {
+            for (uint32_t w = pb[k] / 64; w < pb[k + 1] / 64; ++w) wd[w] = k;
+            for (uint32_t t = tb[k]; t < tb[k + 1]; ++t) td[t] = k;
+        }
         s.batch.draws = (const DrawParams *)dptr;
         s.batch.prim_base = (const uint32_t *)(dptr + n * sizeof(DrawParams));
         s.batch.tile_base = s.batch.prim_base + (n + 1);
+        s.batch.wave_draw = s.batch.tile_base + (n + 1);
+        s.batch.tile_draw = s.batch.wave_draw + acc_p / 64;
         s.batch.n = n;
         s.batch.total_prims = acc_p;
         s.batch.total_tiles = acc_t;

@@ -17,6 +17,8 @@ struct BatchArgs {
     const DrawParams *draws;     // device array of n draws
     const uint32_t *prim_base;   // n + 1 prefix of primitives, padded to multiples of 64
     const uint32_t *tile_base;   // n + 1 prefix of tiles
+    const uint32_t *wave_draw;   // the draw of every 64 primitives (the padded prefix)
+    const uint32_t *tile_draw;   // the draw of every tile
     uint32_t n, total_prims, total_tiles, n_chunks;
     uint32_t *header;            // H_* words
     uint32_t *tile_count;        // all tiles of the group
