@@ -147,6 +147,49 @@ def test_big_triangles_and_overlap(gpu_device):
     assert_parity(spec, gpu_device)
 
 
+def _crowded(n=4000, seed=23, **kw):
+    """n small triangles around the centre of a 1024x1024 target (4096 tiles
+    of 16x16): the four tiles there collect about n/4 entries each, past
+    their bin capacity, so the spill list is read."""
+    vb = random_clip_triangles(n, seed, 1024, 1024, spread=0.006, size_px=(1.5, 5.0), **kw)
+    return vb
+
+
+def test_crowded_tiles_spill_their_bins(gpu_device):
+    vb = _crowded()
+    spec = scenes.PassSpec(1024, 1024, has_depth=True, clear_depth=1.0,
+                           draws=[clip_color_draw(vb, depth=(wgc.LESS, True))], name="crowded_tiles")
+    assert_parity(spec, gpu_device)
+
+
+def test_crowded_tiles_spill_ordered(gpu_device_ext):
+    vb = _crowded(n=3000, seed=29, alpha=(0.2, 0.8))
+    spec = scenes.PassSpec(1024, 1024, clear_color=(0.1, 0.2, 0.3, 1.0),
+                           draws=[clip_color_draw(vb, blend=wgc.BLEND_ALPHA)], semantics=wgc.SEMANTICS_EXTENDED,
+                           name="crowded_tiles_blend")
+    assert_parity(spec, gpu_device_ext)
+
+
+def test_crowded_tiles_spill_batched(gpu_device):
+    """The spill list of a batched group holds global tiles: a crowded pass
+    next to ordinary ones renders like each pass alone."""
+    announce("crowded tiles, batched")
+    crowded = scenes.PassSpec(1024, 1024, has_depth=True, clear_depth=1.0,
+                              draws=[clip_color_draw(_crowded(seed=31), depth=(wgc.LESS, True))], name="crowded_b")
+    others = [scenes.PassSpec(160 + 16 * i, 120, has_depth=True, clear_depth=1.0,
+                              draws=[clip_color_draw(random_clip_triangles(500, 40 + i, 160 + 16 * i, 120),
+                                                     depth=(wgc.LESS, True))], name=f"other_{i}") for i in range(3)]
+    specs = [others[0], crowded, others[1], others[2]]
+    res, total = runner.render_batch(specs, gpu_device)
+    want = 0
+    for s, r in zip(specs, res):
+        ref = bridge.render(s)
+        want += ref.fragments
+        assert np.array_equal(r.color, ref.color), s.name
+        assert np.array_equal(r.depth.view(np.uint32), ref.depth.view(np.uint32)), s.name
+    assert total == want
+
+
 @pytest.mark.parametrize("compare", list(range(1, 9)))
 @pytest.mark.parametrize("write", [True, False])
 def test_depth_compare_and_ties(gpu_device, compare, write):

@@ -131,18 +131,25 @@ __device__ __forceinline__ void activate_tile(const DrawParams &d, uint32_t at, 
 }
 
 // Bins one record from one lane (no wave-wide operations: the clipper's
-// sub-triangles are emitted under divergent control flow).
+// sub-triangles are emitted under divergent control flow).  The adds of a
+// row of up to kMaxBinTiles footprint tiles are issued before any result
+// is used, so the lane waits once per footprint.
 __device__ __forceinline__ void bin_lane(const DrawParams &d, uint32_t key, uint32_t tx0, uint32_t ty0, uint32_t tx1,
                                          uint32_t ty1)
 {
-    for (uint32_t ty = ty0; ty <= ty1; ++ty)
-        for (uint32_t tx = tx0; tx <= tx1; ++tx) {
-            uint32_t t = ty * d.tiles_x + tx;
-            uint32_t slot = atomicAdd(&d.s.tile_count[t], 1u);
-            if (slot < d.s.bin_cap) d.s.bins[(size_t)t * d.s.bin_cap + slot] = key;
-            else spill_entry(d, atomicAdd(&d.s.header[H_SPILL], 1u), t, key);
-            if (slot == 0u) activate_tile(d, atomicAdd(&d.s.header[H_ACTIVE], 1u), t);
-        }
+    uint32_t w = tx1 - tx0 + 1u, ntl = w * (ty1 - ty0 + 1u);
+    uint32_t got[kMaxBinTiles];
+#pragma unroll
+    for (int k = 0; k < kMaxBinTiles; ++k)
+        got[k] = (uint32_t)k < ntl ? atomicAdd(&d.s.tile_count[footprint_tile(d, tx0, ty0, w, (uint32_t)k)], 1u) : 0u;
+#pragma unroll
+    for (int k = 0; k < kMaxBinTiles; ++k) {
+        if ((uint32_t)k >= ntl) break;
+        uint32_t t = footprint_tile(d, tx0, ty0, w, (uint32_t)k), slot = got[k];
+        if (slot < d.s.bin_cap) d.s.bins[(size_t)t * d.s.bin_cap + slot] = key;
+        else spill_entry(d, atomicAdd(&d.s.header[H_SPILL], 1u), t, key);
+        if (slot == 0u) activate_tile(d, atomicAdd(&d.s.header[H_ACTIVE], 1u), t);
+    }
 }
 
 // Bins every lane's record (with `has`) into the tiles of its footprint
@@ -1231,7 +1238,7 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
                                               uint32_t y0)
 {
     uint32_t tid = threadIdx.x;
-    uint32_t e = spread<NT>(tid);  // the triangle this thread stages
+    uint32_t e = tid;  // the triangle this thread stages (a batch takes as few waves as it needs)
     int64_t sx0 = d.scissor[0], sy0 = d.scissor[1];
     int64_t sx1 = sx0 + d.scissor[2], sy1 = sy0 + d.scissor[3];
     uint32_t cnt_rows = 0, first = 0;
