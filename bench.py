@@ -25,7 +25,7 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "wgpu-cpu_amd"))
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["k_raster", "k_tiles", "k_setup", "k_clip", "k_zero", "k_clear"]
+KERNELS = ["k_raster", "k_tiles", "k_setup", "k_clip", "k_clear", "k_end_submission"]
 
 
 def make_spec(name: str):

@@ -94,9 +94,6 @@ template <int PROG, bool MULTI>
 static hipError_t launch_group(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
                                KernelTimer *kt)
 {
-    uint32_t zero_n = max(max(gl.total_tiles, gl.n_counters), (uint32_t)H_NUM);
-    WGC_LAUNCH("k_zero", k_zero, (zero_n + 255) / 256, 256, gl.tile_count, gl.total_tiles, gl.counters,
-               gl.n_counters, gl.header);
     if (!MULTI && d.strip_restart && d.indexed && d.topology == TOPO_TRIANGLE_STRIP && d.count > 0) {
         uint32_t chunks = (d.count + kStripChunk - 1) / kStripChunk;
         WGC_LAUNCH("k_strip_chunks", k_strip_chunks, (chunks + 255) / 256, 256, d, d.s.strip_sep);
@@ -157,9 +154,6 @@ static hipError_t launch_user_t(const UserKernels &k, const DrawParams &d0, cons
     const uint2 *clip_list = gl.clip_list;
     uint32_t clip_cap = gl.clip_cap;
     const int m = MULTI ? 1 : 0;
-    uint32_t zero_n = max(max(gl.total_tiles, gl.n_counters), (uint32_t)H_NUM);
-    WGC_LAUNCH("k_zero", k_zero, (zero_n + 255) / 256, 256, gl.tile_count, gl.total_tiles, gl.counters,
-               gl.n_counters, gl.header);
     if (!MULTI && d.strip_restart && d.indexed && d.topology == TOPO_TRIANGLE_STRIP && d.count > 0) {
         uint32_t chunks = (d.count + kStripChunk - 1) / kStripChunk;
         WGC_LAUNCH("k_strip_chunks", k_strip_chunks, (chunks + 255) / 256, 256, d, d.s.strip_sep);
@@ -297,9 +291,13 @@ hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_take_counters(unsigned long long *frags, uint32_t *error, void *host_out, hipStream_t st)
+hipError_t launch_end_submission(unsigned long long *frags, uint32_t *error, void *host_out, void *zone,
+                                 size_t zone_bytes, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_take_counters, dim3(1), dim3(64), 0, st, frags, error, (unsigned long long *)host_out);
+    uint32_t n16 = (uint32_t)(zone_bytes / 16);
+    uint32_t grid = std::max(1u, std::min((n16 + 1023u) / 1024u, 1024u));
+    hipLaunchKernelGGL(k_end_submission, dim3(grid), dim3(256), 0, st, frags, error, (unsigned long long *)host_out,
+                       (uint4 *)zone, n16);
     return hipGetLastError();
 }
 

@@ -1,7 +1,7 @@
 // kernels_dev.h -- device code of the render-pass executor (gfx950).
 //
-// Per launch group: k_zero -> k_setup -> k_clip -> k_tiles (order-
-// independent draws) or k_raster (ordered draws).  A group is one draw
+// Per launch group: k_setup -> k_clip -> k_tiles (order-independent draws)
+// or k_raster (ordered draws).  A group is one draw
 // (parameters passed by value) or a table of independent draws -- the
 // batched form, where hundreds of render passes share every launch (MULTI).
 //
@@ -279,19 +279,6 @@ __device__ __forceinline__ void write_record(uint32_t *rec, const RasterTri &r, 
     uint4 *o = reinterpret_cast<uint4 *>(rec);
 #pragma unroll
     for (int q = 0; q < kRecWords / 4; ++q) o[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
-}
-
-// ------------------------------------------------------------------
-// k_zero: tile counts of the group, per-draw counters, group header
-// ------------------------------------------------------------------
-
-__global__ void k_zero(uint32_t *tile_count, uint32_t n_tiles, uint32_t *counters, uint32_t n_counters,
-                       uint32_t *header)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_tiles) tile_count[i] = 0u;
-    if (i < n_counters) counters[i] = 0u;
-    if (i < H_NUM) header[i] = 0u;
 }
 
 // ------------------------------------------------------------------
@@ -1610,18 +1597,25 @@ __global__ void k_reset(unsigned long long *frags, uint32_t *error)
     if (threadIdx.x == 0) *error = 0u;
 }
 
-// The submission's counters to host memory (out: kFragSlots fragment
-// counters, then the error word), and zero for the next submission.
-__global__ void k_take_counters(unsigned long long *frags, uint32_t *error, unsigned long long *out)
+// End of a submission: its counters to host memory (out: kFragSlots
+// fragment counters, then the error word) and zero for the next one; the
+// zero region of the scratch arena that its launch groups used (tile
+// counts, group headers, per-draw counters) cleared, n16 16-byte units.
+__global__ __launch_bounds__(256) void k_end_submission(unsigned long long *frags, uint32_t *error,
+                                                        unsigned long long *out, uint4 *zone, uint32_t n16)
 {
-    if (threadIdx.x < kFragSlots) {
-        out[threadIdx.x] = frags[threadIdx.x];
-        frags[threadIdx.x] = 0ull;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < kFragSlots) {
+            out[threadIdx.x] = frags[threadIdx.x];
+            frags[threadIdx.x] = 0ull;
+        }
+        if (threadIdx.x == 0) {
+            reinterpret_cast<uint32_t *>(out + kFragSlots)[0] = *error;
+            *error = 0u;
+        }
     }
-    if (threadIdx.x == 0) {
-        reinterpret_cast<uint32_t *>(out + kFragSlots)[0] = *error;
-        *error = 0u;
-    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += gridDim.x * blockDim.x)
+        zone[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 __global__ void k_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, uint64_t n)

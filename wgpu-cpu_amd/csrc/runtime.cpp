@@ -371,7 +371,9 @@ struct Step {
     BatchArgs batch{};
     size_t table_off = 0;            // offset of this group's tables in the plan's device block
     uint32_t stage = 0;              // steps of one stage are independent and run concurrently
-    size_t arena_off = 0;            // this group's scratch within the arena
+    size_t arena_off = 0;            // this group's scratch within the arena (past the zero region)
+    size_t zone_off = 0;             // this group's words in the arena's zero region (not shared
+                                     // with the plan's other groups; cleared when the submission ends)
     // clear
     uint32_t *color = nullptr;
     uint32_t rgba = 0;
@@ -388,6 +390,7 @@ struct Step {
 struct Plan {
     WgcDevice_T *dev = nullptr;      // set (and retained) while the plan holds pooled memory
     uint64_t arena_gen = UINT64_MAX;
+    size_t zone_used = 0;            // bytes of the arena's zero region its launch groups use
     uint32_t raster_cfg = UINT32_MAX;
     bool batching = false;
     std::vector<Step> steps;
@@ -463,9 +466,12 @@ struct WgcDevice_T : Object {
     hipStream_t side = nullptr;     // second stream for the independent groups of a stage
     hipEvent_t fork = nullptr, join = nullptr;
     std::mutex mu;
-    // per-draw scratch arena, reused by every draw (they run in stream order)
+    // per-draw scratch arena, reused by every draw (they run in stream order);
+    // its first zone_size bytes are zero when a plan starts (the kernel that
+    // ends a submission clears what the submission's plan used there)
     uint8_t *arena = nullptr;
     size_t arena_size = 0;
+    size_t zone_size = 0;
     // persistent device words
     DevCounters *d_counters = nullptr;
     bool counters_dirty = false;            // a submission stopped before taking its counters
@@ -2163,7 +2169,17 @@ static size_t layout_lines(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
     return bytes;
 }
 
-static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t frag_px_size, WgcDevice_T *dev)
+// A group's words in the zero region: its header, per-draw counters and tile
+// counts.
+static size_t zone_bytes(const Step &st)
+{
+    uint64_t tiles = 0;
+    for (const DrawParams &d : st.draws) tiles += (uint64_t)d.tiles_x * d.tiles_y;
+    return align_up(4ull * H_NUM, 256) + align_up(4ull * C_NUM * st.draws.size(), 256) + align_up(4ull * tiles, 256);
+}
+
+static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint32_t *frag_px, size_t frag_px_size,
+                           WgcDevice_T *dev)
 {
     uint32_t n = (uint32_t)st.draws.size();
     uint64_t total_tiles = 0, clip_rec_cap = 0, prims = 0, prims_padded = 0;
@@ -2201,9 +2217,9 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
         off = align_up(off + std::max<size_t>(bytes, 4), 256);
         return o;
     };
-    size_t o_tc = take(4 * total_tiles), o_al = take(4 * total_tiles);
+    size_t o_al = take(4 * total_tiles);
     size_t o_bins = take(4 * bins_total), o_sp = take(8 * spill_cap);
-    size_t o_ctr = take(4ull * C_NUM * n), o_hdr = take(4ull * H_NUM), o_cl = take(8 * prims);
+    size_t o_cl = take(8 * prims);
     std::vector<size_t> o_big(n), o_tri(n), o_ctri(n), o_ct(n), o_cb(n), o_ss(n);
     for (uint32_t i = 0; i < n; ++i) {
         const DrawParams &d = st.draws[i];
@@ -2220,9 +2236,10 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
     BatchArgs &b = st.batch;
     gl.total_tiles = (uint32_t)total_tiles;
     gl.n_counters = C_NUM * n;
-    gl.tile_count = (uint32_t *)(base + o_tc);
-    gl.counters = (uint32_t *)(base + o_ctr);
-    gl.header = (uint32_t *)(base + o_hdr);
+    // zero region: header, counters, tile counts (zone_bytes)
+    gl.header = (uint32_t *)zone;
+    gl.counters = (uint32_t *)(zone + align_up(4ull * H_NUM, 256));
+    gl.tile_count = (uint32_t *)(zone + align_up(4ull * H_NUM, 256) + align_up(4ull * C_NUM * n, 256));
     gl.clip_list = (uint2 *)(base + o_cl);
     gl.clip_cap = (uint32_t)prims;
     gl.vis = nullptr;
@@ -2232,6 +2249,8 @@ static size_t layout_group(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
     b.header = gl.header;
     b.tile_count = gl.tile_count;
     b.active_list = (uint32_t *)(base + o_al);
+    b.counters = gl.counters;
+    b.n_counters = gl.n_counters;
     b.any_clear = any_clear ? 1u : 0u;
     if (!st.multi) {
         b.draws = nullptr;
@@ -2536,7 +2555,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
     }
     // 3. scratch: the groups of a stage side by side, stages reusing the
     //    arena (they run one after another)
-    size_t need = 0;
+    size_t need = 0, zneed = 0;
     size_t tables = 0;
     size_t stage_off = 0;
     uint32_t cur_stage = UINT32_MAX;
@@ -2547,10 +2566,12 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
             stage_off = 0;
         }
         s.arena_off = stage_off;
+        s.zone_off = zneed;
         size_t bytes = s.kind == StepKind::Lines ? layout_lines(s, nullptr, nullptr, 0, dev)
-                                                  : layout_group(s, nullptr, nullptr, 0, dev);
+                                                  : layout_group(s, nullptr, nullptr, nullptr, 0, dev);
         stage_off = align_up(stage_off + bytes, 256);
         need = std::max(need, stage_off);
+        if (s.kind == StepKind::Group) zneed += zone_bytes(s);
         if (s.multi) {
             // draws, primitive and tile prefixes, the draw of every wave of
             // primitives and of every tile
@@ -2563,14 +2584,20 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
             tables = align_up(tables + n * sizeof(DrawParams) + 8 * (n + 1) + 4 * (waves + tiles), 256);
         }
     }
-    if (need > dev->arena_size) {
+    if (zneed > dev->zone_size || dev->zone_size + need > dev->arena_size) {
         HIP_TRY(hipStreamSynchronize(dev->stream), "arena");
         if (dev->arena) HIP_TRY(hipFree(dev->arena), "arena");
         dev->arena = nullptr;
         dev->arena_size = 0;
-        size_t grow = std::max(need, (size_t)64 << 20);
-        if (hipMalloc(&dev->arena, grow) != hipSuccess) return set_error(WGC_ERR_OUT_OF_MEMORY, "scratch arena");
+        size_t zone = align_up(std::max(zneed, dev->zone_size), 4096);
+        size_t grow = zone + std::max(need, (size_t)64 << 20);
+        if (hipMalloc(&dev->arena, grow) != hipSuccess) {
+            dev->zone_size = 0;
+            return set_error(WGC_ERR_OUT_OF_MEMORY, "scratch arena");
+        }
+        HIP_TRY(hipMemsetAsync(dev->arena, 0, zone, dev->stream), "arena zero region");
         dev->arena_size = grow;
+        dev->zone_size = zone;
         dev->arena_gen++;
     }
     // draw tables: pooled device memory, uploaded from pooled pinned staging
@@ -2589,7 +2616,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
     }
     for (Step &s : plan.steps) {
         if (s.kind == StepKind::Lines) {
-            layout_lines(s, dev->arena + s.arena_off, dev->frag_px, dev->frag_px_size, dev);
+            layout_lines(s, dev->arena + dev->zone_size + s.arena_off, dev->frag_px, dev->frag_px_size, dev);
             plan.launches += s.gl.ordered ? 1 : 3;
             plan.prims += s.draws[0].total_prims;
             continue;
@@ -2598,8 +2625,9 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
             plan.launches += 1;
             continue;
         }
-        layout_group(s, dev->arena + s.arena_off, dev->frag_px, dev->frag_px_size, dev);
-        plan.launches += (s.gl.total_prims > 0 ? 4 : 2) + (!s.multi && restart_strip(s.draws[0]) ? 2 : 0);
+        layout_group(s, dev->arena + dev->zone_size + s.arena_off, dev->arena + s.zone_off, dev->frag_px,
+                     dev->frag_px_size, dev);
+        plan.launches += (s.gl.total_prims > 0 ? 3 : 1) + (!s.multi && restart_strip(s.draws[0]) ? 2 : 0);
         for (auto &d : s.draws) plan.prims += d.total_prims;
         if (!s.multi) continue;
         uint32_t n = (uint32_t)s.draws.size();
@@ -2635,6 +2663,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         HIP_TRY(hipMemcpyAsync(plan.d_tables, plan.h_tables, tables, hipMemcpyHostToDevice, dev->stream),
                 "draw tables");
     plan.arena_gen = dev->arena_gen;
+    plan.zone_used = zneed;
     plan.raster_cfg = cfg;
     plan.batching = dev->batching;
     return WGC_OK;
@@ -2746,7 +2775,11 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
     // the counters are zero here: every submission takes (and zeroes) them
     // at its end, or is followed by this reset when it stopped before that
     if (device->counters_dirty) {
+        // a submission that stopped early may also have left launch-group
+        // words in the arena's zero region
         hipError_t e0 = launch_reset(device->d_frags, device->d_error, device->stream);
+        if (e0 == hipSuccess && device->arena && device->zone_size)
+            e0 = hipMemsetAsync(device->arena, 0, device->zone_size, device->stream);
         if (e0 != hipSuccess) {
             device->slot_pool.push_back(f.slot);
             return hip_fail(e0, "counter reset");
@@ -2760,6 +2793,7 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
         }
     f.index = device->next_index++;
     device->counters_dirty = true;
+    size_t zone_dirty = 0;   // bytes of the zero region the plans launched so far have used
     for (uint32_t i = 0; i < count; ++i) {
         WgcCommandBuffer_T *cb = buffers[i];
         if (!cb) return set_error(WGC_ERR_INVALID, "null command buffer");
@@ -2777,15 +2811,22 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
                 if (r != WGC_OK) return r;
             }
         }
+        if (zone_dirty && plan.zone_used) {
+            // the plans of one submission share the zero region
+            HIP_TRY(launch_fill(device->arena, zone_dirty, 0u, device->stream, nullptr), "zero region");
+            zone_dirty = 0;
+        }
         int32_t r = submit_plan(device, plan);
         if (r != WGC_OK) return r;
+        zone_dirty = std::max(zone_dirty, plan.zone_used);
         f.prims += plan.prims;
         f.launches += plan.launches;
         retain(cb);
         f.keep.push_back(cb);
     }
     // the submission's counters, read by the host once its event has passed
-    HIP_TRY(launch_take_counters(device->d_frags, device->d_error, f.slot, device->stream), "counter readback");
+    HIP_TRY(launch_end_submission(device->d_frags, device->d_error, f.slot, device->arena, zone_dirty,
+                                  device->stream), "counter readback");
     device->counters_dirty = false;
     if (!device->event_pool.empty()) {
         f.done = device->event_pool.back();

@@ -88,8 +88,10 @@ uint32_t raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32
 // Zeroes the submission counters (fragment slots and error bits).
 hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t st);
 // Copies the submission counters to host memory (kFragSlots 64-bit fragment
-// counters, then the 32-bit error word) and zeroes them.
-hipError_t launch_take_counters(unsigned long long *frags, uint32_t *error, void *host_out, hipStream_t st);
+// counters, then the 32-bit error word) and zeroes them, and clears the
+// first zone_bytes (a multiple of 16) of the scratch arena's zero region.
+hipError_t launch_end_submission(unsigned long long *frags, uint32_t *error, void *host_out, void *zone,
+                                 size_t zone_bytes, hipStream_t st);
 // Row copy (4-byte multiples; 16-byte accesses where aligned) and word fill,
 // ordered on the stream like the render launches.
 hipError_t launch_copy_rows(const void *src, uint64_t src_pitch, void *dst, uint64_t dst_pitch, uint64_t row_bytes,
