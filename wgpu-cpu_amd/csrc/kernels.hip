@@ -90,6 +90,23 @@ uint32_t raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32
     }
 }
 
+uint32_t setup_clip_blocks_per_cu(uint32_t program)
+{
+    static uint32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (program >= 8) return 0;
+    uint32_t &v = r[program];
+    if (!v) {
+        switch (program) {
+        case PROG_MVP_COLOR: v = occupancy_of(k_setup<PROG_MVP_COLOR, false, true>, 256); break;
+        case PROG_MVP_TEXTURE: v = occupancy_of(k_setup<PROG_MVP_TEXTURE, false, true>, 256); break;
+        case PROG_COLORED_TRIANGLE: v = occupancy_of(k_setup<PROG_COLORED_TRIANGLE, false, true>, 256); break;
+        case PROG_CLIP_COLOR: v = occupancy_of(k_setup<PROG_CLIP_COLOR, false, true>, 256); break;
+        default: return 0;
+        }
+    }
+    return v;
+}
+
 template <int PROG, bool MULTI>
 static hipError_t launch_group(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
                                KernelTimer *kt)
@@ -101,6 +118,12 @@ static hipError_t launch_group(const DrawParams &d, const BatchArgs &b, const Gr
     }
     if (gl.total_prims > 0) {
         uint32_t g = (gl.total_prims + 255u) / 256u;
+        if constexpr (!MULTI) {
+            if (gl.inline_clip) {
+                WGC_LAUNCH("k_setup", (k_setup<PROG, false, true>), g, 256, d, b);
+                return launch_raster<PROG, MULTI>(d, b, gl, st, kt);
+            }
+        }
         WGC_LAUNCH("k_setup", (k_setup<PROG, MULTI>), g, 256, d, b);
         // one thread per clipped primitive at most; the count is only known on the device
         uint32_t gc = min(max(g / 4u, 1u), 1024u);

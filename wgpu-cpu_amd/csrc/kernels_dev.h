@@ -303,7 +303,13 @@ __device__ __forceinline__ void note_big_wave(const DrawParams &d, bool big, con
     }
 }
 
-template <int PROG, bool MULTI>
+template <int PROG>
+__device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act);
+
+// CLIP: the primitives that cross a clip plane are clipped here, by their
+// wave (single draws whose waves all fit on the GPU at once: no k_clip
+// launch); otherwise they go to the clip list.
+template <int PROG, bool MULTI, bool CLIP = false>
 __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
 {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -362,10 +368,13 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
     }
     bool small = has_rec && ntiles <= (uint32_t)kMaxBinTiles;
     bool big = has_rec && !small;
-    uint32_t slot = wave_append(&d.s.header[H_CLIP], needs_clip);
-    if (needs_clip) {
-        if (slot < d.s.clip_list_cap) d.s.clip_list[slot] = make_uint2(di, p);
-        else atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
+    uint32_t slot = 0;
+    if constexpr (!CLIP) {
+        slot = wave_append(&d.s.header[H_CLIP], needs_clip);
+        if (needs_clip) {
+            if (slot < d.s.clip_list_cap) d.s.clip_list[slot] = make_uint2(di, p);
+            else atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
+        }
     }
     bin_wave(d, small, rec.x, rec.y & 0xFFFFu, rec.y >> 16, rec.z & 0xFFFFu, rec.z >> 16);
     slot = wave_append(&d.s.counters[C_BIG], big);
@@ -374,6 +383,10 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
         else atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
     }
     note_big_wave(d, big && slot < d.s.big_cap, rec);
+    if constexpr (CLIP) {
+        // every lane of the wave takes part (the clip slots are reserved per wave)
+        if (__ballot(needs_clip)) clip_primitive<PROG>(d, di, p, needs_clip);
+    }
 }
 
 // ------------------------------------------------------------------
@@ -770,13 +783,24 @@ struct TriPos {
     __device__ void operator()(float p[4]) const { frag_position(d, T, lc, p); }
 };
 
+// Varyings a built-in program's fragment shader reads (at most; the draw's
+// nvar bounds them too).
+template <int PROG>
+struct VarUsed {
+    static constexpr int value = PROG == PROG_MVP_TEXTURE ? 2 : kMaxVar;
+};
+
 // Varyings (fragment.rs:319-377) and the fragment shader.  Returns false
 // when the fragment writes nothing (a WGSL shader discarded it or aborted).
+// Only the varyings the shader can read are interpolated.
 template <int PROG>
 __device__ __forceinline__ bool fragment_color(const DrawParams &d, const TriLds &T, const float lc[3],
                                                float color[4])
 {
+    constexpr int NV = VarUsed<PROG>::value;
     float var[kMaxVar];
+#pragma unroll
+    for (int c = 0; c < kMaxVar; ++c) var[c] = 0.0f;
 #ifdef WGC_USER_PROGRAM
     if constexpr (PROG == PROG_USER) {
         // per word: flat (first vertex), perspective-corrected or linear
@@ -786,6 +810,7 @@ __device__ __forceinline__ bool fragment_color(const DrawParams &d, const TriLds
         float qs = (q0 + q1) + q2;
 #pragma unroll
         for (int c = 0; c < kMaxVar; ++c) {
+            if ((uint32_t)c >= d.nvar) break;
             if ((d.var_flat_mask >> c) & 1u) {
                 var[c] = T.var[0][c];
             } else if ((d.var_persp_mask >> c) & 1u) {
@@ -808,14 +833,15 @@ __device__ __forceinline__ bool fragment_color(const DrawParams &d, const TriLds
 #endif
     if (d.interp == INTERP_FLAT) {
 #pragma unroll
-        for (int c = 0; c < kMaxVar; ++c) var[c] = T.var[0][c];
+        for (int c = 0; c < NV; ++c) var[c] = T.var[0][c];
     } else if (d.interp == INTERP_PERSPECTIVE) {
         float q0 = lc[0] * T.iw[0];
         float q1 = lc[1] * T.iw[1];
         float q2 = lc[2] * T.iw[2];
         float qs = (q0 + q1) + q2;
 #pragma unroll
-        for (int c = 0; c < kMaxVar; ++c) {
+        for (int c = 0; c < NV; ++c) {
+            if ((uint32_t)c >= d.nvar) break;
             float acc = T.var[0][c] * q0;
             acc = acc + T.var[1][c] * q1;
             acc = acc + T.var[2][c] * q2;
@@ -823,7 +849,8 @@ __device__ __forceinline__ bool fragment_color(const DrawParams &d, const TriLds
         }
     } else {
 #pragma unroll
-        for (int c = 0; c < kMaxVar; ++c) {
+        for (int c = 0; c < NV; ++c) {
+            if ((uint32_t)c >= d.nvar) break;
             float acc = T.var[0][c] * lc[0];
             acc = acc + T.var[1][c] * lc[1];
             acc = acc + T.var[2][c] * lc[2];

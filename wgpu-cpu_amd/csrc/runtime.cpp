@@ -2242,6 +2242,13 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint32_t *fra
     gl.tile_count = (uint32_t *)(zone + align_up(4ull * H_NUM, 256) + align_up(4ull * C_NUM * n, 256));
     gl.clip_list = (uint2 *)(base + o_cl);
     gl.clip_cap = (uint32_t)prims;
+    // a single draw of a built-in program whose setup workgroups are all
+    // resident at once clips in setup (no separate clip launch)
+    gl.inline_clip = 0;
+    if (!st.multi && !gl.user && !is_point_or_line(st.draws[0])) {
+        uint64_t blocks = (prims + 255) / 256;
+        gl.inline_clip = blocks <= (uint64_t)dev->cu_count * setup_clip_blocks_per_cu(gl.program) ? 1u : 0u;
+    }
     gl.vis = nullptr;
     b.n = n;
     b.total_tiles = (uint32_t)total_tiles;
@@ -2627,7 +2634,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         }
         layout_group(s, dev->arena + dev->zone_size + s.arena_off, dev->arena + s.zone_off, dev->frag_px,
                      dev->frag_px_size, dev);
-        plan.launches += (s.gl.total_prims > 0 ? 3 : 1) + (!s.multi && restart_strip(s.draws[0]) ? 2 : 0);
+        plan.launches += (s.gl.total_prims > 0 ? (s.gl.inline_clip ? 2 : 3) : 1) + (!s.multi && restart_strip(s.draws[0]) ? 2 : 0);
         for (auto &d : s.draws) plan.prims += d.total_prims;
         if (!s.multi) continue;
         uint32_t n = (uint32_t)s.draws.size();

@@ -334,20 +334,22 @@ __device__ __forceinline__ bool run_vs(const DrawParams &d, uint32_t instance, u
                                        VsOut &o)
 {
     if constexpr (PROG == PROG_MVP_COLOR || PROG == PROG_MVP_TEXTURE) {
+        // both attributes are fetched before either is used
         float pos[4];
-        if (!fetch_attr<0>(d, instance, vertex, pos, 4)) return false;
+        o.var[0] = o.var[1] = o.var[2] = o.var[3] = 0.0f;
+        bool ok0 = fetch_attr<0>(d, instance, vertex, pos, 4);
+        bool ok1 = fetch_attr<1>(d, instance, vertex, o.var, d.nvar);
+        if (!ok0 || !ok1) return false;
         float m[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) m[k] = d.uniform[k];
         mat4_mul_vec4(m, pos, o.clip);
-        o.var[0] = o.var[1] = o.var[2] = o.var[3] = 0.0f;
-        if (!fetch_attr<1>(d, instance, vertex, o.var, d.nvar)) return false;
         return true;
     } else if constexpr (PROG == PROG_CLIP_COLOR) {
-        if (!fetch_attr<0>(d, instance, vertex, o.clip, 4)) return false;
         o.var[0] = o.var[1] = o.var[2] = o.var[3] = 0.0f;
-        if (!fetch_attr<1>(d, instance, vertex, o.var, d.nvar)) return false;
-        return true;
+        bool ok0 = fetch_attr<0>(d, instance, vertex, o.clip, 4);
+        bool ok1 = fetch_attr<1>(d, instance, vertex, o.var, d.nvar);
+        return ok0 && ok1;
     } else {
         // colored_triangle.wgsl
         uint32_t vi = vertex % 3u;
@@ -457,27 +459,40 @@ __device__ __forceinline__ bool shade_primitive(const DrawParams &d, uint32_t p,
         if (pr < 0) atomicOr(d.s.error, (uint32_t)ERR_INDEX);
         return false;
     }
+    // every index first, then every vertex: no vertex's loads wait for
+    // another vertex's
+    uint32_t vertex[N];
+    bool iok = true;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        uint32_t vertex;
-        if (!resolve_index(d, d.first + pos[k], vertex)) {
-            atomicOr(d.s.error, (uint32_t)ERR_INDEX);
-            return false;
-        }
+        bool ok = resolve_index(d, d.first + pos[k], vertex[k]);
+        iok = iok && ok;
+    }
+    if (!iok) {
+        atomicOr(d.s.error, (uint32_t)ERR_INDEX);
+        return false;
+    }
 #ifdef WGC_USER_PROGRAM
-        if constexpr (PROG == PROG_USER) {
-            uint32_t e = user_vertex(d, inst, vertex, v[k]);
-            if (e) {
-                atomicOr(d.s.error, e);
-                return false;
-            }
-            continue;
-        }
-#endif
-        if (!run_vs<PROG>(d, inst, vertex, v[k])) {
-            atomicOr(d.s.error, (uint32_t)ERR_VERTEX);
+    if constexpr (PROG == PROG_USER) {
+        uint32_t e = 0u;
+#pragma unroll
+        for (int k = 0; k < N; ++k) e |= user_vertex(d, inst, vertex[k], v[k]);
+        if (e) {
+            atomicOr(d.s.error, e);
             return false;
         }
+        return true;
+    }
+#endif
+    bool vok = true;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        bool ok = run_vs<PROG>(d, inst, vertex[k], v[k]);
+        vok = vok && ok;
+    }
+    if (!vok) {
+        atomicOr(d.s.error, (uint32_t)ERR_VERTEX);
+        return false;
     }
     return true;
 }
@@ -873,9 +888,12 @@ __device__ __forceinline__ void sample_bilinear(const DrawParams &d, const TexVi
     int32_t iu = (int32_t)fmaxf(fminf(u0f, 1073741824.0f), -1073741824.0f);
     int32_t iv = (int32_t)fmaxf(fminf(v0f, 1073741824.0f), -1073741824.0f);
     uint32_t x0 = (uint32_t)wrap_index(iu, tv.addr_u, tv.w);
-    uint32_t x1 = (uint32_t)wrap_index(iu + 1, tv.addr_u, tv.w);
     uint32_t y0 = (uint32_t)wrap_index(iv, tv.addr_v, tv.h);
-    uint32_t y1 = (uint32_t)wrap_index(iv + 1, tv.addr_v, tv.h);
+    // Repeat: (i + 1) mod n is the next index after i mod n, wrapped
+    uint32_t x1 = tv.addr_u == ADDR_REPEAT ? (x0 + 1u < tv.w ? x0 + 1u : 0u)
+                                           : (uint32_t)wrap_index(iu + 1, tv.addr_u, tv.w);
+    uint32_t y1 = tv.addr_v == ADDR_REPEAT ? (y0 + 1u < tv.h ? y0 + 1u : 0u)
+                                           : (uint32_t)wrap_index(iv + 1, tv.addr_v, tv.h);
     float t00[4], t10[4], t01[4], t11[4];
     fetch_texel(d, tv, x0, y0, t00);
     fetch_texel(d, tv, x1, y0, t10);

@@ -21,6 +21,7 @@ struct GroupLaunch {
     uint32_t *tile_count, *counters, *header;
     uint2 *clip_list;            // (draw, primitive) pairs that cross a clip plane
     uint32_t clip_cap;
+    uint32_t inline_clip;        // single draw whose setup waves all fit at once: setup clips
     unsigned long long *vis;     // points and lines: one visibility value per pixel
 };
 
@@ -85,6 +86,9 @@ hipError_t launch_batch(const DrawParams &d0, const BatchArgs &b, const GroupLau
 // Workgroups of the raster kernel for (program, ordered, batched, tile
 // shape) that fit on one CU at once.
 uint32_t raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32_t cfg);
+// Workgroups per CU of the setup kernel that also clips (single draws of a
+// built-in program); 0 for other programs.
+uint32_t setup_clip_blocks_per_cu(uint32_t program);
 // Zeroes the submission counters (fragment slots and error bits).
 hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t st);
 // Copies the submission counters to host memory (kFragSlots 64-bit fragment
