@@ -122,11 +122,28 @@ __device__ __forceinline__ void spill_entry(const DrawParams &d, uint32_t at, ui
     else atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
 }
 
-// A tile binned for the first time: to the active list (each tile is
-// appended once, by the record that took position 0 of its bin).
-__device__ __forceinline__ void activate_tile(const DrawParams &d, uint32_t at, uint32_t tile)
+// Whether bin position `slot` starts a work item of its tile (position 0,
+// and every `split` positions inside the bin, up to kMaxItems items), and
+// which.  Each position is taken once, so each item is appended once.
+__device__ __forceinline__ bool item_start(const DrawParams &d, uint32_t slot, uint32_t &item)
 {
-    if (at < d.s.active_cap) d.s.active_list[at] = d.s.tile0 + tile;
+    item = 0u;
+    if (slot == 0u) return true;
+    uint32_t sp = d.s.split;
+    if (sp == 0u || slot >= d.s.bin_cap || slot >= kMaxItems * sp) return false;
+    // slot / sp for slot < 16 * sp < 2^24: the binary32 product is within one
+    // of the quotient, one correction either way makes it exact
+    uint32_t q = (uint32_t)((float)slot * d.s.split_inv);
+    if (q * sp > slot) --q;
+    else if ((q + 1u) * sp <= slot) ++q;
+    item = q;
+    return q * sp == slot;
+}
+
+// A work item of a tile to the active list.
+__device__ __forceinline__ void activate_item(const DrawParams &d, uint32_t at, uint32_t tile, uint32_t item)
+{
+    if (at < d.s.active_cap) d.s.active_list[at] = (d.s.tile0 + tile) | (item << kItemShift);
     else atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
 }
 
@@ -148,7 +165,8 @@ __device__ __forceinline__ void bin_lane(const DrawParams &d, uint32_t key, uint
         uint32_t t = footprint_tile(d, tx0, ty0, w, (uint32_t)k), slot = got[k];
         if (slot < d.s.bin_cap) d.s.bins[(size_t)t * d.s.bin_cap + slot] = key;
         else spill_entry(d, atomicAdd(&d.s.header[H_SPILL], 1u), t, key);
-        if (slot == 0u) activate_tile(d, atomicAdd(&d.s.header[H_ACTIVE], 1u), t);
+        uint32_t item;
+        if (item_start(d, slot, item)) activate_item(d, atomicAdd(&d.s.header[H_ACTIVE], 1u), t, item);
     }
 }
 
@@ -160,9 +178,10 @@ __device__ __forceinline__ void bin_lane(const DrawParams &d, uint32_t key, uint
 // and every lane left alone, takes the group's bin positions with one atomic
 // add.  All the adds of the wave are issued before any result is used, so
 // the wave waits for them once.  Then the keys go to their positions, the
-// entries past a full bin to the spill list and the tiles binned for the
-// first time to the active list (one atomic per list per wave).  Every lane
-// of the wave must call it.
+// entries past a full bin to the spill list and the work items the taken
+// positions start (position 0 of a tile, and every item's first position
+// of a tile with more entries than one item holds) to the active list (one
+// atomic per list per wave).  Every lane of the wave must call it.
 __device__ __forceinline__ void bin_wave(const DrawParams &d, bool has, uint32_t key, uint32_t tx0, uint32_t ty0,
                                          uint32_t tx1, uint32_t ty1)
 {
@@ -200,7 +219,8 @@ __device__ __forceinline__ void bin_wave(const DrawParams &d, bool has, uint32_t
         if (act && lead == lane) got[k] = atomicAdd(&d.s.tile_count[t], cnt);
         grp[k] = act ? ((1u << 16) | (rank << 8) | lead) : 0u;
     }
-    uint32_t first = 0u, spilled = 0u;   // bit k: the k-th tile
+    uint32_t first = 0u, spilled = 0u;   // bit k: the k-th tile (starts an item; spilled)
+    uint64_t items = 0u;                  // 4 bits per tile: the item it starts
 #pragma unroll
     for (int k = 0; k < kMaxBinTiles; ++k) {
         if ((uint32_t)k >= nk) break;
@@ -212,13 +232,20 @@ __device__ __forceinline__ void bin_wave(const DrawParams &d, bool has, uint32_t
             uint32_t t = footprint_tile(d, tx0, ty0, w, (uint32_t)k);
             if (slot < d.s.bin_cap) d.s.bins[(size_t)t * d.s.bin_cap + slot] = key;
             else spilled |= 1u << k;
-            if (lead == lane && base == 0u) first |= 1u << k;
+            uint32_t item;
+            if (item_start(d, slot, item)) {
+                first |= 1u << k;
+                items |= (uint64_t)item << (4 * k);
+            }
         }
     }
     uint32_t nf = (uint32_t)__popc(first), ns = (uint32_t)__popc(spilled);
     if (__ballot(nf != 0u)) {
         uint32_t at = wave_add(&d.s.header[H_ACTIVE], nf, nf != 0u);
-        for (uint32_t m = first; m; m &= m - 1u) activate_tile(d, at++, footprint_tile(d, tx0, ty0, w, __ffs(m) - 1u));
+        for (uint32_t m = first; m; m &= m - 1u) {
+            uint32_t k = __ffs(m) - 1u;
+            activate_item(d, at++, footprint_tile(d, tx0, ty0, w, k), (uint32_t)(items >> (4 * k)) & 15u);
+        }
     }
     if (__ballot(ns != 0u)) {
         uint32_t at = wave_add(&d.s.header[H_SPILL], ns, ns != 0u);
@@ -558,26 +585,34 @@ __device__ __forceinline__ bool in_big_union(const DrawParams &d, uint32_t tile)
            ty >= 65535u - d.s.counters[C_BIG_NY0] && ty <= d.s.counters[C_BIG_Y1];
 }
 
-// The sources of a tile's triangles, in the order the raster reads them:
-// its bin, its spill entries (only when the bin overflowed: the spill list
-// is filtered by tile), the big triangles whose footprint holds the tile.
+// The sources of a work item's triangles, in the order the raster reads
+// them: its range of the tile's bin; for the tile's last item the rest of
+// the bin, the spill entries (only when the bin overflowed: the spill list
+// is filtered by tile) and the big triangles whose footprint holds the tile.
 struct TileFeed {
     uint32_t bin0, nbin, pos;     // bin: first slot, entries, next entry
     uint32_t nspill, spos;        // spill list: entries, next entry
     uint32_t nbig, bpos;          // big list: entries, next entry
     uint32_t gt, tx, ty;          // global tile, tile coordinates
+    uint32_t items;               // the tile's work items
 };
 
-__device__ __forceinline__ TileFeed tile_feed(const DrawParams &d, uint32_t tile)
+__device__ __forceinline__ TileFeed tile_feed(const DrawParams &d, uint32_t tile, uint32_t item = 0u)
 {
     TileFeed f;
     uint32_t cnt = d.s.tile_count[tile];
-    f.bin0 = tile * d.s.bin_cap;
-    f.nbin = min(cnt, d.s.bin_cap);
+    uint32_t nbin = min(cnt, d.s.bin_cap), sp = d.s.split;
+    f.items = 1u;
+    if (sp) f.items = max(1u, min((nbin + sp - 1u) / sp, kMaxItems));
+    item = min(item, f.items - 1u);
+    bool last = item == f.items - 1u;
+    uint32_t start = item * sp;
+    f.bin0 = tile * d.s.bin_cap + start;
+    f.nbin = last ? nbin - start : sp;
     f.pos = 0u;
-    f.nspill = cnt > d.s.bin_cap ? min(d.s.header[H_SPILL], d.s.spill_cap) : 0u;
+    f.nspill = last && cnt > d.s.bin_cap ? min(d.s.header[H_SPILL], d.s.spill_cap) : 0u;
     f.spos = 0u;
-    f.nbig = min(d.s.counters[C_BIG], d.s.big_cap);
+    f.nbig = last ? min(d.s.counters[C_BIG], d.s.big_cap) : 0u;
     f.bpos = 0u;
     f.gt = d.s.tile0 + tile;
     f.tx = tile % d.tiles_x;
@@ -1157,6 +1192,7 @@ struct VisShared {
     uint32_t cnt[TS * TS];             // emitted fragments (debug counts only)
     uint32_t wsum[NT / 64];
     uint32_t nkeys;
+    uint32_t last;                     // split tiles: this workgroup finished the tile's last item
 };
 
 // Spreads consecutive items over the waves of the workgroup (item i goes to
@@ -1391,14 +1427,14 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
 // depth are stored where the pass changes them (a winner, or the
 // LoadOp::Clear values).
 template <int PROG, int TS, int NT>
-__device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t tile, VisShared<TS, NT> &sh,
-                                               uint32_t &frag_acc)
+__device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t tile, uint32_t item,
+                                               VisShared<TS, NT> &sh, uint32_t &frag_acc)
 {
     constexpr int PPT = TS * TS / NT;
     constexpr uint32_t NB = VisShared<TS, NT>::NB;
     uint32_t x0 = (tile % d.tiles_x) * TS, y0 = (tile / d.tiles_x) * TS;
     uint32_t tid = threadIdx.x;
-    TileFeed feed = tile_feed(d, tile);
+    TileFeed feed = tile_feed(d, tile, item);
 
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
@@ -1418,6 +1454,44 @@ __device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t til
         uint32_t n = next_batch(d, feed, sh.keys, &sh.nkeys, NB);
         if (n == 0u) break;
         f += vis_batch<TS, NT>(d, sh, n, x0, y0);
+    }
+    frag_acc += f;
+
+    bool split = feed.items > 1u;
+    if (split) {
+        // a split tile: the item's winners merge into the tile's (one 64-bit
+        // maximum of the complemented value per pixel, so that zero means
+        // none); the item that finishes last shades the tile
+        unsigned long long *gv = d.s.gvis + (size_t)tile * (TS * TS);
+        unsigned long long done = 0ull;
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            uint32_t q = tid + (uint32_t)k * NT;
+            unsigned long long v = sh.best[q];
+            if (v != ~0ull) done |= atomicMax(&gv[q], ~v);
+            uint32_t px = x0 + q % TS, py = y0 + q / TS;
+            if (d.s.frag_px && sh.cnt[q] && px < d.width && py < d.height)
+                atomicAdd(&d.s.frag_px[(size_t)py * d.width + px], sh.cnt[q]);
+        }
+        // the maxima have returned (their results are used here), so they
+        // were performed before this item's ticket below
+        asm volatile("" : : "v"(done));
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t last = atomicAdd(&d.s.tile_done[tile], 1u) == feed.items - 1u ? 1u : 0u;
+            if (last) __threadfence();  // the tile's merged winners are read fresh
+            sh.last = last;
+        }
+        __syncthreads();
+        if (sh.last == 0u) return;
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            uint32_t q = tid + (uint32_t)k * NT;
+            unsigned long long g = gv[q];
+            sh.best[q] = g ? ~g : ~0ull;
+            gv[q] = 0ull;  // the buffer is kept zero between uses
+        }
+        // each thread shades the pixels it read: no barrier needed
     }
 
     bool write_depth_mode = d.depth_test && d.depth_write &&
@@ -1442,10 +1516,9 @@ __device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t til
             if (d.color && d.clear_color_on) reinterpret_cast<uint32_t *>(d.color)[pix] = d.clear_rgba;
             if (d.depth && d.clear_depth_on) d.depth[pix] = d.clear_depth;
         }
-        if (d.s.frag_px && sh.cnt[q]) d.s.frag_px[pix] += sh.cnt[q];
+        if (d.s.frag_px && sh.cnt[q] && !split) d.s.frag_px[pix] += sh.cnt[q];
     }
     __syncthreads();  // the next tile resets the per-pixel state
-    frag_acc += f;
 }
 
 // Stores the clear values into one tile.
@@ -1522,10 +1595,14 @@ struct XcdSchedule {
     }
 };
 
+// Active-list entry `item`: its draw, tile and the tile's work item.
 template <bool MULTI>
-__device__ __forceinline__ void locate_tile(const BatchArgs &b, uint32_t item, uint32_t &di, uint32_t &tile)
+__device__ __forceinline__ void locate_tile(const BatchArgs &b, uint32_t item, uint32_t &di, uint32_t &tile,
+                                            uint32_t &it)
 {
-    uint32_t gt = __builtin_amdgcn_readfirstlane(b.active_list[item]);
+    uint32_t e = __builtin_amdgcn_readfirstlane(b.active_list[item]);
+    uint32_t gt = e & ((1u << kItemShift) - 1u);
+    it = e >> kItemShift;
     di = 0;
     tile = gt;
     if constexpr (MULTI) {
@@ -1564,13 +1641,13 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
     __shared__ uint32_t wave_frags[NT / 64];
     __shared__ uint32_t kinds[kClearChunk];
 
-    XcdSchedule w(min(b.header[H_ACTIVE], b.total_tiles));
+    XcdSchedule w(min(b.header[H_ACTIVE], b.total_tiles * kMaxItems));
     uint32_t frag_acc = 0;
     for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
         uint32_t item = w.xcd * w.per_xcd + i;
         if (item >= w.n) break;
-        uint32_t di, tile;
-        locate_tile<MULTI>(b, item, di, tile);
+        uint32_t di, tile, it;
+        locate_tile<MULTI>(b, item, di, tile, it);
         raster_tile<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, olist, olist_n, frag_acc);
     }
     // the last workgroups have the fewest active tiles: chunks from there
@@ -1596,20 +1673,20 @@ __global__ __launch_bounds__(NT, NT == 256 && TS == 16 ? 6 : 1) void k_tiles(Dra
     __shared__ uint32_t wave_frags[NT / 64];
     __shared__ uint32_t kinds[kClearChunk];
 
-    XcdSchedule w(min(b.header[H_ACTIVE], b.total_tiles));
+    XcdSchedule w(min(b.header[H_ACTIVE], b.total_tiles * kMaxItems));
     uint32_t frag_acc = 0;
     for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
         uint32_t item = w.xcd * w.per_xcd + i;
         if (item >= w.n) break;
-        uint32_t di, tile;
-        locate_tile<MULTI>(b, item, di, tile);
-        raster_tile_oi<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, frag_acc);
+        uint32_t di, tile, it;
+        locate_tile<MULTI>(b, item, di, tile, it);
+        raster_tile_oi<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, it, sh, frag_acc);
     }
     // the last workgroups have the fewest active tiles: chunks from there
     if (untouched_pass(b))
         for (uint32_t c = gridDim.x - 1u - blockIdx.x; c < b.n_chunks; c += gridDim.x)
             untouched_chunk<MULTI, TS, NT>(d0, b, c, kinds, [&](const DrawParams &d, uint32_t tile) {
-                raster_tile_oi<PROG, TS, NT>(d, tile, sh, frag_acc);
+                raster_tile_oi<PROG, TS, NT>(d, tile, 0u, sh, frag_acc);
             });
     count_fragments<MULTI, NT>(d0, b, frag_acc, wave_frags);
 }

@@ -374,6 +374,7 @@ struct Step {
     size_t arena_off = 0;            // this group's scratch within the arena (past the zero region)
     size_t zone_off = 0;             // this group's words in the arena's zero region (not shared
                                      // with the plan's other groups; cleared when the submission ends)
+    size_t keep_off = 0;             // this group's split-tile winners in the kept-zero region
     // clear
     uint32_t *color = nullptr;
     uint32_t rgba = 0;
@@ -472,6 +473,7 @@ struct WgcDevice_T : Object {
     uint8_t *arena = nullptr;
     size_t arena_size = 0;
     size_t zone_size = 0;
+    size_t keep_size = 0;   // then keep_size bytes that every use leaves zero (split-tile winners)
     // persistent device words
     DevCounters *d_counters = nullptr;
     bool counters_dirty = false;            // a submission stopped before taking its counters
@@ -2169,17 +2171,49 @@ static size_t layout_lines(Step &st, uint8_t *base, uint32_t *frag_px, size_t fr
     return bytes;
 }
 
-// A group's words in the zero region: its header, per-draw counters and tile
-// counts.
-static size_t zone_bytes(const Step &st)
+static uint64_t group_tiles(const Step &st)
 {
     uint64_t tiles = 0;
     for (const DrawParams &d : st.draws) tiles += (uint64_t)d.tiles_x * d.tiles_y;
-    return align_up(4ull * H_NUM, 256) + align_up(4ull * C_NUM * st.draws.size(), 256) + align_up(4ull * tiles, 256);
+    return tiles;
 }
 
-static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint32_t *frag_px, size_t frag_px_size,
-                           WgcDevice_T *dev)
+// A group's words in the zero region: its header, per-draw counters, tile
+// counts and split-tile tickets.
+static size_t zone_bytes(const Step &st)
+{
+    uint64_t tiles = group_tiles(st);
+    return align_up(4ull * H_NUM, 256) + align_up(4ull * C_NUM * st.draws.size(), 256) + 2 * align_up(4ull * tiles, 256);
+}
+
+// A group's bytes in the kept-zero region: the merged winners of split
+// tiles (order-independent groups only), 8 bytes per tile pixel.
+static size_t keep_bytes(const Step &st)
+{
+    if (st.gl.ordered || st.multi || st.draws.empty()) return 0;
+    uint64_t tile_px = 1ull << (2 * st.draws[0].tile_shift);
+    return align_up(8ull * tile_px * group_tiles(st), 256);
+}
+
+// Bin entries per work item of a tile: two visibility batches of the
+// group's tile shape, so that a crowded tile (three batches or more) spreads
+// over several workgroups while ordinary ones stay whole (a single pass
+// already has about one tile per resident workgroup).  Single
+// order-independent draws only: ordered tiles are never split, and a
+// batched group has many tiles per workgroup anyway.
+static uint32_t split_for(const Step &st)
+{
+    if (st.gl.ordered || st.multi) return 0;
+    if (st.gl.user) return 192;                     // 16x16 tiles of 256 threads: 96-triangle batches
+    switch (st.gl.raster_cfg) {
+    case RASTER_32x32_256: return 256;
+    case RASTER_8x8_64: case RASTER_16x16_64: return 128;
+    default: return 192;
+    }
+}
+
+static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep, uint32_t *frag_px,
+                           size_t frag_px_size, WgcDevice_T *dev)
 {
     uint32_t n = (uint32_t)st.draws.size();
     uint64_t total_tiles = 0, clip_rec_cap = 0, prims = 0, prims_padded = 0;
@@ -2217,7 +2251,7 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint32_t *fra
         off = align_up(off + std::max<size_t>(bytes, 4), 256);
         return o;
     };
-    size_t o_al = take(4 * total_tiles);
+    size_t o_al = take(4 * total_tiles * kMaxItems);
     size_t o_bins = take(4 * bins_total), o_sp = take(8 * spill_cap);
     size_t o_cl = take(8 * prims);
     std::vector<size_t> o_big(n), o_tri(n), o_ctri(n), o_ct(n), o_cb(n), o_ss(n);
@@ -2240,6 +2274,8 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint32_t *fra
     gl.header = (uint32_t *)zone;
     gl.counters = (uint32_t *)(zone + align_up(4ull * H_NUM, 256));
     gl.tile_count = (uint32_t *)(zone + align_up(4ull * H_NUM, 256) + align_up(4ull * C_NUM * n, 256));
+    uint32_t *tile_done = gl.tile_count + align_up(4ull * total_tiles, 256) / 4;
+    uint32_t split = split_for(st);
     gl.clip_list = (uint2 *)(base + o_cl);
     gl.clip_cap = (uint32_t)prims;
     // a single draw of a built-in program whose setup workgroups are all
@@ -2280,6 +2316,10 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint32_t *fra
         DrawParams &d = st.draws[i];
         Scratch &s = d.s;
         s.tile_count = gl.tile_count + tile_base;
+        s.tile_done = tile_done + tile_base;
+        s.gvis = keep_bytes(st) ? (unsigned long long *)keep + tile_base * tile_px : nullptr;
+        s.split = split;
+        s.split_inv = split ? 1.0f / (float)split : 0.0f;
         s.bins = (uint32_t *)(base + o_bins) + bin_base;
         bin_base += bin_cap[i] * std::max<uint64_t>((uint64_t)d.tiles_x * d.tiles_y, 1);
         s.spill = (uint2 *)(base + o_sp);
@@ -2300,7 +2340,7 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint32_t *fra
         s.tile0 = (uint32_t)tile_base;
         s.bin_cap = (uint32_t)bin_cap[i];
         s.spill_cap = (uint32_t)std::min<uint64_t>(spill_cap, 0xFFFFFFFFull);
-        s.active_cap = (uint32_t)total_tiles;
+        s.active_cap = (uint32_t)(total_tiles * kMaxItems);
         s.big_cap = (uint32_t)(d.total_prims + clip_cap[i]);
         s.clip_list_cap = (uint32_t)prims;
         s.clip_tri_cap = (uint32_t)clip_cap[i];
@@ -2562,7 +2602,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
     }
     // 3. scratch: the groups of a stage side by side, stages reusing the
     //    arena (they run one after another)
-    size_t need = 0, zneed = 0;
+    size_t need = 0, zneed = 0, kneed = 0;
     size_t tables = 0;
     size_t stage_off = 0;
     uint32_t cur_stage = UINT32_MAX;
@@ -2574,11 +2614,18 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         }
         s.arena_off = stage_off;
         s.zone_off = zneed;
+        s.keep_off = kneed;
         size_t bytes = s.kind == StepKind::Lines ? layout_lines(s, nullptr, nullptr, 0, dev)
-                                                  : layout_group(s, nullptr, nullptr, nullptr, 0, dev);
+                                                  : layout_group(s, nullptr, nullptr, nullptr, nullptr, 0, dev);
         stage_off = align_up(stage_off + bytes, 256);
         need = std::max(need, stage_off);
-        if (s.kind == StepKind::Group) zneed += zone_bytes(s);
+        if (s.kind == StepKind::Group) {
+            // active-list entries hold the global tile in kItemShift bits
+            if (group_tiles(s) >= (1ull << kItemShift))
+                return set_error(WGC_ERR_INVALID, "too many tiles in one launch group");
+            zneed += zone_bytes(s);
+            kneed += keep_bytes(s);
+        }
         if (s.multi) {
             // draws, primitive and tile prefixes, the draw of every wave of
             // primitives and of every tile
@@ -2591,20 +2638,23 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
             tables = align_up(tables + n * sizeof(DrawParams) + 8 * (n + 1) + 4 * (waves + tiles), 256);
         }
     }
-    if (zneed > dev->zone_size || dev->zone_size + need > dev->arena_size) {
+    if (zneed > dev->zone_size || kneed > dev->keep_size ||
+        dev->zone_size + dev->keep_size + need > dev->arena_size) {
         HIP_TRY(hipStreamSynchronize(dev->stream), "arena");
         if (dev->arena) HIP_TRY(hipFree(dev->arena), "arena");
         dev->arena = nullptr;
         dev->arena_size = 0;
         size_t zone = align_up(std::max(zneed, dev->zone_size), 4096);
-        size_t grow = zone + std::max(need, (size_t)64 << 20);
+        size_t keep = align_up(std::max(kneed, dev->keep_size), 4096);
+        size_t grow = zone + keep + std::max(need, (size_t)64 << 20);
         if (hipMalloc(&dev->arena, grow) != hipSuccess) {
-            dev->zone_size = 0;
+            dev->zone_size = dev->keep_size = 0;
             return set_error(WGC_ERR_OUT_OF_MEMORY, "scratch arena");
         }
-        HIP_TRY(hipMemsetAsync(dev->arena, 0, zone, dev->stream), "arena zero region");
+        HIP_TRY(hipMemsetAsync(dev->arena, 0, zone + keep, dev->stream), "arena zero regions");
         dev->arena_size = grow;
         dev->zone_size = zone;
+        dev->keep_size = keep;
         dev->arena_gen++;
     }
     // draw tables: pooled device memory, uploaded from pooled pinned staging
@@ -2623,7 +2673,8 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
     }
     for (Step &s : plan.steps) {
         if (s.kind == StepKind::Lines) {
-            layout_lines(s, dev->arena + dev->zone_size + s.arena_off, dev->frag_px, dev->frag_px_size, dev);
+            layout_lines(s, dev->arena + dev->zone_size + dev->keep_size + s.arena_off, dev->frag_px,
+                         dev->frag_px_size, dev);
             plan.launches += s.gl.ordered ? 1 : 3;
             plan.prims += s.draws[0].total_prims;
             continue;
@@ -2632,8 +2683,8 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
             plan.launches += 1;
             continue;
         }
-        layout_group(s, dev->arena + dev->zone_size + s.arena_off, dev->arena + s.zone_off, dev->frag_px,
-                     dev->frag_px_size, dev);
+        layout_group(s, dev->arena + dev->zone_size + dev->keep_size + s.arena_off, dev->arena + s.zone_off,
+                     dev->arena + dev->zone_size + s.keep_off, dev->frag_px, dev->frag_px_size, dev);
         plan.launches += (s.gl.total_prims > 0 ? (s.gl.inline_clip ? 2 : 3) : 1) + (!s.multi && restart_strip(s.draws[0]) ? 2 : 0);
         for (auto &d : s.draws) plan.prims += d.total_prims;
         if (!s.multi) continue;
@@ -2785,8 +2836,8 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
         // a submission that stopped early may also have left launch-group
         // words in the arena's zero region
         hipError_t e0 = launch_reset(device->d_frags, device->d_error, device->stream);
-        if (e0 == hipSuccess && device->arena && device->zone_size)
-            e0 = hipMemsetAsync(device->arena, 0, device->zone_size, device->stream);
+        if (e0 == hipSuccess && device->arena && device->zone_size + device->keep_size)
+            e0 = hipMemsetAsync(device->arena, 0, device->zone_size + device->keep_size, device->stream);
         if (e0 != hipSuccess) {
             device->slot_pool.push_back(f.slot);
             return hip_fail(e0, "counter reset");

@@ -122,8 +122,12 @@ struct Scratch {
     unsigned long long *frags;  // kFragSlots fragment counters of the submission (summed on read)
     uint32_t *error;
     uint32_t *frag_px;       // optional per-pixel fragment count
+    unsigned long long *gvis;   // split tiles: the merged winners, TS*TS per tile (kept zero between uses)
+    uint32_t *tile_done;     // split tiles: their items that have finished (zero region)
     uint32_t tile0;          // global index of the draw's first tile
     uint32_t bin_cap;        // slots per tile bin
+    uint32_t split;          // bin entries per work item of a tile (0: one item per tile)
+    float split_inv;         // 1 / split
     uint32_t spill_cap, active_cap;
     uint32_t big_cap, clip_list_cap, clip_tri_cap;
     int32_t *strip_sep;      // restart triangle strips: the last separator position up to the end of
@@ -131,6 +135,13 @@ struct Scratch {
 };
 
 constexpr int kFragSlots = 64;  // fragment counters, spread so that atomics rarely meet
+
+// Work items of the tile kernels: a tile with more bin entries than one
+// item holds is split into items of `split` entries (at most kMaxItems; the
+// last one takes the rest, the spill entries and the big triangles).
+// Active-list entries are global tile | item << kItemShift.
+constexpr uint32_t kMaxItems = 16;
+constexpr uint32_t kItemShift = 24;
 
 // Per-draw counters.  The big-triangle footprint union is kept as maxima of
 // (65535 - min) and max so that zero-initialised counters work.
