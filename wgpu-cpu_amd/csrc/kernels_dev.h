@@ -830,7 +830,7 @@ struct VarUsed {
 // Only the varyings the shader can read are interpolated.
 template <int PROG>
 __device__ __forceinline__ bool fragment_color(const DrawParams &d, const TriLds &T, const float lc[3],
-                                               float color[4])
+                                               float color[4], const float *srgb)
 {
     constexpr int NV = VarUsed<PROG>::value;
     float var[kMaxVar];
@@ -892,7 +892,7 @@ __device__ __forceinline__ bool fragment_color(const DrawParams &d, const TriLds
             var[c] = acc;
         }
     }
-    run_fs<PROG>(d, var, color);
+    run_fs<PROG>(d, var, color, srgb);
     return true;
 }
 
@@ -912,7 +912,7 @@ __device__ __forceinline__ void shade_fragment(const DrawParams &d, const TriLds
     if (ORDERED) {
         if (d.depth_test && !compare(d.depth_compare, z, ps.dep)) return;
         float color[4];
-        if (!fragment_color<PROG>(d, T, lc, color)) return;
+        if (!fragment_color<PROG>(d, T, lc, color, d.srgb_lut)) return;
         if (d.depth_test && d.depth_write) ps.dep = z;
         ps.col = d.blend ? blend_over(color, ps.col, d.color_bgra) : pack_color(color, d.color_bgra);
         ps.has = true;
@@ -948,7 +948,8 @@ __device__ __forceinline__ void shade_fragment(const DrawParams &d, const TriLds
 // read back (from L2, shared by the neighbouring pixels it won) and the same
 // barycentrics are evaluated again, bit for bit.
 template <int PROG>
-__device__ __forceinline__ uint32_t winner_color(const DrawParams &d, uint32_t key, float pxf, float pyf, float &z)
+__device__ __forceinline__ uint32_t winner_color(const DrawParams &d, uint32_t key, float pxf, float pyf, float &z,
+                                                 const float *srgb)
 {
     TriLds T;
     ScanTri S;
@@ -959,7 +960,7 @@ __device__ __forceinline__ uint32_t winner_color(const DrawParams &d, uint32_t k
     float color[4];
     // pipelines whose fragment shader can discard use the ordered path, so
     // the winner always writes (an aborted invocation raised ERR_SHADER)
-    (void)fragment_color<PROG>(d, T, lc, color);
+    (void)fragment_color<PROG>(d, T, lc, color, srgb);
     return pack_color(color, d.color_bgra);
 }
 
@@ -1152,7 +1153,8 @@ __device__ __forceinline__ void raster_tile(const DrawParams &d, uint32_t tile, 
                           (d.resolve != RES_PASS_LAST || d.depth_compare == CMP_EQUAL ||
                            d.depth_compare == CMP_ALWAYS);
             float zw;
-            ps[k].col = ps[k].has ? winner_color<PROG>(d, ps[k].key, (float)px, (float)py, zw) : init_col[k];
+            ps[k].col = ps[k].has ? winner_color<PROG>(d, ps[k].key, (float)px, (float)py, zw, d.srgb_lut)
+                                  : init_col[k];
         }
         if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = ps[k].col;
         if (d.depth) d.depth[pix] = write_depth ? in_vgpr(ps[k].dep) : in_vgpr(ps[k].init_dep);
@@ -1428,7 +1430,7 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
 // LoadOp::Clear values).
 template <int PROG, int TS, int NT>
 __device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t tile, uint32_t item,
-                                               VisShared<TS, NT> &sh, uint32_t &frag_acc)
+                                               VisShared<TS, NT> &sh, const float *srgb, uint32_t &frag_acc)
 {
     constexpr int PPT = TS * TS / NT;
     constexpr uint32_t NB = VisShared<TS, NT>::NB;
@@ -1506,7 +1508,7 @@ __device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t til
         unsigned long long v = sh.best[q];
         if (v != ~0ull) {
             float z;
-            uint32_t col = winner_color<PROG>(d, oi_key(d.resolve, v), (float)px, (float)py, z);
+            uint32_t col = winner_color<PROG>(d, oi_key(d.resolve, v), (float)px, (float)py, z, srgb);
             if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = col;
             if (d.depth) {
                 if (write_depth_mode) d.depth[pix] = z;
@@ -1672,7 +1674,12 @@ __global__ __launch_bounds__(NT, NT == 256 && TS == 16 ? 6 : 1) void k_tiles(Dra
     __shared__ VisShared<TS, NT> sh;
     __shared__ uint32_t wave_frags[NT / 64];
     __shared__ uint32_t kinds[kClearChunk];
+    __shared__ float srgb[256];   // the sRGB decode table (one device table for every draw)
 
+    // read only when shading, after the first tile's barriers
+    const float *srgb_g = draw_at<MULTI>(d0, b, 0).srgb_lut;
+    if (srgb_g)
+        for (uint32_t i = threadIdx.x; i < 256u; i += NT) srgb[i] = srgb_g[i];
     XcdSchedule w(min(b.header[H_ACTIVE], b.total_tiles * kMaxItems));
     uint32_t frag_acc = 0;
     for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
@@ -1680,13 +1687,13 @@ __global__ __launch_bounds__(NT, NT == 256 && TS == 16 ? 6 : 1) void k_tiles(Dra
         if (item >= w.n) break;
         uint32_t di, tile, it;
         locate_tile<MULTI>(b, item, di, tile, it);
-        raster_tile_oi<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, it, sh, frag_acc);
+        raster_tile_oi<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, it, sh, srgb, frag_acc);
     }
     // the last workgroups have the fewest active tiles: chunks from there
     if (untouched_pass(b))
         for (uint32_t c = gridDim.x - 1u - blockIdx.x; c < b.n_chunks; c += gridDim.x)
             untouched_chunk<MULTI, TS, NT>(d0, b, c, kinds, [&](const DrawParams &d, uint32_t tile) {
-                raster_tile_oi<PROG, TS, NT>(d, tile, 0u, sh, frag_acc);
+                raster_tile_oi<PROG, TS, NT>(d, tile, 0u, sh, srgb, frag_acc);
             });
     count_fragments<MULTI, NT>(d0, b, frag_acc, wave_frags);
 }
@@ -2072,7 +2079,7 @@ __device__ __forceinline__ bool lp_color(const DrawParams &d, const VsOut *v, co
         return false;
     }
 #endif
-    run_fs<PROG>(d, var, color);
+    run_fs<PROG>(d, var, color, d.srgb_lut);
     return true;
 }
 

@@ -1983,6 +1983,7 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
     bool extended = dev->semantics == WGC_SEMANTICS_EXTENDED;
     d.program = pl->program;
     d.nvar = pl->nvar;
+    d.srgb_lut = dev->d_srgb;   // every draw: the tile kernels copy it to LDS
     d.interp = pl->interp;
     d.front_face_cw = pd.front_face;
     d.cull_mode = pd.cull_mode;

@@ -832,20 +832,19 @@ __device__ __forceinline__ bool tile_bbox(const DrawParams &d, const ScanTri &s,
 // Fragment stage
 // ------------------------------------------------------------------
 
-// u8 / 255.0 (texture.rs:173-175) for every byte, folded by the compiler with
-// IEEE round-to-nearest, i.e. the same value as the division.
-struct U8Table {
-    float v[256];
-};
-constexpr U8Table make_u8_table()
+// u8 / 255.0 (texture.rs:173-175), correctly rounded: the product with the
+// rounded reciprocal, then one fused correction with the exact residual
+// (Markstein's step).  For every byte this is the division's value (checked
+// against exact rational arithmetic for all 256 inputs); no table, so no
+// memory access.
+__device__ __forceinline__ float u8f(uint32_t b)
 {
-    U8Table t{};
-    for (int i = 0; i < 256; ++i) t.v[i] = (float)i / 255.0f;
-    return t;
+    const float r = 1.0f / 255.0f;
+    float x = (float)(b & 0xFFu);
+    float q = x * r;
+    float e = __builtin_fmaf(-q, 255.0f, x);
+    return __builtin_fmaf(e, r, q);
 }
-__constant__ constexpr U8Table kU8ToF = make_u8_table();
-
-__device__ __forceinline__ float u8f(uint32_t b) { return kU8ToF.v[b & 0xFFu]; }
 
 // A bound texture with its sampler's addressing and filter.
 struct TexView {
@@ -858,14 +857,15 @@ __device__ __forceinline__ TexView builtin_texture(const DrawParams &d)
     return TexView{d.tex, d.tex_w, d.tex_h, d.tex_srgb, d.addr_u, d.addr_v, d.bilinear};
 }
 
+// `srgb`: the draw's sRGB decode table, or a copy of it in LDS.
 __device__ __forceinline__ void fetch_texel(const DrawParams &d, const TexView &tv, uint32_t u, uint32_t v,
-                                            float out[4])
+                                            float out[4], const float *srgb)
 {
     uint32_t t = *reinterpret_cast<const uint32_t *>(tv.tex + ((uint64_t)v * tv.w + u) * 4u);
     if (d.srgb && tv.srgb) {
-        out[0] = d.srgb_lut[t & 0xFFu];
-        out[1] = d.srgb_lut[(t >> 8) & 0xFFu];
-        out[2] = d.srgb_lut[(t >> 16) & 0xFFu];
+        out[0] = srgb[t & 0xFFu];
+        out[1] = srgb[(t >> 8) & 0xFFu];
+        out[2] = srgb[(t >> 16) & 0xFFu];
         out[3] = u8f(t >> 24);
     } else {
         out[0] = u8f(t);   // texture.rs:173-175
@@ -890,7 +890,7 @@ __device__ __forceinline__ int32_t wrap_index(int32_t i, uint32_t mode, uint32_t
 
 // Extension: bilinear, WebGPU texel-centre convention.
 __device__ __forceinline__ void sample_bilinear(const DrawParams &d, const TexView &tv, float u, float v,
-                                                float out[4])
+                                                float out[4], const float *srgb)
 {
     float fu = u * (float)tv.w - 0.5f;
     float fv = v * (float)tv.h - 0.5f;
@@ -906,10 +906,10 @@ __device__ __forceinline__ void sample_bilinear(const DrawParams &d, const TexVi
     uint32_t y1 = tv.addr_v == ADDR_REPEAT ? (y0 + 1u < tv.h ? y0 + 1u : 0u)
                                            : (uint32_t)wrap_index(iv + 1, tv.addr_v, tv.h);
     float t00[4], t10[4], t01[4], t11[4];
-    fetch_texel(d, tv, x0, y0, t00);
-    fetch_texel(d, tv, x1, y0, t10);
-    fetch_texel(d, tv, x0, y1, t01);
-    fetch_texel(d, tv, x1, y1, t11);
+    fetch_texel(d, tv, x0, y0, t00, srgb);
+    fetch_texel(d, tv, x1, y0, t10, srgb);
+    fetch_texel(d, tv, x0, y1, t01, srgb);
+    fetch_texel(d, tv, x1, y1, t11, srgb);
     float su = 1.0f - au, sv = 1.0f - av;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -921,22 +921,29 @@ __device__ __forceinline__ void sample_bilinear(const DrawParams &d, const TexVi
 
 // textureSample (binding.rs:16-164; bilinear is the extension).
 __device__ __forceinline__ void sample_texture(const DrawParams &d, const TexView &tv, float u, float v,
-                                               float out[4])
+                                               float out[4], const float *srgb)
 {
     if (tv.bilinear) {
-        sample_bilinear(d, tv, u, v, out);
+        sample_bilinear(d, tv, u, v, out, srgb);
     } else {
         uint32_t x = texel_coordinate(u, tv.addr_u, tv.w);
         uint32_t y = texel_coordinate(v, tv.addr_v, tv.h);
-        fetch_texel(d, tv, x, y, out);
+        fetch_texel(d, tv, x, y, out, srgb);
     }
 }
 
+__device__ __forceinline__ void sample_texture(const DrawParams &d, const TexView &tv, float u, float v,
+                                               float out[4])
+{
+    sample_texture(d, tv, u, v, out, d.srgb_lut);
+}
+
+// `srgb`: the sRGB decode table to read (the draw's, or a copy in LDS).
 template <int PROG>
-__device__ __forceinline__ void run_fs(const DrawParams &d, const float *var, float out[4])
+__device__ __forceinline__ void run_fs(const DrawParams &d, const float *var, float out[4], const float *srgb)
 {
     if constexpr (PROG == PROG_MVP_TEXTURE) {
-        sample_texture(d, builtin_texture(d), var[0], var[1], out);
+        sample_texture(d, builtin_texture(d), var[0], var[1], out, srgb);
     } else {
         out[0] = var[0];
         out[1] = var[1];
