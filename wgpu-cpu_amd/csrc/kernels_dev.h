@@ -333,6 +333,29 @@ __device__ __forceinline__ void note_big_wave(const DrawParams &d, bool big, con
 template <int PROG>
 __device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act);
 
+// The LoadOp::Clear values into the whole target, spread over the setup
+// grid (16-byte stores; issued before the setup work, they drain while it
+// waits on its loads and atomics).
+__device__ __forceinline__ void clear_target(const DrawParams &d)
+{
+    uint64_t n = (uint64_t)d.width * d.height;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d.color && d.clear_color_on) {
+        uint32_t *c = reinterpret_cast<uint32_t *>(d.color);
+        uint4 v = make_uint4(d.clear_rgba, d.clear_rgba, d.clear_rgba, d.clear_rgba);
+        for (uint64_t i = i0; i < n / 4; i += stride) reinterpret_cast<uint4 *>(c)[i] = v;
+        if (i0 < (n & 3u)) c[(n & ~3ull) + i0] = d.clear_rgba;
+    }
+    if (d.depth && d.clear_depth_on) {
+        uint32_t z = __float_as_uint(d.clear_depth);
+        uint4 v = make_uint4(z, z, z, z);
+        uint32_t *p = reinterpret_cast<uint32_t *>(d.depth);
+        for (uint64_t i = i0; i < n / 4; i += stride) reinterpret_cast<uint4 *>(p)[i] = v;
+        if (i0 < (n & 3u)) p[(n & ~3ull) + i0] = z;
+    }
+}
+
 // CLIP: the primitives that cross a clip plane are clipped here, by their
 // wave (single draws whose waves all fit on the GPU at once: no k_clip
 // launch); otherwise they go to the clip list.
@@ -351,6 +374,7 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
     }
     const DrawParams &d = draw_at<MULTI>(d0, b, di);
     if constexpr (!MULTI) {
+        if (b.preclear) clear_target(d);
         // whole waves past the end leave; the lanes of a partial wave stay
         // (inactive) for the wave-wide steps
         if (__builtin_amdgcn_readfirstlane(g) >= d.total_prims) return;
@@ -1558,7 +1582,7 @@ __device__ __forceinline__ void untouched_chunk(const DrawParams &d0, const Batc
                 tile = gt - b.tile_base[di];
             }
             const DrawParams &d = draw_at<MULTI>(d0, b, di);
-            kind = in_big_union(d, tile) ? 2u : (d.init_clear ? 1u : 0u);
+            kind = in_big_union(d, tile) ? 2u : (d.init_clear && b.any_clear ? 1u : 0u);
         }
         kinds[threadIdx.x] = kind | (di << 2);
     }

@@ -2295,7 +2295,17 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep
     b.active_list = (uint32_t *)(base + o_al);
     b.counters = gl.counters;
     b.n_counters = gl.n_counters;
-    b.any_clear = any_clear ? 1u : 0u;
+    // a single draw clears its whole target in setup when setup has a thread
+    // for every 32 pixels (then the stores drain while setup waits on its
+    // loads and atomics); the tile kernel then only writes the tiles it rasters
+    bool preclear = false;
+    if (!st.multi && any_clear && prims > 0) {
+        uint64_t threads = align_up(prims, 256), px = (uint64_t)st.draws[0].width * st.draws[0].height;
+        preclear = threads * 32 >= px;
+    }
+    b.preclear = preclear ? 1u : 0u;
+    b.any_clear = (any_clear && !b.preclear) ? 1u : 0u;
+    b.pad = 0;
     if (!st.multi) {
         b.draws = nullptr;
         b.prim_base = nullptr;

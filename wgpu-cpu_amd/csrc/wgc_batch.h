@@ -25,7 +25,9 @@ struct BatchArgs {
     uint32_t *active_list;       // tiles with binned records, in the order they were first binned
     uint32_t *counters;          // C_NUM per draw
     uint32_t n_counters;
-    uint32_t any_clear;          // some draw of the group stores LoadOp::Clear values
+    uint32_t any_clear;          // the tile kernel stores LoadOp::Clear values into untouched tiles
+    uint32_t preclear;           // single draw: setup stores the LoadOp::Clear values into the whole target
+    uint32_t pad;
 };
 
 constexpr uint32_t kClearChunk = 8;     // tiles per work item of the untouched-tile pass
