@@ -167,13 +167,23 @@ def main():
     alg_bytes = scenes.algorithmic_bytes(spec)
     roofline = None
     if dominant:
+        # the dominant kernel's own bytes, not the whole pass's
+        kbytes = kernel_bytes(spec, dominant)
         avg_s = prof[dominant]["avg_us"] * 1e-6
-        achieved = alg_bytes / avg_s / 1e9
+        achieved = kbytes / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": measured_traffic(args.config, dominant),
-                    "algorithmic_bytes_per_launch": alg_bytes}
+                    "algorithmic_bytes_per_launch": kbytes,
+                    "kernel_avg_us": round(prof[dominant]["avg_us"], 2)}
     pass_bytes_rate = alg_bytes / (ms_per_step * 1e-3) / 1e9
+    others = {}
+    for k in prof:
+        if k != dominant and k in ("k_setup", "k_clip", "k_tiles", "k_raster"):
+            kb = kernel_bytes(spec, k)
+            gbs = kb / (prof[k]["avg_us"] * 1e-6) / 1e9
+            others[k] = {"algorithmic_bytes_per_launch": kb, "avg_us": round(prof[k]["avg_us"], 2),
+                         "achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
     cpu = None
     if not args.no_cpu:
@@ -233,6 +243,7 @@ def main():
         "pass_algorithmic_GBps": round(pass_bytes_rate, 1),
         "kernels_us": {k: round(v["avg_us"], 2) for k, v in prof.items()},
         "roofline": roofline,
+        "other_kernels_hbm": others,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "extra": extra,
@@ -383,6 +394,49 @@ def profile_batch(n: int):
     print(json.dumps({"batched_submissions": n, "ms_per_submission": round(ms, 4),
                       "fragments_per_submission": pp.total_fragments(),
                       "algorithmic_bytes": batched_algorithmic_bytes(specs)}), flush=True)
+
+
+def _primitives(d) -> int:
+    n = d.count
+    per = {0: n, 1: n // 2, 2: max(n - 1, 0), 3: n // 3, 4: max(n - 2, 0)}[d.topology]
+    return per * d.instance_count
+
+
+def setup_preclears(spec) -> bool:
+    """A single draw with LoadOp::Clear whose setup grid has a thread for every
+    32 pixels has setup store the clear values over the whole target."""
+    px = spec.width * spec.height
+    clears = (spec.has_color and spec.clear_color is not None) or (spec.has_depth and spec.clear_depth is not None)
+    prims = _primitives(spec.draws[0]) if len(spec.draws) == 1 else 0
+    return bool(clears and prims > 0 and -(-prims // 256) * 256 * 32 >= px)
+
+
+def kernel_bytes(spec, kernel: str) -> int:
+    """Algorithmic bytes of one launch of `kernel` in the pass.
+
+    k_setup: the vertex and index reads and the uniform, plus the whole
+    target's colour + depth when setup stores the clear values.
+    k_tiles / k_raster: the texture, and 4 B of colour and 4 B of depth for
+    every pixel the kernel must store: the pixels the draw covers when setup
+    stored the clear (counted on the GPU with fragment counting, on a device
+    of its own), else the whole target."""
+    import wgc
+    from wgc import runner
+    per_px = (4 if spec.has_color else 0) + (4 if spec.has_depth else 0)
+    px = spec.width * spec.height
+    pre = setup_preclears(spec)
+    if kernel in ("k_setup", "k_clip"):
+        b = sum(len(d.vertex_data) + (len(d.index_data) if d.index_data else 0) + (64 if d.uniform else 0)
+                for d in spec.draws)
+        return b + (px * per_px if pre and kernel == "k_setup" else 0)
+    tex = sum(len(d.texture[2]) for d in spec.draws if d.texture)
+    if not pre:
+        return px * per_px + tex
+    dev = wgc.Device(0, spec.semantics)
+    r = runner.render(spec, dev, fragment_counts=True)
+    covered = int((r.frag_count > 0).sum())
+    del dev
+    return covered * per_px + tex
 
 
 def measured_traffic(workload: str, kernel: str):
