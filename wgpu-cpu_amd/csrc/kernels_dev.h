@@ -43,6 +43,27 @@ namespace wgc {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// v from lane `lane`, the same lane for the whole wave (v_readlane: no LDS
+// round trip, unlike a shuffle whose source lane differs per lane).
+__device__ __forceinline__ uint32_t lane_value(uint32_t v, int lane)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+// Inclusive prefix sum over the 64 lanes of a full wave, with DPP: shifts
+// by 1, 2, 4 and 8 lanes within each row of 16, then lane 15 of each row
+// added to the next row and lane 31 to rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return v;
+}
+
 template <bool MULTI>
 __device__ __forceinline__ const DrawParams &draw_at(const DrawParams &d0, const BatchArgs &b, uint32_t i)
 {
@@ -60,7 +81,7 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
     int leader = __ffsll((unsigned long long)mask) - 1;
     uint32_t base = 0;
     if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
+    base = lane_value(base, leader);
     uint64_t below = (lane == 0) ? 0ull : (mask & ((~0ull) >> (64 - lane)));
     return pred ? base + (uint32_t)__popcll(below) : 0xFFFFFFFFu;
 }
@@ -87,8 +108,8 @@ __device__ __forceinline__ uint32_t wave_add(uint32_t *addr, uint32_t v, bool ac
     unsigned long long a = (unsigned long long)addr;
     while (pending) {
         int leader = __ffsll((unsigned long long)pending) - 1;
-        uint32_t lo = (uint32_t)__shfl((int)(uint32_t)a, leader);
-        uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(a >> 32), leader);
+        uint32_t lo = lane_value((uint32_t)a, leader);
+        uint32_t hi = lane_value((uint32_t)(a >> 32), leader);
         bool mine = act && a == (((unsigned long long)hi << 32) | lo);
         uint64_t m = __ballot(mine);
         uint32_t below = 0, total = 0;
@@ -100,7 +121,7 @@ __device__ __forceinline__ uint32_t wave_add(uint32_t *addr, uint32_t v, bool ac
         }
         uint32_t base = 0;
         if ((int)lane == leader) base = atomicAdd(addr, total);
-        base = (uint32_t)__shfl((int)base, leader);
+        base = lane_value(base, leader);
         if (mine) res = base + below;
         pending &= ~m;
     }
@@ -206,7 +227,7 @@ __device__ __forceinline__ void bin_wave(const DrawParams &d, bool has, uint32_t
         uint32_t lead = lane, cnt = 1u, rank = 0u;
         for (int it = 0; it < 8 && pending; ++it) {
             int leader = __ffsll((unsigned long long)pending) - 1;
-            uint32_t tl = (uint32_t)__shfl((int)t, leader);
+            uint32_t tl = lane_value(t, leader);
             bool mine = act && t == tl;
             uint64_t m = __ballot(mine);
             if (mine) {
@@ -621,10 +642,12 @@ struct TileFeed {
     uint32_t items;               // the tile's work items
 };
 
-__device__ __forceinline__ TileFeed tile_feed(const DrawParams &d, uint32_t tile, uint32_t item = 0u)
+// From the tile's count of binned records and the group's spill and big
+// counts.
+__device__ __forceinline__ TileFeed tile_feed_from(const DrawParams &d, uint32_t tile, uint32_t item, uint32_t cnt,
+                                                   uint32_t spill_n, uint32_t big_n)
 {
     TileFeed f;
-    uint32_t cnt = d.s.tile_count[tile];
     uint32_t nbin = min(cnt, d.s.bin_cap), sp = d.s.split;
     f.items = 1u;
     if (sp) f.items = max(1u, min((nbin + sp - 1u) / sp, kMaxItems));
@@ -634,14 +657,21 @@ __device__ __forceinline__ TileFeed tile_feed(const DrawParams &d, uint32_t tile
     f.bin0 = tile * d.s.bin_cap + start;
     f.nbin = last ? nbin - start : sp;
     f.pos = 0u;
-    f.nspill = last && cnt > d.s.bin_cap ? min(d.s.header[H_SPILL], d.s.spill_cap) : 0u;
+    f.nspill = last && cnt > d.s.bin_cap ? min(spill_n, d.s.spill_cap) : 0u;
     f.spos = 0u;
-    f.nbig = last ? min(d.s.counters[C_BIG], d.s.big_cap) : 0u;
+    f.nbig = last ? min(big_n, d.s.big_cap) : 0u;
     f.bpos = 0u;
     f.gt = d.s.tile0 + tile;
     f.tx = tile % d.tiles_x;
     f.ty = tile / d.tiles_x;
     return f;
+}
+
+// The three counts are read together (none waits for another).
+__device__ __forceinline__ TileFeed tile_feed(const DrawParams &d, uint32_t tile, uint32_t item = 0u)
+{
+    uint32_t cnt = d.s.tile_count[tile], spill_n = d.s.header[H_SPILL], big_n = d.s.counters[C_BIG];
+    return tile_feed_from(d, tile, item, cnt, spill_n, big_n);
 }
 
 // Wave 0 compacts the next 64 entries of the spill or big list that belong
@@ -1335,12 +1365,7 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
     // exclusive scan of the row counts in thread order: within each wave,
     // then across waves
     uint32_t lane = lane_id(), wave = tid >> 6;
-    uint32_t v = cnt_rows;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t t = (uint32_t)__shfl_up((int)v, off);
-        if (lane >= (uint32_t)off) v += t;
-    }
+    uint32_t v = wave_inclusive_sum(cnt_rows);
     if (lane == 63u) sh.wsum[wave] = v;
     __syncthreads();
     uint32_t before = 0, total = 0;
@@ -1398,12 +1423,7 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
         return nfr;
     }
     // exclusive scan of the fragment counts in thread order
-    uint32_t c = nfr;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t t = (uint32_t)__shfl_up((int)c, off);
-        if (lane >= (uint32_t)off) c += t;
-    }
+    uint32_t c = wave_inclusive_sum(nfr);
     if (lane == 63u) sh.wsum[wave] = c;  // every thread read the row-count sums before the last barrier
     __syncthreads();
     uint32_t fbase = c - nfr, nfrag = 0;
@@ -1460,7 +1480,15 @@ __device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t til
     constexpr uint32_t NB = VisShared<TS, NT>::NB;
     uint32_t x0 = (tile % d.tiles_x) * TS, y0 = (tile / d.tiles_x) * TS;
     uint32_t tid = threadIdx.x;
-    TileFeed feed = tile_feed(d, tile, item);
+    // the item's first keys, the tile's count and the group's spill and big
+    // counts, all in flight together (key slots are clamped to the bin's
+    // capacity, at least 1; those at or past the count are not used)
+    uint32_t slot0 = min(item * d.s.split + tid, d.s.bin_cap - 1u);
+    uint32_t k0 = d.s.bins[tile * d.s.bin_cap + slot0];
+    uint32_t cnt = d.s.tile_count[tile], spill_n = d.s.header[H_SPILL], big_n = d.s.counters[C_BIG];
+    asm volatile("" : : "v"(k0), "v"(cnt), "v"(spill_n), "v"(big_n));
+    if (tid < NB) sh.keys[tid] = k0;
+    TileFeed feed = tile_feed_from(d, tile, item, cnt, spill_n, big_n);
 
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
@@ -1473,11 +1501,16 @@ __device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t til
         sh.best[q] = ~0ull;
         sh.cnt[q] = 0u;
     }
+    // the first batch: the slots read above (the same as next_batch's)
+    bool pre = item < feed.items && feed.nbin != 0u;
+    uint32_t n = pre ? min(NB, feed.nbin) : 0u;
+    feed.pos = n;
     __syncthreads();
 
     uint32_t f = 0;
     for (;;) {
-        uint32_t n = next_batch(d, feed, sh.keys, &sh.nkeys, NB);
+        if (!pre) n = next_batch(d, feed, sh.keys, &sh.nkeys, NB);
+        pre = false;
         if (n == 0u) break;
         f += vis_batch<TS, NT>(d, sh, n, x0, y0);
     }
@@ -1606,27 +1639,36 @@ __device__ __forceinline__ bool untouched_pass(const BatchArgs &b)
     return b.any_clear != 0u || b.header[H_BIG] != 0u;
 }
 
-// Splits the active tiles into 8 contiguous ranges, one per XCD (workgroup i
-// runs on XCD i mod 8 and the grids are multiples of 8), so that neighbouring
-// tiles share an L2; yields the items of this workgroup.
-struct XcdSchedule {
-    uint32_t per_xcd, xcd, lw, nw, n;
-    __device__ XcdSchedule(uint32_t n_active)
+// The active-list entries of this workgroup: in every round of G entries
+// (G = the grid, a multiple of 8), XCD x (workgroup i runs on XCD i mod 8)
+// takes the contiguous range [x·G/8, (x+1)·G/8), so that tiles binned close
+// in time share an L2.  The first entry's index does not depend on the
+// active count, so it is read together with the count.
+struct ActiveSchedule {
+    uint32_t first, stride, n, e0;
+    __device__ ActiveSchedule(const BatchArgs &b)
     {
-        n = n_active;
-        per_xcd = (n_active + 7u) / 8u;
-        xcd = blockIdx.x & 7u;
-        lw = blockIdx.x >> 3;
-        nw = gridDim.x >> 3;
+        stride = gridDim.x;
+        first = (blockIdx.x & 7u) * (stride >> 3) + (blockIdx.x >> 3);
+        // the list holds kMaxItems entries per tile (a group has at least
+        // one tile); an entry at or beyond the count is not used
+        uint32_t cap = b.total_tiles * kMaxItems;
+        e0 = b.active_list[min(first, cap - 1u)];
+        uint32_t count = b.header[H_ACTIVE];
+        asm volatile("" : : "s"(count), "s"(e0));  // both loads in flight together
+        n = min(count, cap);
+    }
+    __device__ uint32_t entry(const BatchArgs &b, uint32_t item) const
+    {
+        return __builtin_amdgcn_readfirstlane(item == first ? e0 : b.active_list[item]);
     }
 };
 
-// Active-list entry `item`: its draw, tile and the tile's work item.
+// Active-list entry e: its draw, tile and the tile's work item.
 template <bool MULTI>
-__device__ __forceinline__ void locate_tile(const BatchArgs &b, uint32_t item, uint32_t &di, uint32_t &tile,
+__device__ __forceinline__ void locate_tile(const BatchArgs &b, uint32_t e, uint32_t &di, uint32_t &tile,
                                             uint32_t &it)
 {
-    uint32_t e = __builtin_amdgcn_readfirstlane(b.active_list[item]);
     uint32_t gt = e & ((1u << kItemShift) - 1u);
     it = e >> kItemShift;
     di = 0;
@@ -1667,18 +1709,16 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
     __shared__ uint32_t wave_frags[NT / 64];
     __shared__ uint32_t kinds[kClearChunk];
 
-    XcdSchedule w(min(b.header[H_ACTIVE], b.total_tiles * kMaxItems));
+    ActiveSchedule w(b);
     uint32_t frag_acc = 0;
-    for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
-        uint32_t item = w.xcd * w.per_xcd + i;
-        if (item >= w.n) break;
+    for (uint32_t item = w.first; item < w.n; item += w.stride) {
         uint32_t di, tile, it;
-        locate_tile<MULTI>(b, item, di, tile, it);
+        locate_tile<MULTI>(b, w.entry(b, item), di, tile, it);
         raster_tile<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, olist, olist_n, frag_acc);
     }
-    // the last workgroups have the fewest active tiles: chunks from there
+    // workgroups late in the schedule have the fewest active tiles: chunks from there
     if (untouched_pass(b))
-        for (uint32_t c = gridDim.x - 1u - blockIdx.x; c < b.n_chunks; c += gridDim.x)
+        for (uint32_t c = gridDim.x - 1u - w.first; c < b.n_chunks; c += gridDim.x)
             untouched_chunk<MULTI, TS, NT>(d0, b, c, kinds, [&](const DrawParams &d, uint32_t tile) {
                 raster_tile<PROG, TS, NT>(d, tile, sh, olist, olist_n, frag_acc);
             });
@@ -1700,22 +1740,24 @@ __global__ __launch_bounds__(NT, NT == 256 && TS == 16 ? 6 : 1) void k_tiles(Dra
     __shared__ uint32_t kinds[kClearChunk];
     __shared__ float srgb[256];   // the sRGB decode table (one device table for every draw)
 
-    // read only when shading, after the first tile's barriers
+    // read only when shading, after the first tile's barriers; every draw
+    // carries its device's table (allocated with the device, never null)
     const float *srgb_g = draw_at<MULTI>(d0, b, 0).srgb_lut;
-    if (srgb_g)
+    if constexpr (NT == 256) {
+        srgb[threadIdx.x] = srgb_g[threadIdx.x];
+    } else {
         for (uint32_t i = threadIdx.x; i < 256u; i += NT) srgb[i] = srgb_g[i];
-    XcdSchedule w(min(b.header[H_ACTIVE], b.total_tiles * kMaxItems));
+    }
+    ActiveSchedule w(b);
     uint32_t frag_acc = 0;
-    for (uint32_t i = w.lw; i < w.per_xcd; i += w.nw) {
-        uint32_t item = w.xcd * w.per_xcd + i;
-        if (item >= w.n) break;
+    for (uint32_t item = w.first; item < w.n; item += w.stride) {
         uint32_t di, tile, it;
-        locate_tile<MULTI>(b, item, di, tile, it);
+        locate_tile<MULTI>(b, w.entry(b, item), di, tile, it);
         raster_tile_oi<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, it, sh, srgb, frag_acc);
     }
-    // the last workgroups have the fewest active tiles: chunks from there
+    // workgroups late in the schedule have the fewest active tiles: chunks from there
     if (untouched_pass(b))
-        for (uint32_t c = gridDim.x - 1u - blockIdx.x; c < b.n_chunks; c += gridDim.x)
+        for (uint32_t c = gridDim.x - 1u - w.first; c < b.n_chunks; c += gridDim.x)
             untouched_chunk<MULTI, TS, NT>(d0, b, c, kinds, [&](const DrawParams &d, uint32_t tile) {
                 raster_tile_oi<PROG, TS, NT>(d, tile, 0u, sh, srgb, frag_acc);
             });
