@@ -43,6 +43,17 @@ namespace wgc {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Streaming stores (non-temporal) for target pixels that no later step of
+// the same kernel reads back, so that they do not stay dirty in L2 until
+// the end of the kernel (whose write-back then delays the next launch).
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_stream(uint32_t *p, uint32_t v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void store_stream4(uint32_t *p, uint32_t v)
+{
+    u32x4_t q = {v, v, v, v};
+    __builtin_nontemporal_store(q, reinterpret_cast<u32x4_t *>(p));
+}
+
 // v from lane `lane`, the same lane for the whole wave (v_readlane: no LDS
 // round trip, unlike a shuffle whose source lane differs per lane).
 __device__ __forceinline__ uint32_t lane_value(uint32_t v, int lane)
@@ -357,6 +368,15 @@ __device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, boo
 // The LoadOp::Clear values into the whole target, spread over the setup
 // grid (16-byte stores; issued before the setup work, they drain while it
 // waits on its loads and atomics).
+__device__ __forceinline__ void store_color(const DrawParams &d, size_t pix, uint32_t c)
+{
+    store_stream(reinterpret_cast<uint32_t *>(d.color) + pix, c);
+}
+__device__ __forceinline__ void store_depth(const DrawParams &d, size_t pix, float z)
+{
+    store_stream(reinterpret_cast<uint32_t *>(d.depth) + pix, __float_as_uint(z));
+}
+
 __device__ __forceinline__ void clear_target(const DrawParams &d)
 {
     uint64_t n = (uint64_t)d.width * d.height;
@@ -364,16 +384,14 @@ __device__ __forceinline__ void clear_target(const DrawParams &d)
     uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (d.color && d.clear_color_on) {
         uint32_t *c = reinterpret_cast<uint32_t *>(d.color);
-        uint4 v = make_uint4(d.clear_rgba, d.clear_rgba, d.clear_rgba, d.clear_rgba);
-        for (uint64_t i = i0; i < n / 4; i += stride) reinterpret_cast<uint4 *>(c)[i] = v;
-        if (i0 < (n & 3u)) c[(n & ~3ull) + i0] = d.clear_rgba;
+        for (uint64_t i = i0; i < n / 4; i += stride) store_stream4(c + 4 * i, d.clear_rgba);
+        if (i0 < (n & 3u)) store_stream(c + (n & ~3ull) + i0, d.clear_rgba);
     }
     if (d.depth && d.clear_depth_on) {
         uint32_t z = __float_as_uint(d.clear_depth);
-        uint4 v = make_uint4(z, z, z, z);
         uint32_t *p = reinterpret_cast<uint32_t *>(d.depth);
-        for (uint64_t i = i0; i < n / 4; i += stride) reinterpret_cast<uint4 *>(p)[i] = v;
-        if (i0 < (n & 3u)) p[(n & ~3ull) + i0] = z;
+        for (uint64_t i = i0; i < n / 4; i += stride) store_stream4(p + 4 * i, z);
+        if (i0 < (n & 3u)) store_stream(p + (n & ~3ull) + i0, z);
     }
 }
 
@@ -1210,8 +1228,8 @@ __device__ __forceinline__ void raster_tile(const DrawParams &d, uint32_t tile, 
             ps[k].col = ps[k].has ? winner_color<PROG>(d, ps[k].key, (float)px, (float)py, zw, d.srgb_lut)
                                   : init_col[k];
         }
-        if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = ps[k].col;
-        if (d.depth) d.depth[pix] = write_depth ? in_vgpr(ps[k].dep) : in_vgpr(ps[k].init_dep);
+        if (d.color) store_color(d, pix, ps[k].col);
+        if (d.depth) store_depth(d, pix, write_depth ? in_vgpr(ps[k].dep) : in_vgpr(ps[k].init_dep));
         if (d.s.frag_px && nfrag[k]) d.s.frag_px[pix] += nfrag[k];
     });
     frag_acc += f;
@@ -1566,14 +1584,14 @@ __device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t til
         if (v != ~0ull) {
             float z;
             uint32_t col = winner_color<PROG>(d, oi_key(d.resolve, v), (float)px, (float)py, z, srgb);
-            if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = col;
+            if (d.color) store_color(d, pix, col);
             if (d.depth) {
-                if (write_depth_mode) d.depth[pix] = z;
-                else if (d.init_clear && d.clear_depth_on) d.depth[pix] = d.clear_depth;
+                if (write_depth_mode) store_depth(d, pix, z);
+                else if (d.init_clear && d.clear_depth_on) store_depth(d, pix, d.clear_depth);
             }
         } else if (d.init_clear) {
-            if (d.color && d.clear_color_on) reinterpret_cast<uint32_t *>(d.color)[pix] = d.clear_rgba;
-            if (d.depth && d.clear_depth_on) d.depth[pix] = d.clear_depth;
+            if (d.color && d.clear_color_on) store_color(d, pix, d.clear_rgba);
+            if (d.depth && d.clear_depth_on) store_depth(d, pix, d.clear_depth);
         }
         if (d.s.frag_px && sh.cnt[q] && !split) d.s.frag_px[pix] += sh.cnt[q];
     }
@@ -1592,8 +1610,8 @@ __device__ __forceinline__ void clear_tile(const DrawParams &d, uint32_t tile)
         uint32_t px = x0 + q % TS, py = y0 + q / TS;
         if (px >= d.width || py >= d.height) continue;
         size_t pix = (size_t)py * d.width + px;
-        if (d.color && d.clear_color_on) reinterpret_cast<uint32_t *>(d.color)[pix] = d.clear_rgba;
-        if (d.depth && d.clear_depth_on) d.depth[pix] = d.clear_depth;
+        if (d.color && d.clear_color_on) store_color(d, pix, d.clear_rgba);
+        if (d.depth && d.clear_depth_on) store_depth(d, pix, d.clear_depth);
     }
 }
 
