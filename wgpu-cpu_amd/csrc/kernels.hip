@@ -2,6 +2,8 @@
 // built-in programs (the kernels themselves are in kernels_dev.h).
 #include <algorithm>
 
+#include <hip/hip_ext.h>
+
 #include "kernels_dev.h"
 #include "wgc_launch.h"
 
@@ -11,13 +13,19 @@ namespace wgc {
 // launchers
 // ------------------------------------------------------------------
 
-#define WGC_LAUNCH(name, kern, grid, block, ...)                                        \
-    do {                                                                                \
-        if (kt) kt->begin(name, st);                                                    \
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, __VA_ARGS__);          \
-        hipError_t e_ = hipGetLastError();                                              \
-        if (kt) kt->end(st);                                                            \
-        if (e_ != hipSuccess) return e_;                                                \
+// With a kernel timer, the launch records the kernel's start and end in
+// its own events (from the dispatch packet, like a profiler's trace).
+#define WGC_LAUNCH(name, kern, grid, block, ...)                                                      \
+    do {                                                                                              \
+        if (kt) {                                                                                     \
+            hipEvent_t b_ = nullptr, f_ = nullptr;                                                    \
+            kt->pair(name, b_, f_);                                                                   \
+            hipExtLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, b_, f_, 0u, __VA_ARGS__);     \
+        } else {                                                                                      \
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, __VA_ARGS__);                    \
+        }                                                                                             \
+        hipError_t e_ = hipGetLastError();                                                            \
+        if (e_ != hipSuccess) return e_;                                                              \
     } while (0)
 
 template <int PROG, bool MULTI>
@@ -162,9 +170,15 @@ hipError_t launch_batch(const DrawParams &d0, const BatchArgs &b, const GroupLau
 #define WGC_MLAUNCH(name, fn, grid, block, ...)                                                   \
     do {                                                                                          \
         void *args_[] = {__VA_ARGS__};                                                            \
-        if (kt) kt->begin(name, st);                                                              \
-        hipError_t e_ = hipModuleLaunchKernel(fn, (grid), 1, 1, (block), 1, 1, 0, st, args_, nullptr); \
-        if (kt) kt->end(st);                                                                      \
+        hipError_t e_;                                                                            \
+        if (kt) {                                                                                 \
+            hipEvent_t b_ = nullptr, f_ = nullptr;                                                \
+            kt->pair(name, b_, f_);                                                               \
+            uint32_t g_ = (uint32_t)(grid), t_ = (uint32_t)(block);                               \
+            e_ = hipExtModuleLaunchKernel(fn, g_ * t_, 1, 1, t_, 1, 1, 0, st, args_, nullptr, b_, f_, 0u); \
+        } else {                                                                                  \
+            e_ = hipModuleLaunchKernel(fn, (grid), 1, 1, (block), 1, 1, 0, st, args_, nullptr);   \
+        }                                                                                         \
         if (e_ != hipSuccess) return e_;                                                          \
     } while (0)
 

@@ -57,6 +57,15 @@ struct KernelTimer {
         pending.back().end = get();
         (void)hipEventRecord(pending.back().end, st);
     }
+    // Two events for one launch that records them itself (the start and end
+    // of the kernel's execution, from its dispatch packet).
+    void pair(const char *name, hipEvent_t &b, hipEvent_t &e)
+    {
+        Pending p{name, get(), get()};
+        pending.push_back(p);
+        b = p.begin;
+        e = p.end;
+    }
 };
 
 // Kernels of a pipeline whose shaders come from WGSL: the instantiations
