@@ -333,7 +333,7 @@ hipError_t launch_end_submission(unsigned long long *frags, uint32_t *error, voi
 {
     uint32_t n16 = (uint32_t)(zone_bytes / 16);
     uint32_t grid = std::max(1u, std::min((n16 + 1023u) / 1024u, 1024u));
-    hipLaunchKernelGGL(k_end_submission, dim3(grid), dim3(256), 0, st, frags, error, (unsigned long long *)host_out,
+    hipLaunchKernelGGL(k_end_submission, dim3(grid), dim3(256), 0, st, frags, error, (uint4 *)host_out,
                        (uint4 *)zone, n16);
     return hipGetLastError();
 }

@@ -1792,21 +1792,23 @@ __global__ void k_reset(unsigned long long *frags, uint32_t *error)
     if (threadIdx.x == 0) *error = 0u;
 }
 
-// End of a submission: its counters to host memory (out: kFragSlots
-// fragment counters, then the error word) and zero for the next one; the
-// zero region of the scratch arena that its launch groups used (tile
-// counts, group headers, per-draw counters) cleared, n16 16-byte units.
-__global__ __launch_bounds__(256) void k_end_submission(unsigned long long *frags, uint32_t *error,
-                                                        unsigned long long *out, uint4 *zone, uint32_t n16)
+// End of a submission: its counters to host memory (out: the fragment
+// count summed over the kFragSlots counters, its high word, the error word,
+// 0 -- one 16-byte store) and zero for the next one; the zero region of the
+// scratch arena that its launch groups used (tile counts, group headers,
+// per-draw counters) cleared, n16 16-byte units.
+__global__ __launch_bounds__(256) void k_end_submission(unsigned long long *frags, uint32_t *error, uint4 *out,
+                                                        uint4 *zone, uint32_t n16)
 {
-    if (blockIdx.x == 0) {
-        if (threadIdx.x < kFragSlots) {
-            out[threadIdx.x] = frags[threadIdx.x];
-            frags[threadIdx.x] = 0ull;
-        }
+    static_assert(kFragSlots == 64, "one fragment counter per lane of a wave");
+    if (blockIdx.x == 0 && threadIdx.x < 64u) {
+        unsigned long long f = frags[threadIdx.x];
+        frags[threadIdx.x] = 0ull;
+        for (int off = 32; off > 0; off >>= 1) f += __shfl_xor(f, off);
         if (threadIdx.x == 0) {
-            reinterpret_cast<uint32_t *>(out + kFragSlots)[0] = *error;
+            uint32_t e = *error;
             *error = 0u;
+            *out = make_uint4((uint32_t)f, (uint32_t)(f >> 32), e, 0u);
         }
     }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += gridDim.x * blockDim.x)

@@ -434,20 +434,24 @@ struct WgcRenderPass_T : Object {
     ~WgcRenderPass_T() override { pass.drop(); }
 };
 
-// Device-side counters of a submission; copied into a pinned host slot at its
-// end, so that finishing one submission can be observed without draining the
-// stream.
+// Device-side counters of a submission; summed into a pinned host slot at
+// its end (one 16-byte store), so that finishing one submission can be
+// observed without draining the stream.
 struct DevCounters {
     unsigned long long frags[kFragSlots];
     uint32_t error;
     uint32_t pad[63];
+};
+struct HostCounters {
+    uint32_t frags_lo, frags_hi, error, pad;   // written by k_end_submission
+    uint32_t pad2[12];
 };
 
 struct Inflight {
     uint64_t index;
     hipEvent_t done;
     std::vector<Object *> keep;  // command buffers used by the submission
-    DevCounters *slot;           // pinned host copy of the counters
+    HostCounters *slot;          // pinned host copy of the counters
     uint64_t prims, launches;
 };
 
@@ -484,8 +488,8 @@ struct WgcDevice_T : Object {
     uint64_t last_done = 0;
     std::vector<Inflight> inflight;
     std::vector<hipEvent_t> event_pool;
-    std::vector<DevCounters *> slot_pool;   // free pinned counter slots
-    std::vector<DevCounters *> slot_blocks;
+    std::vector<HostCounters *> slot_pool;   // free pinned counter slots
+    std::vector<HostCounters *> slot_blocks;
     BlockPool table_pool;                   // device draw tables of plans
     BlockPool staging_pool;                 // pinned staging of their uploads
     std::vector<Deferred> deferred;
@@ -681,14 +685,14 @@ extern "C" void wgc_device_release(WgcDevice device)
     release(device);
 }
 
-static hipError_t take_slot(WgcDevice_T *d, DevCounters *&slot)
+static hipError_t take_slot(WgcDevice_T *d, HostCounters *&slot)
 {
     if (d->slot_pool.empty()) {
         void *p = nullptr;
-        hipError_t e = hipHostMalloc(&p, 64 * sizeof(DevCounters), hipHostMallocDefault);
+        hipError_t e = hipHostMalloc(&p, 64 * sizeof(HostCounters), hipHostMallocDefault);
         if (e != hipSuccess) return e;
-        d->slot_blocks.push_back((DevCounters *)p);
-        for (int i = 0; i < 64; ++i) d->slot_pool.push_back((DevCounters *)p + i);
+        d->slot_blocks.push_back((HostCounters *)p);
+        for (int i = 0; i < 64; ++i) d->slot_pool.push_back((HostCounters *)p + i);
     }
     slot = d->slot_pool.back();
     d->slot_pool.pop_back();
@@ -717,8 +721,7 @@ static int32_t retire(WgcDevice_T *d, uint64_t upto, bool wait, uint64_t timeout
         d->last_done = f.index;
         d->event_pool.push_back(f.done);
         d->pending_error |= f.slot->error;
-        uint64_t frags = 0;
-        for (int i = 0; i < kFragSlots; ++i) frags += f.slot->frags[i];
+        uint64_t frags = (uint64_t)f.slot->frags_lo | ((uint64_t)f.slot->frags_hi << 32);
         d->last_frags = frags;
         d->total_frags += frags;
         d->total_submissions += 1;

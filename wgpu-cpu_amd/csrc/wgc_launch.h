@@ -100,9 +100,10 @@ uint32_t raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32
 uint32_t setup_clip_blocks_per_cu(uint32_t program);
 // Zeroes the submission counters (fragment slots and error bits).
 hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t st);
-// Copies the submission counters to host memory (kFragSlots 64-bit fragment
-// counters, then the 32-bit error word) and zeroes them, and clears the
-// first zone_bytes (a multiple of 16) of the scratch arena's zero region.
+// Copies the submission counters to host memory (16 bytes, aligned: the
+// fragment count summed over the kFragSlots counters as two 32-bit words,
+// then the error word and 0) and zeroes them, and clears the first
+// zone_bytes (a multiple of 16) of the scratch arena's zero region.
 hipError_t launch_end_submission(unsigned long long *frags, uint32_t *error, void *host_out, void *zone,
                                  size_t zone_bytes, hipStream_t st);
 // Row copy (4-byte multiples; 16-byte accesses where aligned) and word fill,
