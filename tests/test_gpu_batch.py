@@ -105,6 +105,62 @@ def test_dependent_passes_stay_ordered(gpu_device):
     assert np.array_equal(depth.view(np.uint32), ref.depth.view(np.uint32))
 
 
+@pytest.fixture(scope="module")
+def full_batch():
+    """The benchmark's batched workload at its full size: 256 passes, W and H
+    log-uniform in [64, 1024], spheres of 100-50k triangles, half textured;
+    with the oracle's render of every pass alone."""
+    seed = scenes.SEED_BASE + 4
+    specs = [scenes.batched_pass(seed, i, *a) for i, a in enumerate(scenes.batched_sizes(seed, 256))]
+    refs = [bridge.render(s) for s in specs]
+    return specs, refs
+
+
+def _compare_full(specs, refs, res, total):
+    bad = []
+    for s, r, ref in zip(specs, res, refs):
+        if not (np.array_equal(r.color, ref.color) and np.array_equal(r.depth.view(np.uint32),
+                                                                      ref.depth.view(np.uint32))):
+            bad.append(s.name)
+    assert not bad, f"{len(bad)} passes differ from the oracle: {bad[:8]}"
+    assert total == sum(ref.fragments for ref in refs)
+
+
+@pytest.mark.parametrize("cfg", sorted(wgc.RASTER_TILES))
+def test_full_batch_matches_oracle(gpu_device, full_batch, cfg):
+    announce(f"full 256-pass batch, raster cfg {cfg}")
+    specs, refs = full_batch
+    gpu_device.set_option(wgc.OPTION_RASTER_TILE, cfg)
+    try:
+        pp = runner.PreparedPasses(gpu_device, specs)
+        pp.submit()
+        res = pp.results()
+        total = pp.total_fragments()
+        # a replay of the captured graph renders the same images
+        pp.submit()
+        again = pp.results()
+    finally:
+        gpu_device.set_option(wgc.OPTION_RASTER_TILE, 0)
+    _compare_full(specs, refs, res, total)
+    _compare_full(specs, refs, again, total)
+
+
+def test_full_batch_autotuned(gpu_device, full_batch):
+    """The shape the autotuner picks for the full batch renders every pass
+    like the oracle."""
+    announce("full 256-pass batch, autotuned")
+    from wgc import autotune
+    specs, refs = full_batch
+    pp = runner.PreparedPasses(gpu_device, specs)
+    try:
+        autotune.tune(pp, reps=2)
+        pp.submit()
+        res = pp.results()
+        _compare_full(specs, refs, res, pp.total_fragments())
+    finally:
+        gpu_device.set_option(wgc.OPTION_RASTER_TILE, 0)
+
+
 def test_autotuner_picks_a_shape(gpu_device):
     announce("autotune")
     from wgc import autotune

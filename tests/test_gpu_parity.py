@@ -108,6 +108,17 @@ def test_c2_textured_extended(gpu_device_ext):
     assert_parity(spec, gpu_device_ext)
 
 
+def test_c2_textured_extended_1080p(gpu_device_ext):
+    """The benchmark's headline pass at its full size: perspective-correct uv,
+    bilinear Repeat sampling of the sRGB texture, 1920x1080."""
+    assert_parity(scenes.config2_textured(extended=True), gpu_device_ext)
+
+
+def test_c2_textured_extended_1080p_wgsl(gpu_device_ext):
+    """The same pass with the WGSL forms of its shaders (the drop-in path)."""
+    assert_parity(scenes.as_wgsl(scenes.config2_textured(extended=True)), gpu_device_ext)
+
+
 @pytest.mark.parametrize("make", [scenes.perspective_quad, scenes.bilinear_strip])
 def test_extension_closed_form_scenes(gpu_device_ext, make):
     assert_parity(make(), gpu_device_ext)
@@ -125,6 +136,12 @@ def test_c3_overdraw_blend_small(gpu_device_ext):
 
 def test_c3_overdraw_reference_full(gpu_device):
     assert_parity(scenes.config3_overdraw(), gpu_device)
+
+
+def test_c3_overdraw_blend_full(gpu_device_ext):
+    """200k triangles over 2560x2440 blended src-over in API order: every
+    colour byte equal to the oracle's sequential blend."""
+    assert_parity(scenes.config3_overdraw(blend=True), gpu_device_ext)
 
 
 # ---------------------------------------------------------------- pipeline corners

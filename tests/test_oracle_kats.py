@@ -178,3 +178,33 @@ def test_srgb_table_endpoints():
     assert t[0] == 0.0 and t[255] == 1.0
     assert abs(t[10] - (10 / 255) / 12.92) < 1e-7
     assert np.all(np.diff(t) > 0)
+
+
+# sRGB -> linear values of IEC 61966-2-1 for 8-bit codes, as published in
+# sRGB conversion tables (7 decimals).  Codes 10 and 11 sit on either side of
+# the 0.04045 threshold; the others are in the power branch, where an
+# exponent of 2.2 (128 -> 0.2158605 becomes 0.2195) or a shifted threshold
+# fails.
+SRGB_PUBLISHED = {10: 0.0030353, 11: 0.0033465, 64: 0.0512695, 128: 0.2158605,
+                  188: 0.5028865, 200: 0.5775804, 255: 1.0}
+
+
+@pytest.mark.parametrize("code,linear", sorted(SRGB_PUBLISHED.items()))
+def test_srgb_table_published_values(code, linear):
+    t = bridge.srgb_table()
+    # 7-decimal rounding (5e-8) plus one f32 ulp at 1.0 (6e-8)
+    assert abs(float(t[code]) - linear) <= 1.1e-7, (code, float(t[code]), linear)
+
+
+def test_srgb_table_inverts_to_its_codes():
+    """The encoding direction of the same standard (linear -> sRGB:
+    12.92 L below 0.0031308, else 1.055 L^(1/2.4) - 0.055) maps every table
+    entry back to its own code: a different formula, so a mistake in the
+    decoding branch or its threshold shows up as a code that does not come
+    back."""
+    t = bridge.srgb_table().astype(np.float64)
+    enc = np.where(t <= 0.0031308, 12.92 * t, 1.055 * np.power(t, 1.0 / 2.4) - 0.055)
+    back = np.round(enc * 255.0).astype(int)
+    assert np.array_equal(back, np.arange(256))
+    # and the round trip is tight, not merely within half a code
+    assert np.abs(enc * 255.0 - np.arange(256)).max() < 1e-4
