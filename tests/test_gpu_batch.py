@@ -73,6 +73,27 @@ def test_graph_replay_and_batching_switch(gpu_device):
         assert np.array_equal(a.color, b.color) and np.array_equal(a.depth, b.depth)
 
 
+def test_plan_rebuilt_between_unpolled_submissions(gpu_device):
+    """Switching the raster tile rebuilds a command buffer's plan while
+    submissions that replay its graph are still queued: the old graph and
+    draw tables are let go only after those have executed."""
+    announce("plan rebuilt between unpolled submissions")
+    specs = small_batch(12, seed=5)
+    pp = runner.PreparedPasses(gpu_device, specs)
+    try:
+        pp.submit()
+        pp.submit()          # captures the graph
+        pp.submit()          # replays it
+        gpu_device.set_option(wgc.OPTION_RASTER_TILE, 3)
+        pp.submit()          # new plan; the old one is still queued
+        gpu_device.set_option(wgc.OPTION_RASTER_TILE, 0)
+        pp.submit()
+        res = pp.results()
+    finally:
+        gpu_device.set_option(wgc.OPTION_RASTER_TILE, 0)
+    check_against_oracle(specs, res, pp.total_fragments())
+
+
 def test_dependent_passes_stay_ordered(gpu_device):
     """Two passes on the same target: the second loads what the first drew,
     so they must not share a launch group."""

@@ -389,7 +389,7 @@ struct Step {
 };
 
 struct Plan {
-    WgcDevice_T *dev = nullptr;      // set (and retained) while the plan holds pooled memory
+    WgcDevice_T *dev = nullptr;      // set (and retained) while the plan holds pooled memory or a graph
     uint64_t arena_gen = UINT64_MAX;
     size_t zone_used = 0;            // bytes of the arena's zero region its launch groups use
     uint32_t raster_cfg = UINT32_MAX;
@@ -455,13 +455,23 @@ struct Inflight {
     uint64_t prims, launches;
 };
 
-// Pooled memory whose release waits until the submissions before it have
-// executed (they may still read it).
+// Pooled memory whose release, and graphs whose destruction, wait until the
+// submissions before them have executed (they may still read the memory or
+// run the graph).
 struct Deferred {
     uint64_t after;              // submission index that must have retired
     BlockPool *pool;
     uint8_t *ptr;
     uint64_t size;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+
+    void finish()
+    {
+        if (pool) pool->release(ptr, size);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
 };
 
 struct WgcDevice_T : Object {
@@ -492,6 +502,7 @@ struct WgcDevice_T : Object {
     std::vector<HostCounters *> slot_blocks;
     BlockPool table_pool;                   // device draw tables of plans
     BlockPool staging_pool;                 // pinned staging of their uploads
+    std::mutex deferred_mu;                 // plans are reset from any thread
     std::vector<Deferred> deferred;
     std::vector<WgcBuffer_T *> pending_maps;   // held until their copies complete
     uint32_t pending_error = 0;             // error bits of retired submissions not yet reported
@@ -519,6 +530,11 @@ struct WgcDevice_T : Object {
             (void)hipEventDestroy(f.done);
             for (auto *o : f.keep) release(o);
         }
+        for (auto &x : deferred) {   // the stream has drained: nothing reads them any more
+            x.pool = nullptr;
+            x.finish();
+        }
+        deferred.clear();
         table_pool.destroy();
         staging_pool.destroy();
         for (auto *b : slot_blocks) (void)hipHostFree(b);
@@ -596,13 +612,22 @@ WgcHeap_T::~WgcHeap_T()
 // executed, and the graph.
 void Plan::reset()
 {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
     if (dev) {
-        uint64_t after = dev->next_index - 1;
-        if (d_tables) dev->deferred.push_back(Deferred{after, &dev->table_pool, d_tables, tables_size});
-        if (h_tables) dev->deferred.push_back(Deferred{after, &dev->staging_pool, h_tables, tables_size});
-        release(dev);
+        // every submission enqueued so far may still read the tables or run
+        // the graph: they go once those have executed
+        {
+            std::lock_guard<std::mutex> lk(dev->deferred_mu);
+            uint64_t after = dev->next_index - 1;
+            if (d_tables)
+                dev->deferred.push_back(Deferred{after, &dev->table_pool, d_tables, tables_size, nullptr, nullptr});
+            if (h_tables)
+                dev->deferred.push_back(Deferred{after, &dev->staging_pool, h_tables, tables_size, nullptr, nullptr});
+            if (graph || exec) dev->deferred.push_back(Deferred{after, nullptr, nullptr, 0, graph, exec});
+        }
+        release(dev);   // may destroy the device (and with it what was deferred)
+    } else {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
     }
     dev = nullptr;
     exec = nullptr;
@@ -732,11 +757,13 @@ static int32_t retire(WgcDevice_T *d, uint64_t upto, bool wait, uint64_t timeout
         d->inflight.erase(d->inflight.begin());
         for (auto *o : keep) release(o);  // may reset plans, which defer their memory
     }
-    // memory whose last reader has executed goes back to its pool
+    // memory whose last reader has executed goes back to its pool, and
+    // graphs that no submission runs any more are destroyed
+    std::lock_guard<std::mutex> lk(d->deferred_mu);
     for (size_t i = 0; i < d->deferred.size();) {
         Deferred &x = d->deferred[i];
         if (x.after <= d->last_done) {
-            x.pool->release(x.ptr, x.size);
+            x.finish();
             x = d->deferred.back();
             d->deferred.pop_back();
         } else {
@@ -2805,6 +2832,10 @@ static int32_t submit_plan(WgcDevice_T *dev, Plan &plan)
         }
         if (e2 != hipSuccess) return hip_fail(e2, "graph capture");
         plan.graph = g;
+        if (!plan.dev) {   // its graph is destroyed through the device's deferred list
+            plan.dev = dev;
+            retain(dev);
+        }
         HIP_TRY(hipGraphInstantiate(&plan.exec, g, nullptr, nullptr, 0), "graph instantiate");
     }
     HIP_TRY(hipGraphLaunch(plan.exec, dev->stream), "graph launch");
