@@ -388,6 +388,24 @@ void wgc_command_buffer_release(WgcCommandBuffer buffer);
  * be submitted again. */
 int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcCommandBuffer *buffers,
                          uint64_t *submission_index);
+/* The allocation-free variant of wgc_queue_submit.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * running while a HIP graph is captured, tight serving loops, harnesses that
+ * forbid hipMalloc).  It calls no hipMalloc, hipFree, hipHostMalloc or other
+ * allocating routine and creates no HIP objects: the launch groups' scratch
+ * (bins, triangle records, lists, counters) lives in a static device array of
+ * 128 MiB per GPU (held from the library's first use of that GPU until the
+ * process ends), completion events and counter slots come from pools filled
+ * at wgc_device_create (when all are in flight it waits for the oldest
+ * submission), and plans run without batching tables and without graphs.
+ * Results equal wgc_queue_submit's.  A command buffer whose scratch exceeds
+ * the array returns WGC_ERR_OUT_OF_MEMORY before anything is launched.  The
+ * array is shared by every device object on one GPU and keeps the last
+ * submission's contents (it is cleared where it must start at zero):
+ * allocation-free submissions of different device objects that overlap on
+ * one GPU race, those of one device object run in stream order. */
+int32_t wgc_queue_submit_noalloc(WgcDevice device, uint32_t count, const WgcCommandBuffer *buffers,
+                                 uint64_t *submission_index);
 /* Waits until `submission_index` has executed (UINT64_MAX: everything).
  * timeout_ns 0 = no timeout; a timeout of 1 ns checks without waiting
  * (WGC_ERR_TIMEOUT while it is still running). Returns WGC_ERR_EXECUTION if a

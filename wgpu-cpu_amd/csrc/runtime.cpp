@@ -194,6 +194,40 @@ extern "C" uint32_t wgc_abi_version(void) { return WGC_ABI_VERSION; }
 extern "C" const char *wgc_last_error(void) { return g_last_error.c_str(); }
 
 // ------------------------------------------------------------------
+// static scratch of the allocation-free submission
+// ------------------------------------------------------------------
+
+// wgc_queue_submit_noalloc lays its launch groups' scratch (zero region,
+// kept-zero region, bins, records, lists) out in this array instead of the
+// device's growable arena.  Every GPU the library runs on holds its own copy
+// until the process ends.  In host code `scratch` names a host-side stand-in,
+// not a device address: host code only ever uses the address that
+// hipGetSymbolAddress returns for the current device (WgcDevice_T::scratch).
+static constexpr size_t kScratchBytes = 128ull << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::map<int, uint8_t *> g_scratch_address;   // HIP ordinal -> address of `scratch` there
+
+// The address of `scratch` on `hip_device`, which must be current.
+static hipError_t scratch_address(int hip_device, uint8_t *&out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_address.find(hip_device);
+    if (it != g_scratch_address.end()) {
+        out = it->second;
+        return hipSuccess;
+    }
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_address[hip_device] = (uint8_t *)p;
+    out = (uint8_t *)p;
+    return hipSuccess;
+}
+
+// ------------------------------------------------------------------
 // objects
 // ------------------------------------------------------------------
 
@@ -392,8 +426,10 @@ struct Plan {
     WgcDevice_T *dev = nullptr;      // set (and retained) while the plan holds pooled memory or a graph
     uint64_t arena_gen = UINT64_MAX;
     size_t zone_used = 0;            // bytes of the arena's zero region its launch groups use
+    size_t keep_used = 0;            // bytes of the kept-zero region its launch groups use
     uint32_t raster_cfg = UINT32_MAX;
     bool batching = false;
+    bool noalloc = false;            // laid out in the device's static scratch (no tables, no graph)
     std::vector<Step> steps;
     uint8_t *d_tables = nullptr;     // from the device's table pool
     uint8_t *h_tables = nullptr;     // pinned staging of the upload, from the staging pool
@@ -411,9 +447,13 @@ struct Plan {
 struct WgcCommandBuffer_T : Object {
     std::vector<PassCommand> passes;
     std::unique_ptr<Plan> plan{new Plan()};  // launch plan cached for resubmission
+    // the same passes laid out in the static scratch, for the allocation-free
+    // submission (built on its first use)
+    std::unique_ptr<Plan> plan_noalloc{new Plan()};
     ~WgcCommandBuffer_T() override
     {
         plan.reset();
+        plan_noalloc.reset();
         for (auto &p : passes) p.drop();
     }
 };
@@ -504,6 +544,9 @@ struct WgcDevice_T : Object {
     BlockPool staging_pool;                 // pinned staging of their uploads
     std::mutex deferred_mu;                 // plans are reset from any thread
     std::vector<Deferred> deferred;
+    // address of the static scratch array on this device (allocation-free
+    // submissions lay their plans out in it)
+    uint8_t *scratch = nullptr;
     std::vector<WgcBuffer_T *> pending_maps;   // held until their copies complete
     uint32_t pending_error = 0;             // error bits of retired submissions not yet reported
     uint64_t last_frags = 0;                // fragments of the last retired submission
@@ -651,6 +694,9 @@ static float srgb_entry(int i)
     return (float)l;
 }
 
+static hipError_t take_slot(WgcDevice_T *d, HostCounters *&slot);
+static constexpr int kPreparedEvents = 64;
+
 extern "C" int32_t wgc_device_create(const WgcDeviceDescriptor *desc, WgcDevice *out)
 {
     if (!desc || !out) return set_error(WGC_ERR_INVALID, "null argument");
@@ -672,6 +718,7 @@ extern "C" int32_t wgc_device_create(const WgcDeviceDescriptor *desc, WgcDevice 
         e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->hip_device);
         if (e == hipSuccess && cus > 0) d->cu_count = (uint32_t)cus;
     }
+    if (e == hipSuccess) e = scratch_address(d->hip_device, d->scratch);
     if (e == hipSuccess) e = hipMalloc(&d->d_counters, sizeof(DevCounters));
     if (e == hipSuccess) e = hipMalloc(&d->d_srgb, 256 * sizeof(float));
     if (e == hipSuccess) e = stream_zero(d->stream, d->d_counters, sizeof(DevCounters));
@@ -686,6 +733,19 @@ extern "C" int32_t wgc_device_create(const WgcDeviceDescriptor *desc, WgcDevice 
         float lut[256];
         for (int i = 0; i < 256; ++i) lut[i] = srgb_entry(i);
         e = stream_copy(d->stream, d->d_srgb, lut, sizeof lut, hipMemcpyHostToDevice);
+    }
+    // the first block of pinned counter slots and a pool of completion
+    // events, so that the allocation-free submission finds them (it waits
+    // for a submission to retire rather than creating more)
+    if (e == hipSuccess) {
+        HostCounters *slot = nullptr;
+        e = take_slot(d, slot);
+        if (e == hipSuccess) d->slot_pool.push_back(slot);
+    }
+    for (int i = 0; i < kPreparedEvents && e == hipSuccess; ++i) {
+        hipEvent_t ev = nullptr;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) d->event_pool.push_back(ev);
     }
     if (e != hipSuccess) {
         release(d);
@@ -2534,9 +2594,10 @@ static void copy_step(const PassCommand &pc, Step &s)
     }
 }
 
-static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
+static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan, bool noalloc)
 {
     plan.reset();
+    plan.noalloc = noalloc;
     plan.hip_device = dev->hip_device;
     uint32_t cfg = dev->raster_cfg;
     uint32_t shift = raster_tile_shift(cfg);
@@ -2570,7 +2631,8 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         auto mergeable = [](const PassWork &q) {
             return q.draws.size() == 1 && !is_point_or_line(q.draws[0].d) && !restart_strip(q.draws[0].d);
         };
-        bool batchable = dev->batching && mergeable(pw);
+        // the allocation-free form has no draw tables, so no batched groups
+        bool batchable = dev->batching && !noalloc && mergeable(pw);
         if (!batchable) {
             if (pw.copy) {
                 pw.copy_step.stage = stage++;
@@ -2679,7 +2741,21 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
             tables = align_up(tables + n * sizeof(DrawParams) + 8 * (n + 1) + 4 * (waves + tiles), 256);
         }
     }
-    if (zneed > dev->zone_size || kneed > dev->keep_size ||
+    uint8_t *zone_base, *keep_base, *arena_base;
+    if (noalloc) {
+        // the static scratch: zero region, kept-zero region, then the stages
+        size_t zone = align_up(zneed, 256), keep = align_up(kneed, 256);
+        if (tables) return set_error(WGC_ERR_INVALID, "batched launch group in an allocation-free plan");
+        if (zone + keep + need > kScratchBytes)
+            return set_error(WGC_ERR_OUT_OF_MEMORY,
+                             "the command buffer needs " + std::to_string(zone + keep + need) +
+                                 " bytes of scratch; the allocation-free submission has " +
+                                 std::to_string(kScratchBytes));
+        if (!dev->scratch) return set_error(WGC_ERR_DEVICE, "no scratch address for this device");
+        zone_base = dev->scratch;
+        keep_base = dev->scratch + zone;
+        arena_base = dev->scratch + zone + keep;
+    } else if (zneed > dev->zone_size || kneed > dev->keep_size ||
         dev->zone_size + dev->keep_size + need > dev->arena_size) {
         HIP_TRY(hipStreamSynchronize(dev->stream), "arena");
         if (dev->arena) HIP_TRY(hipFree(dev->arena), "arena");
@@ -2698,6 +2774,11 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
         dev->keep_size = keep;
         dev->arena_gen++;
     }
+    if (!noalloc) {
+        zone_base = dev->arena;
+        keep_base = dev->arena + dev->zone_size;
+        arena_base = dev->arena + dev->zone_size + dev->keep_size;
+    }
     // draw tables: pooled device memory, uploaded from pooled pinned staging
     // without a host wait (the upload is ordered before the plan's launches)
     if (tables) {
@@ -2714,8 +2795,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
     }
     for (Step &s : plan.steps) {
         if (s.kind == StepKind::Lines) {
-            layout_lines(s, dev->arena + dev->zone_size + dev->keep_size + s.arena_off, dev->frag_px,
-                         dev->frag_px_size, dev);
+            layout_lines(s, arena_base + s.arena_off, dev->frag_px, dev->frag_px_size, dev);
             plan.launches += s.gl.ordered ? 1 : 3;
             plan.prims += s.draws[0].total_prims;
             continue;
@@ -2724,8 +2804,8 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
             plan.launches += 1;
             continue;
         }
-        layout_group(s, dev->arena + dev->zone_size + dev->keep_size + s.arena_off, dev->arena + s.zone_off,
-                     dev->arena + dev->zone_size + s.keep_off, dev->frag_px, dev->frag_px_size, dev);
+        layout_group(s, arena_base + s.arena_off, zone_base + s.zone_off, keep_base + s.keep_off, dev->frag_px,
+                     dev->frag_px_size, dev);
         plan.launches += (s.gl.total_prims > 0 ? (s.gl.inline_clip ? 2 : 3) : 1) + (!s.multi && restart_strip(s.draws[0]) ? 2 : 0);
         for (auto &d : s.draws) plan.prims += d.total_prims;
         if (!s.multi) continue;
@@ -2763,6 +2843,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan)
                 "draw tables");
     plan.arena_gen = dev->arena_gen;
     plan.zone_used = zneed;
+    plan.keep_used = kneed;
     plan.raster_cfg = cfg;
     plan.batching = dev->batching;
     return WGC_OK;
@@ -2866,15 +2947,53 @@ static bool uses_mapped_buffer(const WgcCommandBuffer_T *cb)
     return false;
 }
 
-extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcCommandBuffer *buffers,
-                                    uint64_t *submission_index)
+// A free pinned counter slot and completion event for a submission.  The
+// allocation-free form never creates either: with none free it waits for
+// the oldest submission in flight to retire and give its own back.
+static int32_t take_slot_and_event(WgcDevice_T *d, bool noalloc, HostCounters *&slot, hipEvent_t &done)
+{
+    if (noalloc) {
+        while ((d->slot_pool.empty() || d->event_pool.empty()) && !d->inflight.empty()) {
+            int32_t r = retire(d, d->inflight.front().index, true, 0);
+            if (r != WGC_OK) return r;
+        }
+        if (d->slot_pool.empty() || d->event_pool.empty())
+            return set_error(WGC_ERR_OUT_OF_MEMORY, "no free counter slot or event");
+    } else if (d->event_pool.empty()) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        d->event_pool.push_back(e);
+    }
+    HIP_TRY(take_slot(d, slot), "pinned counter slot");
+    done = d->event_pool.back();
+    d->event_pool.pop_back();
+    return WGC_OK;
+}
+
+static int32_t submit_plans(WgcDevice_T *device, uint32_t count, const WgcCommandBuffer *buffers, bool noalloc,
+                            Inflight &f);
+
+// Both submission forms.  `noalloc`: plans laid out in the static scratch,
+// launched directly (no graph), no device or pinned allocation.
+static int32_t submit(WgcDevice_T *device, uint32_t count, const WgcCommandBuffer *buffers,
+                      uint64_t *submission_index, bool noalloc)
 {
     if (!device || (count && !buffers)) return set_error(WGC_ERR_INVALID, "null argument");
     std::lock_guard<std::mutex> lk(device->mu);
     HIP_TRY(hipSetDevice(device->hip_device), "hipSetDevice");
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!buffers[i]) return set_error(WGC_ERR_INVALID, "null command buffer");
+        if (uses_mapped_buffer(buffers[i]))
+            return set_error(WGC_ERR_INVALID, "a buffer used by the command buffer is mapped");
+    }
     Inflight f;
     f.prims = f.launches = 0;
-    HIP_TRY(take_slot(device, f.slot), "pinned counter slot");
+    int32_t r0 = take_slot_and_event(device, noalloc, f.slot, f.done);
+    if (r0 != WGC_OK) return r0;
+    auto give_back = [&]() {
+        device->slot_pool.push_back(f.slot);
+        device->event_pool.push_back(f.done);
+    };
     // the counters are zero here: every submission takes (and zeroes) them
     // at its end, or is followed by this reset when it stopped before that
     if (device->counters_dirty) {
@@ -2882,45 +3001,75 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
         // words in the arena's zero region
         hipError_t e0 = launch_reset(device->d_frags, device->d_error, device->stream);
         if (e0 == hipSuccess && device->arena && device->zone_size + device->keep_size)
-            e0 = hipMemsetAsync(device->arena, 0, device->zone_size + device->keep_size, device->stream);
+            e0 = launch_fill(device->arena, device->zone_size + device->keep_size, 0u, device->stream, nullptr);
         if (e0 != hipSuccess) {
-            device->slot_pool.push_back(f.slot);
+            give_back();
             return hip_fail(e0, "counter reset");
         }
         device->counters_dirty = false;
     }
-    for (uint32_t i = 0; i < count; ++i)
-        if (buffers[i] && uses_mapped_buffer(buffers[i])) {
-            device->slot_pool.push_back(f.slot);
-            return set_error(WGC_ERR_INVALID, "a buffer used by the command buffer is mapped");
-        }
     f.index = device->next_index++;
     device->counters_dirty = true;
+    int32_t r = submit_plans(device, count, buffers, noalloc, f);
+    if (r != WGC_OK) {
+        // the counters stay dirty; the next submission resets them.  Kernels
+        // of the plans launched before the failure may still run on the
+        // command buffers' resources: they finish before those are let go.
+        (void)hipStreamSynchronize(device->stream);
+        for (auto *o : f.keep) release(o);
+        give_back();
+        return r;
+    }
+    device->counters_dirty = false;
+    if (submission_index) *submission_index = f.index;
+    device->inflight.push_back(std::move(f));
+    // keep the queue short: retire what is already done
+    return retire(device, UINT64_MAX, false, 0);
+}
+
+// Launches the plans of a submission and its end-of-submission kernel, and
+// records its completion event.
+static int32_t submit_plans(WgcDevice_T *device, uint32_t count, const WgcCommandBuffer *buffers, bool noalloc,
+                            Inflight &f)
+{
+    KernelTimer *kt = (device->timing && !noalloc) ? &device->timer : nullptr;
     size_t zone_dirty = 0;   // bytes of the zero region the plans launched so far have used
+    uint8_t *zone = noalloc ? device->scratch : device->arena;
     for (uint32_t i = 0; i < count; ++i) {
         WgcCommandBuffer_T *cb = buffers[i];
-        if (!cb) return set_error(WGC_ERR_INVALID, "null command buffer");
-        Plan &plan = *cb->plan;
+        Plan &plan = noalloc ? *cb->plan_noalloc : *cb->plan;
         if (plan.arena_gen != device->arena_gen || plan.raster_cfg != device->raster_cfg ||
-            plan.batching != device->batching || plan.hip_device != device->hip_device) {
-            int32_t r = build_plan(device, cb, plan);
+            (!noalloc && plan.batching != device->batching) || plan.hip_device != device->hip_device) {
+            int32_t r = build_plan(device, cb, plan, noalloc);
             if (r != WGC_OK) {
                 plan.reset();
                 return r;
             }
             // growing the arena may have invalidated plans built before
             if (plan.arena_gen != device->arena_gen) {
-                r = build_plan(device, cb, plan);
+                r = build_plan(device, cb, plan, noalloc);
                 if (r != WGC_OK) return r;
             }
+            zone = noalloc ? device->scratch : device->arena;
         }
-        if (zone_dirty && plan.zone_used) {
+        if (noalloc) {
+            // the scratch keeps whatever the last user left in it (another
+            // device object on this GPU, a submission that stopped early):
+            // its zero and kept-zero regions are cleared before each plan
+            size_t z = align_up(plan.zone_used, 256) + align_up(plan.keep_used, 256);
+            HIP_TRY(launch_fill(device->scratch, z, 0u, device->stream, nullptr), "scratch zero regions");
+            zone_dirty = 0;
+        } else if (zone_dirty && plan.zone_used) {
             // the plans of one submission share the zero region
             HIP_TRY(launch_fill(device->arena, zone_dirty, 0u, device->stream, nullptr), "zero region");
             zone_dirty = 0;
         }
-        int32_t r = submit_plan(device, plan);
-        if (r != WGC_OK) return r;
+        if (noalloc) {
+            HIP_TRY(launch_plan(device, plan, kt), "launch");
+        } else {
+            int32_t r = submit_plan(device, plan);
+            if (r != WGC_OK) return r;
+        }
         zone_dirty = std::max(zone_dirty, plan.zone_used);
         f.prims += plan.prims;
         f.launches += plan.launches;
@@ -2928,20 +3077,22 @@ extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcC
         f.keep.push_back(cb);
     }
     // the submission's counters, read by the host once its event has passed
-    HIP_TRY(launch_end_submission(device->d_frags, device->d_error, f.slot, device->arena, zone_dirty,
-                                  device->stream), "counter readback");
-    device->counters_dirty = false;
-    if (!device->event_pool.empty()) {
-        f.done = device->event_pool.back();
-        device->event_pool.pop_back();
-    } else {
-        HIP_TRY(hipEventCreateWithFlags(&f.done, hipEventDisableTiming), "hipEventCreate");
-    }
+    HIP_TRY(launch_end_submission(device->d_frags, device->d_error, f.slot, zone, zone_dirty, device->stream),
+            "counter readback");
     HIP_TRY(hipEventRecord(f.done, device->stream), "hipEventRecord");
-    if (submission_index) *submission_index = f.index;
-    device->inflight.push_back(std::move(f));
-    // keep the queue short: retire what is already done
-    return retire(device, UINT64_MAX, false, 0);
+    return WGC_OK;
+}
+
+extern "C" int32_t wgc_queue_submit(WgcDevice device, uint32_t count, const WgcCommandBuffer *buffers,
+                                    uint64_t *submission_index)
+{
+    return submit(device, count, buffers, submission_index, false);
+}
+
+extern "C" int32_t wgc_queue_submit_noalloc(WgcDevice device, uint32_t count, const WgcCommandBuffer *buffers,
+                                            uint64_t *submission_index)
+{
+    return submit(device, count, buffers, submission_index, true);
 }
 
 extern "C" int32_t wgc_device_set_option(WgcDevice device, uint32_t option, uint32_t value)

@@ -236,6 +236,7 @@ SIGNATURES = {
     "wgc_command_encoder_release": (None, [_P]),
     "wgc_command_buffer_release": (None, [_P]),
     "wgc_queue_submit": (C.c_int32, [_P, C.c_uint32, _PP, C.POINTER(C.c_uint64)]),
+    "wgc_queue_submit_noalloc": (C.c_int32, [_P, C.c_uint32, _PP, C.POINTER(C.c_uint64)]),
     "wgc_device_poll": (C.c_int32, [_P, C.c_uint64, C.c_uint64]),
     "wgc_device_last_pass_stats": (C.c_int32, [_P, C.POINTER(PassStats)]),
     "wgc_device_totals": (C.c_int32, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
@@ -711,6 +712,14 @@ class Device:
         arr = (C.c_void_p * max(1, len(buffers)))(*[b.ptr.value for b in buffers])
         idx = C.c_uint64()
         _check(self._lib.wgc_queue_submit(self._ptr, len(buffers), arr, C.byref(idx)))
+        return idx.value
+
+    def submit_noalloc(self, buffers: Sequence[CommandBuffer]) -> int:
+        """The allocation-free submission (wgc_queue_submit_noalloc): same
+        results, scratch from the library's static per-GPU array."""
+        arr = (C.c_void_p * max(1, len(buffers)))(*[b.ptr.value for b in buffers])
+        idx = C.c_uint64()
+        _check(self._lib.wgc_queue_submit_noalloc(self._ptr, len(buffers), arr, C.byref(idx)))
         return idx.value
 
     def poll(self, submission_index: Optional[int] = None, timeout_ns: int = 0):
