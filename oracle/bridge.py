@@ -45,6 +45,7 @@ class OrDraw(C.Structure):
         ("indexed", C.c_uint32), ("first", C.c_uint32), ("count", C.c_uint32),
         ("base_vertex", C.c_int32), ("first_instance", C.c_uint32), ("instance_count", C.c_uint32),
         ("topology", C.c_uint32), ("strip_restart", C.c_uint32),
+        ("fs_flags", C.c_uint32), ("fs_early", C.c_uint32),
     ]
 
 
@@ -160,6 +161,16 @@ def render(spec, frag_counts: bool = False) -> OracleResult:
                 o.attr[loc] = OrAttribute(0, off, fmt)
                 n_attr = max(n_attr, loc + 1)
             o.n_attr = n_attr
+        if d.instance_buffer is not None:
+            data, stride, attrs = d.instance_buffer
+            ibuf_i = (C.c_uint8 * len(data)).from_buffer_copy(data)
+            keep.append(ibuf_i)
+            slot = o.n_vb
+            o.vb[slot] = OrVertexBuffer(C.addressof(ibuf_i), len(data), stride, 1)
+            o.n_vb = slot + 1
+            for fmt, off, loc in attrs:
+                o.attr[loc] = OrAttribute(slot, off, fmt)
+                o.n_attr = max(o.n_attr, loc + 1)
         if d.index_data is not None:
             ibuf = (C.c_uint8 * len(d.index_data)).from_buffer_copy(d.index_data)
             keep.append(ibuf)
@@ -195,6 +206,7 @@ def render(spec, frag_counts: bool = False) -> OracleResult:
             if d.index_data is not None and d.strip_index_format != d.index_format:
                 raise OracleError("strip_index_format differs from the index format")
             o.strip_restart = 1
+        o.fs_flags, o.fs_early = d.fs_flags, d.fs_early
         o.base_vertex = d.base_vertex
         o.first_instance, o.instance_count = d.first_instance, d.instance_count
     p = OrPass()

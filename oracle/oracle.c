@@ -566,7 +566,7 @@ static void texture_sample(const OrDraw *d, float u, float v, float out[4])
     fetch_texel(d, x, y, out);
 }
 
-static void run_fs(const OrDraw *d, const float *var, int nvar, float out[4])
+static void run_program_fs(const OrDraw *d, const float *var, int nvar, float out[4])
 {
     switch (d->program) {
     case OR_PROG_MVP_TEXTURE:
@@ -579,6 +579,22 @@ static void run_fs(const OrDraw *d, const float *var, int nvar, float out[4])
         out[3] = (nvar >= 4) ? var[3] : 1.0f;
         return;
     }
+}
+
+/* The fragment stage of a draw: the program's shader, then the test
+ * program's steps in the order its WGSL states them.  Returns 0 when the
+ * shader discards (EntryPointError::Killed, fragment.rs:202-213); *frag_depth
+ * receives a written @builtin(frag_depth). */
+static int run_fs(const OrDraw *d, const float *var, int nvar, uint32_t prim, float out[4], float *frag_depth)
+{
+    if (d->fs_flags & OR_FS_DISCARD) {
+        if (var[3] < 0.5f) return 0;
+    }
+    run_program_fs(d, var, nvar, out);
+    if (d->fs_flags & OR_FS_PRIM_RED) out[0] = (float)(prim % 8u) * 0.125f;
+    if (d->fs_flags & OR_FS_DEPTH_ALPHA) *frag_depth = var[3];
+    if (d->fs_flags & OR_FS_DEPTH_CONST) *frag_depth = 0.3125f;
+    return 1;
 }
 
 /* texture.rs:213-218 + 373-411 */
@@ -938,21 +954,38 @@ int oracle_clip_line(const float a[4], const float b[4], float out[8], float alp
     return 1;
 }
 
-/* Depth test and colour write of one fragment (fragment.rs:427-488). */
-static void write_fragment(OrPass *p, const OrDraw *d, uint32_t x, uint32_t y, float z, const float *var, int nvar)
+/* FragmentOutput::depth_test (fragment.rs:427-453): compares `z` with the
+ * stored depth and writes it when the test passes and writes are on; true
+ * without a depth attachment or depth state. */
+static int depth_test(OrPass *p, const OrDraw *d, uint32_t x, uint32_t y, float z)
 {
-    float color[4];
-    run_fs(d, var, nvar, color);
-    int pass = 1;
-    if (p->depth && d->depth_enabled) {
-        float *dp = p->depth + (uint64_t)y * p->width + x;
-        if (compare(d->depth_compare, z, *dp)) {
-            if (d->depth_write) *dp = z;
-        } else {
-            pass = 0;
-        }
+    if (!(p->depth && d->depth_enabled)) return 1;
+    float *dp = p->depth + (uint64_t)y * p->width + x;
+    if (!compare(d->depth_compare, z, *dp)) return 0;
+    if (d->depth_write) *dp = z;
+    return 1;
+}
+
+/* FragmentProcessingState::process (fragment.rs:127-214) with the output
+ * side (fragment.rs:457-488): the early depth test of @early_depth_test
+ * (force: early only; allow: early and late), the fragment shader, then its
+ * outputs in member order -- a frag_depth member replaces the depth the
+ * late test uses only when it comes before the @location member, where
+ * the late test runs -- and the colour write (or src-over blend). */
+static void write_fragment(OrPass *p, const OrDraw *d, uint32_t x, uint32_t y, float z, const float *var, int nvar,
+                           uint32_t prim)
+{
+    int known = 0;
+    if (d->fs_early) {
+        if (!depth_test(p, d, x, y, z)) return;   /* early reject */
+        known = d->fs_early == 1;                 /* force: the early result stands */
     }
-    if (pass && p->color) {
+    float color[4];
+    float frag_depth = z;
+    if (!run_fs(d, var, nvar, prim, color, &frag_depth)) return;
+    float tz = (d->fs_flags & OR_FS_DEPTH_FIRST) ? frag_depth : z;
+    if (!known && !depth_test(p, d, x, y, tz)) return;
+    if (p->color) {
         if (d->blend) blend_pixel(p, x, y, color);
         else put_pixel(p, x, y, color);
     }
@@ -967,7 +1000,7 @@ static int in_scissor(const OrDraw *d, uint32_t x, uint32_t y)
 
 /* PointRasterizer (raster.rs:198-215): no clipping; the vertex's own
  * varyings (NoInterpolation) */
-static int draw_point(OrPass *p, const OrDraw *d, const vout *v)
+static int draw_point(OrPass *p, const OrDraw *d, const vout *v, uint32_t prim)
 {
     if (v->clip[3] == 0.0f) return fail(p, "clip position w = 0");
     uint32_t fb[2];
@@ -976,7 +1009,7 @@ static int draw_point(OrPass *p, const OrDraw *d, const vout *v)
     if (!in_scissor(d, fb[0], fb[1])) return 0;
     p->fragments++;
     if (p->frag_count) p->frag_count[(uint64_t)fb[1] * p->width + fb[0]]++;
-    write_fragment(p, d, fb[0], fb[1], fr[2], v->var, v->nvar);
+    write_fragment(p, d, fb[0], fb[1], fr[2], v->var, v->nvar, prim);
     return 0;
 }
 
@@ -984,7 +1017,7 @@ static int draw_point(OrPass *p, const OrDraw *d, const vout *v)
  * the clipped end points; L = lerp of the clip alphas by t; the fragment is
  * L applied to the clipped end points' (x, y, z, 1/w), the varyings L
  * applied to the unclipped vertices'. */
-static int draw_line(OrPass *p, const OrDraw *d, const vout v[2])
+static int draw_line(OrPass *p, const OrDraw *d, const vout v[2], uint32_t prim)
 {
     float cp[8], al[2];
     if (!oracle_clip_line(v[0].clip, v[1].clip, cp, al)) return 0;
@@ -1029,12 +1062,12 @@ static int draw_line(OrPass *p, const OrDraw *d, const vout v[2])
                 var[c] = a + bb;
             }
         }
-        write_fragment(p, d, xy[0], xy[1], z, var, nvar);
+        write_fragment(p, d, xy[0], xy[1], z, var, nvar, prim);
     }
     return 0;
 }
 
-static int draw_triangle(OrPass *p, const OrDraw *d, const vout vo[3])
+static int draw_triangle(OrPass *p, const OrDraw *d, const vout vo[3], uint32_t prim)
 {
     uint64_t sx0 = d->scissor[0], sy0 = d->scissor[1];
     uint64_t sx1 = sx0 + d->scissor[2], sy1 = sy0 + d->scissor[3];
@@ -1118,22 +1151,7 @@ static int draw_triangle(OrPass *p, const OrDraw *d, const vout vo[3])
                         var[c] = acc;
                     }
                 }
-                float color[4];
-                run_fs(d, var, nvar, color);
-                /* late depth test on the @location write (fragment.rs:427-488) */
-                int pass = 1;
-                if (p->depth && d->depth_enabled) {
-                    float *dp = p->depth + y * p->width + x;
-                    if (compare(d->depth_compare, frag_z, *dp)) {
-                        if (d->depth_write) *dp = frag_z;
-                    } else {
-                        pass = 0;
-                    }
-                }
-                if (pass && p->color) {
-                    if (d->blend) blend_pixel(p, (uint32_t)x, (uint32_t)y, color);
-                    else put_pixel(p, (uint32_t)x, (uint32_t)y, color);
-                }
+                write_fragment(p, d, (uint32_t)x, (uint32_t)y, frag_z, var, nvar, prim);
             }
         }
     }
@@ -1172,9 +1190,11 @@ static int draw(OrPass *p, const OrDraw *d)
                     rc = fail(p, "vertex fetch out of range");
             }
             if (rc) break;
-            if (n == 1) rc = draw_point(p, d, &vo[0]);
-            else if (n == 2) rc = draw_line(p, d, vo);
-            else rc = draw_triangle(p, d, vo);
+            /* primitive_index: the primitive's place in its instance's
+             * assembly order, culled ones counted (state.rs:536-541) */
+            if (n == 1) rc = draw_point(p, d, &vo[0], (uint32_t)prim);
+            else if (n == 2) rc = draw_line(p, d, vo, (uint32_t)prim);
+            else rc = draw_triangle(p, d, vo, (uint32_t)prim);
         }
     }
     free(pos);

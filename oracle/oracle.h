@@ -120,7 +120,20 @@ typedef struct {
     /* primitive state */
     uint32_t topology;       /* OR_TOPO_* */
     uint32_t strip_restart;  /* strip_index_format set: index MAX separates strips */
+    /* fragment test programs: OR_PROG_CLIP_COLOR's vertex stage with a
+     * fragment entry point that reads the colour varying c and
+     * @builtin(primitive_index), built from the OR_FS_* bits below (the
+     * tests hold the WGSL text of each combination) */
+    uint32_t fs_flags;
+    uint32_t fs_early;       /* @early_depth_test: 0 none, 1 force, 2 allow (fragment.rs:166-194) */
 } OrDraw;
+
+/* OR_FS_DISCARD: discard when c.w < 0.5; OR_FS_DEPTH_ALPHA / _CONST: write
+ * @builtin(frag_depth) = c.w / 0.3125; OR_FS_DEPTH_FIRST: that output is a
+ * member before the @location(0) one (so the late depth test sees it);
+ * OR_FS_PRIM_RED: red = f32(primitive_index % 8u) * 0.125. */
+enum { OR_FS_DISCARD = 1, OR_FS_DEPTH_ALPHA = 2, OR_FS_DEPTH_CONST = 4, OR_FS_DEPTH_FIRST = 8,
+       OR_FS_PRIM_RED = 16 };
 
 typedef struct {
     uint32_t width, height;

@@ -212,3 +212,44 @@ def test_default_perspective_interpolation_needs_extended_semantics():
     with pytest.raises(wgc.WgcError) as e:
         wgc.wgsl_check_pipeline(src)
     assert "todo!()" in str(e.value)
+
+
+# ---------------------------------------------------------------- the reference's own shaders
+
+REFERENCE = Path("/root/reference")
+# Interfaces read off the reference's shader files: per file, the entry
+# points' user outputs (location, components), whether the vertex stage
+# reads vertex_index, and the module's resources (group, binding, kind).
+REFERENCE_SHADERS = {
+    "wgpu-cpu-tests/src/tests/colored_triangle.wgsl": ([(0, 4)], True, []),
+    "wgpu-cpu/examples/hello_shader.wgsl": ([(0, 4)], True, []),
+    "wgpu-cpu/examples/hello_mesh.wgsl": ([(0, 4)], True, [(0, 0, "uniform")]),
+    "wgpu-cpu/examples/hello_texture.wgsl": ([(0, 2)], True, [(0, 0, "uniform"), (1, 0, "texture"),
+                                                             (1, 1, "sampler")]),
+}
+
+
+@pytest.mark.parametrize("rel", sorted(REFERENCE_SHADERS))
+def test_reference_shader_files_translate(rel):
+    """The reference's own WGSL files (read in place; skipped where the
+    reference checkout is absent) pass the front end unchanged, with the
+    interface their text declares, and their pipelines compile for gfx950."""
+    path = REFERENCE / rel
+    if not path.exists():
+        pytest.skip("reference checkout not present")
+    _lib_or_skip()
+    src = path.read_text()
+    iface = json.loads(wgc.wgsl_translate(src, 1))
+    outputs, vertex_index, resources = REFERENCE_SHADERS[rel]
+    entries = {e["stage"]: e for e in iface["entries"]}
+    assert set(entries) == {"vertex", "fragment"}
+    vs, fs = entries["vertex"], entries["fragment"]
+    assert vs["name"] == "vs_main" and fs["name"] == "fs_main"
+    assert [(o["location"], o["ncomp"]) for o in vs["outputs"]] == outputs
+    # the fragment stage reads what the vertex stage writes, and writes one vec4
+    assert [(i["location"], i["ncomp"]) for i in fs["inputs"]] == outputs
+    assert [(o["location"], o["ncomp"]) for o in fs["outputs"]] == [(0, 4)]
+    assert vs["position_out"] and vs["vertex_index"] == vertex_index
+    assert not fs["may_discard"] and not fs["frag_depth"]
+    assert [(r["group"], r["binding"], r["kind"]) for r in iface["resources"]] == resources
+    wgc.wgsl_check_pipeline(src)

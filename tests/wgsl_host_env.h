@@ -12,7 +12,7 @@
 #include "wgsl_prelude.h"
 
 struct HostEnv {
-    uint32_t vi = 0, ii = 0;
+    uint32_t vi = 0, ii = 0, pi = 0;
     bool ff = true;
     float frag_pos[4] = {0.0f, 0.0f, 0.0f, 1.0f};
     uint32_t in_loc[16][4] = {};
@@ -27,6 +27,7 @@ struct HostEnv {
 
     uint32_t vertex_index() const { return vi; }
     uint32_t instance_index() const { return ii; }
+    uint32_t primitive_index() const { return pi; }
     bool front_facing() const { return ff; }
     void position(float *p) const { memcpy(p, frag_pos, sizeof frag_pos); }
     void in_location(uint32_t l, uint32_t *w, uint32_t n) const { memcpy(w, in_loc[l], 4 * n); }

@@ -902,7 +902,7 @@ struct VarUsed {
 // Only the varyings the shader can read are interpolated.
 template <int PROG>
 __device__ __forceinline__ bool fragment_color(const DrawParams &d, const TriLds &T, const float lc[3],
-                                               float color[4], const float *srgb)
+                                               float color[4], const float *srgb, float *frag_depth)
 {
     constexpr int NV = VarUsed<PROG>::value;
     float var[kMaxVar];
@@ -932,12 +932,15 @@ __device__ __forceinline__ bool fragment_color(const DrawParams &d, const TriLds
                 var[c] = acc;
             }
         }
-        uint32_t rc = user_fragment(d, TriPos{d, T, lc}, T.ff != 0u, var, color);
+        // primitive_index: the key's primitive within its instance
+        uint32_t prim = (T.key >> 7) % d.nprim;
+        uint32_t rc = user_fragment(d, TriPos{d, T, lc}, T.ff != 0u, prim, var, color, frag_depth);
         if (rc == 0u) return true;
         if (rc != 3u) atomicOr(d.s.error, (uint32_t)ERR_SHADER);   // 3: discard
         return false;
     }
 #endif
+    (void)frag_depth;
     if (d.interp == INTERP_FLAT) {
 #pragma unroll
         for (int c = 0; c < NV; ++c) var[c] = T.var[0][c];
@@ -968,6 +971,40 @@ __device__ __forceinline__ bool fragment_color(const DrawParams &d, const TriLds
     return true;
 }
 
+// The depth tests around a fragment's shader, in API order against the
+// running depth `dep` (fragment.rs:127-214, 427-488): with
+// @early_depth_test the test (and write) before the shader -- force keeps
+// its result, allow tests again after the shader; the late test runs when
+// the colour output is written, on the frag_depth output if the shader wrote
+// one before its colour, else the interpolated depth.  A discarded fragment
+// keeps what an early test wrote.  `shade(&frag_depth)` runs the shader and
+// returns false when it writes nothing.  Returns whether the colour is
+// written.  Without early test or shader-written depth, the test comes
+// first (it does not depend on the shader) so that failing fragments skip
+// the shader.
+template <class Shade>
+__device__ __forceinline__ bool depth_and_shade(const DrawParams &d, float z, float &dep, Shade shade)
+{
+    if (!d.fs_early && !d.fs_depth_tested) {
+        if (d.depth_test && !compare(d.depth_compare, z, dep)) return false;
+        float unused = z;
+        if (!shade(&unused)) return false;
+        if (d.depth_test && d.depth_write) dep = z;
+        return true;
+    }
+    if (d.fs_early && d.depth_test) {
+        if (!compare(d.depth_compare, z, dep)) return false;
+        if (d.depth_write) dep = z;
+    }
+    float frag_depth = z;
+    if (!shade(&frag_depth)) return false;
+    if (d.fs_early != 1u && d.depth_test) {
+        if (!compare(d.depth_compare, frag_depth, dep)) return false;
+        if (d.depth_write) dep = frag_depth;
+    }
+    return true;
+}
+
 // One fragment of triangle T at pixel (pxf, pyf).  Ordered: the depth test
 // against the running depth, the fragment shader and the colour write or
 // blend (fragment.rs:427-488, texture.rs:213-218), in API order.  Otherwise
@@ -982,10 +1019,11 @@ __device__ __forceinline__ void shade_fragment(const DrawParams &d, const TriLds
     float lc[3];
     float z = bary_depth(T, pxf, pyf, lc);
     if (ORDERED) {
-        if (d.depth_test && !compare(d.depth_compare, z, ps.dep)) return;
         float color[4];
-        if (!fragment_color<PROG>(d, T, lc, color, d.srgb_lut)) return;
-        if (d.depth_test && d.depth_write) ps.dep = z;
+        bool ok = depth_and_shade(d, z, ps.dep, [&](float *frag_depth) {
+            return fragment_color<PROG>(d, T, lc, color, d.srgb_lut, frag_depth);
+        });
+        if (!ok) return;
         ps.col = d.blend ? blend_over(color, ps.col, d.color_bgra) : pack_color(color, d.color_bgra);
         ps.has = true;
         return;
@@ -1032,7 +1070,8 @@ __device__ __forceinline__ uint32_t winner_color(const DrawParams &d, uint32_t k
     float color[4];
     // pipelines whose fragment shader can discard use the ordered path, so
     // the winner always writes (an aborted invocation raised ERR_SHADER)
-    (void)fragment_color<PROG>(d, T, lc, color, srgb);
+    float unused = z;
+    (void)fragment_color<PROG>(d, T, lc, color, srgb, &unused);
     return pack_color(color, d.color_bgra);
 }
 
@@ -2153,18 +2192,21 @@ struct VecPos {
 // Fragment shader of a point / line fragment (front_facing is true for
 // them, state.rs:560-561).  False: nothing is written.
 template <int PROG, int N>
-__device__ __forceinline__ bool lp_color(const DrawParams &d, const VsOut *v, const LpFrag &f, float color[4])
+__device__ __forceinline__ bool lp_color(const DrawParams &d, const VsOut *v, const LpFrag &f, uint32_t p,
+                                         float color[4], float *frag_depth)
 {
     float var[kMaxVar];
     lp_varyings<PROG, N>(d, v, f.L, var);
 #ifdef WGC_USER_PROGRAM
     if constexpr (PROG == PROG_USER) {
-        uint32_t rc = user_fragment(d, VecPos{f.pos}, true, var, color);
+        uint32_t rc = user_fragment(d, VecPos{f.pos}, true, p % d.nprim, var, color, frag_depth);
         if (rc == 0u) return true;
         if (rc != 3u) atomicOr(d.s.error, (uint32_t)ERR_SHADER);
         return false;
     }
 #endif
+    (void)p;
+    (void)frag_depth;
     run_fs<PROG>(d, var, color, d.srgb_lut);
     return true;
 }
@@ -2223,7 +2265,8 @@ __global__ __launch_bounds__(256) void k_lp_shade(DrawParams d, const unsigned l
         unsigned long long want = d.resolve == RES_LAST ? (unsigned long long)(~key) : oi_value(d.resolve, f.z, key);
         if (vis[pix] != want) return;
         float color[4];
-        (void)lp_color<PROG, N>(d, v, f, color);
+        float unused = f.z;
+        (void)lp_color<PROG, N>(d, v, f, p, color, &unused);
         if (d.color) reinterpret_cast<uint32_t *>(d.color)[pix] = pack_color(color, d.color_bgra);
         if (d.depth_test && d.depth_write) d.depth[pix] = f.z;
     });
@@ -2300,10 +2343,13 @@ __global__ __launch_bounds__(256) void k_lp_ordered(DrawParams d)
                 uint32_t pix = f.y * d.width + f.x;
                 ++frags;
                 lp_count_fragment(d, pix);
-                if (d.depth_test && !compare(d.depth_compare, f.z, d.depth[pix])) continue;
                 float color[4];
-                if (!lp_color<PROG, N>(d, v, f, color)) continue;
-                if (d.depth_test && d.depth_write) d.depth[pix] = f.z;
+                float dep = d.depth_test ? d.depth[pix] : 0.0f;
+                bool ok = depth_and_shade(d, f.z, dep, [&](float *frag_depth) {
+                    return lp_color<PROG, N>(d, v, f, p, color, frag_depth);
+                });
+                if (d.depth_test && d.depth_write) d.depth[pix] = dep;
+                if (!ok) continue;
                 if (d.color) {
                     uint32_t *cp = reinterpret_cast<uint32_t *>(d.color) + pix;
                     *cp = d.blend ? blend_over(color, *cp, d.color_bgra) : pack_color(color, d.color_bgra);

@@ -2082,7 +2082,11 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
     d.depth_write = pd.depth_write_enabled;
     d.blend = (extended && pd.blend == WGC_BLEND_ALPHA && pc.color) ? 1u : 0u;
     d.resolve = resolve_mode(pd, d.depth_test != 0, d.blend != 0);
-    if (pl->user && pl->user->discards) d.resolve = RES_ORDERED;   // every fragment must run its shader
+    // every fragment must run its shader: a discard, an early depth test
+    // (whose write a later discard keeps) or a shader-written depth makes the
+    // outcome depend on each fragment in API order
+    if (pl->user && (pl->user->discards || pl->user->early_depth || pl->user->frag_depth_tested))
+        d.resolve = RES_ORDERED;
     // vertex buffers (vertex.rs:261-272)
     uint32_t n_vb = (uint32_t)pl->vb_stride.size();
     for (uint32_t b = 0; b < n_vb; ++b)
@@ -2160,6 +2164,8 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
         d.user = &up.kernels;
         d.var_flat_mask = up.flat_mask;
         d.var_persp_mask = up.persp_mask;
+        d.fs_early = up.early_depth;
+        d.fs_depth_tested = up.frag_depth_tested ? 1u : 0u;
         d.srgb = extended ? 1u : 0u;
         d.srgb_lut = dev->d_srgb;
         for (size_t k = 0; k < up.slots.size(); ++k) {

@@ -243,11 +243,9 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
             err = "the fragment shader writes no colour";
             return false;
         }
-        if (fe->frag_depth) {
-            err = "@builtin(frag_depth) is not supported";
-            return false;
-        }
         out.discards = fe->may_discard;
+        out.frag_depth_tested = fe->frag_depth && fe->frag_depth_tested;
+        out.early_depth = fe->early_depth;
     }
     // pipeline resource slots: the union of both stages' bindings
     auto add_slots = [&](const wgsl::Module &m, const wgsl::EntryPoint &e, std::vector<uint32_t> &map) -> bool {
@@ -336,10 +334,13 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
              "    const DrawParams &d;\n"
              "    const Pos &pos;\n"
              "    bool ff;\n"
+             "    uint32_t prim;\n"
              "    const float *var;\n"
              "    float *color;\n"
+             "    float *depth;\n"
              "    __device__ uint32_t vertex_index() const { return 0u; }\n"
              "    __device__ uint32_t instance_index() const { return 0u; }\n"
+             "    __device__ uint32_t primitive_index() const { return prim; }\n"
              "    __device__ bool front_facing() const { return ff; }\n"
              "    __device__ void position(float *p) const { pos(p); }\n"
              "    __device__ void in_location(uint32_t l, uint32_t *w, uint32_t n) const\n"
@@ -353,20 +354,24 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
              "        if (l == 0u)\n"
              "            for (uint32_t k = 0; k < n && k < 4u; ++k) color[k] = __uint_as_float(w[k]);\n"
              "    }\n"
-             "    __device__ void out_frag_depth(float) {}\n" +
+             // a frag_depth member after the colour comes after the late
+             // test and changes nothing (fragment.rs:457-488)
+             + std::string(out.frag_depth_tested ? "    __device__ void out_frag_depth(float z) { *depth = z; }\n"
+                                                 : "    __device__ void out_frag_depth(float) {}\n") +
              env_resources("fs_slot") +
              "};\n"
              "template <class Pos>\n"
              "__device__ __forceinline__ uint32_t user_fragment(const DrawParams &d, const Pos &pos, bool front_facing,\n"
-             "                                                  const float *var, float out[4])\n"
+             "                                                  uint32_t primitive_index, const float *var, float out[4],\n"
+             "                                                  float *frag_depth)\n"
              "{\n"
-             "    FsEnv<Pos> env{d, pos, front_facing, var, out};\n"
+             "    FsEnv<Pos> env{d, pos, front_facing, primitive_index, var, out, frag_depth};\n"
              "    return " + std::string(same ? "umv::" : "umf::") + fe->symbol + "(env);\n"
              "}\n";
     } else {
         g += "template <class Pos>\n"
-             "__device__ __forceinline__ uint32_t user_fragment(const DrawParams &, const Pos &, bool, const float *,\n"
-             "                                                  float out[4])\n"
+             "__device__ __forceinline__ uint32_t user_fragment(const DrawParams &, const Pos &, bool, uint32_t,\n"
+             "                                                  const float *, float out[4], float *)\n"
              "{\n    out[0] = out[1] = out[2] = 0.0f;\n    out[3] = 1.0f;\n    return 0u;\n}\n";
     }
     g += "}  // namespace wgc\n";
@@ -422,6 +427,7 @@ bool build_entry_run(int hip_device, const wgsl::Module &m, const std::string &e
                     "    uint32_t *o;\n"
                     "    __device__ uint32_t vertex_index() const { return index; }\n"
                     "    __device__ uint32_t instance_index() const { return 0u; }\n"
+                    "    __device__ uint32_t primitive_index() const { return 0u; }\n"
                     "    __device__ bool front_facing() const { return true; }\n"
                     "    __device__ void position(float *p) const { p[0] = (float)index; p[1] = 0.0f; p[2] = 0.0f; p[3] = 1.0f; }\n"
                     "    __device__ void in_location(uint32_t, uint32_t *w, uint32_t n) const { for (uint32_t k = 0; k < n; ++k) w[k] = 0u; }\n"
@@ -495,6 +501,9 @@ std::string reflection_json(const wgsl::Module &m)
              ",\"position_in\":" + (e.position_in ? "true" : "false") +
              ",\"position_out\":" + (e.position_out ? "true" : "false") +
              ",\"frag_depth\":" + (e.frag_depth ? "true" : "false") +
+             ",\"frag_depth_tested\":" + (e.frag_depth_tested ? "true" : "false") +
+             ",\"primitive_index\":" + (e.primitive_index ? "true" : "false") +
+             ",\"early_depth_test\":" + std::to_string(e.early_depth) +
              ",\"may_discard\":" + (e.may_discard ? "true" : "false") + ",\"resources\":" + res + "}";
     }
     return o + "]}";

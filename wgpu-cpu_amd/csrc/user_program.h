@@ -33,6 +33,10 @@ struct UserProgram {
     std::vector<wgsl::IoVar> vertex_inputs;
     uint32_t nvar = 0, flat_mask = 0, persp_mask = 0;
     bool discards = false;
+    // the fragment shader's frag_depth output is the depth of the late test
+    // (a member before its colour), and its @early_depth_test mode
+    bool frag_depth_tested = false;
+    uint32_t early_depth = 0;
 };
 
 // Pairs a vertex entry point with an optional fragment entry point (entry

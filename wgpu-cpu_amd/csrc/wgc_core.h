@@ -211,6 +211,9 @@ struct DrawParams {
     // flat (first vertex) or perspective-corrected (else linear)
     ResBind res[kMaxRes];
     uint32_t var_flat_mask, var_persp_mask;
+    // WGSL fragment shaders: @early_depth_test (0 none, 1 force, 2 allow) and
+    // whether a frag_depth output replaces the depth of the late test
+    uint32_t fs_early, fs_depth_tested;
     const void *user;        // host-side program identity (never read on the device)
     Scratch s;
 };
@@ -221,12 +224,15 @@ struct TriLds;
 // Defined by the pipeline's generated glue, compiled with the kernels:
 // returns 0 or the ErrorBits of a failed invocation.
 __device__ __forceinline__ uint32_t user_vertex(const DrawParams &d, uint32_t instance, uint32_t vertex, VsOut &o);
-// A fragment with interpolated varyings `var`; `pos(p)` writes its
-// @builtin(position).  Returns 0 (colour written to `out`), wgsl ABORT_KILL
-// (discard) or another abort code.
+// A fragment with interpolated varyings `var` of primitive
+// `primitive_index`; `pos(p)` writes its @builtin(position).  Returns 0
+// (colour written to `out`, and to *frag_depth the depth the late test uses
+// when the shader writes one before its colour), wgsl ABORT_KILL (discard)
+// or another abort code.
 template <class Pos>
 __device__ __forceinline__ uint32_t user_fragment(const DrawParams &d, const Pos &pos, bool front_facing,
-                                                  const float *var, float out[4]);
+                                                  uint32_t primitive_index, const float *var, float out[4],
+                                                  float *frag_depth);
 #endif
 
 // ------------------------------------------------------------------

@@ -920,7 +920,7 @@ struct TypeInfo {
 enum : int { TY_VOID = 0, TY_BOOL, TY_I32, TY_U32, TY_F32, TY_AINT, TY_AFLOAT };
 
 enum Builtin { BI_NONE, BI_POSITION, BI_VERTEX_INDEX, BI_INSTANCE_INDEX, BI_FRONT_FACING, BI_FRAG_DEPTH,
-               BI_OTHER };
+               BI_PRIMITIVE_INDEX, BI_SAMPLE_INDEX, BI_SAMPLE_MASK, BI_OTHER };
 
 // Stage-interface attributes of a parameter, return value or member.
 struct IoAttrs {
@@ -1001,6 +1001,7 @@ struct FnInfo {
     std::set<int> calls;           // function indices
     std::set<int> slots;           // resource slots used directly
     bool discards = false;
+    uint32_t early_depth = 0;      // fragment entry points: @early_depth_test (EntryPoint::early_depth)
     std::string code;
     int state = 0;                 // call-graph walk: 0 new, 1 visiting, 2 done
 };
@@ -1397,7 +1398,10 @@ IoAttrs Translator::io_attrs(const std::vector<Attr> &attrs)
                        : b == "vertex_index" ? BI_VERTEX_INDEX
                        : b == "instance_index" ? BI_INSTANCE_INDEX
                        : b == "front_facing" ? BI_FRONT_FACING
-                       : b == "frag_depth" ? BI_FRAG_DEPTH : BI_OTHER;
+                       : b == "frag_depth" ? BI_FRAG_DEPTH
+                       : b == "primitive_index" ? BI_PRIMITIVE_INDEX
+                       : b == "sample_index" ? BI_SAMPLE_INDEX
+                       : b == "sample_mask" ? BI_SAMPLE_MASK : BI_OTHER;
         } else if (a.name == "interpolate") {
             std::string m = attr_ident(a);
             io.has_interp = true;
@@ -1600,7 +1604,16 @@ void Translator::declare_functions()
                 f.stage = Stage::Fragment;
             } else if (a.name == "compute") {
                 fail(a.loc, "compute entry points are not part of a render pipeline");
-            } else if (a.name == "early_depth_test" || a.name == "must_use" || a.name == "diagnostic") {
+            } else if (a.name == "early_depth_test") {
+                // naga's EarlyDepthTest: `force` tests (and writes) depth
+                // before the shader only; a conservative-depth argument
+                // allows the early test and keeps the late one
+                // (fragment.rs:166-194)
+                std::string m = attr_ident(a);
+                if (m == "force") f.early_depth = 1;
+                else if (m == "greater_equal" || m == "less_equal" || m == "unchanged") f.early_depth = 2;
+                else fail(a.loc, "unknown @early_depth_test argument '" + m + "'");
+            } else if (a.name == "must_use" || a.name == "diagnostic") {
             } else {
                 fail(a.loc, "unknown function attribute @" + a.name);
             }
@@ -3116,6 +3129,15 @@ void Translator::entry_wrapper(FnInfo &f, EntryPoint &ep, std::string &out)
             } else if (!vs && io.builtin == BI_POSITION && ty == vec_of(TY_F32, 4)) {
                 ep.position_in = true;
                 body += "    env.position(" + dst + ".v);\n";
+            } else if (!vs && io.builtin == BI_PRIMITIVE_INDEX && ty == TY_U32) {
+                // the primitive's index in its instance's assembly order,
+                // culled primitives counted (state.rs:536-541)
+                ep.primitive_index = true;
+                body += "    " + dst + " = env.primitive_index();\n";
+            } else if (!vs && io.builtin == BI_SAMPLE_INDEX && ty == TY_U32) {
+                body += "    " + dst + " = 0u;\n";            // fragment.rs:138-140: single sample
+            } else if (!vs && io.builtin == BI_SAMPLE_MASK && ty == TY_U32) {
+                body += "    " + dst + " = 0xFFFFFFFFu;\n";   // fragment.rs:141: !0
             } else {
                 fail(l, "unsupported input @builtin(" + io.builtin_name + ") of type " + tname(ty) + " in this stage");
             }
@@ -3160,7 +3182,13 @@ void Translator::entry_wrapper(FnInfo &f, EntryPoint &ep, std::string &out)
                 outs += "    env.out_position(" + src + ".v);\n";
             } else if (!vs && io.builtin == BI_FRAG_DEPTH && ty == TY_F32) {
                 ep.frag_depth = true;
+                // outputs are read in member order (runtime.rs:1500-1508):
+                // a depth written before the first location is the one the
+                // late test sees
+                if (ep.outputs.empty()) ep.frag_depth_tested = true;
                 outs += "    env.out_frag_depth(" + src + ");\n";
+            } else if (!vs && io.builtin == BI_SAMPLE_MASK && ty == TY_U32) {
+                outs += "    (void)" + src + ";\n";   // stored and never used (fragment.rs:464-466)
             } else {
                 fail(l, "unsupported output @builtin(" + io.builtin_name + ") of type " + tname(ty) + " in this stage");
             }
@@ -3213,7 +3241,10 @@ void Translator::finish_reachability(EntryPoint &ep, int fi)
     walk(fi);
     ep.resources.assign(slots.begin(), slots.end());
     ep.may_discard = discard;
+    ep.early_depth = fns_[fi].early_depth;
     if (discard && ep.stage != Stage::Fragment) fail(fns_[fi].decl->loc, "discard is only valid in fragment shaders");
+    if (ep.early_depth && ep.stage != Stage::Fragment)
+        fail(fns_[fi].decl->loc, "@early_depth_test is only valid on fragment entry points");
 }
 
 void Translator::run(Module &out)

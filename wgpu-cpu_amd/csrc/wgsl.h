@@ -47,6 +47,15 @@ struct EntryPoint {
     bool front_facing = false, position_in = false;
     bool position_out = false;          // vertex: writes @builtin(position)
     bool frag_depth = false;            // fragment: writes @builtin(frag_depth)
+    // fragment: the frag_depth output comes before the first @location
+    // output in the result's member order, so the late depth test (run when
+    // the first location is written, fragment.rs:465-478) uses it
+    bool frag_depth_tested = false;
+    bool primitive_index = false;       // fragment: reads @builtin(primitive_index)
+    // fragment: @early_depth_test -- 0 none (late test only), 1 force (early
+    // test only), 2 allow with a conservative depth (early and late test),
+    // fragment.rs:166-194
+    uint32_t early_depth = 0;
     bool may_discard = false;           // a discard is reachable
     std::vector<uint32_t> resources;    // slots reachable from the entry point
 };

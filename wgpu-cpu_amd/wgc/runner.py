@@ -12,7 +12,7 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from . import DEPTH32_FLOAT, RGBA8_UNORM, RGBA8_UNORM_SRGB, STEP_VERTEX, Device
+from . import DEPTH32_FLOAT, RGBA8_UNORM, RGBA8_UNORM_SRGB, STEP_INSTANCE, STEP_VERTEX, Device
 from .scenes import PassSpec
 
 
@@ -65,8 +65,11 @@ def record_pass(device: Device, enc, spec: PassSpec, res: _PassResources, keep: 
             module = cached(("module", d.program, d.interp),
                             lambda: device.create_shader_module_builtin(d.program, d.interp))
         layouts = [(d.vertex_stride, STEP_VERTEX, d.attributes)] if d.vertex_data else []
+        if d.instance_buffer is not None:
+            layouts.append((d.instance_buffer[1], STEP_INSTANCE, d.instance_buffer[2]))
         pkey = ("pipeline", d.program, d.interp, d.wgsl, d.vertex_stride, tuple(d.attributes), d.front_face,
-                d.cull_mode, d.depth, spec.color_format, d.blend, d.topology, d.strip_index_format)
+                d.cull_mode, d.depth, spec.color_format, d.blend, d.topology, d.strip_index_format,
+                None if d.instance_buffer is None else (d.instance_buffer[1], tuple(d.instance_buffer[2])))
         pipeline = cached(pkey, lambda: device.create_render_pipeline(
             module, layouts, front_face=d.front_face, cull_mode=d.cull_mode, depth=d.depth,
             target_format=spec.color_format, blend=d.blend, topology=d.topology,
@@ -91,6 +94,10 @@ def record_pass(device: Device, enc, spec: PassSpec, res: _PassResources, keep: 
         if d.vertex_data:
             vb = cached(("buf", d.vertex_data), lambda: device.create_buffer_init(d.vertex_data))
             rp.set_vertex_buffer(0, vb)
+        if d.instance_buffer is not None:
+            ib_data = d.instance_buffer[0]
+            instb = cached(("buf", ib_data), lambda: device.create_buffer_init(ib_data))
+            rp.set_vertex_buffer(len(layouts) - 1, instb)
         if d.viewport is not None:
             rp.set_viewport(*d.viewport)
         if d.scissor is not None:

@@ -46,16 +46,64 @@ def wgsl_source(program: int, interp: int = INTERP_LINEAR, color_components: int
     return src
 
 
+FS_DISCARD, FS_DEPTH_ALPHA, FS_DEPTH_CONST, FS_DEPTH_FIRST, FS_PRIM_RED = 1, 2, 4, 8, 16
+EARLY_NONE, EARLY_FORCE, EARLY_ALLOW = 0, 1, 2
+
+
+def fragment_test_wgsl(flags: int, early: int, interp: int = INTERP_LINEAR) -> str:
+    """WGSL of a fragment test program: clip-space position and a vec4
+    colour c passed through; the fragment stage discards when c.w < 0.5
+    (FS_DISCARD), writes @builtin(frag_depth) = c.w or 0.3125
+    (FS_DEPTH_ALPHA / FS_DEPTH_CONST) as a member before (FS_DEPTH_FIRST) or
+    after the colour, puts f32(primitive_index % 8u) * 0.125 in red
+    (FS_PRIM_RED), and carries @early_depth_test(force) or
+    @early_depth_test(less_equal) (a conservative depth: the early test is
+    allowed and the late one kept)."""
+    mode = {INTERP_LINEAR: "linear", INTERP_PERSPECTIVE: "perspective"}.get(interp, "flat")
+    depth = flags & (FS_DEPTH_ALPHA | FS_DEPTH_CONST)
+    members = ["@location(0) color: vec4f"]
+    if depth:
+        d = "@builtin(frag_depth) depth: f32"
+        members = [d] + members if flags & FS_DEPTH_FIRST else members + [d]
+    attr = {EARLY_NONE: "", EARLY_FORCE: "@early_depth_test(force)\n",
+            EARLY_ALLOW: "@early_depth_test(less_equal)\n"}[early]
+    params = "f: VsOut" + (", @builtin(primitive_index) pi: u32" if flags & FS_PRIM_RED else "")
+    body = []
+    if flags & FS_DISCARD:
+        body.append("    if (f.color.w < 0.5) {\n        discard;\n    }")
+    red = "f32(pi % 8u) * 0.125" if flags & FS_PRIM_RED else "f.color.x"
+    body.append(f"    let c = vec4f({red}, f.color.y, f.color.z, f.color.w);")
+    if depth:
+        dv = "f.color.w" if flags & FS_DEPTH_ALPHA else "0.3125"
+        args = ["c"]
+        args = ([dv] + args) if flags & FS_DEPTH_FIRST else (args + [dv])
+        body.append(f"    return FsOut({', '.join(args)});")
+        ret = "FsOut"
+    else:
+        body.append("    return c;")
+        ret = "@location(0) vec4f"
+    fs_struct = ("struct FsOut {\n    " + ",\n    ".join(members) + ",\n}\n\n") if depth else ""
+    return ("struct VsIn {\n    @location(0) position: vec4f,\n    @location(1) color: vec4f,\n}\n\n"
+            "struct VsOut {\n    @builtin(position) position: vec4f,\n"
+            f"    @location(0) @interpolate({mode}) color: vec4f,\n}}\n\n" + fs_struct +
+            "@vertex\nfn vs_main(v: VsIn) -> VsOut {\n    return VsOut(v.position, v.color);\n}\n\n"
+            f"@fragment\n{attr}fn fs_main({params}) -> {ret} {{\n" + "\n".join(body) + "\n}\n")
+
+
 def as_wgsl(spec: "PassSpec") -> "PassSpec":
     """A copy of `spec` whose draws run the WGSL modules of their programs."""
     import copy
     out = copy.deepcopy(spec)
     for d in out.draws:
         ncol = 0
-        for fmt, _off, loc in d.attributes:
+        attrs = list(d.attributes) + (list(d.instance_buffer[2]) if d.instance_buffer else [])
+        for fmt, _off, loc in attrs:
             if loc == 1:
                 ncol = {FLOAT32X3: 3, FLOAT32X4: 4}.get(fmt, 0)
-        d.wgsl = wgsl_source(d.program, d.interp, ncol)
+        if d.fs_flags or d.fs_early:
+            d.wgsl = fragment_test_wgsl(d.fs_flags, d.fs_early, d.interp)
+        else:
+            d.wgsl = wgsl_source(d.program, d.interp, ncol)
     out.name = (spec.name or "scene") + " (wgsl)"
     return out
 
@@ -92,6 +140,12 @@ class DrawSpec:
     wgsl: Optional[str] = None    # WGSL source replacing the built-in program (same interface)
     topology: int = 3             # wgpu::PrimitiveTopology: 0 points, 1 lines, 2 line strip, 3 triangles, 4 strip
     strip_index_format: int = 0   # 0: none; 2 / 4: the index MAX value restarts strips
+    # a second vertex buffer stepped per instance: (bytes, stride, [(format, offset, location)])
+    instance_buffer: Optional[Tuple[bytes, int, List[Tuple[int, int, int]]]] = None
+    # fragment test programs (with PROGRAM_CLIP_COLOR's vertex stage): FS_* bits and the
+    # @early_depth_test mode (0 none, 1 force, 2 allow); WGSL text from fragment_test_wgsl
+    fs_flags: int = 0
+    fs_early: int = 0
 
     @property
     def indexed(self) -> bool:
@@ -319,6 +373,98 @@ def bilinear_strip(width: int = 32, height: int = 4) -> PassSpec:
                  filter=FILTER_LINEAR)
     return PassSpec(width, height, RGBA8_UNORM, clear_color=(0, 0, 0, 0), draws=[d],
                     semantics=SEMANTICS_EXTENDED, name="bilinear_strip")
+
+
+def addressing_quad(address_u: int, address_v: int, extended: bool = False, bilinear: bool = False,
+                    width: int = 96, height: int = 64, seed: int = SEED_BASE + 21) -> PassSpec:
+    """Screen-filling quad (two triangles, identity camera) whose uv runs
+    from -1.3 to 2.7 across and -0.8 to 1.9 down, over a seeded 8x8 RGBA
+    texture: every texel coordinate outside [0, 1) goes through the
+    sampler's address mode (binding.rs:150-164) on both axes."""
+    rnd = splitmix_u64(seed, 8 * 8 * 4)
+    tex = (rnd & np.uint64(0xFF)).astype(np.uint8).reshape(8, 8, 4)
+    tex[..., 3] = 255
+    u0, u1, v0, v1 = -1.3, 2.7, -0.8, 1.9
+    verts = np.array([[-1, -1, 0.5, 1], [1, -1, 0.5, 1], [1, 1, 0.5, 1],
+                      [-1, -1, 0.5, 1], [1, 1, 0.5, 1], [-1, 1, 0.5, 1]], np.float32)
+    uv = np.array([[u0, v1], [u1, v1], [u1, v0], [u0, v1], [u1, v0], [u0, v0]], np.float32)
+    vb = np.concatenate([verts, uv], axis=1)
+    ident = np.eye(4, dtype=np.float32).T.tobytes()
+    d = DrawSpec(program=PROGRAM_MVP_TEXTURE, interp=INTERP_PERSPECTIVE if extended else INTERP_LINEAR,
+                 vertex_data=vb.tobytes(), vertex_stride=24, attributes=[(FLOAT32X4, 0, 0), (FLOAT32X2, 16, 1)],
+                 count=6, uniform=ident, texture=(8, 8, tex.tobytes(), False), address_u=address_u,
+                 address_v=address_v, filter=FILTER_LINEAR if bilinear else FILTER_NEAREST)
+    return PassSpec(width, height, RGBA8_UNORM, clear_color=(0, 0, 0, 0), draws=[d],
+                    semantics=SEMANTICS_EXTENDED if extended else SEMANTICS_REFERENCE,
+                    name=f"addressing u={address_u} v={address_v} ext={extended} bilinear={bilinear}")
+
+
+def instanced_triangles(n: int = 300, instances: int = 4, first_instance: int = 0, width: int = 160,
+                        height: int = 120, seed: int = SEED_BASE + 22, depth: Optional[Tuple[int, bool]] = None,
+                        blend: bool = False) -> PassSpec:
+    """n clip-space triangles (positions stepped per vertex, constant z)
+    drawn `instances` times; the colour (location 1) comes from a second
+    buffer stepped per instance (vertex.rs:191-195), so every instance
+    covers the same pixels with its own colour: with Less the first
+    instance wins, with LessEqual the last, with src-over all of them in
+    order."""
+    u = splitmix_f32(seed, n * 6).astype(np.float64).reshape(n, 6)
+    cx, cy = u[:, 0] * 1.8 - 0.9, u[:, 1] * 1.8 - 0.9
+    size = 0.05 + 0.25 * u[:, 2]
+    ang = u[:, 3:4] * 2 * np.pi + np.array([0.0, 2.1, 4.2])[None, :]
+    vb = np.zeros((n, 3, 4), np.float32)
+    vb[:, :, 0] = cx[:, None] + size[:, None] * np.cos(ang)
+    vb[:, :, 1] = cy[:, None] + size[:, None] * np.sin(ang)
+    vb[:, :, 2] = 0.25 + 0.5 * u[:, 4:5]
+    vb[:, :, 3] = 1.0
+    total = first_instance + instances
+    cols = splitmix_f32(seed ^ 0x77, total * 4).reshape(total, 4)
+    cols[:, 3] = 0.2 + 0.6 * cols[:, 3]
+    d = DrawSpec(program=PROGRAM_CLIP_COLOR, vertex_data=vb.tobytes(), vertex_stride=16,
+                 attributes=[(FLOAT32X4, 0, 0)], count=3 * n, first_instance=first_instance,
+                 instance_count=instances, instance_buffer=(cols.tobytes(), 16, [(FLOAT32X4, 0, 1)]),
+                 depth=depth, blend=BLEND_ALPHA if blend else BLEND_REPLACE)
+    return PassSpec(width, height, RGBA8_UNORM, clear_color=(0.1, 0.2, 0.3, 1.0), has_depth=depth is not None,
+                    clear_depth=1.0, draws=[d], semantics=SEMANTICS_EXTENDED if blend else SEMANTICS_REFERENCE,
+                    name=f"instanced first={first_instance} n={instances} depth={depth} blend={blend}")
+
+
+def fragment_test_scene(flags: int, early: int, compare: Optional[int] = LESS, write: bool = True, n: int = 600,
+                        width: int = 160, height: int = 120, seed: int = SEED_BASE + 23, topology: int = 3,
+                        z_levels=(0.25, 0.5, 0.75), alpha=(0.1, 0.9), blend: bool = False,
+                        cull_mode: int = CULL_NONE, instances: int = 1) -> PassSpec:
+    """n overlapping clip-space triangles (or as many points / lines, by
+    `topology`) with depths from `z_levels` (ties on purpose) and colour
+    alpha uniform in `alpha`, drawn with the fragment test program of
+    `flags` / `early` (fragment_test_wgsl) over a random initial depth
+    buffer (LoadOp::Load), so that every compare function takes both
+    branches.  `compare` None: no depth attachment."""
+    u = splitmix_f32(seed, n * 8).astype(np.float64).reshape(n, 8)
+    cx, cy = u[:, 0] * 2.2 - 1.1, u[:, 1] * 2.2 - 1.1
+    size = 0.04 + 0.3 * u[:, 2]
+    nv = {0: 1, 1: 2, 2: 2, 3: 3, 4: 3}[topology]
+    ang = u[:, 3:4] * 2 * np.pi + np.array([0.0, 2.1, 4.2])[None, :nv]
+    zq = np.asarray(z_levels)[(u[:, 4] * len(z_levels)).astype(int) % len(z_levels)]
+    vb = np.zeros((n, nv, 8), np.float32)
+    vb[:, :, 0] = cx[:, None] + size[:, None] * np.cos(ang)
+    vb[:, :, 1] = cy[:, None] + size[:, None] * np.sin(ang)
+    vb[:, :, 2] = zq[:, None]
+    vb[:, :, 3] = 1.0
+    col = splitmix_f32(seed ^ 0x5A5A, n * 4).reshape(n, 1, 4)
+    vb[:, :, 4:8] = col
+    vb[:, :, 7] = alpha[0] + col[:, :, 3] * (alpha[1] - alpha[0])
+    d = DrawSpec(program=PROGRAM_CLIP_COLOR, vertex_data=vb.tobytes(), vertex_stride=32,
+                 attributes=[(FLOAT32X4, 0, 0), (FLOAT32X4, 16, 1)], count=n * nv, topology=topology,
+                 depth=None if compare is None else (compare, write),
+                 blend=BLEND_ALPHA if blend else BLEND_REPLACE, fs_flags=flags, fs_early=early,
+                 cull_mode=cull_mode, instance_count=instances)
+    d.wgsl = fragment_test_wgsl(flags, early)
+    init = np.where(splitmix_f32(seed ^ 0x1234, width * height) < 0.5, 0.5, 0.625).astype(np.float32)
+    return PassSpec(width, height, RGBA8_UNORM, clear_color=(0.1, 0.2, 0.3, 1.0),
+                    has_depth=compare is not None, clear_depth=None, depth_init=init.tobytes(), draws=[d],
+                    semantics=SEMANTICS_EXTENDED if blend else SEMANTICS_REFERENCE,
+                    name=f"fragment test flags={flags} early={early} compare={compare} write={write} "
+                         f"topology={topology} blend={blend}")
 
 
 def config3_overdraw(seed: int = SEED_BASE + 3, blend: bool = False, n: int = 200_000,
