@@ -146,6 +146,8 @@ def main():
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration for profiles)")
     ap.add_argument("--profile-batch", type=int, default=0, metavar="N",
                     help="only run N submissions of the batched workload (for counter profiles)")
+    ap.add_argument("--raster-tile", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="raster tile shape of --profile-batch (wgc.RASTER_TILES)")
     args = ap.parse_args()
 
     if args.calibrate_copy:
@@ -160,7 +162,7 @@ def main():
         return
 
     if args.profile_batch:
-        profile_batch(args.profile_batch)
+        profile_batch(args.profile_batch, args.raster_tile)
         return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -399,13 +401,14 @@ def batched_algorithmic_bytes(specs) -> int:
     return sum(scenes.algorithmic_bytes(s, count_texture=False) for s in specs) + 4 * 1024 * 1024
 
 
-def profile_batch(n: int):
-    """The batched workload alone: n submissions with the default raster tile
-    (16x16 px, 256 lanes), for kernel-trace and HBM-counter runs."""
+def profile_batch(n: int, raster_tile: int = 0):
+    """The batched workload alone: n submissions with the given raster tile
+    shape (wgc.RASTER_TILES), for kernel-trace and counter runs."""
     import wgc
     from wgc import runner
     specs = batched_specs()
     dev = wgc.Device(0, wgc.SEMANTICS_REFERENCE)
+    dev.set_option(wgc.OPTION_RASTER_TILE, raster_tile)
     pp = runner.PreparedPasses(dev, specs)
     for _ in range(2):
         pp.submit()
@@ -416,7 +419,8 @@ def profile_batch(n: int):
     dev.timestamp(1)
     dev.poll()
     ms = dev.elapsed_ms(0, 1) / n
-    print(json.dumps({"batched_submissions": n, "ms_per_submission": round(ms, 4),
+    print(json.dumps({"batched_submissions": n, "raster_tile": wgc.RASTER_TILES[raster_tile],
+                      "ms_per_submission": round(ms, 4),
                       "fragments_per_submission": pp.total_fragments(),
                       "algorithmic_bytes": batched_algorithmic_bytes(specs)}), flush=True)
 

@@ -133,8 +133,8 @@ static hipError_t launch_group(const DrawParams &d, const BatchArgs &b, const Gr
             }
         }
         WGC_LAUNCH("k_setup", (k_setup<PROG, MULTI>), g, 256, d, b);
-        // one thread per clipped primitive at most; the count is only known on the device
-        uint32_t gc = min(max(g / 4u, 1u), 1024u);
+        // kClipPerWave primitives per wave; the count is only known on the device
+        uint32_t gc = min(max(g / 2u, 1u), 2048u);
         WGC_LAUNCH("k_clip", (k_clip<PROG, MULTI>), gc, 256, d, b, gl.clip_list, gl.clip_cap);
     }
     return launch_raster<PROG, MULTI>(d, b, gl, st, kt);
@@ -199,7 +199,7 @@ static hipError_t launch_user_t(const UserKernels &k, const DrawParams &d0, cons
     if (gl.total_prims > 0) {
         uint32_t g = (gl.total_prims + 255u) / 256u;
         WGC_MLAUNCH("k_setup", k.setup[m], g, 256, &d, &b);
-        uint32_t gc = min(max(g / 4u, 1u), 1024u);
+        uint32_t gc = min(max(g / 2u, 1u), 2048u);
         WGC_MLAUNCH("k_clip", k.clip[m], gc, 256, &d, &b, &clip_list, &clip_cap);
     }
     if (gl.ordered) WGC_MLAUNCH("k_raster", k.raster[m], gl.raster_grid, 256, &d, &b);

@@ -179,29 +179,6 @@ __device__ __forceinline__ void activate_item(const DrawParams &d, uint32_t at, 
     else atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
 }
 
-// Bins one record from one lane (no wave-wide operations: the clipper's
-// sub-triangles are emitted under divergent control flow).  The adds of a
-// row of up to kMaxBinTiles footprint tiles are issued before any result
-// is used, so the lane waits once per footprint.
-__device__ __forceinline__ void bin_lane(const DrawParams &d, uint32_t key, uint32_t tx0, uint32_t ty0, uint32_t tx1,
-                                         uint32_t ty1)
-{
-    uint32_t w = tx1 - tx0 + 1u, ntl = w * (ty1 - ty0 + 1u);
-    uint32_t got[kMaxBinTiles];
-#pragma unroll
-    for (int k = 0; k < kMaxBinTiles; ++k)
-        got[k] = (uint32_t)k < ntl ? atomicAdd(&d.s.tile_count[footprint_tile(d, tx0, ty0, w, (uint32_t)k)], 1u) : 0u;
-#pragma unroll
-    for (int k = 0; k < kMaxBinTiles; ++k) {
-        if ((uint32_t)k >= ntl) break;
-        uint32_t t = footprint_tile(d, tx0, ty0, w, (uint32_t)k), slot = got[k];
-        if (slot < d.s.bin_cap) d.s.bins[(size_t)t * d.s.bin_cap + slot] = key;
-        else spill_entry(d, atomicAdd(&d.s.header[H_SPILL], 1u), t, key);
-        uint32_t item;
-        if (item_start(d, slot, item)) activate_item(d, atomicAdd(&d.s.header[H_ACTIVE], 1u), t, item);
-    }
-}
-
 // Bins every lane's record (with `has`) into the tiles of its footprint
 // (row-major, at most kMaxBinTiles).  For the k-th tile of every lane's
 // footprint, lanes on the same tile form a group first (up to 8 groups per
@@ -214,6 +191,10 @@ __device__ __forceinline__ void bin_lane(const DrawParams &d, uint32_t key, uint
 // positions start (position 0 of a tile, and every item's first position
 // of a tile with more entries than one item holds) to the active list (one
 // atomic per list per wave).  Every lane of the wave must call it.
+// ANY_DRAW: lanes may bin into different draws of a batched group (the
+// clipper's waves), so lanes group by the tile counter's address rather
+// than by the draw's tile index.
+template <bool ANY_DRAW = false>
 __device__ __forceinline__ void bin_wave(const DrawParams &d, bool has, uint32_t key, uint32_t tx0, uint32_t ty0,
                                          uint32_t tx1, uint32_t ty1)
 {
@@ -235,11 +216,14 @@ __device__ __forceinline__ void bin_wave(const DrawParams &d, bool has, uint32_t
         if (!pending) break;
         nk = (uint32_t)k + 1u;
         uint32_t t = act ? footprint_tile(d, tx0, ty0, w, (uint32_t)k) : 0xFFFFFFFFu;
+        // the global tile identifies the counter in a group (the draws'
+        // tile ranges do not overlap)
+        uint32_t gt = ANY_DRAW ? (act ? d.s.tile0 + t : 0xFFFFFFFFu) : t;
         uint32_t lead = lane, cnt = 1u, rank = 0u;
         for (int it = 0; it < 8 && pending; ++it) {
             int leader = __ffsll((unsigned long long)pending) - 1;
-            uint32_t tl = lane_value(t, leader);
-            bool mine = act && t == tl;
+            uint32_t tl = lane_value(gt, leader);
+            bool mine = act && gt == tl;
             uint64_t m = __ballot(mine);
             if (mine) {
                 lead = (uint32_t)leader;
@@ -362,8 +346,8 @@ __device__ __forceinline__ void note_big_wave(const DrawParams &d, bool big, con
     }
 }
 
-template <int PROG>
-__device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act);
+template <int PROG, class DrawOf>
+__device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act, DrawOf &&draw_of);
 
 // The LoadOp::Clear values into the whole target, spread over the setup
 // grid (16-byte stores; issued before the setup work, they drain while it
@@ -475,7 +459,8 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
     note_big_wave(d, big && slot < d.s.big_cap, rec);
     if constexpr (CLIP) {
         // every lane of the wave takes part (the clip slots are reserved per wave)
-        if (__ballot(needs_clip)) clip_primitive<PROG>(d, di, p, needs_clip);
+        if (__ballot(needs_clip))
+            clip_primitive<PROG>(d, di, p, needs_clip, [&](uint32_t) -> const DrawParams & { return d; });
     }
 }
 
@@ -545,11 +530,17 @@ __device__ int clip_walk(const CTri &root, F &&emit)
 
 // One primitive that crosses a clip plane: the clipper's sub-triangles, in
 // its order, become clip records (positions and clip barycentrics) and
-// triangle records, binned like the primitives of k_setup.  Every lane of
-// the wave calls it (with `act` false for lanes without a primitive), so
-// that the clip slots are taken with one atomic per wave.
-template <int PROG>
-__device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act)
+// triangle records; each leaves a bin job (key and tile footprint) at its
+// clip slot.  Then the wave bins all its lanes' sub-triangles together, 64
+// per round, like k_setup's primitives: one atomic per distinct tile and
+// group of lanes, every add of a round issued before any result is used
+// (rather than two dependent round trips per sub-triangle and lane).  Every
+// lane of the wave calls it (with `act` false for lanes without a
+// primitive), so that the clip slots are taken with one atomic per draw and
+// wave.  `draw_of(di)` is draw di of the group: the lanes of a wave may clip
+// primitives of different draws in the batched form.
+template <int PROG, class DrawOf>
+__device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act, DrawOf &&draw_of)
 {
     VsOut v[3];
     CTri root;
@@ -576,48 +567,86 @@ __device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, boo
         }
     }
     uint32_t base = wave_add(&d.s.counters[C_CLIP_TRI], nleaf, act);
-    if (nleaf == 0u) return;
-    if (base + nleaf > d.s.clip_tri_cap) {
-        atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
-        return;
+    bool fits = nleaf != 0u && base + nleaf <= d.s.clip_tri_cap;
+    if (nleaf != 0u && !fits) atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
+    uint32_t mine = fits ? nleaf : 0u;
+    uint64_t owners = __ballot(mine != 0u);
+    if (!owners) return;
+    if (fits) {
+        d.s.clip_base[p] = base;
+        clip_walk(root, [&](const CTri &t, int leaf) {
+            uint32_t slot = base + (uint32_t)leaf;
+            float *o = d.s.clip_tris + (size_t)slot * kClipFloats;
+            for (int k = 0; k < 3; ++k) {
+                for (int c = 0; c < 4; ++c) o[k * 7 + c] = t.p[k][c];
+                for (int c = 0; c < 3; ++c) o[k * 7 + 4 + c] = t.b[k][c];
+            }
+            uint4 job = make_uint4(0u, 0u, 0u, 0u);
+            RasterTri r;
+            raster_setup(d, t.p, r);
+            ScanTri s = scan_setup(r);
+            uint32_t tx0, ty0, tx1, ty1;
+            if (!coord_ok(r)) {
+                atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE);
+            } else if (r.total != 0.0f && tile_bbox(d, s, tx0, ty0, tx1, ty1)) {
+                uint32_t key = (p << 7) | ((uint32_t)leaf << 1) | 1u;
+                write_record(d.s.clip_tri + (size_t)slot * kRecWords, r, v, key, ff);
+                job = make_uint4(key, tx0 | (ty0 << 16), tx1 | (ty1 << 16), 1u);
+            }
+            d.s.clip_job[slot] = job;
+        });
     }
-    d.s.clip_base[p] = base;
-    clip_walk(root, [&](const CTri &t, int leaf) {
-        uint32_t slot = base + (uint32_t)leaf;
-        float *o = d.s.clip_tris + (size_t)slot * kClipFloats;
-        for (int k = 0; k < 3; ++k) {
-            for (int c = 0; c < 4; ++c) o[k * 7 + c] = t.p[k][c];
-            for (int c = 0; c < 3; ++c) o[k * 7 + 4 + c] = t.b[k][c];
+    // the jobs are read back by other lanes of the wave
+    __threadfence_block();
+    // the wave's sub-triangles in lane order: lane l's are [pre_l, pre_l + mine_l)
+    uint32_t incl = wave_inclusive_sum(mine);
+    uint32_t pre = incl - mine;
+    uint32_t total = lane_value(incl, 63);
+    uint32_t lane = lane_id();
+    for (uint32_t j0 = 0; j0 < total; j0 += 64u) {
+        uint32_t f = j0 + lane;
+        bool has = f < total;
+        // the owner: the last lane with sub-triangles whose range starts at
+        // or before f (wave-uniform loop over those lanes)
+        int owner = 0;
+        for (uint64_t m = owners; m; m &= m - 1u) {
+            int l = __ffsll((unsigned long long)m) - 1;
+            if (lane_value(pre, l) <= f) owner = l;
         }
-        RasterTri r;
-        raster_setup(d, t.p, r);
-        ScanTri s = scan_setup(r);
-        uint32_t tx0, ty0, tx1, ty1;
-        if (!coord_ok(r)) {
-            atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE);
-            return;
-        }
-        if (r.total == 0.0f || !tile_bbox(d, s, tx0, ty0, tx1, ty1)) return;
-        uint32_t key = (p << 7) | ((uint32_t)leaf << 1) | 1u;
-        write_record(d.s.clip_tri + (size_t)slot * kRecWords, r, v, key, ff);
-        uint32_t nt = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-        if (nt <= (uint32_t)kMaxBinTiles) {
-            bin_lane(d, key, tx0, ty0, tx1, ty1);
-        } else {
-            uint32_t bs = atomicAdd(&d.s.counters[C_BIG], 1u);
-            if (bs < d.s.big_cap) {
-                d.s.big[bs] = make_uint4(key, tx0 | (ty0 << 16), tx1 | (ty1 << 16), di);
-                atomicOr(&d.s.header[H_BIG], 1u);
-                atomicMax(&d.s.counters[C_BIG_NX0], 65535u - tx0);
-                atomicMax(&d.s.counters[C_BIG_NY0], 65535u - ty0);
-                atomicMax(&d.s.counters[C_BIG_X1], tx1);
-                atomicMax(&d.s.counters[C_BIG_Y1], ty1);
-            } else {
-                atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
+        uint32_t odi = (uint32_t)__shfl((int)di, owner);
+        uint32_t obase = (uint32_t)__shfl((int)base, owner);
+        uint32_t opre = (uint32_t)__shfl((int)pre, owner);
+        const DrawParams &od = draw_of(odi);
+        uint4 job = make_uint4(0u, 0u, 0u, 0u);
+        if (has) job = od.s.clip_job[obase + (f - opre)];
+        has = has && job.w != 0u;
+        uint32_t tx0 = job.y & 0xFFFFu, ty0 = job.y >> 16, tx1 = job.z & 0xFFFFu, ty1 = job.z >> 16;
+        uint32_t nt = has ? (tx1 - tx0 + 1u) * (ty1 - ty0 + 1u) : 0u;
+        bool small = has && nt <= (uint32_t)kMaxBinTiles;
+        bool big = has && !small;
+        bin_wave<true>(od, small, job.x, tx0, ty0, tx1, ty1);
+        if (__ballot(big)) {
+            uint32_t bs = wave_add(&od.s.counters[C_BIG], 1u, big);
+            if (big) {
+                if (bs < od.s.big_cap) {
+                    od.s.big[bs] = make_uint4(job.x, job.y, job.z, odi);
+                    atomicOr(&od.s.header[H_BIG], 1u);
+                    atomicMax(&od.s.counters[C_BIG_NX0], 65535u - tx0);
+                    atomicMax(&od.s.counters[C_BIG_NY0], 65535u - ty0);
+                    atomicMax(&od.s.counters[C_BIG_X1], tx1);
+                    atomicMax(&od.s.counters[C_BIG_Y1], ty1);
+                } else {
+                    atomicOr(od.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
+                }
             }
         }
-    });
+    }
 }
+
+// Primitives of the clip list per wave: the clipper's walk is a long serial
+// chain per lane, so few lanes per wave spread the list over many waves
+// (each wave's walk takes as long as its longest lane's).
+constexpr uint32_t kClipPerWave = 8;
 
 // Every lane runs the same number of iterations, so that the wave-wide slot
 // reservations in clip_primitive see the whole wave.
@@ -625,13 +654,14 @@ template <int PROG, bool MULTI>
 __global__ __launch_bounds__(256) void k_clip(DrawParams d0, BatchArgs b, const uint2 *clip_list, uint32_t cap)
 {
     uint32_t n = min(b.header[H_CLIP], cap);
-    uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t first = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);  // the wave's first thread
-    for (uint32_t w0 = first; w0 < n; w0 += stride) {
+    uint32_t waves = gridDim.x * (blockDim.x / 64u);
+    uint32_t wave = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u;
+    for (uint32_t w0 = wave * kClipPerWave; w0 < n; w0 += waves * kClipPerWave) {
         uint32_t i = w0 + lane_id();
-        bool act = i < n;
+        bool act = lane_id() < kClipPerWave && i < n;
         uint2 e = act ? clip_list[i] : make_uint2(0u, 0u);
-        clip_primitive<PROG>(draw_at<MULTI>(d0, b, e.x), e.x, e.y, act);
+        clip_primitive<PROG>(draw_at<MULTI>(d0, b, e.x), e.x, e.y, act,
+                             [&](uint32_t di) -> const DrawParams & { return draw_at<MULTI>(d0, b, di); });
     }
 }
 
@@ -1302,7 +1332,6 @@ struct VisShared {
     uint32_t keys[NB];
     unsigned long long best[TS * TS];  // encoded winner (see oi_value), ~0 = none
     float idep[TS * TS];               // depth before the draw
-    uint32_t cnt[TS * TS];             // emitted fragments (debug counts only)
     uint32_t wsum[NT / 64];
     uint32_t nkeys;
     uint32_t last;                     // split tiles: this workgroup finished the tile's last item
@@ -1439,8 +1468,9 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
     __syncthreads();
 
     // the span of every (triangle, row) entry, tile-relative; this thread's
-    // entries are tid, tid + NT, ... and its fragments follow each other
-    bool counts = d.s.frag_px != nullptr;
+    // entries are tid, tid + NT, ... and its fragments follow each other.
+    // Debug fragment counts (rare) go straight to the per-pixel buffer.
+    uint32_t *counts = d.s.frag_px;
     uint32_t nfr = 0;
     for (uint32_t idx = tid; idx < total; idx += NT) {
         uint32_t o = sh.owner[idx];
@@ -1468,7 +1498,7 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
             uint32_t q0 = row * TS + start;
             unsigned long long val = (unsigned long long)(~sh.tkey[te]);
             for (uint32_t i = 0; i < len; ++i) {
-                if (counts) atomicAdd(&sh.cnt[q0 + i], 1u);
+                if (counts) atomicAdd(&counts[(size_t)(y0 + row) * d.width + x0 + start + i], 1u);
                 atomicMin(&sh.best[q0 + i], val);
             }
             continue;
@@ -1511,7 +1541,7 @@ __device__ __forceinline__ uint32_t vis_batch(const DrawParams &d, VisShared<TS,
             uint32_t te = o & 0xFFu, row = (o >> 8) & 0xFFu;
             uint32_t px = ((o >> 16) & 0xFFu) + (fl >> 16);
             uint32_t q = row * TS + px;
-            if (counts) atomicAdd(&sh.cnt[q], 1u);
+            if (counts) atomicAdd(&counts[(size_t)(y0 + row) * d.width + x0 + px], 1u);
             uint32_t key = sh.tkey[te];
             float z = vis_depth<TS, NT>(sh, te, (float)(x0 + px), (float)(y0 + row));
             if (oi_pass(d, z, sh.idep[q])) atomicMin(&sh.best[q], oi_value(d.resolve, z, key));
@@ -1556,7 +1586,6 @@ __device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t til
             z = (d.init_clear && d.clear_depth_on) ? d.clear_depth : d.depth[(size_t)py * d.width + px];
         sh.idep[q] = z;
         sh.best[q] = ~0ull;
-        sh.cnt[q] = 0u;
     }
     // the first batch: the slots read above (the same as next_batch's)
     bool pre = item < feed.items && feed.nbin != 0u;
@@ -1585,9 +1614,6 @@ __device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t til
             uint32_t q = tid + (uint32_t)k * NT;
             unsigned long long v = sh.best[q];
             if (v != ~0ull) done |= atomicMax(&gv[q], ~v);
-            uint32_t px = x0 + q % TS, py = y0 + q / TS;
-            if (d.s.frag_px && sh.cnt[q] && px < d.width && py < d.height)
-                atomicAdd(&d.s.frag_px[(size_t)py * d.width + px], sh.cnt[q]);
         }
         // the maxima have returned (their results are used here), so they
         // were performed before this item's ticket below
@@ -1626,13 +1652,12 @@ __device__ __forceinline__ void raster_tile_oi(const DrawParams &d, uint32_t til
             if (d.color) store_color(d, pix, col);
             if (d.depth) {
                 if (write_depth_mode) store_depth(d, pix, z);
-                else if (d.init_clear && d.clear_depth_on) store_depth(d, pix, d.clear_depth);
+                else if (d.init_clear && d.clear_depth_on && !d.preclear) store_depth(d, pix, d.clear_depth);
             }
-        } else if (d.init_clear) {
+        } else if (d.init_clear && !d.preclear) {
             if (d.color && d.clear_color_on) store_color(d, pix, d.clear_rgba);
             if (d.depth && d.clear_depth_on) store_depth(d, pix, d.clear_depth);
         }
-        if (d.s.frag_px && sh.cnt[q] && !split) d.s.frag_px[pix] += sh.cnt[q];
     }
     __syncthreads();  // the next tile resets the per-pixel state
 }

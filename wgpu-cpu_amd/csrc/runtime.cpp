@@ -2351,7 +2351,7 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep
     size_t o_al = take(4 * total_tiles * kMaxItems);
     size_t o_bins = take(4 * bins_total), o_sp = take(8 * spill_cap);
     size_t o_cl = take(8 * prims);
-    std::vector<size_t> o_big(n), o_tri(n), o_ctri(n), o_ct(n), o_cb(n), o_ss(n);
+    std::vector<size_t> o_big(n), o_tri(n), o_ctri(n), o_ct(n), o_cj(n), o_cb(n), o_ss(n);
     for (uint32_t i = 0; i < n; ++i) {
         const DrawParams &d = st.draws[i];
         uint64_t rc = d.total_prims + clip_cap[i];
@@ -2360,6 +2360,7 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep
         o_tri[i] = take(4ull * kRecWords * d.total_prims);
         o_ctri[i] = take(4ull * kRecWords * clip_cap[i]);
         o_ct[i] = take(4ull * kClipFloats * clip_cap[i]);
+        o_cj[i] = take(16ull * clip_cap[i]);
         o_cb[i] = take(4ull * d.total_prims);
     }
     if (!base) return off;
@@ -2393,7 +2394,10 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep
     b.n_counters = gl.n_counters;
     // a single draw clears its whole target in setup when setup has a thread
     // for every 32 pixels (then the stores drain while setup waits on its
-    // loads and atomics); the tile kernel then only writes the tiles it rasters
+    // loads and atomics); the tile kernel then skips the untouched tiles and
+    // stores only the pixels the draw writes.  (A batched group leaves its
+    // clears to the tile kernel, where they hide behind latency-bound work:
+    // stored separately they cost more than they save.)
     bool preclear = false;
     if (!st.multi && any_clear && prims > 0) {
         uint64_t threads = align_up(prims, 256), px = (uint64_t)st.draws[0].width * st.draws[0].height;
@@ -2422,6 +2426,7 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep
     for (uint32_t i = 0; i < n; ++i) {
         DrawParams &d = st.draws[i];
         Scratch &s = d.s;
+        d.preclear = (preclear && d.init_clear) ? 1u : 0u;
         s.tile_count = gl.tile_count + tile_base;
         s.tile_done = tile_done + tile_base;
         s.gvis = keep_bytes(st) ? (unsigned long long *)keep + tile_base * tile_px : nullptr;
@@ -2437,6 +2442,7 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep
         s.tri = (uint32_t *)(base + o_tri[i]);
         s.clip_tri = (uint32_t *)(base + o_ctri[i]);
         s.clip_tris = (float *)(base + o_ct[i]);
+        s.clip_job = (uint4 *)(base + o_cj[i]);
         s.clip_base = (uint32_t *)(base + o_cb[i]);
         s.strip_sep = restart_strip(d) ? (int32_t *)(base + o_ss[i]) : nullptr;
         s.counters = gl.counters + C_NUM * i;

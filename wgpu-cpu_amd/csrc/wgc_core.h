@@ -117,6 +117,7 @@ struct Scratch {
     uint32_t *tri;           // triangle records of unclipped primitives (kRecWords each, by primitive)
     uint32_t *clip_tri;      // triangle records of clipped sub-triangles (by clip slot)
     float *clip_tris;        // clip_tri_cap x kClipFloats: positions and clip barycentrics
+    uint4 *clip_job;         // clip_tri_cap: the sub-triangle's key and tile footprint (w = 1: to bin)
     uint32_t *clip_base;     // per primitive: first sub-triangle slot
     uint32_t *counters;      // see Counter
     unsigned long long *frags;  // kFragSlots fragment counters of the submission (summed on read)
@@ -204,6 +205,8 @@ struct DrawParams {
     uint32_t color_bgra;
     float *depth;
     uint32_t init_clear;     // this raster launch applies the LoadOp::Clear values
+    uint32_t preclear;       // setup stored them into the whole target: the tile kernels store only
+                             // the pixels the draw writes
     uint32_t clear_color_on, clear_depth_on;
     uint32_t clear_rgba;     // packed in memory byte order
     float clear_depth;

@@ -18,7 +18,9 @@ import os
 from pathlib import Path
 from typing import Optional, Sequence
 
-_LIB_PATH = Path(__file__).resolve().parent / "libwgc.so"
+# WGC_LIBRARY names another build of the library (for example an
+# instrumented one); the default is the in-tree build next to this file.
+_LIB_PATH = Path(os.environ.get("WGC_LIBRARY") or Path(__file__).resolve().parent / "libwgc.so")
 
 # ---- enums (include/wgc.h) ----
 SEMANTICS_REFERENCE = 0
