@@ -1313,11 +1313,18 @@ enum VisField {
 };
 enum VisScan { S_AX = 0, S_AY, S_BX, S_BY, S_CX, S_CY, S_MODE, S_NUM };
 
-// Triangles staged per visibility batch: 96 for 16x16 tiles of 256 threads
-// (so that six workgroups fit a CU's LDS), else 128 (64 for one wave).
 template <int TS, int NT>
 struct VisBatch {
-    static constexpr int value = (TS == 16 && NT == 256) ? 96 : (NT >= 128 ? 128 : NT);
+    static constexpr int value = (int)vis_batch_size(TS, NT);
+};
+
+// Waves per SIMD the tile kernel's registers are budgeted for (the second
+// __launch_bounds__ argument): six, i.e. six workgroups per CU for 16x16
+// tiles of 256 threads and three for 32x32 tiles of 512; other shapes as the
+// compiler sees fit.
+template <int TS, int NT>
+struct TilesOccupancy {
+    static constexpr int value = ((NT == 256 && TS == 16) || NT == 512) ? 6 : 1;
 };
 
 template <int TS, int NT>
@@ -1810,9 +1817,8 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
 // Order-independent draws: visibility and shading of every active tile
 // (raster_tile_oi), then the untouched tiles in chunks, strided over the
 // whole grid.
-// Six waves per SIMD for 256 threads (registers for six resident workgroups).
 template <int PROG, bool MULTI, int TS, int NT>
-__global__ __launch_bounds__(NT, NT == 256 && TS == 16 ? 6 : 1) void k_tiles(DrawParams d0, BatchArgs b)
+__global__ __launch_bounds__(NT, (TilesOccupancy<TS, NT>::value)) void k_tiles(DrawParams d0, BatchArgs b)
 {
     constexpr int PPT = TS * TS / NT;
     static_assert(NT >= kBatch && PPT >= 1 && TS * TS == PPT * NT, "tile shape");

@@ -2301,12 +2301,8 @@ static size_t keep_bytes(const Step &st)
 static uint32_t split_for(const Step &st)
 {
     if (st.gl.ordered || st.multi) return 0;
-    if (st.gl.user) return 192;                     // 16x16 tiles of 256 threads: 96-triangle batches
-    switch (st.gl.raster_cfg) {
-    case RASTER_32x32_256: return 256;
-    case RASTER_8x8_64: case RASTER_16x16_64: return 128;
-    default: return 192;
-    }
+    uint32_t cfg = st.gl.user ? (uint32_t)RASTER_16x16_256 : st.gl.raster_cfg;
+    return 2u * vis_batch_size(1u << raster_tile_shift(cfg), raster_threads(cfg));
 }
 
 static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep, uint32_t *frag_px,

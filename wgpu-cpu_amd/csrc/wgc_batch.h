@@ -38,12 +38,37 @@ enum RasterCfg : uint32_t {
     RASTER_8x8_64 = 1,
     RASTER_16x16_64 = 2,
     RASTER_32x32_256 = 3,
-    RASTER_NUM_CFG = 4,
+    RASTER_16x16_128 = 4,
+    RASTER_32x32_128 = 5,
+    RASTER_32x32_512 = 6,
+    RASTER_NUM_CFG = 7,
 };
+
+// Threads per workgroup of a tile shape.
+__host__ __device__ inline uint32_t raster_threads(uint32_t cfg)
+{
+    switch (cfg) {
+    case RASTER_8x8_64: case RASTER_16x16_64: return 64u;
+    case RASTER_16x16_128: case RASTER_32x32_128: return 128u;
+    case RASTER_32x32_512: return 512u;
+    default: return 256u;
+    }
+}
+
+// Triangles staged per visibility batch of the order-independent tile
+// kernel: 96 for 16x16 tiles of 256 threads (so that six workgroups fit a
+// CU's LDS), 64 for 16x16 tiles of at most 128 threads, else 128 (64 for
+// one wave).
+__host__ __device__ constexpr uint32_t vis_batch_size(uint32_t ts, uint32_t nt)
+{
+    return (ts == 16u && nt == 256u) ? 96u : (ts == 16u || nt < 128u) ? 64u : 128u;
+}
 
 __host__ __device__ inline uint32_t raster_tile_shift(uint32_t cfg)
 {
-    return cfg == RASTER_8x8_64 ? 3u : (cfg == RASTER_32x32_256 ? 5u : 4u);
+    return cfg == RASTER_8x8_64 ? 3u
+         : (cfg == RASTER_32x32_256 || cfg == RASTER_32x32_128 || cfg == RASTER_32x32_512) ? 5u
+                                                                                           : 4u;
 }
 
 }  // namespace wgc
