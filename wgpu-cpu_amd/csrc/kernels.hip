@@ -1,174 +1,145 @@
-// kernels.hip -- host-side launchers of the render-pass kernels for the
-// built-in programs (the kernels themselves are in kernels_dev.h).
+// kernels.hip -- host-side launchers: dispatch of the built-in programs'
+// launch groups to their per-program units (kernels_prog.hip), the launch
+// groups of WGSL pipelines, and the program-independent kernels (counters,
+// clears, copies, the restart-strip pre-pass).
 #include <algorithm>
 
-#include <hip/hip_ext.h>
-
 #include "kernels_dev.h"
-#include "wgc_launch.h"
+#include "kernels_prog.h"
 
 namespace wgc {
 
 // ------------------------------------------------------------------
-// launchers
+// program-independent kernels
 // ------------------------------------------------------------------
 
-// With a kernel timer, the launch records the kernel's start and end in
-// its own events (from the dispatch packet, like a profiler's trace).
-#define WGC_LAUNCH(name, kern, grid, block, ...)                                                      \
-    do {                                                                                              \
-        if (kt) {                                                                                     \
-            hipEvent_t b_ = nullptr, f_ = nullptr;                                                    \
-            kt->pair(name, b_, f_);                                                                   \
-            hipExtLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, b_, f_, 0u, __VA_ARGS__);     \
-        } else {                                                                                      \
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, __VA_ARGS__);                    \
-        }                                                                                             \
-        hipError_t e_ = hipGetLastError();                                                            \
-        if (e_ != hipSuccess) return e_;                                                              \
-    } while (0)
-
-template <int PROG, bool MULTI>
-static hipError_t launch_raster(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
-                                KernelTimer *kt)
+__global__ void k_reset(unsigned long long *frags, uint32_t *error)
 {
-    if (gl.ordered) {
-        WGC_LAUNCH("k_raster", (k_raster<PROG, MULTI, 16, 256>), gl.raster_grid, 256, d, b);
-        return hipSuccess;
+    if (threadIdx.x < kFragSlots) frags[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) *error = 0u;
+}
+
+// End of a submission: its counters to host memory (out: the fragment
+// count summed over the kFragSlots counters, its high word, the error word,
+// 0 -- one 16-byte store) and zero for the next one; the zero region of the
+// scratch arena that its launch groups used (tile counts, group headers,
+// per-draw counters) cleared, n16 16-byte units.
+__global__ __launch_bounds__(256) void k_end_submission(unsigned long long *frags, uint32_t *error, uint4 *out,
+                                                        uint4 *zone, uint32_t n16)
+{
+    static_assert(kFragSlots == 64, "one fragment counter per lane of a wave");
+    if (blockIdx.x == 0 && threadIdx.x < 64u) {
+        unsigned long long f = frags[threadIdx.x];
+        frags[threadIdx.x] = 0ull;
+        for (int off = 32; off > 0; off >>= 1) f += __shfl_xor(f, off);
+        if (threadIdx.x == 0) {
+            uint32_t e = *error;
+            *error = 0u;
+            *out = make_uint4((uint32_t)f, (uint32_t)(f >> 32), e, 0u);
+        }
     }
-    switch (gl.raster_cfg) {
-    case RASTER_8x8_64: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 8, 64>), gl.raster_grid, 64, d, b); break;
-    case RASTER_16x16_64: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 16, 64>), gl.raster_grid, 64, d, b); break;
-    case RASTER_32x32_256: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 32, 256>), gl.raster_grid, 256, d, b); break;
-    case RASTER_16x16_128: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 16, 128>), gl.raster_grid, 128, d, b); break;
-    case RASTER_32x32_128: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 32, 128>), gl.raster_grid, 128, d, b); break;
-    case RASTER_32x32_512: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 32, 512>), gl.raster_grid, 512, d, b); break;
-    default: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 16, 256>), gl.raster_grid, 256, d, b); break;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += gridDim.x * blockDim.x)
+        zone[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+__global__ void k_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, uint64_t n)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (color) color[i] = rgba;
+    if (depth) depth[i] = dv;
+}
+
+
+
+// ------------------------------------------------------------------
+// Restart triangle strips: separator summary per chunk of positions
+// ------------------------------------------------------------------
+
+// Last separator position inside each chunk (-1: none); one thread per chunk.
+__global__ __launch_bounds__(256) void k_strip_chunks(DrawParams d, int32_t *last)
+{
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n_chunks = (d.count + kStripChunk - 1) / kStripChunk;
+    if (c >= n_chunks) return;
+    uint32_t sep = d.index_format == 2u ? 0xFFFFu : 0xFFFFFFFFu;
+    int32_t found = -1;
+    uint32_t end = min(d.count, (c + 1) * kStripChunk);
+    for (uint32_t i = c * kStripChunk; i < end; ++i) {
+        uint32_t idx;
+        if (raw_index(d, d.first + i, idx) && idx == sep) found = (int32_t)i;
     }
+    last[c] = found;
+}
+
+// Inclusive prefix maximum over the chunks (one thread: a few thousand
+// chunks at most per draw).
+__global__ void k_strip_prefix(DrawParams d, int32_t *last)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t n_chunks = (d.count + kStripChunk - 1) / kStripChunk;
+    int32_t m = -1;
+    for (uint32_t c = 0; c < n_chunks; ++c) {
+        m = max(m, last[c]);
+        last[c] = m;
+    }
+}
+
+hipError_t launch_strip_prepass(const DrawParams &d, hipStream_t st, KernelTimer *kt)
+{
+    uint32_t chunks = (d.count + kStripChunk - 1) / kStripChunk;
+    WGC_LAUNCH("k_strip_chunks", k_strip_chunks, (chunks + 255) / 256, 256, d, d.s.strip_sep);
+    WGC_LAUNCH("k_strip_prefix", k_strip_prefix, 1, 64, d, d.s.strip_sep);
     return hipSuccess;
 }
 
-// Workgroups of a kernel that fit on one CU at once (registers, LDS, waves);
-// the persistent grids are sized to what can be resident, so that the static
-// schedule never waits for a second round of workgroups.
-template <class K>
-static uint32_t occupancy_of(K kernel, int threads)
-{
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1) n = 1;
-    return (uint32_t)n;
-}
+// ------------------------------------------------------------------
+// built-in programs: one unit per (program, form)
+// ------------------------------------------------------------------
 
-template <int PROG, bool MULTI, int TS, int NT>
-static uint32_t occupancy_tiles()
-{
-    static uint32_t r = 0;
-    if (!r) r = occupancy_of(k_tiles<PROG, MULTI, TS, NT>, NT);
-    return r;
-}
+#define WGC_EACH_PROGRAM(X) X(PROG_MVP_COLOR) X(PROG_MVP_TEXTURE) X(PROG_COLORED_TRIANGLE) X(PROG_CLIP_COLOR)
 
-template <int PROG, bool MULTI>
-static uint32_t occupancy_cfg(bool ordered, uint32_t cfg)
+hipError_t launch_draw(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
+                       KernelTimer *kt)
 {
-    if (ordered) {
-        static uint32_t r = 0;
-        if (!r) r = occupancy_of(k_raster<PROG, MULTI, 16, 256>, 256);
-        return r;
+    switch (gl.program) {
+#define X(P) case P: return launch_program<P, false>(d, b, gl, st, kt);
+    WGC_EACH_PROGRAM(X)
+#undef X
+    default: return hipErrorInvalidValue;
     }
-    switch (cfg) {
-    case RASTER_8x8_64: return occupancy_tiles<PROG, MULTI, 8, 64>();
-    case RASTER_16x16_64: return occupancy_tiles<PROG, MULTI, 16, 64>();
-    case RASTER_32x32_256: return occupancy_tiles<PROG, MULTI, 32, 256>();
-    case RASTER_16x16_128: return occupancy_tiles<PROG, MULTI, 16, 128>();
-    case RASTER_32x32_128: return occupancy_tiles<PROG, MULTI, 32, 128>();
-    case RASTER_32x32_512: return occupancy_tiles<PROG, MULTI, 32, 512>();
-    default: return occupancy_tiles<PROG, MULTI, 16, 256>();
+}
+
+hipError_t launch_batch(const DrawParams &d0, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
+                        KernelTimer *kt)
+{
+    switch (gl.program) {
+#define X(P) case P: return launch_program<P, true>(d0, b, gl, st, kt);
+    WGC_EACH_PROGRAM(X)
+#undef X
+    default: return hipErrorInvalidValue;
     }
 }
 
 uint32_t raster_blocks_per_cu(uint32_t program, bool ordered, bool multi, uint32_t cfg)
 {
     switch (program) {
-    case PROG_MVP_COLOR:
-        return multi ? occupancy_cfg<PROG_MVP_COLOR, true>(ordered, cfg) : occupancy_cfg<PROG_MVP_COLOR, false>(ordered, cfg);
-    case PROG_MVP_TEXTURE:
-        return multi ? occupancy_cfg<PROG_MVP_TEXTURE, true>(ordered, cfg)
-                     : occupancy_cfg<PROG_MVP_TEXTURE, false>(ordered, cfg);
-    case PROG_COLORED_TRIANGLE:
-        return multi ? occupancy_cfg<PROG_COLORED_TRIANGLE, true>(ordered, cfg)
-                     : occupancy_cfg<PROG_COLORED_TRIANGLE, false>(ordered, cfg);
-    case PROG_CLIP_COLOR:
-        return multi ? occupancy_cfg<PROG_CLIP_COLOR, true>(ordered, cfg)
-                     : occupancy_cfg<PROG_CLIP_COLOR, false>(ordered, cfg);
+#define X(P)                                                                                           \
+    case P: return multi ? raster_blocks_per_cu_of<P, true>(ordered, cfg) : raster_blocks_per_cu_of<P, false>(ordered, cfg);
+    WGC_EACH_PROGRAM(X)
+#undef X
     default: return 1;
     }
 }
 
 uint32_t setup_clip_blocks_per_cu(uint32_t program)
 {
-    static uint32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (program >= 8) return 0;
-    uint32_t &v = r[program];
-    if (!v) {
-        switch (program) {
-        case PROG_MVP_COLOR: v = occupancy_of(k_setup<PROG_MVP_COLOR, false, true>, 256); break;
-        case PROG_MVP_TEXTURE: v = occupancy_of(k_setup<PROG_MVP_TEXTURE, false, true>, 256); break;
-        case PROG_COLORED_TRIANGLE: v = occupancy_of(k_setup<PROG_COLORED_TRIANGLE, false, true>, 256); break;
-        case PROG_CLIP_COLOR: v = occupancy_of(k_setup<PROG_CLIP_COLOR, false, true>, 256); break;
-        default: return 0;
-        }
+    switch (program) {
+#define X(P) case P: return setup_clip_blocks_per_cu_of<P>();
+    WGC_EACH_PROGRAM(X)
+#undef X
+    default: return 0;
     }
-    return v;
-}
-
-template <int PROG, bool MULTI>
-static hipError_t launch_group(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
-                               KernelTimer *kt)
-{
-    if (!MULTI && d.strip_restart && d.indexed && d.topology == TOPO_TRIANGLE_STRIP && d.count > 0) {
-        uint32_t chunks = (d.count + kStripChunk - 1) / kStripChunk;
-        WGC_LAUNCH("k_strip_chunks", k_strip_chunks, (chunks + 255) / 256, 256, d, d.s.strip_sep);
-        WGC_LAUNCH("k_strip_prefix", k_strip_prefix, 1, 64, d, d.s.strip_sep);
-    }
-    if (gl.total_prims > 0) {
-        uint32_t g = (gl.total_prims + 255u) / 256u;
-        if constexpr (!MULTI) {
-            if (gl.inline_clip) {
-                WGC_LAUNCH("k_setup", (k_setup<PROG, false, true>), g, 256, d, b);
-                return launch_raster<PROG, MULTI>(d, b, gl, st, kt);
-            }
-        }
-        WGC_LAUNCH("k_setup", (k_setup<PROG, MULTI>), g, 256, d, b);
-        // kClipPerWave primitives per wave; the count is only known on the device
-        uint32_t gc = min(max(g / 2u, 1u), 2048u);
-        WGC_LAUNCH("k_clip", (k_clip<PROG, MULTI>), gc, 256, d, b, gl.clip_list, gl.clip_cap);
-    }
-    return launch_raster<PROG, MULTI>(d, b, gl, st, kt);
-}
-
-template <bool MULTI>
-static hipError_t launch_group_prog(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl,
-                                    hipStream_t st, KernelTimer *kt)
-{
-    switch (gl.program) {
-    case PROG_MVP_COLOR: return launch_group<PROG_MVP_COLOR, MULTI>(d, b, gl, st, kt);
-    case PROG_MVP_TEXTURE: return launch_group<PROG_MVP_TEXTURE, MULTI>(d, b, gl, st, kt);
-    case PROG_COLORED_TRIANGLE: return launch_group<PROG_COLORED_TRIANGLE, MULTI>(d, b, gl, st, kt);
-    case PROG_CLIP_COLOR: return launch_group<PROG_CLIP_COLOR, MULTI>(d, b, gl, st, kt);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_draw(const DrawParams &d, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
-                       KernelTimer *kt)
-{
-    return launch_group_prog<false>(d, b, gl, st, kt);
-}
-
-hipError_t launch_batch(const DrawParams &d0, const BatchArgs &b, const GroupLaunch &gl, hipStream_t st,
-                        KernelTimer *kt)
-{
-    return launch_group_prog<true>(d0, b, gl, st, kt);
 }
 
 // ---- WGSL pipelines: the shader kernels are module functions ----
@@ -198,9 +169,8 @@ static hipError_t launch_user_t(const UserKernels &k, const DrawParams &d0, cons
     uint32_t clip_cap = gl.clip_cap;
     const int m = MULTI ? 1 : 0;
     if (!MULTI && d.strip_restart && d.indexed && d.topology == TOPO_TRIANGLE_STRIP && d.count > 0) {
-        uint32_t chunks = (d.count + kStripChunk - 1) / kStripChunk;
-        WGC_LAUNCH("k_strip_chunks", k_strip_chunks, (chunks + 255) / 256, 256, d, d.s.strip_sep);
-        WGC_LAUNCH("k_strip_prefix", k_strip_prefix, 1, 64, d, d.s.strip_sep);
+        hipError_t e = launch_strip_prepass(d, st, kt);
+        if (e != hipSuccess) return e;
     }
     if (gl.total_prims > 0) {
         uint32_t g = (gl.total_prims + 255u) / 256u;
@@ -275,28 +245,6 @@ hipError_t launch_fill(void *dst, uint64_t bytes, uint32_t value, hipStream_t st
 
 // ---- points and lines ----
 
-template <int PROG, int N>
-static hipError_t launch_lines_t(const DrawParams &d, const GroupLaunch &gl, hipStream_t st, KernelTimer *kt)
-{
-    uint32_t g = (d.total_prims + 255u) / 256u;
-    if (gl.ordered) {
-        WGC_LAUNCH("k_lp_ordered", (k_lp_ordered<PROG, N>), 1, 256, d);
-        return hipSuccess;
-    }
-    if (!g) return hipSuccess;
-    hipError_t e = launch_fill(gl.vis, 8ull * d.width * d.height, 0xFFFFFFFFu, st, kt);
-    if (e != hipSuccess) return e;
-    WGC_LAUNCH("k_lp_visibility", (k_lp_visibility<PROG, N>), g, 256, d, gl.vis);
-    WGC_LAUNCH("k_lp_shade", (k_lp_shade<PROG, N>), g, 256, d, (const unsigned long long *)gl.vis);
-    return hipSuccess;
-}
-
-template <int PROG>
-static hipError_t launch_lines_p(const DrawParams &d, const GroupLaunch &gl, hipStream_t st, KernelTimer *kt)
-{
-    return d.topology == TOPO_POINT_LIST ? launch_lines_t<PROG, 1>(d, gl, st, kt) : launch_lines_t<PROG, 2>(d, gl, st, kt);
-}
-
 static hipError_t launch_lines_user(const UserKernels &k, const DrawParams &d0, const GroupLaunch &gl, hipStream_t st,
                                     KernelTimer *kt)
 {
@@ -320,10 +268,9 @@ hipError_t launch_lines(const DrawParams &d, const GroupLaunch &gl, hipStream_t 
 {
     if (gl.user) return launch_lines_user(*gl.user, d, gl, st, kt);
     switch (d.program) {
-    case PROG_MVP_COLOR: return launch_lines_p<PROG_MVP_COLOR>(d, gl, st, kt);
-    case PROG_MVP_TEXTURE: return launch_lines_p<PROG_MVP_TEXTURE>(d, gl, st, kt);
-    case PROG_COLORED_TRIANGLE: return launch_lines_p<PROG_COLORED_TRIANGLE>(d, gl, st, kt);
-    case PROG_CLIP_COLOR: return launch_lines_p<PROG_CLIP_COLOR>(d, gl, st, kt);
+#define X(P) case P: return launch_lines_of<P>(d, gl, st, kt);
+    WGC_EACH_PROGRAM(X)
+#undef X
     default: return hipErrorInvalidValue;
     }
 }

@@ -1853,82 +1853,6 @@ __global__ __launch_bounds__(NT, (TilesOccupancy<TS, NT>::value)) void k_tiles(D
 }
 
 // ------------------------------------------------------------------
-// k_clear / k_reset
-// ------------------------------------------------------------------
-
-__global__ void k_reset(unsigned long long *frags, uint32_t *error)
-{
-    if (threadIdx.x < kFragSlots) frags[threadIdx.x] = 0ull;
-    if (threadIdx.x == 0) *error = 0u;
-}
-
-// End of a submission: its counters to host memory (out: the fragment
-// count summed over the kFragSlots counters, its high word, the error word,
-// 0 -- one 16-byte store) and zero for the next one; the zero region of the
-// scratch arena that its launch groups used (tile counts, group headers,
-// per-draw counters) cleared, n16 16-byte units.
-__global__ __launch_bounds__(256) void k_end_submission(unsigned long long *frags, uint32_t *error, uint4 *out,
-                                                        uint4 *zone, uint32_t n16)
-{
-    static_assert(kFragSlots == 64, "one fragment counter per lane of a wave");
-    if (blockIdx.x == 0 && threadIdx.x < 64u) {
-        unsigned long long f = frags[threadIdx.x];
-        frags[threadIdx.x] = 0ull;
-        for (int off = 32; off > 0; off >>= 1) f += __shfl_xor(f, off);
-        if (threadIdx.x == 0) {
-            uint32_t e = *error;
-            *error = 0u;
-            *out = make_uint4((uint32_t)f, (uint32_t)(f >> 32), e, 0u);
-        }
-    }
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += gridDim.x * blockDim.x)
-        zone[i] = make_uint4(0u, 0u, 0u, 0u);
-}
-
-__global__ void k_clear(uint32_t *color, uint32_t rgba, float *depth, float dv, uint64_t n)
-{
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (color) color[i] = rgba;
-    if (depth) depth[i] = dv;
-}
-
-
-
-// ------------------------------------------------------------------
-// Restart triangle strips: separator summary per chunk of positions
-// ------------------------------------------------------------------
-
-// Last separator position inside each chunk (-1: none); one thread per chunk.
-__global__ __launch_bounds__(256) void k_strip_chunks(DrawParams d, int32_t *last)
-{
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n_chunks = (d.count + kStripChunk - 1) / kStripChunk;
-    if (c >= n_chunks) return;
-    uint32_t sep = d.index_format == 2u ? 0xFFFFu : 0xFFFFFFFFu;
-    int32_t found = -1;
-    uint32_t end = min(d.count, (c + 1) * kStripChunk);
-    for (uint32_t i = c * kStripChunk; i < end; ++i) {
-        uint32_t idx;
-        if (raw_index(d, d.first + i, idx) && idx == sep) found = (int32_t)i;
-    }
-    last[c] = found;
-}
-
-// Inclusive prefix maximum over the chunks (one thread: a few thousand
-// chunks at most per draw).
-__global__ void k_strip_prefix(DrawParams d, int32_t *last)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint32_t n_chunks = (d.count + kStripChunk - 1) / kStripChunk;
-    int32_t m = -1;
-    for (uint32_t c = 0; c < n_chunks; ++c) {
-        m = max(m, last[c]);
-        last[c] = m;
-    }
-}
-
-// ------------------------------------------------------------------
 // Points and lines (raster.rs:184-265, clipper.rs:207-407,
 // util/bresenham.rs:78-153)
 // ------------------------------------------------------------------

@@ -2210,8 +2210,11 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
         return set_error(WGC_ERR_INVALID, "scissor rect outside the framebuffer");
     d.width = w;
     d.height = h;
-    // the ordered (sorted, sequential) raster exists for 16x16 tiles only
-    // the ordered raster and WGSL programs use 16x16 tiles
+    // 16x16 tiles for the ordered (sorted, sequential) raster, whatever the
+    // raster tile option: of the seven shapes it is the fastest for it (200k
+    // blended triangles over 2560x2440: 0.63 ms, the others 0.74-3.0 ms) and
+    // holds the most keys per pixel; WGSL pipelines' kernels are compiled for
+    // 16x16 tiles only
     d.tile_shift = (d.resolve == RES_ORDERED || pl->user) ? 4u : tile_shift;
     // points and lines: unsigned pixel indices (their keys and visibility buffer)
     if (d.topology <= TOPO_LINE_STRIP && (uint64_t)w * h > 0xFFFFFFFFull)

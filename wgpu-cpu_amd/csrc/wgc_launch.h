@@ -1,6 +1,7 @@
 // wgc_launch.h -- host-side interface between the runtime and the kernels.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -67,6 +68,23 @@ struct KernelTimer {
         e = p.end;
     }
 };
+
+// Launches a kernel on stream `st`; with a kernel timer `kt`, the launch
+// records the kernel's start and end in its own events (from the dispatch
+// packet, like a profiler's trace).  Returns the launch error from the
+// enclosing function.
+#define WGC_LAUNCH(name, kern, grid, block, ...)                                                      \
+    do {                                                                                              \
+        if (kt) {                                                                                     \
+            hipEvent_t b_ = nullptr, f_ = nullptr;                                                    \
+            kt->pair(name, b_, f_);                                                                   \
+            hipExtLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, b_, f_, 0u, __VA_ARGS__);     \
+        } else {                                                                                      \
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, __VA_ARGS__);                    \
+        }                                                                                             \
+        hipError_t e_ = hipGetLastError();                                                            \
+        if (e_ != hipSuccess) return e_;                                                              \
+    } while (0)
 
 // Kernels of a pipeline whose shaders come from WGSL: the instantiations
 // for PROG_USER of the kernels that run shader code, compiled with hipRTC
