@@ -202,13 +202,13 @@ __device__ __forceinline__ void bin_wave(const DrawParams &d, bool has, uint32_t
     uint32_t ntl = has ? w * (ty1 - ty0 + 1u) : 0u;
     uint32_t lane = lane_id();
     uint64_t lt = lanes_below();
+    // got[k] is written only by the lanes that take an atomic and read only
+    // from those lanes: it has no initial value, so that no merge of the
+    // results with one waits for the atomics one by one
     uint32_t got[kMaxBinTiles], grp[kMaxBinTiles];
     uint32_t nk = 0;
 #pragma unroll
-    for (int k = 0; k < kMaxBinTiles; ++k) {
-        got[k] = 0u;
-        grp[k] = 0u;
-    }
+    for (int k = 0; k < kMaxBinTiles; ++k) grp[k] = 0u;
 #pragma unroll
     for (int k = 0; k < kMaxBinTiles; ++k) {
         bool act = (uint32_t)k < ntl;
@@ -347,7 +347,7 @@ __device__ __forceinline__ void note_big_wave(const DrawParams &d, bool big, con
 }
 
 template <int PROG, class DrawOf>
-__device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act, DrawOf &&draw_of);
+__device__ __forceinline__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act, DrawOf &&draw_of);
 
 // The LoadOp::Clear values into the whole target, spread over the setup
 // grid (16-byte stores; issued before the setup work, they drain while it
@@ -540,7 +540,7 @@ __device__ int clip_walk(const CTri &root, F &&emit)
 // wave.  `draw_of(di)` is draw di of the group: the lanes of a wave may clip
 // primitives of different draws in the batched form.
 template <int PROG, class DrawOf>
-__device__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act, DrawOf &&draw_of)
+__device__ __forceinline__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act, DrawOf &&draw_of)
 {
     VsOut v[3];
     CTri root;

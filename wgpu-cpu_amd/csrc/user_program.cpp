@@ -297,19 +297,24 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
          "    __device__ uint32_t instance_index() const { return inst; }\n"
          "    __device__ bool front_facing() const { return true; }\n"
          "    __device__ void position(float *p) const { p[0] = p[1] = p[2] = 0.0f; p[3] = 1.0f; }\n"
-         // vertex.rs:139-152: the attribute's bytes, unconverted
+         // vertex.rs:139-152: the attribute's bytes, unconverted; the load is
+         // issued whether or not the attribute lies inside the buffer (outside,
+         // it reads the buffer's start and fails), so that no branch on the
+         // address keeps the vertices' loads apart
          "    __device__ void in_location(uint32_t l, uint32_t *w, uint32_t n)\n"
          "    {\n"
-         "        const AttrBind &at = d.attr[l];\n"
-         "        uint64_t start = at.step_instance ? (uint64_t)inst * at.stride : (uint64_t)vert * at.stride;\n"
-         "        uint64_t off = start + at.offset;\n"
-         "        if (l >= d.n_attr || !at.data || off + 4ull * n > at.size) {\n"
+         "        if (l >= d.n_attr || !d.attr[l].data || d.attr[l].size < 4ull * n) {\n"
          "            ok = false;\n"
          "            for (uint32_t k = 0; k < n; ++k) w[k] = 0u;\n"
          "            return;\n"
          "        }\n"
-         "        const uint32_t *p = reinterpret_cast<const uint32_t *>(at.data + off);\n"
-         "        for (uint32_t k = 0; k < n; ++k) w[k] = p[k];\n"
+         "        const AttrBind &at = d.attr[l];\n"
+         "        uint64_t start = at.step_instance ? (uint64_t)inst * at.stride : (uint64_t)vert * at.stride;\n"
+         "        uint64_t off = start + at.offset;\n"
+         "        bool in = off + 4ull * n <= at.size;\n"
+         "        ok = ok && in;\n"
+         "        const uint32_t *p = reinterpret_cast<const uint32_t *>(at.data + (in ? off : 0ull));\n"
+         "        for (uint32_t k = 0; k < n; ++k) w[k] = in ? p[k] : 0u;\n"
          "    }\n"
          "    __device__ void out_position(const float *p) { for (int k = 0; k < 4; ++k) o.clip[k] = p[k]; }\n"
          "    __device__ void out_location(uint32_t l, const uint32_t *w, uint32_t n)\n"

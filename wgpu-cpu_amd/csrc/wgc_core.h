@@ -319,21 +319,25 @@ struct VsOut {
 };
 
 // vertex.rs:191-195: buffer[instance * stride_i + vertex * stride_v + offset..]
+// The load is issued whether or not the attribute lies inside the buffer
+// (outside, it reads the buffer's start and the result is false): no branch
+// on the address, so the loads of every vertex of a primitive are in flight
+// together.
 template <uint32_t LOC>
 __device__ __forceinline__ bool fetch_attr(const DrawParams &d, uint32_t instance, uint32_t vertex,
                                            float *dst, uint32_t want)
 {
     if (LOC >= d.n_attr) return false;
     const AttrBind at = d.attr[LOC];
-    if (!at.data || at.ncomp < want) return false;
+    if (!at.data || at.ncomp < want || at.size < 4ull * at.ncomp) return false;
     uint64_t start = at.step_instance ? (uint64_t)instance * at.stride : (uint64_t)vertex * at.stride;
     uint64_t off = start + at.offset;
-    if (off + 4ull * at.ncomp > at.size) return false;
-    const float *p = reinterpret_cast<const float *>(at.data + off);
+    bool in = off + 4ull * at.ncomp <= at.size;
+    const float *p = reinterpret_cast<const float *>(at.data + (in ? off : 0ull));
 #pragma unroll
     for (uint32_t k = 0; k < (uint32_t)kMaxVar; ++k)
         if (k < want) dst[k] = p[k];
-    return true;
+    return in;
 }
 
 // naga-cranelift binary.rs:299-321: column sum with separate fmul / fadd
@@ -349,9 +353,11 @@ __device__ __forceinline__ void mat4_mul_vec4(const float *m, const float *v, fl
     }
 }
 
+// The built-in programs' vertex shaders; `m` is the MVP matrix of the two
+// mesh programs (read once per primitive, by the caller).
 template <int PROG>
 __device__ __forceinline__ bool run_vs(const DrawParams &d, uint32_t instance, uint32_t vertex,
-                                       VsOut &o)
+                                       VsOut &o, const float *m)
 {
     if constexpr (PROG == PROG_MVP_COLOR || PROG == PROG_MVP_TEXTURE) {
         // both attributes are fetched before either is used
@@ -359,12 +365,8 @@ __device__ __forceinline__ bool run_vs(const DrawParams &d, uint32_t instance, u
         o.var[0] = o.var[1] = o.var[2] = o.var[3] = 0.0f;
         bool ok0 = fetch_attr<0>(d, instance, vertex, pos, 4);
         bool ok1 = fetch_attr<1>(d, instance, vertex, o.var, d.nvar);
-        if (!ok0 || !ok1) return false;
-        float m[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) m[k] = d.uniform[k];
         mat4_mul_vec4(m, pos, o.clip);
-        return true;
+        return ok0 && ok1;
     } else if constexpr (PROG == PROG_CLIP_COLOR) {
         o.var[0] = o.var[1] = o.var[2] = o.var[3] = 0.0f;
         bool ok0 = fetch_attr<0>(d, instance, vertex, o.clip, 4);
@@ -387,19 +389,36 @@ __device__ __forceinline__ bool run_vs(const DrawParams &d, uint32_t instance, u
     }
 }
 
-// index.rs:45-70
-__device__ __forceinline__ bool resolve_index(const DrawParams &d, uint32_t i, uint32_t &vertex)
+// index.rs:45-70 for the N positions i[k]: every index is loaded before
+// any is checked (a position outside the index buffer loads its first index
+// and fails), so the N loads are in flight together.
+template <int N>
+__device__ __forceinline__ bool resolve_indices(const DrawParams &d, const uint32_t i[N], uint32_t vertex[N])
 {
-    if (!d.indexed) { vertex = i; return true; }
-    uint64_t off = (uint64_t)i * d.index_format;
-    if (off + d.index_format > d.index_size) return false;
-    uint32_t idx;
-    if (d.index_format == 2u) idx = *reinterpret_cast<const uint16_t *>(d.index_data + off);
-    else idx = *reinterpret_cast<const uint32_t *>(d.index_data + off);
-    int64_t v = (int64_t)idx + d.base_vertex;
-    if (v < 0 || v > 0xFFFFFFFFll) return false;
-    vertex = (uint32_t)v;
-    return true;
+    if (!d.indexed) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) vertex[k] = i[k];
+        return true;
+    }
+    if (d.index_size < d.index_format) return false;
+    bool ok = true;
+    uint32_t idx[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        uint64_t off = (uint64_t)i[k] * d.index_format;
+        bool in = off + d.index_format <= d.index_size;
+        ok = ok && in;
+        const uint8_t *q = d.index_data + (in ? off : 0ull);
+        idx[k] = d.index_format == 2u ? (uint32_t)*reinterpret_cast<const uint16_t *>(q)
+                                      : *reinterpret_cast<const uint32_t *>(q);
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        int64_t v = (int64_t)idx[k] + d.base_vertex;
+        ok = ok && v >= 0 && v <= 0xFFFFFFFFll;
+        vertex[k] = (uint32_t)v;
+    }
+    return ok;
 }
 
 // The raw index value at position i of the draw (restart separators are
@@ -481,13 +500,10 @@ __device__ __forceinline__ bool shade_primitive(const DrawParams &d, uint32_t p,
     }
     // every index first, then every vertex: no vertex's loads wait for
     // another vertex's
-    uint32_t vertex[N];
-    bool iok = true;
+    uint32_t vertex[N], at[N];
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-        bool ok = resolve_index(d, d.first + pos[k], vertex[k]);
-        iok = iok && ok;
-    }
+    for (int k = 0; k < N; ++k) at[k] = d.first + pos[k];
+    bool iok = resolve_indices<N>(d, at, vertex);
     if (!iok) {
         atomicOr(d.s.error, (uint32_t)ERR_INDEX);
         return false;
@@ -504,10 +520,15 @@ __device__ __forceinline__ bool shade_primitive(const DrawParams &d, uint32_t p,
         return true;
     }
 #endif
+    float m[16];
+    if constexpr (PROG == PROG_MVP_COLOR || PROG == PROG_MVP_TEXTURE) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) m[k] = d.uniform[k];
+    }
     bool vok = true;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        bool ok = run_vs<PROG>(d, inst, vertex[k], v[k]);
+        bool ok = run_vs<PROG>(d, inst, vertex[k], v[k], m);
         vok = vok && ok;
     }
     if (!vok) {
