@@ -378,6 +378,7 @@ def run_serving(devs, args):
         srv.run(warm, batched=batched)
         out[mode] = srv.run(trace, batched=batched).summary()
         srv.heap.release()
+        print(f"bench.py: serving {mode} done: {out[mode]}", file=sys.stderr, flush=True)
     # saturation: 10x the rate for a shorter run
     hi = serving.make_trace(scenes.SEED_BASE + 7, args.serve_rate * 10, max(1.0, args.serve_seconds / 5),
                             len(pool.meshes))
@@ -385,6 +386,7 @@ def run_serving(devs, args):
         srv = serving.Server(dev, pool, budget, max_batch=256, max_inflight=2)
         out[mode] = srv.run(hi, batched=batched).summary()
         srv.heap.release()
+        print(f"bench.py: serving {mode} done: {out[mode]}", file=sys.stderr, flush=True)
     return out
 
 
