@@ -1747,16 +1747,21 @@ struct ActiveSchedule {
         asm volatile("" : : "s"(count), "s"(e0));  // both loads in flight together
         n = min(count, cap);
     }
-    __device__ uint32_t entry(const BatchArgs &b, uint32_t item) const
+    // The entry of `item` as loaded (0 past the end): the loops read each
+    // item's entry one item ahead and take it into scalar registers when
+    // they reach it, so the load overlaps the tile before.
+    __device__ uint32_t load(const BatchArgs &b, uint32_t item) const
     {
-        return __builtin_amdgcn_readfirstlane(item == first ? e0 : b.active_list[item]);
+        return item < n ? b.active_list[item] : 0u;
     }
 };
 
-// Active-list entry e: its draw, tile and the tile's work item.
+// Active-list entry e: its draw, tile and the tile's work item.  The
+// draw's first global tile comes with its parameters (scalar loads the
+// tile needs anyway), not from a table of its own.
 template <bool MULTI>
-__device__ __forceinline__ void locate_tile(const BatchArgs &b, uint32_t e, uint32_t &di, uint32_t &tile,
-                                            uint32_t &it)
+__device__ __forceinline__ void locate_tile(const DrawParams &d0, const BatchArgs &b, uint32_t e, uint32_t &di,
+                                            uint32_t &tile, uint32_t &it)
 {
     uint32_t gt = e & ((1u << kItemShift) - 1u);
     it = e >> kItemShift;
@@ -1764,7 +1769,7 @@ __device__ __forceinline__ void locate_tile(const BatchArgs &b, uint32_t e, uint
     tile = gt;
     if constexpr (MULTI) {
         di = __builtin_amdgcn_readfirstlane(b.tile_draw[gt]);
-        tile = gt - b.tile_base[di];
+        tile = gt - draw_at<MULTI>(d0, b, di).s.tile0;
     }
 }
 
@@ -1800,9 +1805,12 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
 
     ActiveSchedule w(b);
     uint32_t frag_acc = 0;
+    uint32_t raw = w.e0;
     for (uint32_t item = w.first; item < w.n; item += w.stride) {
+        uint32_t e = __builtin_amdgcn_readfirstlane(raw);
+        raw = w.load(b, item + w.stride);
         uint32_t di, tile, it;
-        locate_tile<MULTI>(b, w.entry(b, item), di, tile, it);
+        locate_tile<MULTI>(d0, b, e, di, tile, it);
         raster_tile<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, olist, olist_n, frag_acc);
     }
     // workgroups late in the schedule have the fewest active tiles: chunks from there
@@ -1838,9 +1846,12 @@ __global__ __launch_bounds__(NT, (TilesOccupancy<TS, NT>::value)) void k_tiles(D
     }
     ActiveSchedule w(b);
     uint32_t frag_acc = 0;
+    uint32_t raw = w.e0;
     for (uint32_t item = w.first; item < w.n; item += w.stride) {
+        uint32_t e = __builtin_amdgcn_readfirstlane(raw);
+        raw = w.load(b, item + w.stride);
         uint32_t di, tile, it;
-        locate_tile<MULTI>(b, w.entry(b, item), di, tile, it);
+        locate_tile<MULTI>(d0, b, e, di, tile, it);
         raster_tile_oi<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, it, sh, srgb, frag_acc);
     }
     // workgroups late in the schedule have the fewest active tiles: chunks from there
