@@ -83,7 +83,7 @@ def test_overdraw_in_api_order_per_tile_shape(gpu_device, tile_cfg):
 def test_crowded_ordered_tile_per_tile_shape(gpu_device_ext, tile_cfg):
     """2500 small triangles on the four central 16x16 tiles of a 1024x1024
     target: past every bin's capacity, so most keys come from the spill
-    list, and still under the 4096 keys an ordered tile sorts."""
+    list, and under the 4096 keys an ordered tile sorts at once."""
     spec = blend_pass(triangles(2500, 77, 1024, 1024, spread=0.006, size_px=(1.5, 5.0)), 1024, 1024, "crowded")
     announce(f"crowded ordered tile, raster cfg {tile_cfg}")
     assert_same(spec, runner.render(spec, gpu_device_ext), bridge.render(spec))
@@ -114,4 +114,58 @@ def test_ordered_passes_batched_per_tile_shape(gpu_device_ext, tile_cfg):
         assert_same(s, r, ref, per_pass_count=False)   # the batch reports one total
         if ref.depth is not None:
             assert np.array_equal(r.depth.view(np.uint32), ref.depth.view(np.uint32)), s.name
+    assert total == want
+
+
+def one_tile(vb, cx=136, cy=136, size=256):
+    """Moves triangles generated around the centre of a size x size target
+    to pixel (cx, cy), well inside one 16x16 tile."""
+    vb = vb.copy()
+    vb[:, :, 0] += (cx + 0.5) / size * 2 - 1
+    vb[:, :, 1] -= (cy + 0.5) / size * 2 - 1
+    return vb
+
+
+@pytest.mark.parametrize("n", [4096, 4097, 9000])
+def test_ordered_tile_beyond_one_sort(gpu_device_ext, n):
+    """More blended triangles on one tile than one sort holds: the tile
+    takes them in chunks of 4096 in API order (n = 4096 is one full chunk,
+    4097 leaves one key for a second)."""
+    vb = one_tile(triangles(n, 500 + n, 256, 256, spread=0.01, size_px=(1.5, 3.0)))
+    spec = blend_pass(vb, 256, 256, f"one tile, {n} blended triangles")
+    announce(spec.name)
+    assert_same(spec, runner.render(spec, gpu_device_ext), bridge.render(spec))
+
+
+def test_not_equal_write_beyond_one_sort(gpu_device_ext):
+    """NotEqual + depth write (applied in API order) with 7000 triangles on
+    one tile over four depth levels."""
+    vb = one_tile(triangles(7000, 611, 256, 256, spread=0.01, size_px=(1.5, 3.0), alpha=(1.0, 1.0)))
+    u = scenes.splitmix_f32(612, vb.shape[0])
+    vb[:, :, 2] = (np.array([0.2, 0.4, 0.6, 0.8], np.float32)[(u * 4).astype(int) % 4])[:, None]
+    d = scenes.DrawSpec(program=wgc.PROGRAM_CLIP_COLOR, vertex_data=vb.tobytes(), vertex_stride=32,
+                        attributes=[(wgc.FLOAT32X4, 0, 0), (wgc.FLOAT32X4, 16, 1)], count=vb.shape[0] * 3,
+                        depth=(wgc.NOT_EQUAL, True))
+    spec = scenes.PassSpec(256, 256, has_depth=True, clear_depth=0.5, draws=[d], semantics=wgc.SEMANTICS_EXTENDED,
+                           name="not_equal_write_one_tile")
+    announce(spec.name)
+    ref = bridge.render(spec)
+    got = runner.render(spec, gpu_device_ext)
+    assert_same(spec, got, ref)
+    assert np.array_equal(got.depth.view(np.uint32), ref.depth.view(np.uint32))
+
+
+def test_ordered_tile_beyond_one_sort_batched(gpu_device_ext):
+    """The crowded blended pass between two ordinary ones in one submission
+    (its spill entries are global tiles of the group)."""
+    announce("one tile beyond one sort, batched")
+    specs = [blend_pass(triangles(500, 700, 96, 80), 96, 80, "small_a"),
+             blend_pass(one_tile(triangles(6000, 701, 256, 256, spread=0.01, size_px=(1.5, 3.0))), 256, 256, "crowded"),
+             blend_pass(triangles(700, 702, 128, 64), 128, 64, "small_b")]
+    res, total = runner.render_batch(specs, gpu_device_ext)
+    want = 0
+    for s, r in zip(specs, res):
+        ref = bridge.render(s)
+        want += ref.fragments
+        assert_same(s, r, ref, per_pass_count=False)
     assert total == want

@@ -1191,6 +1191,83 @@ __device__ __forceinline__ void raster_batch(const DrawParams &d, RasterShared<T
     __syncthreads();
 }
 
+// fn(key) for every key of the tile (its bin slots, its entries of the
+// spill list, the big triangles whose footprint holds it), each key by one
+// thread of the workgroup.
+template <class F>
+__device__ __forceinline__ void for_tile_keys(const DrawParams &d, const TileFeed &f, F &&fn)
+{
+    for (uint32_t i = threadIdx.x; i < f.nbin; i += blockDim.x) fn(d.s.bins[f.bin0 + i]);
+    for (uint32_t i = threadIdx.x; i < f.nspill; i += blockDim.x) {
+        uint2 e = d.s.spill[i];
+        if (e.x == f.gt) fn(e.y);
+    }
+    for (uint32_t i = threadIdx.x; i < f.nbig; i += blockDim.x) {
+        uint4 r = d.s.big[i];
+        if (f.tx >= (r.y & 0xFFFFu) && f.tx <= (r.z & 0xFFFFu) && f.ty >= (r.y >> 16) && f.ty <= (r.z >> 16))
+            fn(r.x);
+    }
+}
+
+// The r-th smallest (from 1) of the tile's keys at or above lo, when at
+// least r of them are: a radix select over four 8-bit digits, one pass over
+// the keys per digit with a histogram in LDS (`hist`: 256 counters and two
+// words for the choice).  Every thread of the workgroup calls it.
+__device__ __forceinline__ uint32_t tile_key_select(const DrawParams &d, const TileFeed &f, uint64_t lo, uint32_t r,
+                                                    uint32_t *hist)
+{
+    uint32_t prefix = 0u, pmask = 0u;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) hist[i] = 0u;
+        __syncthreads();
+        for_tile_keys(d, f, [&](uint32_t key) {
+            if ((uint64_t)key >= lo && (key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+        });
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0u, sel = 255u;
+            for (uint32_t v = 0; v < 256u; ++v) {
+                if (acc + hist[v] >= r) { sel = v; break; }
+                acc += hist[v];
+            }
+            hist[256] = sel;
+            hist[257] = acc;
+        }
+        __syncthreads();
+        uint32_t sel = hist[256], below = hist[257];
+        __syncthreads();  // the histogram is cleared next round
+        prefix |= sel << shift;
+        pmask |= 255u << shift;
+        r -= below;
+    }
+    return prefix;
+}
+
+// Sorts olist[0..total) (total <= kOrderedCap) into ascending order: a
+// bitonic sort padded with UINT32_MAX to a power of two.
+template <int NT>
+__device__ __forceinline__ void sort_olist(uint32_t *olist, uint32_t total)
+{
+    uint32_t tid = threadIdx.x;
+    uint32_t P = 1u;
+    while (P < total) P <<= 1;
+    for (uint32_t i = total + tid; i < P; i += NT) olist[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < P; i += NT) {
+                uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    uint32_t a = olist[i], c = olist[ixj];
+                    bool up = (i & k) == 0u;
+                    if ((a > c) == up) { olist[i] = c; olist[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // One tile of an ordered draw: load or clear, apply its triangles in API
 // order, store.
 template <int PROG, int TS, int NT>
@@ -1247,35 +1324,49 @@ __device__ __forceinline__ void raster_tile(const DrawParams &d, uint32_t tile, 
         if (tid == 0 && !overflow) olist_n = base + n;
         __syncthreads();
     }
-    uint32_t total = olist_n;
+    // more keys than the list holds: the tile's keys in ascending chunks of
+    // at most kOrderedCap, each chunk's end found by a radix select over the
+    // keys (chunks in key order are API order; the pixels' state carries
+    // over from one chunk to the next)
+    uint64_t lo = 0u, hi = 1ull << 32;
     if (overflow) {
-        if (tid == 0) atomicOr(d.s.error, (uint32_t)ERR_ORDERED_OVERFLOW);
-        total = 0u;
+        __syncthreads();  // every thread has read olist_n
+        hi = (uint64_t)tile_key_select(d, feed, 0u, (uint32_t)kOrderedCap, olist) + 1u;
     }
-    // bitonic sort of olist[0..P), padded with UINT32_MAX
-    uint32_t P = 1u;
-    while (P < total) P <<= 1;
-    for (uint32_t i = total + tid; i < P; i += NT) olist[i] = 0xFFFFFFFFu;
-    __syncthreads();
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = tid; i < P; i += NT) {
-                uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    uint32_t a = olist[i], c = olist[ixj];
-                    bool up = (i & k) == 0u;
-                    if ((a > c) == up) { olist[i] = c; olist[ixj] = a; }
+    for (;;) {
+        if (overflow) {
+            if (tid == 0) olist_n = 0u;
+            __syncthreads();
+            for_tile_keys(d, feed, [&](uint32_t key) {
+                if ((uint64_t)key >= lo && (uint64_t)key < hi) {
+                    uint32_t at = atomicAdd(&olist_n, 1u);
+                    if (at < (uint32_t)kOrderedCap) olist[at] = key;   // (keys are distinct: at most kOrderedCap)
                 }
-            }
+            });
             __syncthreads();
         }
-    }
-
-    for (uint32_t fpos = 0; fpos < total;) {
-        uint32_t n = min((uint32_t)kBatch, total - fpos);
-        if (tid < n) sh.keys[tid] = olist[fpos + tid];
-        fpos += n;
-        raster_batch<PROG, ORDERED, TS, NT>(d, sh, n, x0, y0, ps, nfrag);
+        uint32_t total = min(olist_n, (uint32_t)kOrderedCap);
+        sort_olist<NT>(olist, total);
+        for (uint32_t fpos = 0; fpos < total;) {
+            uint32_t n = min((uint32_t)kBatch, total - fpos);
+            if (tid < n) sh.keys[tid] = olist[fpos + tid];
+            fpos += n;
+            raster_batch<PROG, ORDERED, TS, NT>(d, sh, n, x0, y0, ps, nfrag);
+        }
+        if (!overflow || hi > 0xFFFFFFFFull) break;
+        // the next chunk: the keys from hi on, at most kOrderedCap of them
+        lo = hi;
+        if (tid == 0) olist_n = 0u;
+        __syncthreads();
+        for_tile_keys(d, feed, [&](uint32_t key) {
+            if ((uint64_t)key >= lo) atomicAdd(&olist_n, 1u);
+        });
+        __syncthreads();
+        uint32_t rest = olist_n;
+        __syncthreads();  // every thread has read olist_n
+        if (rest == 0u) break;
+        hi = rest <= (uint32_t)kOrderedCap ? (1ull << 32)
+                                           : (uint64_t)tile_key_select(d, feed, lo, (uint32_t)kOrderedCap, olist) + 1u;
     }
 
     uint32_t f = 0;

@@ -861,7 +861,6 @@ static int32_t check_device_error(WgcDevice_T *d)
     if (err & ERR_W_ZERO) msg += " clip position w == 0 (reference: expect(\"w=0\"));";
     if (err & ERR_CLIP_OVERFLOW) msg += " clipped sub-triangle buffer full;";
     if (err & ERR_RECORD_OVERFLOW) msg += " triangle record buffer full;";
-    if (err & ERR_ORDERED_OVERFLOW) msg += " more than 4096 triangles on one tile in ordered (blend) mode;";
     if (err & ERR_COORD_RANGE) msg += " framebuffer coordinate beyond 2^30;";
     if (err & ERR_PRIM_RANGE) msg += " more than 2^25 primitives in one draw;";
     if (err & ERR_SHADER)

@@ -32,7 +32,7 @@ constexpr int kMaxVar = 4;        // user varying floats of the built-in program
 constexpr int kMaxTileDim = 32;   // tiles are 8x8, 16x16 or 32x32 pixels
 constexpr int kBatch = 64;        // triangles staged in LDS per raster step
 constexpr int kMaxBinTiles = 16;  // larger footprints go to the big-triangle list
-constexpr int kOrderedCap = 4096; // LDS list capacity of the ordered (blend) path
+constexpr int kOrderedCap = 4096; // keys the ordered (blend) path sorts at once (a tile with more takes several chunks)
 constexpr int kClipFloats = 24;   // one clipped sub-triangle: 3 x (xyzw + b0 b1 b2), padded
 
 enum Program : uint32_t {
@@ -76,7 +76,7 @@ enum ErrorBits : uint32_t {
     ERR_W_ZERO = 1u << 2,         // clip position w == 0 (reference: expect("w=0"))
     ERR_CLIP_OVERFLOW = 1u << 3,  // clipped sub-triangle buffer full
     ERR_RECORD_OVERFLOW = 1u << 4,
-    ERR_ORDERED_OVERFLOW = 1u << 5,
+    // 1u << 5 unused (the ordered path has no key limit)
     ERR_COORD_RANGE = 1u << 6,    // framebuffer coordinate >= 2^30
     ERR_PRIM_RANGE = 1u << 7,     // more than 2^25 primitives in a draw
     ERR_SHADER = 1u << 8,         // a shader invocation aborted (not discard): division by zero,
