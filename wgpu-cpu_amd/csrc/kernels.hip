@@ -174,9 +174,13 @@ static hipError_t launch_user_t(const UserKernels &k, const DrawParams &d0, cons
     }
     if (gl.total_prims > 0) {
         uint32_t g = (gl.total_prims + 255u) / 256u;
-        WGC_MLAUNCH("k_setup", k.setup[m], g, 256, &d, &b);
-        uint32_t gc = min(max(g / 2u, 1u), 2048u);
-        WGC_MLAUNCH("k_clip", k.clip[m], gc, 256, &d, &b, &clip_list, &clip_cap);
+        if (!MULTI && gl.inline_clip) {
+            WGC_MLAUNCH("k_setup", k.setup_clip, g, 256, &d, &b);
+        } else {
+            WGC_MLAUNCH("k_setup", k.setup[m], g, 256, &d, &b);
+            uint32_t gc = min(max(g / 2u, 1u), 2048u);
+            WGC_MLAUNCH("k_clip", k.clip[m], gc, 256, &d, &b, &clip_list, &clip_cap);
+        }
     }
     if (gl.ordered) WGC_MLAUNCH("k_raster", k.raster[m], gl.raster_grid, 256, &d, &b);
     else WGC_MLAUNCH("k_tiles", k.tiles[m], gl.raster_grid, 256, &d, &b);

@@ -2374,12 +2374,13 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep
     uint32_t split = split_for(st);
     gl.clip_list = (uint2 *)(base + o_cl);
     gl.clip_cap = (uint32_t)prims;
-    // a single draw of a built-in program whose setup workgroups are all
-    // resident at once clips in setup (no separate clip launch)
+    // a single triangle draw whose setup workgroups are all resident at
+    // once clips in setup (no separate clip launch)
     gl.inline_clip = 0;
-    if (!st.multi && !gl.user && !is_point_or_line(st.draws[0])) {
+    if (!st.multi && !is_point_or_line(st.draws[0])) {
         uint64_t blocks = (prims + 255) / 256;
-        gl.inline_clip = blocks <= (uint64_t)dev->cu_count * setup_clip_blocks_per_cu(gl.program) ? 1u : 0u;
+        uint32_t per_cu = gl.user ? gl.user->occ_setup_clip : setup_clip_blocks_per_cu(gl.program);
+        gl.inline_clip = blocks <= (uint64_t)dev->cu_count * per_cu ? 1u : 0u;
     }
     gl.vis = nullptr;
     b.n = n;

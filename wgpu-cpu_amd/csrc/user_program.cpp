@@ -390,6 +390,7 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
         "wgc::k_lp_visibility<5, 1>",      "wgc::k_lp_visibility<5, 2>",
         "wgc::k_lp_shade<5, 1>",           "wgc::k_lp_shade<5, 2>",
         "wgc::k_lp_ordered<5, 1>",         "wgc::k_lp_ordered<5, 2>",
+        "wgc::k_setup<5, false, true>",
     };
     Compiled c;
     if (!rtc_build(hip_device, src, {{"wgc_user.h", g}}, names, c, err)) return false;
@@ -408,6 +409,8 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
         k.occ_tiles[m] = occupancy(k.tiles[m], 256);
         k.occ_raster[m] = occupancy(k.raster[m], 256);
     }
+    k.setup_clip = c.fns[14];
+    k.occ_setup_clip = occupancy(k.setup_clip, 256);
     return true;
 }
 

@@ -93,7 +93,8 @@ struct UserKernels {
     hipModule_t module = nullptr;
     hipFunction_t setup[2]{}, clip[2]{}, tiles[2]{}, raster[2]{};
     hipFunction_t lp_vis[2]{}, lp_shade[2]{}, lp_ordered[2]{};   // index: points 0, lines 1
-    uint32_t occ_tiles[2]{}, occ_raster[2]{};
+    hipFunction_t setup_clip{};   // single draws: setup that clips in place
+    uint32_t occ_tiles[2]{}, occ_raster[2]{}, occ_setup_clip{};
 };
 
 // Launch group of a WGSL pipeline (16x16 tiles): the shader kernels from
