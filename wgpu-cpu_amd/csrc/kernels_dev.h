@@ -1819,17 +1819,24 @@ __device__ __forceinline__ bool untouched_pass(const BatchArgs &b)
     return b.any_clear != 0u || b.header[H_BIG] != 0u;
 }
 
-// The active-list entries of this workgroup: in every round of G entries
-// (G = the grid, a multiple of 8), XCD x (workgroup i runs on XCD i mod 8)
-// takes the contiguous range [x·G/8, (x+1)·G/8), so that tiles binned close
-// in time share an L2.  The first entry's index does not depend on the
-// active count, so it is read together with the count.
+// The workgroups' share of the active list: round r gives XCD x the
+// entries [r*G + x*G/8, r*G + (x+1)*G/8) (G = the grid, a multiple of 8;
+// workgroup i runs on XCD i mod 8), so that tiles binned close in time
+// (neighbouring triangles of a mesh, the texels they sample) share an L2.
+// Round 0 is static: workgroup j of XCD x takes entry x*G/8 + j, read
+// together with the active count.  When there are more entries than
+// workgroups, the rest of each XCD's share goes out one entry per ticket
+// of that XCD's counter (index_of maps the t-th ticket to its entry), so
+// workgroups that drew heavy tiles take fewer: one same-address atomic
+// per item, eight addresses, spread over the kernel.
 struct ActiveSchedule {
-    uint32_t first, stride, n, e0;
+    uint32_t first, stride, per, xcd, n, e0;
     __device__ ActiveSchedule(const BatchArgs &b)
     {
         stride = gridDim.x;
-        first = (blockIdx.x & 7u) * (stride >> 3) + (blockIdx.x >> 3);
+        per = stride >> 3;
+        xcd = blockIdx.x & 7u;
+        first = xcd * per + (blockIdx.x >> 3);
         // the list holds kMaxItems entries per tile (a group has at least
         // one tile); an entry at or beyond the count is not used
         uint32_t cap = b.total_tiles * kMaxItems;
@@ -1838,9 +1845,14 @@ struct ActiveSchedule {
         asm volatile("" : : "s"(count), "s"(e0));  // both loads in flight together
         n = min(count, cap);
     }
-    // The entry of `item` as loaded (0 past the end): the loops read each
-    // item's entry one item ahead and take it into scalar registers when
-    // they reach it, so the load overlaps the tile before.
+    __device__ bool dynamic() const { return n > stride; }
+    // The entry of this XCD's share that ticket t (counted after round 0) names.
+    __device__ uint32_t index_of(uint32_t t) const
+    {
+        uint32_t u = per + t;
+        return (u / per) * stride + xcd * per + u % per;
+    }
+    __device__ uint32_t *queue(const BatchArgs &b) const { return b.header + H_QUEUE + kQueueStride * xcd; }
     __device__ uint32_t load(const BatchArgs &b, uint32_t item) const
     {
         return item < n ? b.active_list[item] : 0u;
@@ -1891,18 +1903,28 @@ __global__ __launch_bounds__(NT) void k_raster(DrawParams d0, BatchArgs b)
     __shared__ RasterShared<TS> sh;
     __shared__ uint32_t olist[kOrderedCap];
     __shared__ uint32_t olist_n;
+    __shared__ uint32_t ticket;
     __shared__ uint32_t wave_frags[NT / 64];
     __shared__ uint32_t kinds[kClearChunk];
 
     ActiveSchedule w(b);
     uint32_t frag_acc = 0;
+    bool dyn = w.dynamic();
     uint32_t raw = w.e0;
-    for (uint32_t item = w.first; item < w.n; item += w.stride) {
+    for (uint32_t item = w.first; item < w.n;) {
         uint32_t e = __builtin_amdgcn_readfirstlane(raw);
-        raw = w.load(b, item + w.stride);
+        // the ticket of this workgroup's next item, taken while this one is
+        // processed (thread 0 takes it, the workgroup reads it after)
+        uint32_t tk = 0u;
+        if (dyn && threadIdx.x == 0) tk = atomicAdd(w.queue(b), 1u);
         uint32_t di, tile, it;
         locate_tile<MULTI>(d0, b, e, di, tile, it);
         raster_tile<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, sh, olist, olist_n, frag_acc);
+        if (!dyn) break;
+        if (threadIdx.x == 0) ticket = tk;
+        __syncthreads();
+        item = w.index_of(ticket);
+        raw = w.load(b, item);
     }
     // workgroups late in the schedule have the fewest active tiles: chunks from there
     if (untouched_pass(b))
@@ -1923,6 +1945,7 @@ __global__ __launch_bounds__(NT, (TilesOccupancy<TS, NT>::value)) void k_tiles(D
     static_assert(NT >= kBatch && PPT >= 1 && TS * TS == PPT * NT, "tile shape");
     static_assert(VisShared<TS, NT>::NB <= 256 && TS <= 32, "owner entries hold 8-bit triangle and row indices");
     __shared__ VisShared<TS, NT> sh;
+    __shared__ uint32_t ticket;
     __shared__ uint32_t wave_frags[NT / 64];
     __shared__ uint32_t kinds[kClearChunk];
     __shared__ float srgb[256];   // the sRGB decode table (one device table for every draw)
@@ -1937,13 +1960,22 @@ __global__ __launch_bounds__(NT, (TilesOccupancy<TS, NT>::value)) void k_tiles(D
     }
     ActiveSchedule w(b);
     uint32_t frag_acc = 0;
+    bool dyn = w.dynamic();
     uint32_t raw = w.e0;
-    for (uint32_t item = w.first; item < w.n; item += w.stride) {
+    for (uint32_t item = w.first; item < w.n;) {
         uint32_t e = __builtin_amdgcn_readfirstlane(raw);
-        raw = w.load(b, item + w.stride);
+        // the ticket of this workgroup's next item, taken while this one is
+        // processed (thread 0 takes it, the workgroup reads it after)
+        uint32_t tk = 0u;
+        if (dyn && threadIdx.x == 0) tk = atomicAdd(w.queue(b), 1u);
         uint32_t di, tile, it;
         locate_tile<MULTI>(d0, b, e, di, tile, it);
         raster_tile_oi<PROG, TS, NT>(draw_at<MULTI>(d0, b, di), tile, it, sh, srgb, frag_acc);
+        if (!dyn) break;
+        if (threadIdx.x == 0) ticket = tk;
+        __syncthreads();
+        item = w.index_of(ticket);
+        raw = w.load(b, item);
     }
     // workgroups late in the schedule have the fewest active tiles: chunks from there
     if (untouched_pass(b))

@@ -149,8 +149,11 @@ constexpr uint32_t kItemShift = 24;
 enum Counter { C_BIG = 0, C_CLIP_TRI = 1, C_BIG_NX0 = 2, C_BIG_NY0 = 3, C_BIG_X1 = 4, C_BIG_Y1 = 5, C_NUM = 8 };
 
 // Group header words: active tiles, primitives to clip, spilled bin
-// entries, big triangles of every draw of the group.
-enum Header { H_ACTIVE = 0, H_CLIP = 1, H_SPILL = 2, H_BIG = 3, H_NUM = 4 };
+// entries, big triangles of every draw of the group; then the tile
+// kernels' work queues, one ticket counter per XCD, 128 bytes apart
+// (H_QUEUE + kQueueStride * xcd).
+constexpr uint32_t kQueueStride = 32;
+enum Header { H_ACTIVE = 0, H_CLIP = 1, H_SPILL = 2, H_BIG = 3, H_QUEUE = 32, H_NUM = 32 + 8 * 32 };
 
 // Triangle record: everything the raster needs about one (sub-)triangle,
 // computed once in setup (the same values the raster would compute).
