@@ -181,11 +181,10 @@ __device__ __forceinline__ void activate_item(const DrawParams &d, uint32_t at, 
 
 // Bins every lane's record (with `has`) into the tiles of its footprint
 // (row-major, at most kMaxBinTiles).  For the k-th tile of every lane's
-// footprint, lanes on the same tile form a group first (up to 8 groups per
-// round, in registers: neighbouring triangles of a mesh mostly share tiles,
-// and one tile may collect thousands of them); every group's first lane,
-// and every lane left alone, takes the group's bin positions with one atomic
-// add.  All the adds of the wave are issued before any result is used, so
+// footprint, runs of consecutive lanes on the same tile form a group
+// (consecutive primitives of a mesh mostly share tiles, and one tile may
+// collect thousands of them); every run's first lane takes the run's bin
+// positions with one atomic add.  All the adds of the wave are issued before any result is used, so
 // the wave waits for them once.  Then the keys go to their positions, the
 // entries past a full bin to the spill list and the work items the taken
 // positions start (position 0 of a tile, and every item's first position
@@ -219,19 +218,16 @@ __device__ __forceinline__ void bin_wave(const DrawParams &d, bool has, uint32_t
         // the global tile identifies the counter in a group (the draws'
         // tile ranges do not overlap)
         uint32_t gt = ANY_DRAW ? (act ? d.s.tile0 + t : 0xFFFFFFFFu) : t;
-        uint32_t lead = lane, cnt = 1u, rank = 0u;
-        for (int it = 0; it < 8 && pending; ++it) {
-            int leader = __ffsll((unsigned long long)pending) - 1;
-            uint32_t tl = lane_value(gt, leader);
-            bool mine = act && gt == tl;
-            uint64_t m = __ballot(mine);
-            if (mine) {
-                lead = (uint32_t)leader;
-                rank = (uint32_t)__popcll(m & lt);
-                cnt = (uint32_t)__popcll(m);
-            }
-            pending &= ~m;
-        }
+        // runs of consecutive lanes on the same counter (inactive lanes carry
+        // an id no tile has, so they end every run): one shuffle and one
+        // ballot, no loop
+        uint32_t prev = (uint32_t)__shfl_up((int)gt, 1);
+        uint64_t bounds = __ballot(lane == 0u || gt != prev);
+        uint64_t le = lt | (1ull << lane);
+        uint32_t start = 63u - (uint32_t)__clzll(bounds & le);
+        uint64_t up = bounds & ~le;
+        uint32_t end = up ? (uint32_t)(__ffsll((unsigned long long)up) - 1) : 64u;
+        uint32_t lead = start, cnt = end - start, rank = lane - start;
         if (act && lead == lane) got[k] = atomicAdd(&d.s.tile_count[t], cnt);
         grp[k] = act ? ((1u << 16) | (rank << 8) | lead) : 0u;
     }
