@@ -2294,17 +2294,17 @@ static size_t keep_bytes(const Step &st)
     return align_up(8ull * tile_px * group_tiles(st), 256);
 }
 
-// Bin entries per work item of a tile: two visibility batches of the
-// group's tile shape, so that a crowded tile (three batches or more) spreads
-// over several workgroups while ordinary ones stay whole (a single pass
-// already has about one tile per resident workgroup).  Single
-// order-independent draws only: ordered tiles are never split, and a
-// batched group has many tiles per workgroup anyway.
+// Bin entries per work item of a tile: one visibility batch of the group's
+// tile shape, so that a tile of two batches or more spreads over several
+// workgroups (a single pass has about one tile per resident workgroup, and
+// its kernel lasts as long as its slowest tile) while ordinary ones stay
+// whole.  Single order-independent draws only: ordered tiles are never
+// split, and a batched group has many tiles per workgroup anyway.
 static uint32_t split_for(const Step &st)
 {
     if (st.gl.ordered || st.multi) return 0;
     uint32_t cfg = st.gl.user ? (uint32_t)RASTER_16x16_256 : st.gl.raster_cfg;
-    return 2u * vis_batch_size(1u << raster_tile_shift(cfg), raster_threads(cfg));
+    return vis_batch_size(1u << raster_tile_shift(cfg), raster_threads(cfg));
 }
 
 static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep, uint32_t *frag_px,
