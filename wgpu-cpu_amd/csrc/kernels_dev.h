@@ -1240,11 +1240,31 @@ __device__ __forceinline__ uint32_t tile_key_select(const DrawParams &d, const T
 }
 
 // Sorts olist[0..total) (total <= kOrderedCap) into ascending order: a
-// bitonic sort padded with UINT32_MAX to a power of two.
+// bitonic sort padded with UINT32_MAX to a power of two.  Up to 64 keys
+// (most tiles) are sorted by one wave in registers, the network running over
+// its lanes with shuffles and no barrier between steps; longer lists by the
+// workgroup in LDS.
 template <int NT>
 __device__ __forceinline__ void sort_olist(uint32_t *olist, uint32_t total)
 {
     uint32_t tid = threadIdx.x;
+    if (total <= 64u) {
+        if (tid < 64u) {
+            uint32_t v = tid < total ? olist[tid] : 0xFFFFFFFFu;
+#pragma unroll
+            for (uint32_t k = 2; k <= 64u; k <<= 1) {
+#pragma unroll
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    uint32_t o = (uint32_t)__shfl_xor((int)v, (int)j);
+                    bool up = (tid & k) == 0u, lower = (tid & j) == 0u;
+                    v = lower == up ? min(v, o) : max(v, o);
+                }
+            }
+            if (tid < total) olist[tid] = v;
+        }
+        __syncthreads();
+        return;
+    }
     uint32_t P = 1u;
     while (P < total) P <<= 1;
     for (uint32_t i = total + tid; i < P; i += NT) olist[i] = 0xFFFFFFFFu;
