@@ -146,8 +146,8 @@ def main():
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration for profiles)")
     ap.add_argument("--profile-batch", type=int, default=0, metavar="N",
                     help="only run N submissions of the batched workload (for counter profiles)")
-    ap.add_argument("--raster-tile", type=int, default=0, choices=range(7),
-                    help="raster tile shape of --profile-batch (wgc.RASTER_TILES: 6 is 32x32 tiles of 512 threads)")
+    ap.add_argument("--raster-tile", type=int, default=0, choices=range(8),
+                    help="raster tile shape of --profile-batch (wgc.RASTER_TILES: 6 is 32x32 tiles of 512 threads, 7 of 64)")
     args = ap.parse_args()
 
     if args.calibrate_copy:

@@ -423,8 +423,9 @@ int32_t wgc_device_totals(WgcDevice device, uint64_t *fragments, uint64_t *submi
  *      pass at a time on one thread, engine.rs:26-46) ---- */
 typedef enum WgcOption {
     /* raster tile shape: 0 = 16x16 px / 256 lanes, 1 = 8x8 / 64, 2 = 16x16 / 64,
-     * 3 = 32x32 / 256, 4 = 16x16 / 128, 5 = 32x32 / 128, 6 = 32x32 / 512 (the
-     * autotuner's search space) */
+     * 3 = 32x32 / 256, 4 = 16x16 / 128, 5 = 32x32 / 128, 6 = 32x32 / 512,
+     * 7 = 32x32 / 64 (the autotuner's search space: tiles of 8, 16 and 32
+     * pixels with 1, 2, 4 or 8 waves) */
     WGC_OPTION_RASTER_TILE = 1,
     /* 1 (default): consecutive independent passes with one draw each share
      * their launches (the batched form); 0: one launch set per draw */

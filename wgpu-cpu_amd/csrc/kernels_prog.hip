@@ -29,6 +29,7 @@ static hipError_t launch_raster(const DrawParams &d, const BatchArgs &b, const G
     case RASTER_16x16_128: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 16, 128>), gl.raster_grid, 128, d, b); break;
     case RASTER_32x32_128: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 32, 128>), gl.raster_grid, 128, d, b); break;
     case RASTER_32x32_512: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 32, 512>), gl.raster_grid, 512, d, b); break;
+    case RASTER_32x32_64: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 32, 64>), gl.raster_grid, 64, d, b); break;
     default: WGC_LAUNCH("k_tiles", (k_tiles<PROG, MULTI, 16, 256>), gl.raster_grid, 256, d, b); break;
     }
     return hipSuccess;
@@ -68,6 +69,7 @@ uint32_t raster_blocks_per_cu_of(bool ordered, uint32_t cfg)
     case RASTER_16x16_128: return occupancy_tiles<PROG, MULTI, 16, 128>();
     case RASTER_32x32_128: return occupancy_tiles<PROG, MULTI, 32, 128>();
     case RASTER_32x32_512: return occupancy_tiles<PROG, MULTI, 32, 512>();
+    case RASTER_32x32_64: return occupancy_tiles<PROG, MULTI, 32, 64>();
     default: return occupancy_tiles<PROG, MULTI, 16, 256>();
     }
 }

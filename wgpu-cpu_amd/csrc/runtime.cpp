@@ -2210,7 +2210,7 @@ static int32_t build_draw(WgcDevice_T *dev, const PassCommand &pc, const PassSta
     d.width = w;
     d.height = h;
     // 16x16 tiles for the ordered (sorted, sequential) raster, whatever the
-    // raster tile option: of the seven shapes it is the fastest for it (200k
+    // raster tile option: of the tile shapes it is the fastest for it (200k
     // blended triangles over 2560x2440: 0.63 ms, the others 0.74-3.0 ms) and
     // holds the most keys per pixel; WGSL pipelines' kernels are compiled for
     // 16x16 tiles only

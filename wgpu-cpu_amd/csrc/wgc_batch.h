@@ -41,14 +41,15 @@ enum RasterCfg : uint32_t {
     RASTER_16x16_128 = 4,
     RASTER_32x32_128 = 5,
     RASTER_32x32_512 = 6,
-    RASTER_NUM_CFG = 7,
+    RASTER_32x32_64 = 7,
+    RASTER_NUM_CFG = 8,
 };
 
 // Threads per workgroup of a tile shape.
 __host__ __device__ inline uint32_t raster_threads(uint32_t cfg)
 {
     switch (cfg) {
-    case RASTER_8x8_64: case RASTER_16x16_64: return 64u;
+    case RASTER_8x8_64: case RASTER_16x16_64: case RASTER_32x32_64: return 64u;
     case RASTER_16x16_128: case RASTER_32x32_128: return 128u;
     case RASTER_32x32_512: return 512u;
     default: return 256u;
@@ -67,8 +68,9 @@ __host__ __device__ constexpr uint32_t vis_batch_size(uint32_t ts, uint32_t nt)
 __host__ __device__ inline uint32_t raster_tile_shift(uint32_t cfg)
 {
     return cfg == RASTER_8x8_64 ? 3u
-         : (cfg == RASTER_32x32_256 || cfg == RASTER_32x32_128 || cfg == RASTER_32x32_512) ? 5u
-                                                                                           : 4u;
+         : (cfg == RASTER_32x32_256 || cfg == RASTER_32x32_128 || cfg == RASTER_32x32_512 || cfg == RASTER_32x32_64)
+               ? 5u
+               : 4u;
 }
 
 }  // namespace wgc

@@ -92,7 +92,8 @@ OPTION_RASTER_TILE = 1
 OPTION_BATCHING = 2
 OPTION_GRAPHS = 3
 # raster tile shapes (tile side, lanes per workgroup)
-RASTER_TILES = {0: (16, 256), 1: (8, 64), 2: (16, 64), 3: (32, 256), 4: (16, 128), 5: (32, 128), 6: (32, 512)}
+RASTER_TILES = {0: (16, 256), 1: (8, 64), 2: (16, 64), 3: (32, 256), 4: (16, 128), 5: (32, 128), 6: (32, 512),
+                7: (32, 64)}
 
 BINDING_BUFFER = 1
 BINDING_TEXTURE = 2
