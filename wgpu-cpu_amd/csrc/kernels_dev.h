@@ -342,8 +342,8 @@ __device__ __forceinline__ void note_big_wave(const DrawParams &d, bool big, con
     }
 }
 
-template <int PROG, class DrawOf>
-__device__ __forceinline__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act, DrawOf &&draw_of);
+template <int PROG, bool ANY_DRAW, class DrawOf>
+__device__ __forceinline__ void clip_wave(uint32_t di, uint32_t p, bool act, DrawOf &&draw_of);
 
 // The LoadOp::Clear values into the whole target, spread over the setup
 // grid (16-byte stores; issued before the setup work, they drain while it
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams d0, BatchArgs b)
     if constexpr (CLIP) {
         // every lane of the wave takes part (the clip slots are reserved per wave)
         if (__ballot(needs_clip))
-            clip_primitive<PROG>(d, di, p, needs_clip, [&](uint32_t) -> const DrawParams & { return d; });
+            clip_wave<PROG, false>(di, p, needs_clip, [&](uint32_t) -> const DrawParams & { return d; });
     }
 }
 
@@ -476,176 +476,194 @@ __device__ __forceinline__ void pick(CTri &t, const CTri out[2], uint32_t c)
     }
 }
 
-// The clipper's output in the order of the reference's plane-by-plane queue
-// (clipper.rs:756-775), which is lexicographic in the split choices (plane 0
-// most significant): a depth-first walk that keeps the triangle entering each
-// plane in registers (levels are indexed only by unrolled loops, so nothing
-// goes to scratch memory).  Every node of the split tree is clipped once,
-// plus once more when its second piece is taken.
-template <class F>
-__device__ int clip_walk(const CTri &root, F &&emit)
+// The clipper's output (clipper.rs:625-775) by split paths.  The reference
+// clips a triangle against the six planes in turn, each plane turning every
+// triangle of its queue into zero, one or two; its output order is
+// lexicographic in the choices of piece at the levels where a triangle was
+// split in two (plane 0 first).  Here a lane follows one path: it clips its
+// triangle against the six planes in order and at each split takes the
+// piece its next choice bit names (the first bit at the first split), with
+// `bits` choices available.  The lanes of a group that take the choices 0
+// .. 2^bits - 1 in rank order then hold every sub-triangle with at most
+// `bits` splits on its path, in the clipper's order: two paths part at a
+// level where both split after the same splits, so the order of the choice
+// bits is the order of the paths.  A lane whose path dies (a plane leaves
+// nothing) or does not exist (a choice of the second piece of a triangle
+// that was not split, or a choice bit its path never used set) holds
+// nothing; `over` marks a path that splits more often than `bits` allows.
+// Each sub-triangle is computed by the same plane clips as the reference's.
+struct ClipPath {
+    CTri t;
+    bool valid, over;
+};
+
+__device__ __forceinline__ ClipPath clip_path(const CTri &root, uint32_t rank, uint32_t bits, bool act)
 {
-    CTri t[7];
-    uint32_t nn[6], ch[6];
-    t[0] = root;
+    ClipPath r;
+    r.t = root;
+    r.valid = act;
+    r.over = false;
+    uint32_t used = 0;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) { nn[k] = 0u; ch[k] = 0u; }
-    int leaf = 0;
-    int from = 0;  // first level to (re)compute, its choice already set in ch
-    for (;;) {
-        int dead = 6;  // level whose choice has no piece, 6 = reached a leaf
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            if (k >= from && dead == 6) {
-                CTri out[2];
-                nn[k] = (uint32_t)clip_against_plane(t[k], k, out);
-                if (ch[k] >= nn[k]) dead = k;
-                else pick(t[k + 1], out, ch[k]);
-            }
+    for (int k = 0; k < 6; ++k) {
+        CTri out[2];
+        uint32_t nn = (uint32_t)clip_against_plane(r.t, k, out);
+        uint32_t ch = 0u;
+        if (nn == 2u) {
+            if (used < bits) ch = (rank >> (bits - 1u - used)) & 1u;
+            else if (r.valid) r.over = true;
+            ++used;
         }
-        if (dead == 6) {
-            emit(t[6], leaf);
-            ++leaf;
-        }
-        // backtrack: the deepest level above the end of this path whose first
-        // piece was taken and that has a second piece
-        int next = -1;
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-            if (k < dead && ch[k] == 0u && nn[k] == 2u) next = k;
-        if (next < 0) break;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            if (k == next) ch[k] = 1u;
-            if (k > next) ch[k] = 0u;
-        }
-        from = next;
+        if (nn == 0u) r.valid = false;
+        else pick(r.t, out, ch);
     }
-    return leaf;
+    // choice bits past the path's splits must be 0 (one lane per path)
+    uint32_t spare = used >= bits ? 0u : bits - used;
+    r.valid = r.valid && !r.over && (rank & ((1u << spare) - 1u)) == 0u;
+    return r;
 }
 
-// One primitive that crosses a clip plane: the clipper's sub-triangles, in
-// its order, become clip records (positions and clip barycentrics) and
-// triangle records; each leaves a bin job (key and tile footprint) at its
-// clip slot.  Then the wave bins all its lanes' sub-triangles together, 64
-// per round, like k_setup's primitives: one atomic per distinct tile and
-// group of lanes, every add of a round issued before any result is used
-// (rather than two dependent round trips per sub-triangle and lane).  Every
-// lane of the wave calls it (with `act` false for lanes without a
-// primitive), so that the clip slots are taken with one atomic per draw and
-// wave.  `draw_of(di)` is draw di of the group: the lanes of a wave may clip
-// primitives of different draws in the batched form.
-template <int PROG, class DrawOf>
-__device__ __forceinline__ void clip_primitive(const DrawParams &d, uint32_t di, uint32_t p, bool act, DrawOf &&draw_of)
+// Primitives that cross a clip plane, clipped by one wave: the lanes with
+// `act` hold them (draw di, primitive p).  Eight at a time, each by a group
+// of eight lanes following its paths of up to three splits (clip_path); a
+// primitive with a path of more splits is clipped again by the whole wave
+// (64 lanes, six choice bits: every path).  A primitive's sub-triangles, in
+// the clipper's order, become clip records (positions and clip
+// barycentrics) and triangle records at consecutive clip slots, and are
+// binned like k_setup's primitives, every lane of the wave together.  Every
+// lane of the wave calls it.  ANY_DRAW: the primitives may belong to
+// different draws of a batched group; `draw_of(di)` is draw di.
+template <int PROG, bool ANY_DRAW, class DrawOf>
+__device__ __forceinline__ void clip_wave(uint32_t di, uint32_t p, bool act, DrawOf &&draw_of)
 {
-    VsOut v[3];
-    CTri root;
-    uint32_t nleaf = 0;
-    bool ff = false;
-    if (act && shade_primitive<PROG>(d, p, v)) {
-        if constexpr (PROG == PROG_USER) {
-            // @builtin(front_facing) of the sub-triangles (only a WGSL
-            // fragment shader can read it)
-            bool culled;
-            (void)face_and_cull(d, v, ff, culled);
-        }
+    uint32_t lane = lane_id();
+    uint64_t lt = lanes_below();
+    uint64_t todo = __ballot(act), redo = 0u;
+    while (todo | redo) {
+        // the groups of this round and the lane holding each one's primitive
+        uint32_t bits, gshift;
+        int holder = -1;
+        if (todo) {
+            bits = 3u;
+            gshift = 3u;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) root.p[k][c] = v[k].clip[c];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) root.b[k][c] = (c == k) ? 1.0f : 0.0f;
-        }
-        nleaf = (uint32_t)clip_walk(root, [](const CTri &, int) {});
-        if (nleaf > 63u) {  // the record key holds 6 bits of sub-triangle index
-            atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
-            nleaf = 0;
-        }
-    }
-    uint32_t base = wave_add(&d.s.counters[C_CLIP_TRI], nleaf, act);
-    bool fits = nleaf != 0u && base + nleaf <= d.s.clip_tri_cap;
-    if (nleaf != 0u && !fits) atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
-    uint32_t mine = fits ? nleaf : 0u;
-    uint64_t owners = __ballot(mine != 0u);
-    if (!owners) return;
-    if (fits) {
-        d.s.clip_base[p] = base;
-        clip_walk(root, [&](const CTri &t, int leaf) {
-            uint32_t slot = base + (uint32_t)leaf;
-            float *o = d.s.clip_tris + (size_t)slot * kClipFloats;
-            for (int k = 0; k < 3; ++k) {
-                for (int c = 0; c < 4; ++c) o[k * 7 + c] = t.p[k][c];
-                for (int c = 0; c < 3; ++c) o[k * 7 + 4 + c] = t.b[k][c];
+            for (uint32_t q = 0; q < 8u; ++q) {
+                if (!todo) break;
+                int h = __ffsll((unsigned long long)todo) - 1;
+                todo &= todo - 1u;
+                if ((lane >> 3) == q) holder = h;
             }
-            uint4 job = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            bits = 6u;
+            gshift = 6u;
+            holder = __ffsll((unsigned long long)redo) - 1;
+            redo &= redo - 1u;
+        }
+        uint32_t rank = lane & ((1u << gshift) - 1u);
+        uint32_t first = lane - rank;   // the group's first lane
+        uint64_t gmask = gshift == 6u ? ~0ull : (0xFFull << first);
+        bool has = holder >= 0;
+        uint32_t gdi = (uint32_t)__shfl((int)di, has ? holder : 0);
+        uint32_t gp = (uint32_t)__shfl((int)p, has ? holder : 0);
+        const DrawParams &d = draw_of(gdi);
+
+        VsOut v[3];
+        CTri root{};
+        bool ff = false;
+        bool ok = has && shade_primitive<PROG>(d, gp, v);
+        if (ok) {
+            if constexpr (PROG == PROG_USER) {
+                // @builtin(front_facing) of the sub-triangles (only a WGSL
+                // fragment shader can read it)
+                bool culled;
+                (void)face_and_cull(d, v, ff, culled);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) root.p[k][c] = v[k].clip[c];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) root.b[k][c] = (c == k) ? 1.0f : 0.0f;
+            }
+        }
+        ClipPath path = clip_path(root, rank, bits, ok);
+        uint64_t overs = __ballot(path.over);
+        bool group_over = (overs & gmask) != 0u;
+        // primitives with a path of more splits go round again (by their holder lane)
+        uint64_t again = __ballot(rank == 0u && group_over && has);
+        for (uint64_t m = again; m; m &= m - 1u) {
+            int l = __ffsll((unsigned long long)m) - 1;
+            redo |= 1ull << lane_value((uint32_t)holder, l);
+        }
+        bool valid = path.valid && !group_over;
+        uint64_t leaves = __ballot(valid);
+        uint32_t nleaf = (uint32_t)__popcll(leaves & gmask);
+        uint32_t leaf = (uint32_t)__popcll(leaves & gmask & lt);
+        bool lead = rank == 0u && has && !group_over;
+        if (lead && nleaf > 63u) {   // the record key holds 6 bits of sub-triangle index
+            atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
+        }
+        bool take = lead && nleaf != 0u && nleaf <= 63u;
+        uint32_t base = wave_add(&d.s.counters[C_CLIP_TRI], take ? nleaf : 0u, take);
+        bool fits = take && base + nleaf <= d.s.clip_tri_cap;
+        if (take && !fits) atomicOr(d.s.error, (uint32_t)ERR_CLIP_OVERFLOW);
+        if (fits) d.s.clip_base[gp] = base;
+        // the group's slot range, from its first lane
+        base = (uint32_t)__shfl((int)base, (int)first);
+        bool gfits = __shfl((int)(fits ? 1 : 0), (int)first) != 0;
+        bool emit = valid && gfits;
+
+        bool bin = false;
+        uint32_t key = 0u, tx0 = 0u, ty0 = 0u, tx1 = 0u, ty1 = 0u;
+        if (emit) {
+            uint32_t slot = base + leaf;
+            float *o = d.s.clip_tris + (size_t)slot * kClipFloats;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) o[k * 7 + c] = path.t.p[k][c];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) o[k * 7 + 4 + c] = path.t.b[k][c];
+            }
             RasterTri r;
-            raster_setup(d, t.p, r);
+            raster_setup(d, path.t.p, r);
             ScanTri s = scan_setup(r);
-            uint32_t tx0, ty0, tx1, ty1;
             if (!coord_ok(r)) {
                 atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE);
             } else if (r.total != 0.0f && tile_bbox(d, s, tx0, ty0, tx1, ty1)) {
-                uint32_t key = (p << 7) | ((uint32_t)leaf << 1) | 1u;
+                key = (gp << 7) | (leaf << 1) | 1u;
                 write_record(d.s.clip_tri + (size_t)slot * kRecWords, r, v, key, ff);
-                job = make_uint4(key, tx0 | (ty0 << 16), tx1 | (ty1 << 16), 1u);
+                bin = true;
             }
-            d.s.clip_job[slot] = job;
-        });
-    }
-    // the jobs are read back by other lanes of the wave
-    __threadfence_block();
-    // the wave's sub-triangles in lane order: lane l's are [pre_l, pre_l + mine_l)
-    uint32_t incl = wave_inclusive_sum(mine);
-    uint32_t pre = incl - mine;
-    uint32_t total = lane_value(incl, 63);
-    uint32_t lane = lane_id();
-    for (uint32_t j0 = 0; j0 < total; j0 += 64u) {
-        uint32_t f = j0 + lane;
-        bool has = f < total;
-        // the owner: the last lane with sub-triangles whose range starts at
-        // or before f (wave-uniform loop over those lanes)
-        int owner = 0;
-        for (uint64_t m = owners; m; m &= m - 1u) {
-            int l = __ffsll((unsigned long long)m) - 1;
-            if (lane_value(pre, l) <= f) owner = l;
         }
-        uint32_t odi = (uint32_t)__shfl((int)di, owner);
-        uint32_t obase = (uint32_t)__shfl((int)base, owner);
-        uint32_t opre = (uint32_t)__shfl((int)pre, owner);
-        const DrawParams &od = draw_of(odi);
-        uint4 job = make_uint4(0u, 0u, 0u, 0u);
-        if (has) job = od.s.clip_job[obase + (f - opre)];
-        has = has && job.w != 0u;
-        uint32_t tx0 = job.y & 0xFFFFu, ty0 = job.y >> 16, tx1 = job.z & 0xFFFFu, ty1 = job.z >> 16;
-        uint32_t nt = has ? (tx1 - tx0 + 1u) * (ty1 - ty0 + 1u) : 0u;
-        bool small = has && nt <= (uint32_t)kMaxBinTiles;
-        bool big = has && !small;
-        bin_wave<true>(od, small, job.x, tx0, ty0, tx1, ty1);
+        uint32_t nt = bin ? (tx1 - tx0 + 1u) * (ty1 - ty0 + 1u) : 0u;
+        bool small = bin && nt <= (uint32_t)kMaxBinTiles;
+        bool big = bin && !small;
+        bin_wave<ANY_DRAW>(d, small, key, tx0, ty0, tx1, ty1);
         if (__ballot(big)) {
-            uint32_t bs = wave_add(&od.s.counters[C_BIG], 1u, big);
+            uint32_t bs = wave_add(&d.s.counters[C_BIG], 1u, big);
             if (big) {
-                if (bs < od.s.big_cap) {
-                    od.s.big[bs] = make_uint4(job.x, job.y, job.z, odi);
-                    atomicOr(&od.s.header[H_BIG], 1u);
-                    atomicMax(&od.s.counters[C_BIG_NX0], 65535u - tx0);
-                    atomicMax(&od.s.counters[C_BIG_NY0], 65535u - ty0);
-                    atomicMax(&od.s.counters[C_BIG_X1], tx1);
-                    atomicMax(&od.s.counters[C_BIG_Y1], ty1);
+                if (bs < d.s.big_cap) {
+                    d.s.big[bs] = make_uint4(key, tx0 | (ty0 << 16), tx1 | (ty1 << 16), gdi);
+                    atomicOr(&d.s.header[H_BIG], 1u);
+                    atomicMax(&d.s.counters[C_BIG_NX0], 65535u - tx0);
+                    atomicMax(&d.s.counters[C_BIG_NY0], 65535u - ty0);
+                    atomicMax(&d.s.counters[C_BIG_X1], tx1);
+                    atomicMax(&d.s.counters[C_BIG_Y1], ty1);
                 } else {
-                    atomicOr(od.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
+                    atomicOr(d.s.error, (uint32_t)ERR_RECORD_OVERFLOW);
                 }
             }
         }
     }
 }
 
-// Primitives of the clip list per wave: the clipper's walk is a long serial
-// chain per lane, so few lanes per wave spread the list over many waves
-// (each wave's walk takes as long as its longest lane's).
+// Primitives of the clip list per wave: one round of clip_wave's groups of
+// eight lanes.
 constexpr uint32_t kClipPerWave = 8;
 
-// Every lane runs the same number of iterations, so that the wave-wide slot
-// reservations in clip_primitive see the whole wave.
+// Every lane runs the same number of iterations, so that clip_wave's
+// wave-wide steps see the whole wave.
 template <int PROG, bool MULTI>
 __global__ __launch_bounds__(256) void k_clip(DrawParams d0, BatchArgs b, const uint2 *clip_list, uint32_t cap)
 {
@@ -656,8 +674,8 @@ __global__ __launch_bounds__(256) void k_clip(DrawParams d0, BatchArgs b, const 
         uint32_t i = w0 + lane_id();
         bool act = lane_id() < kClipPerWave && i < n;
         uint2 e = act ? clip_list[i] : make_uint2(0u, 0u);
-        clip_primitive<PROG>(draw_at<MULTI>(d0, b, e.x), e.x, e.y, act,
-                             [&](uint32_t di) -> const DrawParams & { return draw_at<MULTI>(d0, b, di); });
+        clip_wave<PROG, MULTI>(e.x, e.y, act,
+                               [&](uint32_t di) -> const DrawParams & { return draw_at<MULTI>(d0, b, di); });
     }
 }
 

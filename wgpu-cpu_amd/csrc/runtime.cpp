@@ -2349,7 +2349,7 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep
     size_t o_al = take(4 * total_tiles * kMaxItems);
     size_t o_bins = take(4 * bins_total), o_sp = take(8 * spill_cap);
     size_t o_cl = take(8 * prims);
-    std::vector<size_t> o_big(n), o_tri(n), o_ctri(n), o_ct(n), o_cj(n), o_cb(n), o_ss(n);
+    std::vector<size_t> o_big(n), o_tri(n), o_ctri(n), o_ct(n), o_cb(n), o_ss(n);
     for (uint32_t i = 0; i < n; ++i) {
         const DrawParams &d = st.draws[i];
         uint64_t rc = d.total_prims + clip_cap[i];
@@ -2358,7 +2358,6 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep
         o_tri[i] = take(4ull * kRecWords * d.total_prims);
         o_ctri[i] = take(4ull * kRecWords * clip_cap[i]);
         o_ct[i] = take(4ull * kClipFloats * clip_cap[i]);
-        o_cj[i] = take(16ull * clip_cap[i]);
         o_cb[i] = take(4ull * d.total_prims);
     }
     if (!base) return off;
@@ -2441,7 +2440,6 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep
         s.tri = (uint32_t *)(base + o_tri[i]);
         s.clip_tri = (uint32_t *)(base + o_ctri[i]);
         s.clip_tris = (float *)(base + o_ct[i]);
-        s.clip_job = (uint4 *)(base + o_cj[i]);
         s.clip_base = (uint32_t *)(base + o_cb[i]);
         s.strip_sep = restart_strip(d) ? (int32_t *)(base + o_ss[i]) : nullptr;
         s.counters = gl.counters + C_NUM * i;

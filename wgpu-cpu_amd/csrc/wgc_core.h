@@ -117,7 +117,6 @@ struct Scratch {
     uint32_t *tri;           // triangle records of unclipped primitives (kRecWords each, by primitive)
     uint32_t *clip_tri;      // triangle records of clipped sub-triangles (by clip slot)
     float *clip_tris;        // clip_tri_cap x kClipFloats: positions and clip barycentrics
-    uint4 *clip_job;         // clip_tri_cap: the sub-triangle's key and tile footprint (w = 1: to bin)
     uint32_t *clip_base;     // per primitive: first sub-triangle slot
     uint32_t *counters;      // see Counter
     unsigned long long *frags;  // kFragSlots fragment counters of the submission (summed on read)
