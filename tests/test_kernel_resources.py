@@ -57,12 +57,17 @@ def test_kernels_keep_their_state_in_registers(tmp_path):
 
 
 @pytest.mark.skipif(not objects() or not (LLVM / "llvm-readelf").exists(), reason="library not built here")
-def test_setup_kernels_fit_their_occupancy(tmp_path):
-    """The setup kernel that clips in place runs one wave per SIMD at most
-    (its clipper keeps each plane's triangle in registers); the others must
-    leave room for several."""
+def test_setup_and_clip_kernels_fit_their_occupancy(tmp_path):
+    """Setup and the clipper keep several waves per SIMD: the clipper follows
+    one split path per lane (one triangle in registers, not the seven of a
+    walk over the split tree), so the clip kernel and the setup kernel that
+    clips in place stay within 128 VGPRs (four waves per SIMD); setup
+    without the clipper within 96."""
+    seen = 0
     for obj in objects():
         for name, m in kernel_metadata(obj, tmp_path).items():
-            if "k_setup" in name:
-                clips = re.search(r"k_setupILi\d+ELb0ELb1E", name) is not None
-                assert m["vgpr"] <= (512 if clips else 128), (obj.name, name, m)
+            if "k_setup" in name or "k_clip" in name:
+                seen += 1
+                clips = "k_clip" in name or re.search(r"k_setupILi\d+ELb[01]ELb1E", name) is not None
+                assert m["vgpr"] <= (128 if clips else 96), (obj.name, name, m)
+    assert seen >= 20, seen
