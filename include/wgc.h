@@ -430,8 +430,9 @@ typedef enum WgcOption {
     /* 1 (default): consecutive independent passes with one draw each share
      * their launches (the batched form); 0: one launch set per draw */
     WGC_OPTION_BATCHING = 2,
-    /* 1 (default): a command buffer's launches are captured once into a
-     * hipGraph and replayed on every later submission */
+    /* 1: a command buffer's launches are captured once into a hipGraph and
+     * replayed on every later submission; 0 (default): launched directly
+     * (cheaper on the host and on the GPU with this ROCm) */
     WGC_OPTION_GRAPHS = 3,
 } WgcOption;
 int32_t wgc_device_set_option(WgcDevice device, uint32_t option, uint32_t value);

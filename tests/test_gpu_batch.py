@@ -52,23 +52,25 @@ def test_batch_matches_oracle_per_pass(gpu_device, cfg):
 def test_graph_replay_and_batching_switch(gpu_device):
     announce("graph replay")
     specs = small_batch(12, seed=99)
-    pp = runner.PreparedPasses(gpu_device, specs)
-    pp.submit()
-    first = pp.results()
-    for _ in range(3):
-        pp.submit()
-    again = pp.results()
-    for a, b in zip(first, again):
-        assert np.array_equal(a.color, b.color) and np.array_equal(a.depth, b.depth)
-    # the same passes one launch set at a time, no graph
-    gpu_device.set_option(wgc.OPTION_BATCHING, 0)
-    gpu_device.set_option(wgc.OPTION_GRAPHS, 0)
+    graphs = gpu_device.get_option(wgc.OPTION_GRAPHS)
+    gpu_device.set_option(wgc.OPTION_GRAPHS, 1)
     try:
+        pp = runner.PreparedPasses(gpu_device, specs)
+        pp.submit()
+        first = pp.results()
+        for _ in range(3):   # the second captures the graph, the others replay it
+            pp.submit()
+        again = pp.results()
+        for a, b in zip(first, again):
+            assert np.array_equal(a.color, b.color) and np.array_equal(a.depth, b.depth)
+        # the same passes one launch set at a time, no graph
+        gpu_device.set_option(wgc.OPTION_BATCHING, 0)
+        gpu_device.set_option(wgc.OPTION_GRAPHS, 0)
         pp.submit()
         plain = pp.results()
     finally:
         gpu_device.set_option(wgc.OPTION_BATCHING, 1)
-        gpu_device.set_option(wgc.OPTION_GRAPHS, 1)
+        gpu_device.set_option(wgc.OPTION_GRAPHS, graphs)
     for a, b in zip(first, plain):
         assert np.array_equal(a.color, b.color) and np.array_equal(a.depth, b.depth)
 
@@ -79,6 +81,8 @@ def test_plan_rebuilt_between_unpolled_submissions(gpu_device):
     draw tables are let go only after those have executed."""
     announce("plan rebuilt between unpolled submissions")
     specs = small_batch(12, seed=5)
+    graphs = gpu_device.get_option(wgc.OPTION_GRAPHS)
+    gpu_device.set_option(wgc.OPTION_GRAPHS, 1)
     pp = runner.PreparedPasses(gpu_device, specs)
     try:
         pp.submit()
@@ -91,6 +95,7 @@ def test_plan_rebuilt_between_unpolled_submissions(gpu_device):
         res = pp.results()
     finally:
         gpu_device.set_option(wgc.OPTION_RASTER_TILE, 0)
+        gpu_device.set_option(wgc.OPTION_GRAPHS, graphs)
     check_against_oracle(specs, res, pp.total_fragments())
 
 
@@ -157,7 +162,7 @@ def test_full_batch_matches_oracle(gpu_device, full_batch, cfg):
         pp.submit()
         res = pp.results()
         total = pp.total_fragments()
-        # a replay of the captured graph renders the same images
+        # a second submission renders the same images
         pp.submit()
         again = pp.results()
     finally:

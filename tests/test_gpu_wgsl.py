@@ -98,21 +98,28 @@ def test_wgsl_programs_extended(gpu_device_ext, make):
 
 
 def test_wgsl_batched_passes(gpu_device):
-    """The batched form (one launch group for many passes, hipGraph replay)."""
+    """The batched form (one launch group for many passes), launched directly
+    and replayed as a hipGraph."""
     announce("wgsl batched")
     specs = []
     for i, (w, h, tris, tex) in enumerate(scenes.batched_sizes(0x5EED_0004, 12)):
         w, h, tris = max(16, w // 4), max(16, h // 4), min(tris, 5000.0)
         specs.append(scenes.as_wgsl(scenes.batched_pass(0x5EED_0004, i, w, h, tris, tex)))
-    pp = runner.PreparedPasses(gpu_device, specs)
-    for _ in range(2):      # the second submission replays the captured graph
-        pp.reset_attachments()
-        pp.submit()
-        results = pp.results()
-        for s, r in zip(specs, results):
-            ref = bridge.render(s)
-            assert np.array_equal(r.color, ref.color), s.name
-            assert np.array_equal(r.depth.view(np.uint32), ref.depth.view(np.uint32)), s.name
+    graphs = gpu_device.get_option(wgc.OPTION_GRAPHS)
+    try:
+        for use_graphs in (0, 1):
+            gpu_device.set_option(wgc.OPTION_GRAPHS, use_graphs)
+            pp = runner.PreparedPasses(gpu_device, specs)
+            for _ in range(3):      # with graphs: the second captures the graph, the third replays it
+                pp.reset_attachments()
+                pp.submit()
+                results = pp.results()
+                for s, r in zip(specs, results):
+                    ref = bridge.render(s)
+                    assert np.array_equal(r.color, ref.color), s.name
+                    assert np.array_equal(r.depth.view(np.uint32), ref.depth.view(np.uint32)), s.name
+    finally:
+        gpu_device.set_option(wgc.OPTION_GRAPHS, graphs)
 
 
 # ---------------------------------------------------------------- WGSL-only features

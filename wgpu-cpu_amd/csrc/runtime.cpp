@@ -561,7 +561,12 @@ struct WgcDevice_T : Object {
     uint32_t cu_count = 256;        // compute units of the device
     uint32_t raster_cfg = RASTER_16x16_256;
     bool batching = true;          // merge independent single-draw passes into launch groups
-    bool use_graphs = true;        // replay cached plans as hipGraphs
+    // replay cached plans as hipGraphs (WGC_OPTION_GRAPHS).  Off by default:
+    // on this ROCm a graph launch costs 22-27 us of host time per
+    // submission against 7 us for the same launches made directly, and the
+    // GPU runs the replayed launches slower (the 1080p pass 47.1 against
+    // 41.3 us per frame, the 256x256 pass 32.2 against 26.4)
+    bool use_graphs = false;
     uint32_t *frag_px = nullptr;   // optional per-pixel fragment counts (debug)
     size_t frag_px_size = 0;
 

@@ -2,8 +2,8 @@
 
 ``PreparedPasses`` uploads every resource once, records all passes into one
 command buffer and can submit it repeatedly; with several independent passes
-the runtime merges them into shared launches (the batched form) and replays
-the recorded launches as a hipGraph.
+the runtime merges them into shared launches (the batched form), launched
+directly or replayed as a hipGraph (``OPTION_GRAPHS``).
 """
 from __future__ import annotations
 
