@@ -81,6 +81,36 @@ def test_blended_lines_and_points(gpu_device_ext):
     both(spec, gpu_device_ext)
 
 
+@pytest.mark.parametrize("kind", ["blend", "not_equal"])
+def test_ordered_lines_over_many_tiles(gpu_device_ext, kind):
+    """The ordered point/line path over a target of 20x15 screen tiles:
+    long lines (up to 700 px, many leaving the target and clipped) cross
+    many tiles and overlap each other, so every pixel must see its
+    fragments in API order whichever tile applies them."""
+    w, h = 640, 480
+    u = scenes.splitmix_f32(4242, 1500 * 4).reshape(1500, 4).astype(np.float64)
+    vb = np.zeros((3000, 8), np.float32)
+    vb[0::2, 0] = u[:, 0] * 3.0 - 1.5
+    vb[0::2, 1] = u[:, 1] * 3.0 - 1.5
+    vb[1::2, 0] = u[:, 2] * 3.0 - 1.5
+    vb[1::2, 1] = u[:, 3] * 3.0 - 1.5
+    vb[:, 2] = np.where(np.arange(3000) % 3 == 0, 0.5, 0.25)
+    vb[:, 3] = 1.0
+    vb[:, 4:8] = scenes.splitmix_f32(4243, 3000 * 4).reshape(3000, 4)
+    vb[:, 7] = 0.4
+    pts = vertices(400, 4244, w, h)
+    pts[:, 7] = 0.6
+    if kind == "blend":
+        draws = [draw(vb, LINES, blend=wgc.BLEND_ALPHA), draw(pts, POINTS, blend=wgc.BLEND_ALPHA)]
+        spec = scenes.PassSpec(w, h, draws=draws, semantics=wgc.SEMANTICS_EXTENDED, name="ordered lines, blend")
+    else:
+        init = np.where(scenes.splitmix_f32(4245, w * h) < 0.5, 0.5, 0.25).astype(np.float32)
+        draws = [draw(vb, LINES, depth=(wgc.NOT_EQUAL, True)), draw(pts, POINTS, depth=(wgc.NOT_EQUAL, True))]
+        spec = scenes.PassSpec(w, h, has_depth=True, clear_depth=None, depth_init=init.tobytes(), draws=draws,
+                               semantics=wgc.SEMANTICS_EXTENDED, name="ordered lines, not-equal + write")
+    both(spec, gpu_device_ext)
+
+
 def test_triangle_strip(gpu_device):
     w, h = 256, 192
     vb = vertices(2000, 41, w, h, spread=1.5, w_range=(0.4, 2.0))

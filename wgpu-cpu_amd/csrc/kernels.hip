@@ -254,10 +254,14 @@ static hipError_t launch_lines_user(const UserKernels &k, const DrawParams &d0, 
 {
     DrawParams d = d0;
     unsigned long long *vis = gl.vis;
+    void *lp = gl.lp;
     const int n = d.topology == TOPO_POINT_LIST ? 0 : 1;
     uint32_t g = (d.total_prims + 255u) / 256u;
     if (gl.ordered) {
-        WGC_MLAUNCH("k_lp_ordered", k.lp_ordered[n], 1, 256, &d);
+        if (!g) return hipSuccess;
+        WGC_MLAUNCH("k_lp_prep", k.lp_prep[n], g, 256, &d, &lp);
+        uint32_t tiles = ((d.width + kLpTile - 1u) / kLpTile) * ((d.height + kLpTile - 1u) / kLpTile);
+        WGC_MLAUNCH("k_lp_ordered", k.lp_ordered[n], std::min(tiles, 65535u), 256, &d, &lp);
         return hipSuccess;
     }
     if (!g) return hipSuccess;

@@ -2396,91 +2396,169 @@ __global__ __launch_bounds__(256) void k_lp_shade(DrawParams d, const unsigned l
 }
 
 // Ordered points / lines (blending, NotEqual + write, or a fragment shader
-// that may discard): one workgroup takes the primitives in API order; thread
-// 0 sets up each primitive and lists its fragments, all threads shade them
-// (the fragments of one primitive are distinct pixels), in batches.
+// that may discard).  k_lp_prep runs every primitive's vertex stage and
+// set-up once; then each workgroup of k_lp_ordered owns one kLpTile-square
+// screen tile and takes the primitives in API order, 256 at a time: those
+// whose pixel bounds meet the tile, in order, have thread 0 walk them and
+// list their fragments inside the tile, all threads shade those (the
+// fragments of one primitive are distinct pixels), in batches.  A pixel
+// belongs to one tile, so its fragments are applied in API order as by one
+// walk over every primitive.
 constexpr int kLpBatch = 512;
+constexpr uint32_t kLpTile = 32;
+
+// A set-up point or line: its vertices, clipped end points and the pixel
+// bounds of its walk (x0 > x1: nothing to draw).
+template <int N>
+struct LpPrim {
+    VsOut v[N];
+    LpSetup s;
+    uint32_t x0, y0, x1, y1;
+};
+static_assert(sizeof(LpPrim<2>) <= kLpPrimBytes && sizeof(LpPrim<1>) <= kLpPrimBytes, "ordered point / line scratch");
 
 template <int PROG, int N>
-__global__ __launch_bounds__(256) void k_lp_ordered(DrawParams d)
+__global__ __launch_bounds__(256) void k_lp_prep(DrawParams d, LpPrim<N> *prims)
 {
-    __shared__ VsOut v[N];
-    __shared__ LpSetup s;
-    __shared__ LpFrag fr[kLpBatch];
-    __shared__ int ok, nf, done;
-    __shared__ Bres b;
-    __shared__ uint32_t step;
-    unsigned long long frags = 0;
-    for (uint32_t p = 0; p < d.total_prims; ++p) {
-        if (threadIdx.x == 0) {
-            ok = shade_primitive<PROG, N>(d, p, v) && lp_setup<N>(d, v, s) && s.visible;
-            done = 0;
-            if (ok && N == 2) b.init(s.fb[0], s.fb[1]);
-            step = 0;
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.total_prims) return;
+    LpPrim<N> r;
+    bool ok = shade_primitive<PROG, N>(d, p, r.v) && lp_setup<N>(d, r.v, r.s) && r.s.visible;
+    r.x0 = 1u;
+    r.x1 = 0u;
+    r.y0 = r.y1 = 0u;
+    if (ok) {
+        uint32_t e = N == 1 ? 0u : 1u;
+        r.x0 = min(r.s.fb[0][0], r.s.fb[e][0]);
+        r.x1 = max(r.s.fb[0][0], r.s.fb[e][0]);
+        r.y0 = min(r.s.fb[0][1], r.s.fb[e][1]);
+        r.y1 = max(r.s.fb[0][1], r.s.fb[e][1]);
+        // a line of more than kLineSteps steps: its walk stops there with an
+        // error (lp_walk), wherever its later steps fall
+        if (N == 2) {
+            uint32_t dx = r.x1 - r.x0, dy = r.y1 - r.y0;
+            if (max(dx, dy) > kLineSteps) atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE);
         }
-        __syncthreads();
-        while (ok && !done) {
-            if (threadIdx.x == 0) {
-                nf = 0;
-                if (N == 1) {
-                    lp_walk<N>(d, s, [&](const LpFrag &f) { fr[nf++] = f; });
-                    done = 1;
-                } else {
-                    uint32_t x, y;
-                    float t;
-                    while (nf < kLpBatch) {
-                        if (!b.next(x, y, t)) {
-                            done = 1;
-                            break;
+    }
+    prims[p] = r;
+}
+
+template <int PROG, int N>
+__global__ __launch_bounds__(256) void k_lp_ordered(DrawParams d, const LpPrim<N> *prims)
+{
+    __shared__ LpPrim<N> cur;
+    __shared__ LpFrag fr[kLpBatch];
+    __shared__ int nf, done;
+    __shared__ Bres b;
+    __shared__ uint32_t step, entered;
+    __shared__ unsigned long long touch[4];
+    unsigned long long frags = 0;
+    uint32_t tiles_x = (d.width + kLpTile - 1u) / kLpTile, tiles_y = (d.height + kLpTile - 1u) / kLpTile;
+    for (uint32_t tile = blockIdx.x; tile < tiles_x * tiles_y; tile += gridDim.x) {
+        // the tile's pixels, inclusive
+        uint32_t tx0 = (tile % tiles_x) * kLpTile, ty0 = (tile / tiles_x) * kLpTile;
+        uint32_t tx1 = min(tx0 + kLpTile, d.width) - 1u, ty1 = min(ty0 + kLpTile, d.height) - 1u;
+        for (uint32_t base = 0; base < d.total_prims; base += 256u) {
+            uint32_t p = base + threadIdx.x;
+            bool t = false;
+            if (p < d.total_prims) {
+                const LpPrim<N> &q = prims[p];
+                t = q.x0 <= q.x1 && q.x0 <= tx1 && q.x1 >= tx0 && q.y0 <= ty1 && q.y1 >= ty0;
+            }
+            uint64_t m = __ballot(t);
+            if (lane_id() == 0) touch[threadIdx.x >> 6] = m;
+            __syncthreads();
+#pragma unroll 1
+            for (uint32_t w = 0; w < 4u; ++w) {
+                for (uint64_t mm = touch[w]; mm; mm &= mm - 1u) {
+                    uint32_t pp = base + w * 64u + (uint32_t)(__ffsll((unsigned long long)mm) - 1);
+                    if (threadIdx.x == 0) {
+                        cur = prims[pp];
+                        done = 0;
+                        step = 0;
+                        entered = 0;
+                        if (N == 2) b.init(cur.s.fb[0], cur.s.fb[1]);
+                    }
+                    __syncthreads();
+                    while (!done) {
+                        if (threadIdx.x == 0) {
+                            nf = 0;
+                            if (N == 1) {
+                                lp_walk<N>(d, cur.s, [&](const LpFrag &f) {
+                                    if (f.x >= tx0 && f.x <= tx1 && f.y >= ty0 && f.y <= ty1) fr[nf++] = f;
+                                });
+                                done = 1;
+                            } else {
+                                // the steps inside the tile are consecutive (both
+                                // coordinates move monotonically): the walk ends
+                                // at the first step past them
+                                uint32_t x, y;
+                                float t;
+                                while (nf < kLpBatch) {
+                                    if (!b.next(x, y, t)) {
+                                        done = 1;
+                                        break;
+                                    }
+                                    uint32_t k = step++;
+                                    if (k >= kLineSteps) {   // the error is raised by k_lp_prep
+                                        done = 1;
+                                        break;
+                                    }
+                                    bool in = x >= tx0 && x <= tx1 && y >= ty0 && y <= ty1;
+                                    if (!in) {
+                                        if (entered) {
+                                            done = 1;
+                                            break;
+                                        }
+                                        continue;
+                                    }
+                                    entered = 1;
+                                    if (!in_scissor(d, x, y)) continue;
+                                    // raster.rs:251-254, as lp_walk
+                                    float u = 1.0f - t;
+                                    float l0 = cur.s.al[0] * u;
+                                    float l1 = cur.s.al[1] * t;
+                                    float L = l0 + l1;
+                                    float mv = 1.0f - L;
+                                    LpFrag f;
+                                    f.x = x;
+                                    f.y = y;
+                                    f.k = k;
+                                    for (int c = 0; c < 4; ++c) {
+                                        float a = cur.s.fr[0][c] * mv;
+                                        float bb = cur.s.fr[1][c] * L;
+                                        f.pos[c] = a + bb;
+                                    }
+                                    f.z = f.pos[2];
+                                    f.L = L;
+                                    fr[nf++] = f;
+                                }
+                            }
                         }
-                        uint32_t k = step++;
-                        if (k >= kLineSteps) {
-                            atomicOr(d.s.error, (uint32_t)ERR_COORD_RANGE);
-                            done = 1;
-                            break;
+                        __syncthreads();
+                        for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+                            const LpFrag &f = fr[i];
+                            uint32_t pix = f.y * d.width + f.x;
+                            ++frags;
+                            lp_count_fragment(d, pix);
+                            float color[4];
+                            float dep = d.depth_test ? d.depth[pix] : 0.0f;
+                            bool ok = depth_and_shade(d, f.z, dep, [&](float *frag_depth) {
+                                return lp_color<PROG, N>(d, cur.v, f, pp, color, frag_depth);
+                            });
+                            if (d.depth_test && d.depth_write) d.depth[pix] = dep;
+                            if (!ok) continue;
+                            if (d.color) {
+                                uint32_t *cp = reinterpret_cast<uint32_t *>(d.color) + pix;
+                                *cp = d.blend ? blend_over(color, *cp, d.color_bgra) : pack_color(color, d.color_bgra);
+                            }
                         }
-                        if (!in_scissor(d, x, y)) continue;
-                        float u = 1.0f - t;
-                        float l0 = s.al[0] * u;
-                        float l1 = s.al[1] * t;
-                        float L = l0 + l1;
-                        float m = 1.0f - L;
-                        LpFrag f;
-                        f.x = x;
-                        f.y = y;
-                        f.k = k;
-                        for (int c = 0; c < 4; ++c) {
-                            float a = s.fr[0][c] * m;
-                            float bb = s.fr[1][c] * L;
-                            f.pos[c] = a + bb;
-                        }
-                        f.z = f.pos[2];
-                        f.L = L;
-                        fr[nf++] = f;
+                        __syncthreads();
                     }
                 }
             }
-            __syncthreads();
-            for (int i = threadIdx.x; i < nf; i += blockDim.x) {
-                const LpFrag &f = fr[i];
-                uint32_t pix = f.y * d.width + f.x;
-                ++frags;
-                lp_count_fragment(d, pix);
-                float color[4];
-                float dep = d.depth_test ? d.depth[pix] : 0.0f;
-                bool ok = depth_and_shade(d, f.z, dep, [&](float *frag_depth) {
-                    return lp_color<PROG, N>(d, v, f, p, color, frag_depth);
-                });
-                if (d.depth_test && d.depth_write) d.depth[pix] = dep;
-                if (!ok) continue;
-                if (d.color) {
-                    uint32_t *cp = reinterpret_cast<uint32_t *>(d.color) + pix;
-                    *cp = d.blend ? blend_over(color, *cp, d.color_bgra) : pack_color(color, d.color_bgra);
-                }
-            }
-            __syncthreads();
+            __syncthreads();   // touch[] is written again by the next chunk
         }
-        __syncthreads();
     }
     unsigned long long tot = frags;
     for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);

@@ -117,7 +117,10 @@ static hipError_t launch_lines_t(const DrawParams &d, const GroupLaunch &gl, hip
 {
     uint32_t g = (d.total_prims + 255u) / 256u;
     if (gl.ordered) {
-        WGC_LAUNCH("k_lp_ordered", (k_lp_ordered<PROG, N>), 1, 256, d);
+        if (!g) return hipSuccess;
+        WGC_LAUNCH("k_lp_prep", (k_lp_prep<PROG, N>), g, 256, d, (LpPrim<N> *)gl.lp);
+        uint32_t tiles = ((d.width + kLpTile - 1u) / kLpTile) * ((d.height + kLpTile - 1u) / kLpTile);
+        WGC_LAUNCH("k_lp_ordered", (k_lp_ordered<PROG, N>), min(tiles, 65535u), 256, d, (const LpPrim<N> *)gl.lp);
         return hipSuccess;
     }
     if (!g) return hipSuccess;

@@ -2262,13 +2262,17 @@ static bool restart_strip(const DrawParams &d)
 
 static bool is_point_or_line(const DrawParams &d) { return d.topology <= TOPO_LINE_STRIP; }
 
-// Scratch of a point / line draw: the per-pixel visibility values.
+// Scratch of a point / line draw: the per-pixel visibility values, or for
+// an ordered draw the set-up primitives.
 static size_t layout_lines(Step &st, uint8_t *base, uint32_t *frag_px, size_t frag_px_size, WgcDevice_T *dev)
 {
     DrawParams &d = st.draws[0];
-    size_t bytes = d.resolve == RES_ORDERED ? 256 : align_up(8ull * d.width * d.height, 256);
+    bool ordered = d.resolve == RES_ORDERED;
+    size_t bytes = ordered ? align_up(std::max<uint64_t>((uint64_t)kLpPrimBytes * d.total_prims, 256), 256)
+                           : align_up(8ull * d.width * d.height, 256);
     if (!base) return bytes;
-    st.gl.vis = d.resolve == RES_ORDERED ? nullptr : (unsigned long long *)base;
+    st.gl.vis = ordered ? nullptr : (unsigned long long *)base;
+    st.gl.lp = ordered ? (void *)base : nullptr;
     d.s.frags = dev->d_frags;
     d.s.error = dev->d_error;
     d.s.frag_px = (frag_px && frag_px_size == (size_t)d.width * d.height * 4u) ? frag_px : nullptr;
@@ -2387,6 +2391,7 @@ static size_t layout_group(Step &st, uint8_t *base, uint8_t *zone, uint8_t *keep
         gl.inline_clip = blocks <= (uint64_t)dev->cu_count * per_cu ? 1u : 0u;
     }
     gl.vis = nullptr;
+    gl.lp = nullptr;
     b.n = n;
     b.total_tiles = (uint32_t)total_tiles;
     b.n_chunks = (uint32_t)((total_tiles + kClearChunk - 1) / kClearChunk);
@@ -2810,7 +2815,7 @@ static int32_t build_plan(WgcDevice_T *dev, WgcCommandBuffer_T *cb, Plan &plan, 
     for (Step &s : plan.steps) {
         if (s.kind == StepKind::Lines) {
             layout_lines(s, arena_base + s.arena_off, dev->frag_px, dev->frag_px_size, dev);
-            plan.launches += s.gl.ordered ? 1 : 3;
+            plan.launches += s.draws[0].total_prims == 0 ? 0 : (s.gl.ordered ? 2 : 3);
             plan.prims += s.draws[0].total_prims;
             continue;
         }

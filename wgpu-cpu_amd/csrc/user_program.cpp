@@ -391,6 +391,7 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
         "wgc::k_lp_shade<5, 1>",           "wgc::k_lp_shade<5, 2>",
         "wgc::k_lp_ordered<5, 1>",         "wgc::k_lp_ordered<5, 2>",
         "wgc::k_setup<5, false, true>",
+        "wgc::k_lp_prep<5, 1>",            "wgc::k_lp_prep<5, 2>",
     };
     Compiled c;
     if (!rtc_build(hip_device, src, {{"wgc_user.h", g}}, names, c, err)) return false;
@@ -410,6 +411,8 @@ bool build_user_program(int hip_device, const wgsl::Module &vm, const std::strin
         k.occ_raster[m] = occupancy(k.raster[m], 256);
     }
     k.setup_clip = c.fns[14];
+    k.lp_prep[0] = c.fns[15];
+    k.lp_prep[1] = c.fns[16];
     k.occ_setup_clip = occupancy(k.setup_clip, 256);
     return true;
 }

@@ -50,6 +50,7 @@ enum Topology : uint32_t {
 };
 constexpr uint32_t kStripChunk = 256;   // index positions per separator summary (restart strips)
 constexpr uint32_t kLineSteps = 8192;   // Bresenham steps per line (the fragment key holds 13 bits)
+constexpr uint32_t kLpPrimBytes = 192;   // scratch per primitive of an ordered point / line draw
 enum Compare : uint32_t {
     CMP_NEVER = 1, CMP_LESS = 2, CMP_EQUAL = 3, CMP_LESS_EQUAL = 4,
     CMP_GREATER = 5, CMP_NOT_EQUAL = 6, CMP_GREATER_EQUAL = 7, CMP_ALWAYS = 8,

@@ -24,6 +24,7 @@ struct GroupLaunch {
     uint32_t clip_cap;
     uint32_t inline_clip;        // single draw whose setup waves all fit at once: setup clips
     unsigned long long *vis;     // points and lines: one visibility value per pixel
+    void *lp;                    // ordered points and lines: kLpPrimBytes per primitive (LpPrim)
 };
 
 // Brackets launches with HIP events on the launching stream; the runtime
@@ -92,7 +93,7 @@ struct KernelTimer {
 struct UserKernels {
     hipModule_t module = nullptr;
     hipFunction_t setup[2]{}, clip[2]{}, tiles[2]{}, raster[2]{};
-    hipFunction_t lp_vis[2]{}, lp_shade[2]{}, lp_ordered[2]{};   // index: points 0, lines 1
+    hipFunction_t lp_vis[2]{}, lp_shade[2]{}, lp_ordered[2]{}, lp_prep[2]{};   // index: points 0, lines 1
     hipFunction_t setup_clip{};   // single draws: setup that clips in place
     uint32_t occ_tiles[2]{}, occ_raster[2]{}, occ_setup_clip{};
 };
