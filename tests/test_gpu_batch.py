@@ -1,6 +1,7 @@
 """The batched form: many independent passes in one submission share their
-launches and are replayed as a hipGraph.  Every pass must equal the
-oracle's render of it alone, for every raster tile shape."""
+launches (launched directly, or replayed as a hipGraph when that option is
+set).  Every pass must equal the oracle's render of it alone, for every
+raster tile shape."""
 import sys
 
 import numpy as np
