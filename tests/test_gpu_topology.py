@@ -163,6 +163,24 @@ def test_instanced_points_and_lines_with_viewport_and_scissor(gpu_device):
     both(spec, gpu_device)
 
 
+def test_blended_lines_with_viewport_scissor_and_instances(gpu_device_ext):
+    """The ordered point/line path under a viewport and a scissor rect that
+    cut the screen tiles it walks (fragments outside the scissor are
+    skipped within a tile's run of steps), with instancing: every pixel's
+    blends in API order."""
+    w, h = 300, 200
+    vb = vertices(800, 53, w, h, spread=1.4)
+    vb[:, 7] = 0.45
+    d1 = draw(vb, LINE_STRIP, blend=wgc.BLEND_ALPHA)
+    d1.instance_count = 2
+    d1.viewport = (15.0, 25.0, 260.0, 150.0, 0.0, 1.0)
+    d1.scissor = (37, 41, 190, 101)
+    d2 = draw(vertices(500, 59, w, h), POINTS, blend=wgc.BLEND_ALPHA)
+    d2.scissor = (5, 70, 250, 60)
+    spec = scenes.PassSpec(w, h, draws=[d1, d2], semantics=wgc.SEMANTICS_EXTENDED, name="blended lines, scissor")
+    both(spec, gpu_device_ext)
+
+
 def test_strip_restart_format_mismatch_is_an_error(gpu_device):
     announce("restart format mismatch")
     w, h = 64, 64
