@@ -290,12 +290,14 @@ hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t 
 }
 
 hipError_t launch_end_submission(unsigned long long *frags, uint32_t *error, void *host_out, void *zone,
-                                 size_t zone_bytes, hipStream_t st)
+                                 size_t zone_bytes, hipStream_t st, hipEvent_t done)
 {
     uint32_t n16 = (uint32_t)(zone_bytes / 16);
     uint32_t grid = std::max(1u, std::min((n16 + 1023u) / 1024u, 1024u));
-    hipLaunchKernelGGL(k_end_submission, dim3(grid), dim3(256), 0, st, frags, error, (uint4 *)host_out,
-                       (uint4 *)zone, n16);
+    // the completion event rides on the kernel's own dispatch (no marker
+    // packet of its own after it)
+    hipExtLaunchKernelGGL(k_end_submission, dim3(grid), dim3(256), 0, st, nullptr, done, 0u, frags, error,
+                          (uint4 *)host_out, (uint4 *)zone, n16);
     return hipGetLastError();
 }
 

@@ -123,9 +123,10 @@ hipError_t launch_reset(unsigned long long *frags, uint32_t *error, hipStream_t 
 // Copies the submission counters to host memory (16 bytes, aligned: the
 // fragment count summed over the kFragSlots counters as two 32-bit words,
 // then the error word and 0) and zeroes them, and clears the first
-// zone_bytes (a multiple of 16) of the scratch arena's zero region.
+// zone_bytes (a multiple of 16) of the scratch arena's zero region; `done`
+// is recorded when the kernel completes (the submission's completion).
 hipError_t launch_end_submission(unsigned long long *frags, uint32_t *error, void *host_out, void *zone,
-                                 size_t zone_bytes, hipStream_t st);
+                                 size_t zone_bytes, hipStream_t st, hipEvent_t done);
 // Row copy (4-byte multiples; 16-byte accesses where aligned) and word fill,
 // ordered on the stream like the render launches.
 hipError_t launch_copy_rows(const void *src, uint64_t src_pitch, void *dst, uint64_t dst_pitch, uint64_t row_bytes,
