@@ -59,8 +59,14 @@ def time_gpu(dev, pp, steps: int, warmup: int):
     return ev_ms, wall * 1e3
 
 
-def kernel_profile(dev, pp, reps: int):
+def kernel_profile(dev, pp, reps: int, warmup: int = 5):
+    """Average event-timed duration of each kernel over `reps` submissions,
+    after `warmup` untimed ones in the same (kernel-timing) launch mode."""
     dev.set_kernel_timing(True)
+    for _ in range(warmup):
+        pp.submit()
+    dev.poll()
+    dev.set_kernel_timing(True)   # clears what the warm-up launches recorded
     for _ in range(reps):
         pp.submit()
     dev.poll()
