@@ -84,6 +84,35 @@ def test_noalloc_several_passes_and_resubmission(gpu_device):
     assert dev.last_pass_stats().fragments == want_frags
 
 
+def test_noalloc_ordered_points_and_lines(gpu_device_ext):
+    """Blended lines and points (the ordered point/line path, whose set-up
+    primitives are laid out in the static array too), submitted twice."""
+    announce("noalloc: blended lines and points")
+    w, h = 320, 240
+    u = scenes.splitmix_f32(9100, 600 * 4).reshape(600, 4)
+    vb = np.zeros((1200, 8), np.float32)
+    vb[0::2, 0], vb[0::2, 1] = u[:, 0] * 2.4 - 1.2, u[:, 1] * 2.4 - 1.2
+    vb[1::2, 0], vb[1::2, 1] = u[:, 2] * 2.4 - 1.2, u[:, 3] * 2.4 - 1.2
+    vb[:, 2], vb[:, 3] = 0.5, 1.0
+    vb[:, 4:8] = scenes.splitmix_f32(9101, 1200 * 4).reshape(1200, 4)
+    vb[:, 7] = 0.5
+    draws = []
+    for topology, count in ((1, 1200), (0, 300)):   # lines, then points
+        draws.append(scenes.DrawSpec(program=wgc.PROGRAM_CLIP_COLOR, vertex_data=vb.tobytes(), vertex_stride=32,
+                                     attributes=[(wgc.FLOAT32X4, 0, 0), (wgc.FLOAT32X4, 16, 1)],
+                                     blend=wgc.BLEND_ALPHA, count=count, topology=topology))
+    spec = scenes.PassSpec(w, h, draws=draws, semantics=wgc.SEMANTICS_EXTENDED, name="noalloc blended lines")
+    pp = runner.PreparedPass(gpu_device_ext, spec)
+    want, want_frags = _render(pp, noalloc=False)
+    for _ in range(2):
+        pp.reset_attachments()
+        got, got_frags = _render(pp, noalloc=True)
+        _same(got, want)
+        assert got_frags == want_frags
+    ref = bridge.render(spec)
+    assert np.array_equal(got[0].color, ref.color) and ref.fragments == got_frags
+
+
 def test_noalloc_refuses_what_does_not_fit(gpu_device):
     """200k triangles over 2560x2440 need more scratch than the static array
     holds: refused with WGC_ERR_OUT_OF_MEMORY, nothing launched, and the
