@@ -100,6 +100,30 @@ def test_plan_rebuilt_between_unpolled_submissions(gpu_device):
     check_against_oracle(specs, res, pp.total_fragments())
 
 
+def test_batched_clipping_of_every_plane(gpu_device):
+    """Passes of triangles that cross every clip plane (near and far
+    included), batched into one launch group: the clip kernel's lanes clip
+    primitives of different draws, and the primitives whose split paths
+    have more than three splits go round again on the whole wave."""
+    announce("batched clipping, every plane")
+    sys.path.insert(0, str(__import__("pathlib").Path(__file__).parent))
+    from test_gpu_parity import clip_color_draw, random_clip_triangles
+    specs = []
+    for i, (w, h) in enumerate(((160, 120), (96, 200), (240, 64), (128, 128))):
+        vb = random_clip_triangles(1500, 300 + i, w, h, w_range=(0.3, 3.0), spread=2.0, size_px=(5.0, 300.0))
+        vb[:, :, 2] = (vb[:, :, 2] / vb[:, :, 3] * 2.0 - 0.5) * vb[:, :, 3]
+        specs.append(scenes.PassSpec(w, h, has_depth=True, clear_depth=1.0,
+                                     draws=[clip_color_draw(vb, depth=(wgc.LESS, True))], name=f"clip_batch_{i}"))
+    res, total = runner.render_batch(specs, gpu_device)
+    want = 0
+    for s, r in zip(specs, res):
+        ref = bridge.render(s)
+        want += ref.fragments
+        assert np.array_equal(r.color, ref.color), s.name
+        assert np.array_equal(r.depth.view(np.uint32), ref.depth.view(np.uint32)), s.name
+    assert total == want
+
+
 def test_dependent_passes_stay_ordered(gpu_device):
     """Two passes on the same target: the second loads what the first drew,
     so they must not share a launch group."""
